@@ -1,0 +1,240 @@
+/* isaac_hip.h — C-ABI of the MI355X (gfx950) in-situ ray-casting renderer.
+ *
+ * One context = one rank's IsaacVisualization render state on one GPU: the
+ * per-voxel classification pass, the particle/iso/SSAO/shading/volume passes
+ * of the IceT draw callback, and the steering setters that feed them.  There
+ * is no IceT, MPI, JSON, JPEG or TCP behind this interface: the caller (the
+ * C++ shim include/isaac_mi355x.hpp, Python through ctypes, or a cgo/JNI
+ * binding) owns those.
+ *
+ * Conventions
+ *   - every function returns 0 on success and a negative ISAAC_HIP_ERR_* on
+ *     failure; isaac_hip_last_error(ctx) then holds a message.  Setters that
+ *     the reference exposes as bool return 1/0 in that position instead.
+ *   - device pointers are plain `const void*` / `const float*`; streams are
+ *     passed as `void*` (a hipStream_t).  No torch or HIP types appear here.
+ *   - calls on one context are not thread safe; several contexts may coexist
+ *     (the reference's `static myself` limit, lib/isaac.hpp:1307,3029, does
+ *     not apply).
+ *   - source indices: volume sources 0..nv-1, then field sources, then
+ *     particle sources (the reference's combined source list order,
+ *     lib/isaac.hpp:720-731).
+ */
+#ifndef ISAAC_HIP_H
+#define ISAAC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C"
+{
+#endif
+
+#define ISAAC_HIP_MAX_SOURCES 16 /* ISAAC_MAX_SOURCES   lib/isaac/isaac_defines.hpp:36-38 */
+#define ISAAC_HIP_MAX_FUNCTORS 3 /* ISAAC_MAX_FUNCTORS  lib/isaac/isaac_defines.hpp:27-29 */
+#define ISAAC_HIP_MAX_CLIPPING 16 /* ISAAC_MAX_CLIPPING  lib/isaac/isaac_defines.hpp:44-46 */
+
+    enum isaac_hip_status
+    {
+        ISAAC_HIP_OK = 0,
+        ISAAC_HIP_ERR_ARG = -1,
+        ISAAC_HIP_ERR_DEVICE = -2,
+        ISAAC_HIP_ERR_STATE = -3,
+        ISAAC_HIP_ERR_UNSUPPORTED = -4,
+        ISAAC_HIP_ERR_ALLOC = -5
+    };
+
+    enum isaac_hip_dtype
+    {
+        ISAAC_HIP_F32 = 0,
+        ISAAC_HIP_F16 = 1,
+        ISAAC_HIP_BF16 = 2
+    };
+
+    /* counters filled when counting is enabled (isaac_hip_set_counting) */
+    enum isaac_hip_counter
+    {
+        ISAAC_HIP_CNT_VOL_SAMPLES = 0,
+        ISAAC_HIP_CNT_ISO_SAMPLES = 1,
+        ISAAC_HIP_CNT_ISO_HITS = 2,
+        ISAAC_HIP_CNT_CELLS_VISITED = 3,
+        ISAAC_HIP_CNT_SPHERE_TESTS = 4,
+        ISAAC_HIP_CNT_VOL_RAYS = 5,
+        ISAAC_HIP_CNT_ISO_RAYS = 6,
+        ISAAC_HIP_CNT_PART_RAYS = 7,
+        ISAAC_HIP_CNT_COUNT = 8
+    };
+
+    typedef struct isaac_hip_ctx isaac_hip_ctx;
+
+    /* Replaces the IsaacVisualization constructor's size arguments
+     * (lib/isaac.hpp:565-583) and the source-list template parameters
+     * (lib/isaac.hpp:58-69). */
+    typedef struct isaac_hip_config
+    {
+        int32_t framebuffer_size[2]; /* framebufferSize (W, H) */
+        int64_t global_size[3]; /* globalSize */
+        int64_t local_size[3]; /* localSize */
+        int64_t local_particle_size[3]; /* localParticleSize (particle cell grid) */
+        int32_t position[3]; /* position of the local box in the global box */
+        float scale[3]; /* scale */
+        int32_t transfer_size; /* T_transferSize */
+        int32_t n_volume_sources; /* size of T_VolumeSourceList */
+        int32_t n_field_sources; /* size of T_FieldSourceList (rendered as volume sources, no advection) */
+        int32_t n_particle_sources; /* size of T_ParticleList */
+    } isaac_hip_config;
+
+    /* A volume source as the precompiled library sees it: the reference's
+     * `operator[](isaac_int3)` (example/example.cpp:39-143) reduced to a strided
+     * device array.  Element (x,y,z) lives at data + (x*sx + y*sy + z*sz)*feature_dim
+     * components; coordinates -guard_size .. local+guard_size-1 must be readable. */
+    typedef struct isaac_hip_volume_desc
+    {
+        const void* data; /* device pointer at local voxel (0,0,0) */
+        int32_t dtype; /* ISAAC_HIP_F32 */
+        int32_t feature_dim; /* 1..4 (featureDim) */
+        int64_t stride[3]; /* element strides x, y, z */
+        int32_t guard_size; /* guardSize */
+        int32_t persistent; /* persistent: read at every classification; else copied
+                               functor-applied into a library texture (UpdatePersistendTextureKernel) */
+    } isaac_hip_volume_desc;
+
+    /* A particle source: raw particles in local voxel coordinates; the library
+     * builds the per-cell lists that the reference's getIterator(cell) yields
+     * (example/example.cpp:148-228). */
+    typedef struct isaac_hip_particle_desc
+    {
+        const float* position; /* device, count x 3, local voxel coordinates */
+        const float* attribute; /* device, count x feature_dim (getAttribute) */
+        int32_t feature_dim; /* 1..4 */
+        const float* radius; /* device, count radii, or NULL */
+        float constant_radius; /* getRadius() when radius == NULL */
+        int64_t count;
+    } isaac_hip_particle_desc;
+
+    /* Per-phase wall times in microseconds (the reference's public timing
+     * members, lib/isaac.hpp:2129-2137) */
+    typedef struct isaac_hip_timings
+    {
+        double kernel_us; /* drawCallBack kernels */
+        double copy_us; /* framebuffer copy */
+        double buffer_us; /* source update + particle binning */
+        double optimization_buffer_us; /* classification */
+    } isaac_hip_timings;
+
+    /* ---------------------------------------------------------------- lifetime */
+    int isaac_hip_create(const isaac_hip_config* cfg, int device, isaac_hip_ctx** out);
+    void isaac_hip_destroy(isaac_hip_ctx* ctx);
+    const char* isaac_hip_last_error(const isaac_hip_ctx* ctx);
+    int isaac_hip_set_stream(isaac_hip_ctx* ctx, void* hip_stream);
+
+    /* ---------------------------------------------------------------- sources */
+    int isaac_hip_set_volume_source(isaac_hip_ctx* ctx, int idx, const isaac_hip_volume_desc* desc);
+    int isaac_hip_set_particle_source(isaac_hip_ctx* ctx, int idx, const isaac_hip_particle_desc* desc);
+
+    /* ---------------------------------------------------------------- steering
+     * (the JSON keys of doVisualization, lib/isaac.hpp:1564-1795, as setters) */
+    /* updateFunctions for one source (lib/isaac.hpp:1079-1241); *error_code gets
+     * functions[i].errorCode: 0 ok, 1 too many functors, 2 too many params,
+     * -1 missing ')', -2 bad number / unknown functor */
+    int isaac_hip_set_function(isaac_hip_ctx* ctx, int idx, const char* chain, int32_t* error_code);
+    /* updateTransfer from control points (lib/isaac.hpp:1244-1266) */
+    int isaac_hip_set_transfer_points(isaac_hip_ctx* ctx, int idx, int n, const uint32_t* positions, const float* rgba);
+    /* a ready LUT of transfer_size RGBA floats (host memory) */
+    int isaac_hip_set_transfer_lut(isaac_hip_ctx* ctx, int idx, const float* lut_rgba);
+    int isaac_hip_set_weight(isaac_hip_ctx* ctx, int idx, float weight);
+    int isaac_hip_set_iso_threshold(isaac_hip_ctx* ctx, int idx, float threshold);
+    int isaac_hip_set_step(isaac_hip_ctx* ctx, float step); /* clamped to >= 0.01 */
+    int isaac_hip_set_interpolation(isaac_hip_ctx* ctx, int trilinear);
+    int isaac_hip_set_dither_mode(isaac_hip_ctx* ctx, int mode); /* 0 round, 1 ordered 4^3 */
+    int isaac_hip_set_ao(isaac_hip_ctx* ctx, int enabled, float weight);
+    int isaac_hip_set_render_mode(isaac_hip_ctx* ctx, int mode); /* 0..8 */
+    /* clipping planes (lib/isaac.hpp:958-1017): add/edit return 1 on success, 0 as the reference's false */
+    int isaac_hip_add_clipping(isaac_hip_ctx* ctx, float px, float py, float pz, float nx, float ny, float nz);
+    int isaac_hip_edit_clipping(
+        isaac_hip_ctx* ctx,
+        uint32_t nr,
+        float px,
+        float py,
+        float pz,
+        float nx,
+        float ny,
+        float nz);
+    int isaac_hip_remove_clipping(isaac_hip_ctx* ctx, uint32_t nr);
+    int isaac_hip_clipping_count(const isaac_hip_ctx* ctx);
+    /* updatePosition / updateLocalSize / updateLocalParticleSize (lib/isaac.hpp:1056-1076) */
+    int isaac_hip_update_position(isaac_hip_ctx* ctx, const int32_t position[3]);
+    int isaac_hip_update_local_size(isaac_hip_ctx* ctx, const int64_t local_size[3]);
+    int isaac_hip_update_local_particle_size(isaac_hip_ctx* ctx, const int64_t local_particle_size[3]);
+
+    /* ---------------------------------------------------------------- per frame
+     * doVisualization's buffer section (lib/isaac.hpp:1805-1948): non-persistent
+     * source copies, particle cell lists, classification into the combined
+     * RGBA8 volume/iso textures.  Runs on the context stream. */
+    int isaac_hip_update_buffers(isaac_hip_ctx* ctx);
+    /* drawCallBack (lib/isaac.hpp:2140-2543): proj/mv column-major doubles as
+     * IceT passes them, bg as IceT passes it, viewport = readbackViewport
+     * (x, y, w, h).  The framebuffer (RGBA8, premultiplied, row 0 = bottom) is
+     * copied to out_rgba (W*H*4 bytes, host or device per out_is_device) unless
+     * out_rgba is NULL.  A host copy waits for completion; otherwise the call
+     * returns after enqueueing on the context stream. */
+    int isaac_hip_render(
+        isaac_hip_ctx* ctx,
+        const double proj[16],
+        const double modelview[16],
+        const float bg[4],
+        const int32_t viewport[4],
+        uint8_t* out_rgba,
+        int out_is_device);
+    int isaac_hip_synchronize(isaac_hip_ctx* ctx);
+
+    /* ---------------------------------------------------------------- introspection */
+    int isaac_hip_get_timings(const isaac_hip_ctx* ctx, isaac_hip_timings* out);
+    int isaac_hip_set_counting(isaac_hip_ctx* ctx, int enabled);
+    int isaac_hip_get_counters(isaac_hip_ctx* ctx, uint64_t out[ISAAC_HIP_CNT_COUNT]);
+    /* device pointers of the G-buffer (colour RGBA8, depth f32, normal 3xf32, ao f32), W*H each */
+    int isaac_hip_get_gbuffer(
+        const isaac_hip_ctx* ctx,
+        const void** color,
+        const void** depth,
+        const void** normal,
+        const void** ao);
+    /* device pointers of the combined RGBA8 textures, x-fastest local voxels */
+    int isaac_hip_get_combined_textures(const isaac_hip_ctx* ctx, const void** volume, const void** iso);
+    /* device pointers of a particle source's exact cell list: per-cell count
+     * (int32) and exclusive offset (int32, ncells+1), and the stable
+     * permutation of particle indices (int32) */
+    int isaac_hip_get_cell_list(
+        const isaac_hip_ctx* ctx,
+        int particle_idx,
+        const void** counts,
+        const void** offsets,
+        const void** perm);
+
+    /* ---------------------------------------------------------------- host helpers (no GPU needed) */
+    /* DefaultController projection: glm::perspective(45.0, w/h, 0.002, 100)
+     * (lib/isaac/isaac_controllers.hpp:33-37), column-major */
+    void isaac_hip_default_projection(int32_t w, int32_t h, double out[16]);
+    /* updateModelview (lib/isaac.hpp:2878-2889); rotation column-major 3x3 */
+    void isaac_hip_modelview(const double look_at[3], const double rotation[9], double distance, double out[16]);
+    /* the parser of updateFunctions for one chain; returns errorCode */
+    int32_t isaac_hip_parse_functor_chain(const char* chain, int32_t bytecode[3], float params[12]);
+    /* updateTransfer's LUT build; returns 0 */
+    int isaac_hip_build_transfer_lut(int32_t T, int32_t n, const uint32_t* positions, const float* rgba, float* lut);
+    /* default TF control points of source i of n (lib/isaac.hpp:772-792): 2 points */
+    int isaac_hip_default_transfer(
+        int32_t i,
+        int32_t n_sources,
+        int32_t is_particle,
+        int32_t T,
+        uint32_t positions[2],
+        float rgba[8]);
+    /* library version string */
+    const char* isaac_hip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ISAAC_HIP_H */

@@ -1,0 +1,221 @@
+// Shared host/device definitions of the gfx950 render path.
+//
+// All arithmetic helpers spell out the association order of the reference's
+// glm expressions; the library is compiled with -ffp-contract=off so each
+// + - * / below is exactly one IEEE-754 binary32 operation, as on the
+// reference's CPU path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+namespace isaac_hip
+{
+    constexpr int kMaxSources = 16;
+    constexpr int kMaxFunctors = 3;
+    constexpr int kMaxClipping = 16;
+    constexpr float kZNear = 0.002f; // ISAAC_Z_NEAR lib/isaac/isaac_defines.hpp:53
+
+    enum Op : int32_t
+    {
+        OP_IDEM = 0,
+        OP_ADD = 1,
+        OP_MUL = 2,
+        OP_LENGTH = 3,
+        OP_POW = 4,
+        OP_SUM = 5
+    };
+
+    // ------------------------------------------------------------------ kernel parameter blocks
+    struct ChainDesc
+    {
+        int32_t op[kMaxFunctors];
+        float p[kMaxFunctors][4];
+    };
+
+    struct ClassifySource
+    {
+        const float* data; // local voxel (0,0,0)
+        int64_t stride[3]; // element strides
+        int32_t dim;
+        int32_t active; // weight + iso > 0
+        float weight;
+        float iso;
+        ChainDesc chain;
+        int32_t lutSlot; // index of the LUT in the staged table
+    };
+
+    struct ClassifyParams
+    {
+        int32_t X, Y, Z;
+        int32_t nsrc;
+        float totalWeight;
+        int32_t ditherMode;
+        int32_t T;
+        int32_t nLut; // LUTs staged in LDS (0 = read from global)
+        uint32_t divX_mul, divX_shift, divXY_mul, divXY_shift;
+        const float4* luts; // nsrc * T
+        uint32_t* volTex;
+        uint32_t* isoTex;
+        int64_t nvox;
+        ClassifySource src[kMaxSources];
+    };
+
+    // Per-frame ray / G-buffer state (drawCallBack, lib/isaac.hpp:2140-2283).
+    struct FrameParams
+    {
+        float inv[16]; // inverse MVP, column major
+        int32_t W, H;
+        int32_t vx0, vy0, vx1, vy1; // viewport clipped to the framebuffer
+        int32_t tilesX, tilesY; // 16x16 pixel tiles over the viewport
+        float fbW, fbH;
+        float maxSize; // maxGlobalSizeScaled * 0.5
+        float off[3]; // position_offset (clipRay)
+        float bbMin[3], bbMax[3]; // local box in scaled voxels
+        int32_t outer[3]; // face i is an outer face for every sign (see clipRay)
+        int32_t outerLow[3], outerHigh[3];
+        int32_t nclip;
+        float cpos[kMaxClipping][3]; // plane positions, scaled and offset
+        float cnorm[kMaxClipping][3];
+        float scale[3];
+        float step;
+        int32_t linear;
+        // textures
+        int32_t X, Y, Z;
+        const uint32_t* volTex;
+        const uint32_t* isoTex;
+        int32_t isoEnabled;
+        int32_t volumeEnabled;
+        float minSize;
+        float totalWeightAll;
+        // shading
+        int32_t renderMode;
+        int32_t aoEnabled;
+        float aoWeight;
+        float bg[4]; // bgColor (channel reversed)
+        uint32_t bgClear; // transformColor(bg with w = 0)
+        float maxGlobalSizeScaled;
+        float rankHue[4]; // getHSVA(halton(rank,3)*2pi,1,1,1) for render modes 7/8
+        // G-buffer
+        uint32_t* color;
+        float* depth;
+        float* normal; // 3 floats per pixel
+        float* ao;
+        unsigned long long* counters;
+    };
+
+    struct ParticleSourceDev
+    {
+        const int32_t* voff; // ncells + 1 exclusive offsets into recs (visible particles)
+        const float4* recs; // rel.xyz (cell units), lut index bits in .w
+        const float* radius; // per rec or nullptr
+        float constRadius;
+        float weight;
+        const float4* lut; // T entries
+    };
+
+    struct ParticleParams
+    {
+        int32_t nsrc; // active (weight != 0) particle sources
+        int32_t grid[3];
+        float pscale[3];
+        ParticleSourceDev src[4];
+    };
+
+    // ------------------------------------------------------------------ device math
+    struct F3
+    {
+        float x, y, z;
+    };
+
+    __host__ __device__ inline F3 f3(float x, float y, float z)
+    {
+        return F3{x, y, z};
+    }
+    __host__ __device__ inline F3 add(F3 a, F3 b)
+    {
+        return F3{a.x + b.x, a.y + b.y, a.z + b.z};
+    }
+    __host__ __device__ inline F3 sub(F3 a, F3 b)
+    {
+        return F3{a.x - b.x, a.y - b.y, a.z - b.z};
+    }
+    __host__ __device__ inline F3 mul(F3 a, F3 b)
+    {
+        return F3{a.x * b.x, a.y * b.y, a.z * b.z};
+    }
+    __host__ __device__ inline F3 divv(F3 a, F3 b)
+    {
+        return F3{a.x / b.x, a.y / b.y, a.z / b.z};
+    }
+    __host__ __device__ inline F3 muls(F3 a, float s)
+    {
+        return F3{a.x * s, a.y * s, a.z * s};
+    }
+    __host__ __device__ inline F3 divs(F3 a, float s)
+    {
+        return F3{a.x / s, a.y / s, a.z / s};
+    }
+    // glm dot for vec3: (x*x' + y*y') + z*z'
+    __host__ __device__ inline float dot3(F3 a, F3 b)
+    {
+        float tx = a.x * b.x, ty = a.y * b.y, tz = a.z * b.z;
+        return tx + ty + tz;
+    }
+    __host__ __device__ inline float len3(F3 a)
+    {
+        return sqrtf(dot3(a, a));
+    }
+    __host__ __device__ inline F3 normalize3(F3 a)
+    {
+        return muls(a, 1.0f / sqrtf(dot3(a, a)));
+    }
+    __host__ __device__ inline float gmin(float a, float b)
+    {
+        return (b < a) ? b : a;
+    }
+    __host__ __device__ inline float gmax(float a, float b)
+    {
+        return (a < b) ? b : a;
+    }
+    __host__ __device__ inline float gclamp(float x, float lo, float hi)
+    {
+        return gmin(gmax(x, lo), hi);
+    }
+    __host__ __device__ inline float gsign(float x)
+    {
+        return float((0.0f < x) ? 1 : 0) - float((x < 0.0f) ? 1 : 0);
+    }
+    __host__ __device__ inline int iclamp(int v, int lo, int hi)
+    {
+        return v < lo ? lo : (v > hi ? hi : v);
+    }
+
+    // transformColor float4 -> byte4 (truncating, lib/isaac/isaac_helper.hpp:71-74)
+    __host__ __device__ inline uint32_t to_byte4(float r, float g, float b, float a)
+    {
+        uint32_t q0 = (uint32_t) (gclamp(r, 0.0f, 1.0f) * 255.0f);
+        uint32_t q1 = (uint32_t) (gclamp(g, 0.0f, 1.0f) * 255.0f);
+        uint32_t q2 = (uint32_t) (gclamp(b, 0.0f, 1.0f) * 255.0f);
+        uint32_t q3 = (uint32_t) (gclamp(a, 0.0f, 1.0f) * 255.0f);
+        return q0 | (q1 << 8) | (q2 << 16) | (q3 << 24);
+    }
+
+    // fast unsigned division by an invariant divisor d >= 1 (round-up multiplier method)
+    __host__ inline void make_fastdiv(uint32_t d, uint32_t& mul, uint32_t& shift)
+    {
+        uint32_t s = 0;
+        while((1ull << s) < d)
+            s++;
+        uint64_t m = ((1ull << (32 + s)) + d - 1) / d;
+        mul = (uint32_t) m; // m < 2^33; low 32 bits, the implicit 2^32 is added back in fastdiv
+        shift = s;
+    }
+    __device__ inline uint32_t fastdiv(uint32_t n, uint32_t mul, uint32_t shift)
+    {
+        uint64_t hi = __umulhi(n, mul);
+        return (uint32_t) ((hi + n) >> shift);
+    }
+} // namespace isaac_hip

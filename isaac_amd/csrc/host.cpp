@@ -1,0 +1,1303 @@
+// Host side of the gfx950 render library: the C-ABI declared in
+// include/isaac_hip.h.  It owns the device buffers of one rank's render state
+// (the reference's IsaacVisualization members, lib/isaac.hpp:565-945) and
+// turns the steering setters into kernel parameter blocks.
+#include "isaac_hip.h"
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+using namespace isaac_hip;
+
+namespace
+{
+    const char* kFunctorNames[6] = {"idem", "add", "mul", "length", "pow", "sum"}; // lib/isaac/isaac_functors.hpp:185-208
+
+    // ------------------------------------------------------------------ host math (glm op order)
+    struct M4
+    {
+        float m[4][4];
+    };
+
+    M4 mat_mul(const M4& a, const M4& b)
+    {
+        M4 r;
+        for(int c = 0; c < 4; c++)
+            for(int row = 0; row < 4; row++)
+            {
+                float s = a.m[0][row] * b.m[c][0];
+                s = s + a.m[1][row] * b.m[c][1];
+                s = s + a.m[2][row] * b.m[c][2];
+                s = s + a.m[3][row] * b.m[c][3];
+                r.m[c][row] = s;
+            }
+        return r;
+    }
+
+    // glm compute_inverse<4,4> in binary32 (lib/isaac.hpp:2177 calls glm::inverse)
+    M4 mat_inverse(const M4& M)
+    {
+        auto m = [&](int c, int r) { return M.m[c][r]; };
+        const float c00 = m(2, 2) * m(3, 3) - m(3, 2) * m(2, 3);
+        const float c02 = m(1, 2) * m(3, 3) - m(3, 2) * m(1, 3);
+        const float c03 = m(1, 2) * m(2, 3) - m(2, 2) * m(1, 3);
+        const float c04 = m(2, 1) * m(3, 3) - m(3, 1) * m(2, 3);
+        const float c06 = m(1, 1) * m(3, 3) - m(3, 1) * m(1, 3);
+        const float c07 = m(1, 1) * m(2, 3) - m(2, 1) * m(1, 3);
+        const float c08 = m(2, 1) * m(3, 2) - m(3, 1) * m(2, 2);
+        const float c10 = m(1, 1) * m(3, 2) - m(3, 1) * m(1, 2);
+        const float c11 = m(1, 1) * m(2, 2) - m(2, 1) * m(1, 2);
+        const float c12 = m(2, 0) * m(3, 3) - m(3, 0) * m(2, 3);
+        const float c14 = m(1, 0) * m(3, 3) - m(3, 0) * m(1, 3);
+        const float c15 = m(1, 0) * m(2, 3) - m(2, 0) * m(1, 3);
+        const float c16 = m(2, 0) * m(3, 2) - m(3, 0) * m(2, 2);
+        const float c18 = m(1, 0) * m(3, 2) - m(3, 0) * m(1, 2);
+        const float c19 = m(1, 0) * m(2, 2) - m(2, 0) * m(1, 2);
+        const float c20 = m(2, 0) * m(3, 1) - m(3, 0) * m(2, 1);
+        const float c22 = m(1, 0) * m(3, 1) - m(3, 0) * m(1, 1);
+        const float c23 = m(1, 0) * m(2, 1) - m(2, 0) * m(1, 1);
+        const float f0[4] = {c00, c00, c02, c03}, f1[4] = {c04, c04, c06, c07}, f2[4] = {c08, c08, c10, c11};
+        const float f3_[4] = {c12, c12, c14, c15}, f4[4] = {c16, c16, c18, c19}, f5[4] = {c20, c20, c22, c23};
+        const float v0[4] = {m(1, 0), m(0, 0), m(0, 0), m(0, 0)};
+        const float v1[4] = {m(1, 1), m(0, 1), m(0, 1), m(0, 1)};
+        const float v2[4] = {m(1, 2), m(0, 2), m(0, 2), m(0, 2)};
+        const float v3[4] = {m(1, 3), m(0, 3), m(0, 3), m(0, 3)};
+        float col[4][4];
+        const float sa[4] = {1.f, -1.f, 1.f, -1.f}, sb[4] = {-1.f, 1.f, -1.f, 1.f};
+        for(int k = 0; k < 4; k++)
+        {
+            const float i0 = v1[k] * f0[k] - v2[k] * f1[k] + v3[k] * f2[k];
+            const float i1 = v0[k] * f0[k] - v2[k] * f3_[k] + v3[k] * f4[k];
+            const float i2 = v0[k] * f1[k] - v1[k] * f3_[k] + v3[k] * f5[k];
+            const float i3 = v0[k] * f2[k] - v1[k] * f4[k] + v2[k] * f5[k];
+            col[0][k] = i0 * sa[k];
+            col[1][k] = i1 * sb[k];
+            col[2][k] = i2 * sa[k];
+            col[3][k] = i3 * sb[k];
+        }
+        const float d0[4] = {M.m[0][0] * col[0][0], M.m[0][1] * col[1][0], M.m[0][2] * col[2][0], M.m[0][3] * col[3][0]};
+        const float d1 = (d0[0] + d0[1]) + (d0[2] + d0[3]);
+        const float inv = 1.0f / d1;
+        M4 r;
+        for(int c = 0; c < 4; c++)
+            for(int k = 0; k < 4; k++)
+                r.m[c][k] = col[c][k] * inv;
+        return r;
+    }
+
+    float halton(uint32_t index, uint32_t base)
+    {
+        float result = 0, f = 1;
+        while(index > 0)
+        {
+            f /= float(base);
+            result += f * float(index % base);
+            index /= base;
+        }
+        return result;
+    }
+
+    // getHSVA (lib/isaac/isaac_helper.hpp:221-264)
+    void hsva(float h, float s, float v, float a, float out[4])
+    {
+        const double pi3 = M_PI / 3.0;
+        const int hi = int(std::floor(double(h) / pi3));
+        const float f = float(double(h) / pi3 - double(float(hi)));
+        const float p = v * (1.0f - s);
+        const float q = v * (1.0f - s * f);
+        const float t = v * (1.0f - s * (1.0f - f));
+        float r = 0, g = 0, b = 0;
+        switch(hi)
+        {
+        case 0:
+        case 6:
+            r = v, g = t, b = p;
+            break;
+        case 1:
+            r = q, g = v, b = p;
+            break;
+        case 2:
+            r = p, g = v, b = t;
+            break;
+        case 3:
+            r = p, g = q, b = v;
+            break;
+        case 4:
+            r = t, g = p, b = v;
+            break;
+        case 5:
+            r = v, g = p, b = q;
+            break;
+        }
+        out[0] = r;
+        out[1] = g;
+        out[2] = b;
+        out[3] = a;
+    }
+
+    // updateFunctions' per-source parser (lib/isaac.hpp:1086-1186)
+    int32_t parse_chain(const std::string& src, int32_t bytecode[3], float params[12])
+    {
+        int32_t err = 0;
+        float P[3][4] = {};
+        int32_t bc[3] = {0, 0, 0};
+        std::string source = src;
+        int elem = 0;
+        bool again = true;
+        while(again)
+        {
+            size_t pos = source.find('|');
+            std::string token;
+            if(pos == std::string::npos)
+            {
+                again = false;
+                token = source;
+            }
+            else
+            {
+                token = source.substr(0, pos);
+                source.erase(0, pos + 1);
+            }
+            if(elem >= 3)
+            {
+                err = 1;
+                break;
+            }
+            token.erase(std::remove_if(token.begin(), token.end(), [](unsigned char ch) { return std::isspace(ch); }), token.end());
+            const size_t tb = token.find('(');
+            if(tb == std::string::npos)
+            {
+                for(int k = 0; k < 4; k++)
+                    P[elem][k] = 0.0f;
+            }
+            else
+            {
+                const size_t te = token.find(')');
+                if(te == std::string::npos)
+                {
+                    err = -1;
+                    break;
+                }
+                if(te - tb == 1)
+                {
+                    for(int k = 0; k < 4; k++)
+                        P[elem][k] = 0.0f;
+                }
+                else
+                {
+                    std::string parameters = token.substr(tb + 1, te - tb - 1);
+                    float* arr = P[elem];
+                    int pe = 0;
+                    bool pagain = true;
+                    while(pagain)
+                    {
+                        size_t pp = parameters.find(',');
+                        std::string par;
+                        if(pp == std::string::npos)
+                        {
+                            pagain = false;
+                            par = parameters;
+                        }
+                        else
+                        {
+                            par = parameters.substr(0, pp);
+                            parameters.erase(0, pp + 1);
+                        }
+                        if(pe >= 4)
+                        {
+                            err = 2;
+                            break;
+                        }
+                        try
+                        {
+                            arr[pe] = std::stof(par);
+                        }
+                        catch(const std::exception&)
+                        {
+                            err = -2;
+                            pe++;
+                            break;
+                        }
+                        pe++;
+                    }
+                    for(; pe < 4; pe++)
+                        arr[pe] = arr[pe - 1];
+                }
+            }
+            if(tb != std::string::npos)
+                token = token.substr(0, tb);
+            bool found = false;
+            for(int k = 0; k < 6; k++)
+                if(token == kFunctorNames[k])
+                {
+                    bc[elem] = k;
+                    found = true;
+                    break;
+                }
+            if(!found)
+            {
+                err = -2;
+                break;
+            }
+            elem++;
+        }
+        for(; elem < 3; elem++)
+        {
+            bc[elem] = 0;
+            for(int k = 0; k < 4; k++)
+                P[elem][k] = 0.0f;
+        }
+        for(int k = 0; k < 3; k++)
+        {
+            bytecode[k] = bc[k];
+            for(int c = 0; c < 4; c++)
+                params[k * 4 + c] = P[k][c];
+        }
+        return err;
+    }
+
+    // updateTransfer (lib/isaac.hpp:1244-1266); std::map keeps the first of duplicate keys
+    void build_lut(int T, int n, const uint32_t* pos, const float* rgba, float* lut)
+    {
+        std::map<uint32_t, const float*> d;
+        for(int i = 0; i < n; i++)
+            d.insert(std::make_pair(pos[i], rgba + 4 * i));
+        std::fill(lut, lut + size_t(T) * 4, 0.0f);
+        if(d.empty())
+            return;
+        auto before = d.begin();
+        auto next = before;
+        for(next++; next != d.end(); next++)
+        {
+            const uint32_t width = next->first - before->first;
+            for(size_t j = 0; j < size_t(width) && size_t(j + before->first) < size_t(T); j++)
+                for(int c = 0; c < 4; c++)
+                    lut[(before->first + j) * 4 + c]
+                        = (before->second[c] * float(width - j) + next->second[c] * float(j)) / float(width);
+            before = next;
+        }
+    }
+
+    struct VolSlot
+    {
+        isaac_hip_volume_desc desc{};
+        bool set = false;
+        float* npTex = nullptr; // non-persistent functor-applied texture (with guard)
+        size_t npBytes = 0;
+    };
+
+    struct PartSlot
+    {
+        isaac_hip_particle_desc desc{};
+        bool set = false;
+        int64_t capN = 0, capCells = 0;
+        BinBuffers B{};
+        void* blocks[12] = {};
+    };
+} // namespace
+
+struct isaac_hip_ctx
+{
+    isaac_hip_config cfg{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool ownStream = false;
+    std::string err;
+    int nv = 0, nf = 0, np = 0, nsrc = 0;
+    std::vector<VolSlot> vols;
+    std::vector<PartSlot> parts;
+    std::vector<ChainDesc> chain;
+    std::vector<int32_t> chainErr;
+    std::vector<std::vector<float>> lutHost;
+    float4* lutDev = nullptr;
+    bool lutDirty = true;
+    std::vector<float> weight, iso;
+    float step = 0.5f;
+    bool interpolation = false;
+    int ditherMode = 1;
+    bool aoEnabled = false;
+    float aoWeight = 0.5f;
+    int renderMode = 0;
+    int nclip = 0;
+    float clipPos[kMaxClipping][3];
+    float clipNormal[kMaxClipping][3];
+    float clipSaved[kMaxClipping][3];
+    // sizes
+    int64_t global[3], local[3], pgrid[3];
+    int32_t position[3];
+    float scale[3];
+    int64_t gsScaled[3], lsScaled[3], maxGsScaled;
+    int32_t posScaled[3];
+    // device buffers
+    uint32_t* volTex = nullptr;
+    uint32_t* isoTex = nullptr;
+    size_t texVox = 0;
+    uint32_t* color = nullptr;
+    float* depth = nullptr;
+    float* normal = nullptr;
+    float* ao = nullptr;
+    unsigned long long* counters = nullptr;
+    bool counting = false;
+    bool classified = false;
+    hipEvent_t ev[8] = {};
+    bool evRecorded[4] = {false, false, false, false};
+};
+
+namespace
+{
+    int fail(isaac_hip_ctx* c, int code, const std::string& msg)
+    {
+        if(c)
+            c->err = msg;
+        return code;
+    }
+
+    int hip_fail(isaac_hip_ctx* c, hipError_t e, const char* what)
+    {
+        return fail(c, ISAAC_HIP_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    }
+
+#define HIPCHK(ctx, call)                                                                                             \
+    do                                                                                                                \
+    {                                                                                                                 \
+        hipError_t e_ = (call);                                                                                       \
+        if(e_ != hipSuccess)                                                                                          \
+            return hip_fail(ctx, e_, #call);                                                                          \
+    } while(0)
+
+    void derive_sizes(isaac_hip_ctx* c)
+    {
+        // the ctor's float conversions (lib/isaac.hpp:659-661,796-797)
+        for(int i = 0; i < 3; i++)
+        {
+            c->gsScaled[i] = int64_t(float(c->global[i]) * c->scale[i]);
+            c->lsScaled[i] = int64_t(float(c->local[i]) * c->scale[i]);
+            c->posScaled[i] = int32_t(float(c->position[i]) * c->scale[i]);
+        }
+        c->maxGsScaled = std::max(c->gsScaled[0], std::max(c->gsScaled[1], c->gsScaled[2]));
+    }
+
+    int alloc_textures(isaac_hip_ctx* c)
+    {
+        const size_t nvox = size_t(c->local[0]) * size_t(c->local[1]) * size_t(c->local[2]);
+        if(c->volTex)
+            (void) hipFree(c->volTex);
+        if(c->isoTex)
+            (void) hipFree(c->isoTex);
+        c->volTex = c->isoTex = nullptr;
+        HIPCHK(c, hipMalloc(&c->volTex, std::max<size_t>(nvox, 1) * 4));
+        HIPCHK(c, hipMalloc(&c->isoTex, std::max<size_t>(nvox, 1) * 4));
+        HIPCHK(c, hipMemsetAsync(c->volTex, 0, std::max<size_t>(nvox, 1) * 4, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->isoTex, 0, std::max<size_t>(nvox, 1) * 4, c->stream));
+        c->texVox = nvox;
+        c->classified = false;
+        return 0;
+    }
+
+    int ensure_np_texture(isaac_hip_ctx* c, VolSlot& v)
+    {
+        const int64_t g = v.desc.guard_size;
+        const size_t n = size_t(c->local[0] + 2 * g) * size_t(c->local[1] + 2 * g) * size_t(c->local[2] + 2 * g);
+        if(v.npTex && v.npBytes == n * 4)
+            return 0;
+        if(v.npTex)
+            (void) hipFree(v.npTex);
+        v.npTex = nullptr;
+        HIPCHK(c, hipMalloc(&v.npTex, n * 4));
+        v.npBytes = n * 4;
+        return 0;
+    }
+
+    void free_part(PartSlot& p)
+    {
+        for(void*& b : p.blocks)
+        {
+            if(b)
+                (void) hipFree(b);
+            b = nullptr;
+        }
+        p.B = BinBuffers{};
+        p.capN = p.capCells = 0;
+    }
+
+    int ensure_part_buffers(isaac_hip_ctx* c, PartSlot& p)
+    {
+        const int64_t n = std::max<int64_t>(p.desc.count, 1);
+        const int64_t nc = c->pgrid[0] * c->pgrid[1] * c->pgrid[2];
+        const bool needRad = p.desc.radius != nullptr;
+        if(p.capN >= n && p.capCells == nc && (p.B.recRadius != nullptr) == needRad)
+            return 0;
+        free_part(p);
+        size_t sizes[12] = {size_t(n) * 4, size_t(n) * 16, size_t(nc) * 4, size_t(nc) * 4, size_t(nc + 1) * 4,
+                            size_t(nc + 1) * 4, size_t(nc) * 4, size_t(n) * 4, size_t(n) * 16,
+                            needRad ? size_t(n) * 4 : 0, bin_scan_workspace_ints(nc) * 4, 0};
+        for(int i = 0; i < 11; i++)
+            if(sizes[i])
+                HIPCHK(c, hipMalloc(&p.blocks[i], sizes[i]));
+        p.B.cellOf = (int32_t*) p.blocks[0];
+        p.B.rel = (float4*) p.blocks[1];
+        p.B.count = (int32_t*) p.blocks[2];
+        p.B.vcount = (int32_t*) p.blocks[3];
+        p.B.offset = (int32_t*) p.blocks[4];
+        p.B.voffset = (int32_t*) p.blocks[5];
+        p.B.cursor = (int32_t*) p.blocks[6];
+        p.B.perm = (int32_t*) p.blocks[7];
+        p.B.recs = (float4*) p.blocks[8];
+        p.B.recRadius = (float*) p.blocks[9];
+        p.B.scanTmp = (int32_t*) p.blocks[10];
+        p.capN = n;
+        p.capCells = nc;
+        return 0;
+    }
+
+    ChainDesc to_chain(const int32_t bc[3], const float params[12])
+    {
+        ChainDesc d{};
+        for(int k = 0; k < 3; k++)
+        {
+            d.op[k] = bc[k];
+            for(int i = 0; i < 4; i++)
+                d.p[k][i] = params[k * 4 + i];
+        }
+        return d;
+    }
+
+    int check_idx(isaac_hip_ctx* c, int idx)
+    {
+        if(!c)
+            return ISAAC_HIP_ERR_ARG;
+        if(idx < 0 || idx >= c->nsrc)
+            return fail(c, ISAAC_HIP_ERR_ARG, "source index out of range");
+        return 0;
+    }
+
+    float elapsed_us(hipEvent_t a, hipEvent_t b)
+    {
+        float ms = 0.f;
+        if(hipEventElapsedTime(&ms, a, b) != hipSuccess)
+            return 0.f;
+        return ms * 1000.0f;
+    }
+} // namespace
+
+extern "C"
+{
+    const char* isaac_hip_version(void)
+    {
+        return "isaac-hip 0.1.0 (gfx950)";
+    }
+
+    void isaac_hip_default_projection(int32_t w, int32_t h, double out[16])
+    {
+        // glm::perspective(45.0 [radians in glm 0.9.9], w/h, ISAAC_Z_NEAR, ISAAC_Z_FAR), RH, depth -1..1
+        const double fovy = 45.0, aspect = double(w) / double(h), zn = double(kZNear), zf = double(100.0f);
+        const double t = std::tan(fovy / 2.0);
+        for(int i = 0; i < 16; i++)
+            out[i] = 0.0;
+        out[0] = 1.0 / (aspect * t);
+        out[5] = 1.0 / t;
+        out[10] = -(zf + zn) / (zf - zn);
+        out[11] = -1.0;
+        out[14] = -(2.0 * zf * zn) / (zf - zn);
+    }
+
+    void isaac_hip_modelview(const double look_at[3], const double rotation[9], double distance, double out[16])
+    {
+        // distanceM * rotationM * translationM in double, glm column-major
+        double T[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {look_at[0], look_at[1], look_at[2], 1}};
+        double R[4][4] = {};
+        for(int c = 0; c < 3; c++)
+            for(int r = 0; r < 3; r++)
+                R[c][r] = rotation[c * 3 + r];
+        R[3][3] = 1.0;
+        double D[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, distance, 1}};
+        auto mm = [](const double A[4][4], const double B[4][4], double C[4][4])
+        {
+            for(int c = 0; c < 4; c++)
+                for(int r = 0; r < 4; r++)
+                {
+                    double s = A[0][r] * B[c][0];
+                    s = s + A[1][r] * B[c][1];
+                    s = s + A[2][r] * B[c][2];
+                    s = s + A[3][r] * B[c][3];
+                    C[c][r] = s;
+                }
+        };
+        double DR[4][4], M[4][4];
+        mm(D, R, DR);
+        mm(DR, T, M);
+        for(int c = 0; c < 4; c++)
+            for(int r = 0; r < 4; r++)
+                out[c * 4 + r] = M[c][r];
+    }
+
+    int32_t isaac_hip_parse_functor_chain(const char* chain, int32_t bytecode[3], float params[12])
+    {
+        return parse_chain(chain ? std::string(chain) : std::string(), bytecode, params);
+    }
+
+    int isaac_hip_build_transfer_lut(int32_t T, int32_t n, const uint32_t* positions, const float* rgba, float* lut)
+    {
+        if(T <= 0 || n < 0 || !lut || (n > 0 && (!positions || !rgba)))
+            return ISAAC_HIP_ERR_ARG;
+        build_lut(T, n, positions, rgba, lut);
+        return 0;
+    }
+
+    int isaac_hip_default_transfer(
+        int32_t i,
+        int32_t n_sources,
+        int32_t is_particle,
+        int32_t T,
+        uint32_t positions[2],
+        float rgba[8])
+    {
+        if(n_sources <= 0 || i < 0 || i >= n_sources)
+            return ISAAC_HIP_ERR_ARG;
+        // hue = 2*pi*i/S computed as float(2i) * M_PI / float(S) (lib/isaac.hpp:778)
+        const float h = float(double(float(2 * i)) * M_PI / double(float(n_sources)));
+        hsva(h, 1.0f, 1.0f, is_particle ? 1.0f : 0.0f, rgba);
+        hsva(h, 1.0f, 1.0f, 1.0f, rgba + 4);
+        positions[0] = 0;
+        positions[1] = uint32_t(T);
+        return 0;
+    }
+
+    int isaac_hip_create(const isaac_hip_config* cfg, int device, isaac_hip_ctx** out)
+    {
+        if(!cfg || !out)
+            return ISAAC_HIP_ERR_ARG;
+        *out = nullptr;
+        if(cfg->n_volume_sources < 0 || cfg->n_particle_sources < 0 || cfg->n_field_sources < 0)
+            return ISAAC_HIP_ERR_ARG;
+        if(cfg->n_volume_sources + cfg->n_field_sources > ISAAC_HIP_MAX_SOURCES
+           || cfg->n_particle_sources > ISAAC_HIP_MAX_SOURCES)
+            return ISAAC_HIP_ERR_ARG;
+        if(cfg->n_field_sources > 0)
+            return ISAAC_HIP_ERR_UNSUPPORTED; // vector-field advection is not part of this library
+        if(cfg->framebuffer_size[0] <= 0 || cfg->framebuffer_size[1] <= 0 || cfg->transfer_size <= 0)
+            return ISAAC_HIP_ERR_ARG;
+        for(int i = 0; i < 3; i++)
+            if(cfg->global_size[i] <= 0 || cfg->local_size[i] <= 0 || cfg->local_particle_size[i] <= 0
+               || cfg->local_size[i] > (1 << 20) || cfg->scale[i] <= 0.0f)
+                return ISAAC_HIP_ERR_ARG;
+        auto* c = new isaac_hip_ctx();
+        c->cfg = *cfg;
+        c->device = device;
+        hipError_t e = hipSetDevice(device);
+        if(e != hipSuccess)
+        {
+            delete c;
+            return ISAAC_HIP_ERR_DEVICE;
+        }
+        if(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        {
+            delete c;
+            return ISAAC_HIP_ERR_DEVICE;
+        }
+        c->ownStream = true;
+        for(auto& ev : c->ev)
+            (void) hipEventCreate(&ev);
+        c->nv = cfg->n_volume_sources;
+        c->nf = cfg->n_field_sources;
+        c->np = cfg->n_particle_sources;
+        c->nsrc = c->nv + c->nf + c->np;
+        for(int i = 0; i < 3; i++)
+        {
+            c->global[i] = cfg->global_size[i];
+            c->local[i] = cfg->local_size[i];
+            c->pgrid[i] = cfg->local_particle_size[i];
+            c->position[i] = cfg->position[i];
+            c->scale[i] = cfg->scale[i];
+        }
+        derive_sizes(c);
+        c->vols.resize(c->nv + c->nf);
+        c->parts.resize(c->np);
+        c->chain.resize(c->nsrc);
+        c->chainErr.assign(c->nsrc, 0);
+        c->lutHost.assign(c->nsrc, std::vector<float>(size_t(cfg->transfer_size) * 4, 0.0f));
+        c->weight.assign(c->nsrc, 2.0f); // ISAAC_DEFAULT_WEIGHT
+        c->iso.assign(c->nsrc, 0.0f);
+        const int T = cfg->transfer_size;
+        for(int i = 0; i < c->nsrc; i++)
+        {
+            // default functions and transfer functions (lib/isaac.hpp:720-731,759-794)
+            const bool isPart = i >= c->nv + c->nf;
+            const bool isField = i >= c->nv && !isPart;
+            int32_t bc[3];
+            float pr[12];
+            parse_chain(isField ? "length" : "idem", bc, pr);
+            c->chain[i] = to_chain(bc, pr);
+            uint32_t pos[2];
+            float rgba[8];
+            isaac_hip_default_transfer(i, c->nsrc, isPart ? 1 : 0, T, pos, rgba);
+            build_lut(T, 2, pos, rgba, c->lutHost[i].data());
+        }
+        int rc = 0;
+        const size_t npx = size_t(cfg->framebuffer_size[0]) * size_t(cfg->framebuffer_size[1]);
+        if((e = hipMalloc(&c->lutDev, size_t(std::max(c->nsrc, 1)) * T * sizeof(float4))) != hipSuccess
+           || (e = hipMalloc(&c->color, npx * 4)) != hipSuccess || (e = hipMalloc(&c->depth, npx * 4)) != hipSuccess
+           || (e = hipMalloc(&c->normal, npx * 12)) != hipSuccess || (e = hipMalloc(&c->ao, npx * 4)) != hipSuccess
+           || (e = hipMalloc(&c->counters, 64 * sizeof(unsigned long long))) != hipSuccess)
+        {
+            isaac_hip_destroy(c);
+            return ISAAC_HIP_ERR_ALLOC;
+        }
+        (void) hipMemsetAsync(c->color, 0, npx * 4, c->stream);
+        (void) hipMemsetAsync(c->depth, 0, npx * 4, c->stream);
+        (void) hipMemsetAsync(c->normal, 0, npx * 12, c->stream);
+        (void) hipMemsetAsync(c->ao, 0, npx * 4, c->stream);
+        (void) hipMemsetAsync(c->counters, 0, 64 * sizeof(unsigned long long), c->stream);
+        rc = alloc_textures(c);
+        if(rc != 0)
+        {
+            isaac_hip_destroy(c);
+            return rc;
+        }
+        (void) hipStreamSynchronize(c->stream);
+        *out = c;
+        return 0;
+    }
+
+    void isaac_hip_destroy(isaac_hip_ctx* c)
+    {
+        if(!c)
+            return;
+        (void) hipSetDevice(c->device);
+        if(c->stream)
+            (void) hipStreamSynchronize(c->stream);
+        for(auto& v : c->vols)
+            if(v.npTex)
+                (void) hipFree(v.npTex);
+        for(auto& p : c->parts)
+            free_part(p);
+        void* bufs[] = {c->lutDev, c->volTex, c->isoTex, c->color, c->depth, c->normal, c->ao, c->counters};
+        for(void* b : bufs)
+            if(b)
+                (void) hipFree(b);
+        for(auto& ev : c->ev)
+            if(ev)
+                (void) hipEventDestroy(ev);
+        if(c->ownStream && c->stream)
+            (void) hipStreamDestroy(c->stream);
+        delete c;
+    }
+
+    const char* isaac_hip_last_error(const isaac_hip_ctx* c)
+    {
+        return c ? c->err.c_str() : "null context";
+    }
+
+    int isaac_hip_set_stream(isaac_hip_ctx* c, void* s)
+    {
+        if(!c)
+            return ISAAC_HIP_ERR_ARG;
+        (void) hipStreamSynchronize(c->stream);
+        if(c->ownStream && c->stream)
+            (void) hipStreamDestroy(c->stream);
+        if(s)
+        {
+            c->stream = (hipStream_t) s;
+            c->ownStream = false;
+        }
+        else
+        {
+            if(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+                return fail(c, ISAAC_HIP_ERR_DEVICE, "stream creation failed");
+            c->ownStream = true;
+        }
+        return 0;
+    }
+
+    int isaac_hip_set_volume_source(isaac_hip_ctx* c, int idx, const isaac_hip_volume_desc* d)
+    {
+        if(!c || !d)
+            return ISAAC_HIP_ERR_ARG;
+        if(idx < 0 || idx >= c->nv + c->nf)
+            return fail(c, ISAAC_HIP_ERR_ARG, "volume source index out of range");
+        if(d->dtype != ISAAC_HIP_F32)
+            return fail(c, ISAAC_HIP_ERR_UNSUPPORTED, "the plain render path reads float32 sources");
+        if(d->feature_dim < 1 || d->feature_dim > 4 || !d->data || d->guard_size < 0)
+            return fail(c, ISAAC_HIP_ERR_ARG, "bad volume source descriptor");
+        c->vols[idx].desc = *d;
+        c->vols[idx].set = true;
+        return 0;
+    }
+
+    int isaac_hip_set_particle_source(isaac_hip_ctx* c, int idx, const isaac_hip_particle_desc* d)
+    {
+        if(!c || !d)
+            return ISAAC_HIP_ERR_ARG;
+        if(idx < 0 || idx >= c->np)
+            return fail(c, ISAAC_HIP_ERR_ARG, "particle source index out of range");
+        if(d->feature_dim < 1 || d->feature_dim > 4 || d->count < 0 || d->count > INT32_MAX
+           || (d->count > 0 && (!d->position || !d->attribute)))
+            return fail(c, ISAAC_HIP_ERR_ARG, "bad particle source descriptor");
+        c->parts[idx].desc = *d;
+        c->parts[idx].set = true;
+        return 0;
+    }
+
+    int isaac_hip_set_function(isaac_hip_ctx* c, int idx, const char* chain, int32_t* error_code)
+    {
+        int rc = check_idx(c, idx);
+        if(rc)
+            return rc;
+        int32_t bc[3];
+        float pr[12];
+        const int32_t e = parse_chain(chain ? std::string(chain) : std::string(), bc, pr);
+        c->chain[idx] = to_chain(bc, pr);
+        c->chainErr[idx] = e;
+        if(error_code)
+            *error_code = e;
+        return 0;
+    }
+
+    int isaac_hip_set_transfer_points(isaac_hip_ctx* c, int idx, int n, const uint32_t* positions, const float* rgba)
+    {
+        int rc = check_idx(c, idx);
+        if(rc)
+            return rc;
+        if(n < 0 || (n > 0 && (!positions || !rgba)))
+            return fail(c, ISAAC_HIP_ERR_ARG, "bad transfer points");
+        build_lut(c->cfg.transfer_size, n, positions, rgba, c->lutHost[idx].data());
+        c->lutDirty = true;
+        return 0;
+    }
+
+    int isaac_hip_set_transfer_lut(isaac_hip_ctx* c, int idx, const float* lut)
+    {
+        int rc = check_idx(c, idx);
+        if(rc)
+            return rc;
+        if(!lut)
+            return fail(c, ISAAC_HIP_ERR_ARG, "null lut");
+        std::memcpy(c->lutHost[idx].data(), lut, size_t(c->cfg.transfer_size) * 16);
+        c->lutDirty = true;
+        return 0;
+    }
+
+    int isaac_hip_set_weight(isaac_hip_ctx* c, int idx, float w)
+    {
+        int rc = check_idx(c, idx);
+        if(rc)
+            return rc;
+        c->weight[idx] = w;
+        return 0;
+    }
+
+    int isaac_hip_set_iso_threshold(isaac_hip_ctx* c, int idx, float t)
+    {
+        int rc = check_idx(c, idx);
+        if(rc)
+            return rc;
+        c->iso[idx] = t;
+        return 0;
+    }
+
+    int isaac_hip_set_step(isaac_hip_ctx* c, float step)
+    {
+        if(!c)
+            return ISAAC_HIP_ERR_ARG;
+        c->step = step < 0.01f ? 0.01f : step; // lib/isaac.hpp:1629-1633
+        return 0;
+    }
+
+    int isaac_hip_set_interpolation(isaac_hip_ctx* c, int on)
+    {
+        if(!c)
+            return ISAAC_HIP_ERR_ARG;
+        c->interpolation = on != 0;
+        return 0;
+    }
+
+    int isaac_hip_set_dither_mode(isaac_hip_ctx* c, int mode)
+    {
+        if(!c)
+            return ISAAC_HIP_ERR_ARG;
+        if(mode != 0 && mode != 1)
+            return fail(c, ISAAC_HIP_ERR_UNSUPPORTED, "dither modes 0 (round) and 1 (ordered 4^3) are implemented");
+        c->ditherMode = mode;
+        return 0;
+    }
+
+    int isaac_hip_set_ao(isaac_hip_ctx* c, int enabled, float weight)
+    {
+        if(!c)
+            return ISAAC_HIP_ERR_ARG;
+        c->aoEnabled = enabled != 0;
+        c->aoWeight = weight;
+        return 0;
+    }
+
+    int isaac_hip_set_render_mode(isaac_hip_ctx* c, int mode)
+    {
+        if(!c)
+            return ISAAC_HIP_ERR_ARG;
+        c->renderMode = mode;
+        return 0;
+    }
+
+    int isaac_hip_edit_clipping(isaac_hip_ctx* c, uint32_t nr, float px, float py, float pz, float nx, float ny, float nz)
+    {
+        if(!c)
+            return ISAAC_HIP_ERR_ARG;
+        if(nr >= uint32_t(kMaxClipping))
+            return 0;
+        // editClipping (lib/isaac.hpp:958-986)
+        const float sx = nx * c->scale[0], sy = ny * c->scale[1], sz = nz * c->scale[2];
+        const float l = sqrtf(sx * sx + sy * sy + sz * sz);
+        if(l == 0.0f)
+            return 0;
+        c->clipPos[nr][0] = px;
+        c->clipPos[nr][1] = py;
+        c->clipPos[nr][2] = pz;
+        c->clipNormal[nr][0] = sx / l;
+        c->clipNormal[nr][1] = sy / l;
+        c->clipNormal[nr][2] = sz / l;
+        c->clipSaved[nr][0] = nx;
+        c->clipSaved[nr][1] = ny;
+        c->clipSaved[nr][2] = nz;
+        return 1;
+    }
+
+    int isaac_hip_add_clipping(isaac_hip_ctx* c, float px, float py, float pz, float nx, float ny, float nz)
+    {
+        if(!c)
+            return ISAAC_HIP_ERR_ARG;
+        if(isaac_hip_edit_clipping(c, uint32_t(c->nclip), px, py, pz, nx, ny, nz) == 1)
+        {
+            c->nclip++;
+            return 1;
+        }
+        return 0;
+    }
+
+    int isaac_hip_remove_clipping(isaac_hip_ctx* c, uint32_t nr)
+    {
+        if(!c)
+            return ISAAC_HIP_ERR_ARG;
+        if(nr >= uint32_t(c->nclip))
+            return 0;
+        c->nclip--;
+        for(int i = int(nr); i < c->nclip; i++)
+            for(int k = 0; k < 3; k++)
+            {
+                c->clipPos[i][k] = c->clipPos[i + 1][k];
+                c->clipNormal[i][k] = c->clipNormal[i + 1][k];
+                c->clipSaved[i][k] = c->clipSaved[i + 1][k];
+            }
+        return 0;
+    }
+
+    int isaac_hip_clipping_count(const isaac_hip_ctx* c)
+    {
+        return c ? c->nclip : ISAAC_HIP_ERR_ARG;
+    }
+
+    int isaac_hip_update_position(isaac_hip_ctx* c, const int32_t position[3])
+    {
+        if(!c || !position)
+            return ISAAC_HIP_ERR_ARG;
+        for(int i = 0; i < 3; i++)
+            c->position[i] = position[i];
+        derive_sizes(c);
+        return 0;
+    }
+
+    int isaac_hip_update_local_size(isaac_hip_ctx* c, const int64_t ls[3])
+    {
+        if(!c || !ls)
+            return ISAAC_HIP_ERR_ARG;
+        for(int i = 0; i < 3; i++)
+            if(ls[i] <= 0 || ls[i] > (1 << 20))
+                return fail(c, ISAAC_HIP_ERR_ARG, "bad local size");
+        for(int i = 0; i < 3; i++)
+            c->local[i] = ls[i];
+        derive_sizes(c);
+        return alloc_textures(c);
+    }
+
+    int isaac_hip_update_local_particle_size(isaac_hip_ctx* c, const int64_t ps[3])
+    {
+        if(!c || !ps)
+            return ISAAC_HIP_ERR_ARG;
+        for(int i = 0; i < 3; i++)
+            if(ps[i] <= 0)
+                return fail(c, ISAAC_HIP_ERR_ARG, "bad particle grid");
+        for(int i = 0; i < 3; i++)
+            c->pgrid[i] = ps[i];
+        return 0;
+    }
+
+    int isaac_hip_update_buffers(isaac_hip_ctx* c)
+    {
+        if(!c)
+            return ISAAC_HIP_ERR_ARG;
+        HIPCHK(c, hipSetDevice(c->device));
+        const int T = c->cfg.transfer_size;
+        const int nvf = c->nv + c->nf;
+        for(int s = 0; s < nvf; s++)
+            if(!c->vols[s].set && (c->weight[s] + c->iso[s] > 0.0f))
+                return fail(c, ISAAC_HIP_ERR_STATE, "volume source " + std::to_string(s) + " has no data");
+        if(c->lutDirty)
+        {
+            std::vector<float> all(size_t(c->nsrc) * T * 4);
+            for(int s = 0; s < c->nsrc; s++)
+                std::memcpy(all.data() + size_t(s) * T * 4, c->lutHost[s].data(), size_t(T) * 16);
+            HIPCHK(c, hipMemcpyAsync(c->lutDev, all.data(), all.size() * 4, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            c->lutDirty = false;
+        }
+        HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+        // non-persistent sources (UpdatePersistentTextureIterator, lib/isaac.hpp:249-290)
+        for(int s = 0; s < nvf; s++)
+        {
+            VolSlot& v = c->vols[s];
+            const bool enabled = c->weight[s] != 0.0f || c->iso[s] != 0.0f;
+            if(!v.set || v.desc.persistent || !enabled)
+                continue;
+            int rc = ensure_np_texture(c, v);
+            if(rc)
+                return rc;
+            const int g = v.desc.guard_size;
+            HIPCHK(
+                c,
+                launch_persistent_copy(
+                    (const float*) v.desc.data, v.desc.stride, v.desc.feature_dim, c->chain[s], int(c->local[0] + 2 * g),
+                    int(c->local[1] + 2 * g), int(c->local[2] + 2 * g), g, v.npTex, c->stream));
+        }
+        // particle cell lists
+        for(int s = 0; s < c->np; s++)
+        {
+            PartSlot& p = c->parts[s];
+            const int gi = nvf + s;
+            if(!p.set || c->weight[gi] == 0.0f)
+                continue;
+            int rc = ensure_part_buffers(c, p);
+            if(rc)
+                return rc;
+            BinParams P{};
+            P.pos = p.desc.position;
+            P.attr = p.desc.attribute;
+            P.radius = p.desc.radius;
+            P.dim = p.desc.feature_dim;
+            P.n = p.desc.count;
+            for(int i = 0; i < 3; i++)
+            {
+                P.grid[i] = int32_t(c->pgrid[i]);
+                P.cw[i] = float(c->local[i]) / float(c->pgrid[i]);
+            }
+            P.chain = c->chain[gi];
+            P.lut = c->lutDev + size_t(gi) * T;
+            P.T = T;
+            P.filter = 1;
+            HIPCHK(c, launch_bin(P, p.B, c->stream));
+        }
+        HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+        // classification (lib/isaac.hpp:1916-1947)
+        ClassifyParams P{};
+        P.X = int32_t(c->local[0]);
+        P.Y = int32_t(c->local[1]);
+        P.Z = int32_t(c->local[2]);
+        P.nvox = int64_t(P.X) * P.Y * P.Z;
+        if(P.nvox >= (int64_t(1) << 32))
+            return fail(c, ISAAC_HIP_ERR_UNSUPPORTED, "local volume above 2^32 voxels");
+        P.nsrc = nvf;
+        float tw = 0.0f;
+        for(int s = 0; s < nvf; s++)
+            tw += c->weight[s];
+        P.totalWeight = tw;
+        P.ditherMode = c->ditherMode;
+        P.T = T;
+        make_fastdiv(uint32_t(P.X), P.divX_mul, P.divX_shift);
+        make_fastdiv(uint32_t(P.X * P.Y), P.divXY_mul, P.divXY_shift);
+        P.luts = c->lutDev;
+        P.volTex = c->volTex;
+        P.isoTex = c->isoTex;
+        const int maxLds = int((64 * 1024) / (size_t(T) * sizeof(float4)));
+        P.nLut = std::min(nvf, std::max(0, maxLds));
+        for(int s = 0; s < nvf; s++)
+        {
+            ClassifySource& S = P.src[s];
+            const VolSlot& v = c->vols[s];
+            S.weight = c->weight[s];
+            S.iso = c->iso[s];
+            S.active = (S.weight + S.iso > 0.0f) ? 1 : 0;
+            S.lutSlot = s;
+            if(!v.set)
+            {
+                S.active = 0;
+                S.data = nullptr;
+                S.dim = 1;
+                continue;
+            }
+            if(v.desc.persistent)
+            {
+                S.data = (const float*) v.desc.data;
+                S.dim = v.desc.feature_dim;
+                for(int k = 0; k < 3; k++)
+                    S.stride[k] = v.desc.stride[k];
+                S.chain = c->chain[s];
+            }
+            else
+            {
+                const int64_t g = v.desc.guard_size;
+                const int64_t gX = c->local[0] + 2 * g, gY = c->local[1] + 2 * g;
+                S.data = v.npTex ? v.npTex + (g + g * gX + g * gX * gY) : nullptr;
+                S.dim = 1;
+                S.stride[0] = 1;
+                S.stride[1] = gX;
+                S.stride[2] = gX * gY;
+                S.chain = ChainDesc{};
+                if(!S.data)
+                    S.active = 0;
+            }
+        }
+        HIPCHK(c, launch_classify(P, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+        c->evRecorded[0] = true;
+        c->classified = true;
+        return 0;
+    }
+
+    int isaac_hip_render(
+        isaac_hip_ctx* c,
+        const double proj[16],
+        const double mv[16],
+        const float bg[4],
+        const int32_t viewport[4],
+        uint8_t* out_rgba,
+        int out_is_device)
+    {
+        if(!c || !proj || !mv)
+            return ISAAC_HIP_ERR_ARG;
+        HIPCHK(c, hipSetDevice(c->device));
+        if(!c->classified)
+            return fail(c, ISAAC_HIP_ERR_STATE, "isaac_hip_update_buffers must run before the first render");
+        const int W = c->cfg.framebuffer_size[0], H = c->cfg.framebuffer_size[1];
+        FrameParams F{};
+        // inverse MVP in binary32 (lib/isaac.hpp:2174-2178)
+        M4 P, MV;
+        for(int col = 0; col < 4; col++)
+            for(int r = 0; r < 4; r++)
+            {
+                P.m[col][r] = float(proj[col * 4 + r]);
+                MV.m[col][r] = float(mv[col * 4 + r]);
+            }
+        M4 inv = mat_inverse(mat_mul(P, MV));
+        for(int col = 0; col < 4; col++)
+            for(int r = 0; r < 4; r++)
+                F.inv[col * 4 + r] = inv.m[col][r];
+        F.W = W;
+        F.H = H;
+        int vp[4] = {0, 0, W, H};
+        if(viewport)
+            for(int i = 0; i < 4; i++)
+                vp[i] = viewport[i];
+        F.vx0 = std::max(0, vp[0]);
+        F.vy0 = std::max(0, vp[1]);
+        F.vx1 = std::min(W, vp[0] + vp[2]);
+        F.vy1 = std::min(H, vp[1] + vp[3]);
+        if(F.vx1 < F.vx0)
+            F.vx1 = F.vx0;
+        if(F.vy1 < F.vy0)
+            F.vy1 = F.vy0;
+        F.tilesX = (F.vx1 - F.vx0 + 15) / 16;
+        F.tilesY = (F.vy1 - F.vy0 + 15) / 16;
+        F.fbW = float(W);
+        F.fbH = float(H);
+        F.maxSize = float(c->maxGsScaled) * 0.5f;
+        for(int i = 0; i < 3; i++)
+        {
+            F.off[i] = float(int32_t(c->gsScaled[i]) / 2 - c->posScaled[i]);
+            F.bbMin[i] = float(std::max(int32_t(0), -c->posScaled[i]));
+            F.bbMax[i] = float(std::min(int32_t(c->lsScaled[i]), int32_t(c->gsScaled[i]) - c->posScaled[i]));
+            F.outerLow[i] = c->position[i] == 0;
+            F.outerHigh[i] = (int64_t(c->position[i]) + c->local[i] == c->global[i]);
+            F.scale[i] = c->scale[i];
+        }
+        F.nclip = c->nclip;
+        for(int i = 0; i < c->nclip; i++)
+            for(int k = 0; k < 3; k++)
+            {
+                F.cpos[i][k] = c->clipPos[i][k] * float(c->gsScaled[k]) * 0.5f + F.off[k];
+                F.cnorm[i][k] = c->clipNormal[i][k];
+            }
+        F.step = c->step;
+        F.linear = c->interpolation ? 1 : 0;
+        F.X = int32_t(c->local[0]);
+        F.Y = int32_t(c->local[1]);
+        F.Z = int32_t(c->local[2]);
+        F.volTex = c->volTex;
+        F.isoTex = c->isoTex;
+        const int nvf = c->nv + c->nf;
+        F.isoEnabled = 0;
+        for(int s = 0; s < nvf; s++)
+            if(c->iso[s] > 0.0f)
+                F.isoEnabled = 1;
+        float twAll = 0.0f;
+        for(int s = 0; s < c->nsrc; s++)
+            twAll += c->weight[s];
+        F.totalWeightAll = twAll;
+        F.volumeEnabled = twAll > 0.0f;
+        F.minSize = float(std::min(int(c->global[0]), std::min(int(c->global[1]), int(c->global[2]))));
+        F.renderMode = c->renderMode;
+        F.aoEnabled = (c->aoEnabled && c->aoWeight > 0.0f) ? 1 : 0;
+        F.aoWeight = c->aoWeight;
+        float bgv[4] = {0, 0, 0, 0};
+        if(bg)
+            for(int i = 0; i < 4; i++)
+                bgv[i] = bg[i];
+        // bgColor = {bg[3], bg[2], bg[1], bg[0]} (lib/isaac.hpp:2245-2249)
+        F.bg[0] = bgv[3];
+        F.bg[1] = bgv[2];
+        F.bg[2] = bgv[1];
+        F.bg[3] = bgv[0];
+        F.bgClear = to_byte4(F.bg[0], F.bg[1], F.bg[2], 0.0f);
+        F.maxGlobalSizeScaled = float(c->maxGsScaled);
+        hsva(halton(0, 3) * float(2 * M_PI), 1.0f, 1.0f, 1.0f, F.rankHue);
+        F.color = c->color;
+        F.depth = c->depth;
+        F.normal = c->normal;
+        F.ao = c->ao;
+        F.counters = c->counters;
+        // particles
+        ParticleParams PP{};
+        PP.nsrc = 0;
+        for(int i = 0; i < 3; i++)
+        {
+            PP.grid[i] = int32_t(c->pgrid[i]);
+            PP.pscale[i] = float(c->lsScaled[i]) / float(c->pgrid[i]);
+        }
+        const int T = c->cfg.transfer_size;
+        for(int s = 0; s < c->np; s++)
+        {
+            const PartSlot& p = c->parts[s];
+            const int gi = nvf + s;
+            if(c->weight[gi] == 0.0f || !p.set)
+                continue;
+            if(p.capCells != c->pgrid[0] * c->pgrid[1] * c->pgrid[2])
+                return fail(c, ISAAC_HIP_ERR_STATE, "particle cell lists are stale; call isaac_hip_update_buffers");
+            if(PP.nsrc >= 4)
+                return fail(c, ISAAC_HIP_ERR_UNSUPPORTED, "at most 4 active particle sources per frame");
+            ParticleSourceDev& S = PP.src[PP.nsrc++];
+            S.voff = p.B.voffset;
+            S.recs = p.B.recs;
+            S.radius = p.B.recRadius;
+            S.constRadius = p.desc.constant_radius;
+            S.weight = c->weight[gi];
+            S.lut = c->lutDev + size_t(gi) * T;
+        }
+        if(c->counting)
+            HIPCHK(c, launch_fill_u32(reinterpret_cast<uint32_t*>(c->counters), 0u, 16, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+        HIPCHK(c, launch_frame(F, PP, c->counting, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+        if(out_rgba)
+        {
+            const size_t bytes = size_t(W) * size_t(H) * 4;
+            HIPCHK(c, hipMemcpyAsync(out_rgba, c->color, bytes, out_is_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
+        }
+        HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
+        c->evRecorded[1] = true;
+        if(out_rgba && !out_is_device)
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        return 0;
+    }
+
+    int isaac_hip_synchronize(isaac_hip_ctx* c)
+    {
+        if(!c)
+            return ISAAC_HIP_ERR_ARG;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return 0;
+    }
+
+    int isaac_hip_get_timings(const isaac_hip_ctx* c, isaac_hip_timings* out)
+    {
+        if(!c || !out)
+            return ISAAC_HIP_ERR_ARG;
+        (void) hipStreamSynchronize(c->stream);
+        *out = isaac_hip_timings{};
+        if(c->evRecorded[0])
+        {
+            out->buffer_us = elapsed_us(c->ev[0], c->ev[1]);
+            out->optimization_buffer_us = elapsed_us(c->ev[1], c->ev[2]);
+        }
+        if(c->evRecorded[1])
+        {
+            out->kernel_us = elapsed_us(c->ev[3], c->ev[4]);
+            out->copy_us = elapsed_us(c->ev[4], c->ev[5]);
+        }
+        return 0;
+    }
+
+    int isaac_hip_set_counting(isaac_hip_ctx* c, int enabled)
+    {
+        if(!c)
+            return ISAAC_HIP_ERR_ARG;
+        c->counting = enabled != 0;
+        return 0;
+    }
+
+    int isaac_hip_get_counters(isaac_hip_ctx* c, uint64_t out[ISAAC_HIP_CNT_COUNT])
+    {
+        if(!c || !out)
+            return ISAAC_HIP_ERR_ARG;
+        unsigned long long h[16];
+        HIPCHK(c, hipMemcpyAsync(h, c->counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for(int i = 0; i < ISAAC_HIP_CNT_COUNT; i++)
+            out[i] = h[i];
+        return 0;
+    }
+
+    int isaac_hip_get_gbuffer(const isaac_hip_ctx* c, const void** color, const void** depth, const void** normal, const void** ao)
+    {
+        if(!c)
+            return ISAAC_HIP_ERR_ARG;
+        if(color)
+            *color = c->color;
+        if(depth)
+            *depth = c->depth;
+        if(normal)
+            *normal = c->normal;
+        if(ao)
+            *ao = c->ao;
+        return 0;
+    }
+
+    int isaac_hip_get_combined_textures(const isaac_hip_ctx* c, const void** volume, const void** iso)
+    {
+        if(!c)
+            return ISAAC_HIP_ERR_ARG;
+        if(volume)
+            *volume = c->volTex;
+        if(iso)
+            *iso = c->isoTex;
+        return 0;
+    }
+
+    int isaac_hip_get_cell_list(const isaac_hip_ctx* c, int idx, const void** counts, const void** offsets, const void** perm)
+    {
+        if(!c || idx < 0 || idx >= c->np)
+            return ISAAC_HIP_ERR_ARG;
+        const PartSlot& p = c->parts[idx];
+        if(counts)
+            *counts = p.B.count;
+        if(offsets)
+            *offsets = p.B.offset;
+        if(perm)
+            *perm = p.B.perm;
+        return 0;
+    }
+}

@@ -1,0 +1,56 @@
+// Launch entry points of the gfx950 kernels (host side of the HIP library).
+#pragma once
+
+#include "common.hpp"
+
+namespace isaac_hip
+{
+    hipError_t launch_classify(const ClassifyParams& P, hipStream_t s);
+    hipError_t launch_persistent_copy(
+        const float* src,
+        const int64_t stride[3],
+        int dim,
+        const ChainDesc& chain,
+        int gX,
+        int gY,
+        int gZ,
+        int guard,
+        float* dst,
+        hipStream_t s);
+    hipError_t launch_frame(const FrameParams& F, const ParticleParams& PP, bool counting, hipStream_t s);
+
+    // ---- particle cell lists
+    struct BinParams
+    {
+        const float* pos; // n x 3
+        const float* attr; // n x dim
+        const float* radius; // n or nullptr
+        int32_t dim;
+        int64_t n;
+        int32_t grid[3];
+        float cw[3]; // cell width = local / grid (float)
+        ChainDesc chain;
+        const float4* lut; // T entries, device
+        int32_t T;
+        int32_t filter; // drop particles whose TF alpha < 0.5
+    };
+
+    struct BinBuffers
+    {
+        int32_t* cellOf; // n
+        float4* rel; // n: rel.xyz, lut index (or -1 if invisible) in .w
+        int32_t* count; // ncells
+        int32_t* vcount; // ncells
+        int32_t* offset; // ncells + 1
+        int32_t* voffset; // ncells + 1
+        int32_t* cursor; // ncells
+        int32_t* perm; // n
+        float4* recs; // visible records in cell order
+        float* recRadius; // per record radius or nullptr
+        int32_t* scanTmp; // scan workspace (>= 2 * ceil(ncells/1024) + 64 ints)
+    };
+
+    size_t bin_scan_workspace_ints(int64_t ncells);
+    hipError_t launch_bin(const BinParams& P, const BinBuffers& B, hipStream_t s);
+    hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s);
+} // namespace isaac_hip

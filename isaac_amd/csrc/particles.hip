@@ -1,0 +1,291 @@
+// Particle cell lists on gfx950.
+//
+// The reference asks the application for a per-cell particle iterator
+// (getIterator(cell), example/example.cpp:148-228) and walks it in
+// ParticleRenderKernel (lib/isaac/isaac_particle_kernel.hpp:32-114).  This
+// library builds those lists itself from raw particles:
+//
+//   1. bin_count   per particle: cell = clamp(floor(p / cw)), rel = p / cw - cell,
+//                  TF index of the functor-applied attribute, atomic per-cell counts
+//                  (all particles, and those whose TF alpha >= 0.5)
+//   2. scan        exclusive prefix sums of both count arrays (3-phase block scan)
+//   3. bin_scatter atomic slot claim inside each cell (arbitrary order)
+//   4. bin_sort    per cell: sort the claimed indices ascending, which makes the
+//                  permutation equal to a stable counting sort by cell (bit-exact),
+//                  then write the render records of the visible particles in that order
+//
+// A particle whose TF alpha is below 0.5 can never produce a hit in the render
+// kernel (the alpha test follows the geometric test and changes nothing when
+// it fails), so dropping it from the render records leaves every pixel
+// unchanged while removing its sphere tests.
+#include "kernels.hpp"
+
+namespace isaac_hip
+{
+    __device__ inline float apply_chain_p(const ChainDesc& c, const float* e, int n)
+    {
+        float v[4] = {e[0], n > 1 ? e[1] : 0.f, n > 2 ? e[2] : 0.f, n > 3 ? e[3] : 0.f};
+#pragma unroll
+        for(int k = 0; k < kMaxFunctors; k++)
+        {
+            const int op = c.op[k];
+            if(op == OP_ADD)
+            {
+                for(int i = 0; i < n; i++)
+                    v[i] = v[i] + c.p[k][i];
+            }
+            else if(op == OP_MUL)
+            {
+                for(int i = 0; i < n; i++)
+                    v[i] = v[i] * c.p[k][i];
+            }
+            else if(op == OP_LENGTH)
+            {
+                float d;
+                if(n == 1)
+                    d = v[0] * v[0];
+                else if(n == 2)
+                    d = v[0] * v[0] + v[1] * v[1];
+                else if(n == 3)
+                    d = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                else
+                    d = (v[0] * v[0] + v[1] * v[1]) + (v[2] * v[2] + v[3] * v[3]);
+                v[0] = sqrtf(d);
+                n = 1;
+            }
+            else if(op == OP_POW)
+            {
+                for(int i = 0; i < n; i++)
+                    v[i] = powf(v[i], c.p[k][i]);
+            }
+            else if(op == OP_SUM)
+            {
+                float s = 0.0f;
+                for(int i = 0; i < n; i++)
+                    s = s + v[i];
+                v[0] = s;
+                n = 1;
+            }
+        }
+        return v[0];
+    }
+
+    __device__ inline int cell_coord(float p, float cw, int n, float& q)
+    {
+        q = p / cw;
+        int c = int(floorf(q));
+        return iclamp(c, 0, n - 1);
+    }
+
+    __global__ __launch_bounds__(256) void bin_count_kernel(const BinParams P, const BinBuffers B)
+    {
+        for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < P.n; i += int64_t(gridDim.x) * blockDim.x)
+        {
+            float qx, qy, qz;
+            const int cx = cell_coord(P.pos[3 * i + 0], P.cw[0], P.grid[0], qx);
+            const int cy = cell_coord(P.pos[3 * i + 1], P.cw[1], P.grid[1], qy);
+            const int cz = cell_coord(P.pos[3 * i + 2], P.cw[2], P.grid[2], qz);
+            const int c = cx + P.grid[0] * (cy + P.grid[1] * cz);
+            const float v = apply_chain_p(P.chain, P.attr + i * P.dim, P.dim);
+            const float lv = roundf(v * float(P.T));
+            const int li = (lv != lv) ? 0 : (int) gclamp(lv, 0.0f, float(P.T - 1));
+            const bool visible = !P.filter || P.lut[li].w >= 0.5f;
+            B.cellOf[i] = c;
+            B.rel[i] = make_float4(qx - float(cx), qy - float(cy), qz - float(cz), __int_as_float(visible ? li : -1));
+            atomicAdd(&B.count[c], 1);
+            if(visible)
+                atomicAdd(&B.vcount[c], 1);
+        }
+    }
+
+    // ---- exclusive scan of int32 counts into offsets[0..n] (offsets[n] = total)
+    constexpr int kScanBlock = 1024;
+
+    __device__ inline int block_inclusive_scan(int v, int* sh)
+    {
+        const int tid = threadIdx.x;
+        sh[tid] = v;
+        __syncthreads();
+        for(int off = 1; off < kScanBlock; off <<= 1)
+        {
+            int t = (tid >= off) ? sh[tid - off] : 0;
+            __syncthreads();
+            sh[tid] += t;
+            __syncthreads();
+        }
+        return sh[tid];
+    }
+
+    __global__ __launch_bounds__(kScanBlock) void scan_reduce_kernel(const int32_t* in, int64_t n, int32_t* blockSums)
+    {
+        __shared__ int sh[kScanBlock];
+        const int64_t i = int64_t(blockIdx.x) * kScanBlock + threadIdx.x;
+        int v = i < n ? in[i] : 0;
+        int s = block_inclusive_scan(v, sh);
+        if(threadIdx.x == kScanBlock - 1)
+            blockSums[blockIdx.x] = s;
+    }
+
+    // single workgroup: exclusive scan of the block sums, in place
+    __global__ __launch_bounds__(kScanBlock) void scan_blocksums_kernel(int32_t* blockSums, int nb)
+    {
+        __shared__ int sh[kScanBlock];
+        int carry = 0;
+        for(int base = 0; base < nb; base += kScanBlock)
+        {
+            const int i = base + threadIdx.x;
+            int v = i < nb ? blockSums[i] : 0;
+            int s = block_inclusive_scan(v, sh);
+            if(i < nb)
+                blockSums[i] = carry + s - v;
+            __syncthreads();
+            carry += sh[kScanBlock - 1];
+            __syncthreads();
+        }
+    }
+
+    __global__ __launch_bounds__(kScanBlock) void scan_apply_kernel(
+        const int32_t* in,
+        int64_t n,
+        const int32_t* blockSums,
+        int32_t* out)
+    {
+        __shared__ int sh[kScanBlock];
+        const int64_t i = int64_t(blockIdx.x) * kScanBlock + threadIdx.x;
+        int v = i < n ? in[i] : 0;
+        int s = block_inclusive_scan(v, sh);
+        const int base = blockSums[blockIdx.x];
+        if(i < n)
+            out[i] = base + s - v;
+        if(i == n - 1)
+            out[n] = base + s;
+    }
+
+    __global__ __launch_bounds__(256) void bin_scatter_kernel(const BinParams P, const BinBuffers B)
+    {
+        for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < P.n; i += int64_t(gridDim.x) * blockDim.x)
+        {
+            const int c = B.cellOf[i];
+            const int slot = B.offset[c] + atomicAdd(&B.cursor[c], 1);
+            B.perm[slot] = int32_t(i);
+        }
+    }
+
+    // one thread per cell: ascending sort of the cell's particle indices, then
+    // the visible render records in that order
+    __global__ __launch_bounds__(256) void bin_sort_kernel(const BinParams P, const BinBuffers B, int64_t ncells)
+    {
+        for(int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; c < ncells; c += int64_t(gridDim.x) * blockDim.x)
+        {
+            const int k0 = B.offset[c], k1 = B.offset[c + 1];
+            const int len = k1 - k0;
+            int32_t* a = B.perm + k0;
+            if(len <= 64)
+            {
+                for(int i = 1; i < len; i++)
+                {
+                    const int32_t key = a[i];
+                    int j = i - 1;
+                    while(j >= 0 && a[j] > key)
+                    {
+                        a[j + 1] = a[j];
+                        j--;
+                    }
+                    a[j + 1] = key;
+                }
+            }
+            else
+            {
+                // Shell sort (Ciura gaps) for crowded cells
+                const int gaps[8] = {701, 301, 132, 57, 23, 10, 4, 1};
+                for(int g = 0; g < 8; g++)
+                {
+                    const int gap = gaps[g];
+                    for(int i = gap; i < len; i++)
+                    {
+                        const int32_t key = a[i];
+                        int j = i;
+                        while(j >= gap && a[j - gap] > key)
+                        {
+                            a[j] = a[j - gap];
+                            j -= gap;
+                        }
+                        a[j] = key;
+                    }
+                }
+            }
+            int w = B.voffset[c];
+            for(int k = 0; k < len; k++)
+            {
+                const int32_t i = a[k];
+                const float4 r = B.rel[i];
+                if(__float_as_int(r.w) >= 0)
+                {
+                    B.recs[w] = r;
+                    if(B.recRadius)
+                        B.recRadius[w] = P.radius[i];
+                    w++;
+                }
+            }
+        }
+    }
+
+    __global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* p, uint32_t value, size_t n)
+    {
+        for(size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+            p[i] = value;
+    }
+
+    hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s)
+    {
+        if(n == 0)
+            return hipSuccess;
+        size_t blocks = (n + 255) / 256;
+        if(blocks > 8192)
+            blocks = 8192;
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(unsigned(blocks)), dim3(256), 0, s, p, value, n);
+        return hipGetLastError();
+    }
+
+    size_t bin_scan_workspace_ints(int64_t ncells)
+    {
+        return size_t((ncells + kScanBlock - 1) / kScanBlock) + 64;
+    }
+
+    static hipError_t launch_scan(const int32_t* in, int64_t n, int32_t* out, int32_t* tmp, hipStream_t s)
+    {
+        const int nb = int((n + kScanBlock - 1) / kScanBlock);
+        hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(kScanBlock), 0, s, in, n, tmp);
+        hipLaunchKernelGGL(scan_blocksums_kernel, dim3(1), dim3(kScanBlock), 0, s, tmp, nb);
+        hipLaunchKernelGGL(scan_apply_kernel, dim3(nb), dim3(kScanBlock), 0, s, in, n, tmp, out);
+        return hipGetLastError();
+    }
+
+    hipError_t launch_bin(const BinParams& P, const BinBuffers& B, hipStream_t s)
+    {
+        const int64_t ncells = int64_t(P.grid[0]) * P.grid[1] * P.grid[2];
+        hipError_t e;
+        if((e = launch_fill_u32(reinterpret_cast<uint32_t*>(B.count), 0u, size_t(ncells), s)) != hipSuccess)
+            return e;
+        if((e = launch_fill_u32(reinterpret_cast<uint32_t*>(B.vcount), 0u, size_t(ncells), s)) != hipSuccess)
+            return e;
+        if((e = launch_fill_u32(reinterpret_cast<uint32_t*>(B.cursor), 0u, size_t(ncells), s)) != hipSuccess)
+            return e;
+        int64_t pb = (P.n + 255) / 256;
+        if(pb > 8192)
+            pb = 8192;
+        if(P.n > 0)
+            hipLaunchKernelGGL(bin_count_kernel, dim3(unsigned(pb)), dim3(256), 0, s, P, B);
+        if((e = launch_scan(B.count, ncells, B.offset, B.scanTmp, s)) != hipSuccess)
+            return e;
+        if((e = launch_scan(B.vcount, ncells, B.voffset, B.scanTmp, s)) != hipSuccess)
+            return e;
+        if(P.n > 0)
+            hipLaunchKernelGGL(bin_scatter_kernel, dim3(unsigned(pb)), dim3(256), 0, s, P, B);
+        int64_t cb = (ncells + 255) / 256;
+        if(cb > 8192)
+            cb = 8192;
+        hipLaunchKernelGGL(bin_sort_kernel, dim3(unsigned(cb)), dim3(256), 0, s, P, B, ncells);
+        return hipGetLastError();
+    }
+} // namespace isaac_hip

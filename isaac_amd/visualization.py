@@ -1,0 +1,494 @@
+"""Python mirror of the reference's per-rank IsaacVisualization interface.
+
+`IsaacVisualization` keeps the reference's constructor arguments, steering
+state and setters (lib/isaac.hpp:565-1295) and its per-frame call
+`do_visualization` (lib/isaac.hpp:1298-2104) for one rank: the buffer update
+(source refresh, particle cell lists, classification) followed by the draw
+callback (lib/isaac.hpp:2140-2543).  Everything runs on the gfx950 library
+through the C-ABI in include/isaac_hip.h; PyTorch only provides device memory
+for the sources.  There is no server, JSON, MPI or IceT layer.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+ISAAC_MAX_FUNCTORS = 3
+
+
+# --------------------------------------------------------------------------- sources
+class VolumeSource:
+    """A volume source: `feature_dim`, `guard_size`, `persistent`, `get_name()`,
+    `update(enabled, pointer)` (example/example.cpp:39-143) plus `tensor`, a
+    CUDA float32 tensor of shape (Z+2g, Y+2g, X+2g[, C]) standing in for the
+    reference's device-side `operator[]`."""
+
+    feature_dim = 1
+    guard_size = 0
+    persistent = True
+
+    def __init__(self, name: str, tensor, persistent: bool = True, guard_size: int = 0):
+        import torch
+
+        if not tensor.is_cuda or tensor.dtype != torch.float32:
+            raise TypeError("volume source tensors must be CUDA float32")
+        self.name = name
+        self.tensor = tensor.contiguous()
+        self.persistent = persistent
+        self.guard_size = guard_size
+        self.feature_dim = 1 if tensor.dim() == 3 else int(tensor.shape[3])
+
+    def get_name(self) -> str:
+        return self.name
+
+    def update(self, enabled: bool, pointer=None) -> None:
+        pass
+
+    def descriptor(self) -> _lib.VolumeDesc:
+        t = self.tensor
+        g = self.guard_size
+        Zg, Yg, Xg = int(t.shape[0]), int(t.shape[1]), int(t.shape[2])
+        d = _lib.VolumeDesc()
+        off = (g + g * Xg + g * Xg * Yg) * self.feature_dim * 4
+        d.data = t.data_ptr() + off
+        d.dtype = 0
+        d.feature_dim = self.feature_dim
+        d.stride[0], d.stride[1], d.stride[2] = 1, Xg, Xg * Yg
+        d.guard_size = g
+        d.persistent = 1 if self.persistent else 0
+        return d
+
+
+class ParticleSource:
+    """A particle source: raw particles (local voxel coordinates) whose cell
+    lists the library builds (the reference's getIterator(cell),
+    example/example.cpp:148-228)."""
+
+    def __init__(self, name: str, pos, attr, radius=None, constant_radius: float = 0.5):
+        self.name = name
+        self.pos = pos.contiguous()
+        self.attr = attr.contiguous()
+        self.radius = radius.contiguous() if radius is not None else None
+        self.constant_radius = constant_radius
+        self.feature_dim = int(attr.shape[1])
+
+    def get_name(self) -> str:
+        return self.name
+
+    def update(self, enabled: bool, pointer=None) -> None:
+        pass
+
+    def descriptor(self) -> _lib.ParticleDesc:
+        d = _lib.ParticleDesc()
+        d.position = self.pos.data_ptr()
+        d.attribute = self.attr.data_ptr()
+        d.feature_dim = self.feature_dim
+        d.radius = self.radius.data_ptr() if self.radius is not None else None
+        d.constant_radius = self.constant_radius
+        d.count = int(self.pos.shape[0])
+        return d
+
+
+# --------------------------------------------------------------------------- host helpers
+def default_projection(w: int, h: int) -> np.ndarray:
+    out = np.zeros(16, np.float64)
+    lib().isaac_hip_default_projection(int(w), int(h), out.ctypes.data)
+    return out
+
+
+def modelview(look_at, rotation: np.ndarray, distance: float) -> np.ndarray:
+    """updateModelview; `rotation` is the math 3x3 matrix R (rows = output axes)."""
+    la = np.asarray(look_at, np.float64)
+    rot = np.ascontiguousarray(np.asarray(rotation, np.float64).T).reshape(9)  # column-major
+    out = np.zeros(16, np.float64)
+    lib().isaac_hip_modelview(la.ctypes.data, rot.ctypes.data, float(distance), out.ctypes.data)
+    return out
+
+
+def parse_functor_chain(chain: str) -> Tuple[List[int], List[List[float]], int]:
+    bc = np.zeros(3, np.int32)
+    pr = np.zeros(12, np.float32)
+    err = lib().isaac_hip_parse_functor_chain(chain.encode(), bc.ctypes.data, pr.ctypes.data)
+    return bc.tolist(), pr.reshape(3, 4).tolist(), int(err)
+
+
+def build_transfer_lut(points: Sequence[Tuple[int, Sequence[float]]], T: int) -> np.ndarray:
+    pos = np.array([int(p) for p, _ in points], np.uint32)
+    rgba = np.array([list(c) for _, c in points], np.float32).reshape(-1)
+    lut = np.zeros((T, 4), np.float32)
+    check(lib().isaac_hip_build_transfer_lut(T, len(points), pos.ctypes.data, rgba.ctypes.data, lut.ctypes.data))
+    return lut
+
+
+# --------------------------------------------------------------------------- visualization
+class IsaacVisualization:
+    """One rank's render state (IsaacVisualization<...>, lib/isaac.hpp:58-3036)."""
+
+    def __init__(
+        self,
+        name: str,
+        framebuffer_size: Tuple[int, int],
+        global_size: Tuple[int, int, int],
+        local_size: Tuple[int, int, int],
+        local_particle_size: Tuple[int, int, int],
+        position: Tuple[int, int, int],
+        volume_sources: Sequence[VolumeSource],
+        field_sources: Sequence[VolumeSource],
+        particle_sources: Sequence[ParticleSource],
+        scale: Tuple[float, float, float] = (1.0, 1.0, 1.0),
+        transfer_size: int = 1024,
+        device: int = 0,
+        master: int = 0,
+        server_url: Optional[str] = None,
+        server_port: Optional[int] = None,
+    ):
+        self.name = name
+        self.framebuffer_size = tuple(int(v) for v in framebuffer_size)
+        self.global_size = tuple(int(v) for v in global_size)
+        self.local_size = tuple(int(v) for v in local_size)
+        self.local_particle_size = tuple(int(v) for v in local_particle_size)
+        self.position = tuple(int(v) for v in position)
+        self.scale = tuple(float(v) for v in scale)
+        self.transfer_size = int(transfer_size)
+        self.volume_sources = list(volume_sources)
+        self.field_sources = list(field_sources)
+        self.particle_sources = list(particle_sources)
+        self.device = device
+        n = len(self.volume_sources) + len(self.field_sources) + len(self.particle_sources)
+        self.combined_source_list_size = n
+        cfg = _lib.Config()
+        cfg.framebuffer_size[:] = self.framebuffer_size
+        cfg.global_size[:] = self.global_size
+        cfg.local_size[:] = self.local_size
+        cfg.local_particle_size[:] = self.local_particle_size
+        cfg.position[:] = self.position
+        cfg.scale[:] = self.scale
+        cfg.transfer_size = self.transfer_size
+        cfg.n_volume_sources = len(self.volume_sources)
+        cfg.n_field_sources = len(self.field_sources)
+        cfg.n_particle_sources = len(self.particle_sources)
+        ctx = ctypes.c_void_p()
+        check(lib().isaac_hip_create(ctypes.byref(cfg), device, ctypes.byref(ctx)), None, "isaac_hip_create")
+        self._ctx = ctx
+        # steering state with the reference's defaults (Appendix of defaults: lib/isaac.hpp:608-609,720-794,2987-2988)
+        nv, nf = len(self.volume_sources), len(self.field_sources)
+        self.functions = ["idem"] * nv + ["length"] * nf + ["idem"] * len(self.particle_sources)
+        self.function_errors = [0] * n
+        self.source_weight = [2.0] * n
+        self.iso_threshold = [0.0] * n
+        self.transfer_points: List[Dict[int, Tuple[float, float, float, float]]] = []
+        for i in range(n):
+            pos = np.zeros(2, np.uint32)
+            rgba = np.zeros(8, np.float32)
+            lib().isaac_hip_default_transfer(i, n, 1 if i >= nv + nf else 0, self.transfer_size, pos.ctypes.data, rgba.ctypes.data)
+            self.transfer_points.append({int(pos[0]): tuple(rgba[:4].tolist()), int(pos[1]): tuple(rgba[4:].tolist())})
+        self.interpolation = False
+        self.step = 0.5
+        self.dither_mode = 1
+        self.render_mode = 0
+        self.ambient_occlusion = {"isEnabled": False, "weight": 0.5}
+        self.background_color = [0.0, 0.0, 0.0, 1.0]
+        self.look_at = np.zeros(3)
+        self.rotation = np.eye(3)
+        self.distance = -4.5
+        self.projection = default_projection(*self.framebuffer_size)
+        self.modelview = modelview(self.look_at, self.rotation, self.distance)
+        self.clipping: List[Tuple[Tuple[float, float, float], Tuple[float, float, float]]] = []
+        self.kernel_time = self.copy_time = self.buffer_time = self.optimization_buffer_time = 0.0
+        self._sources_bound = False
+        self.update_functions()
+
+    # ---------------------------------------------------------------- lifetime
+    def close(self) -> None:
+        if getattr(self, "_ctx", None):
+            lib().isaac_hip_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init(self, communicator_setting=None) -> int:
+        """No server connection in this library; always succeeds."""
+        return 0
+
+    # ---------------------------------------------------------------- setters
+    def update_functions(self) -> None:
+        for i, f in enumerate(self.functions):
+            err = ctypes.c_int32(0)
+            check(lib().isaac_hip_set_function(self._ctx, i, f.encode(), ctypes.byref(err)), self._ctx, "set_function")
+            self.function_errors[i] = int(err.value)
+
+    def update_transfer(self) -> None:
+        for i, pts in enumerate(self.transfer_points):
+            items = sorted(pts.items())
+            pos = np.array([k for k, _ in items], np.uint32)
+            rgba = np.array([v for _, v in items], np.float32).reshape(-1)
+            check(
+                lib().isaac_hip_set_transfer_points(self._ctx, i, len(items), pos.ctypes.data, rgba.ctypes.data),
+                self._ctx,
+                "set_transfer_points",
+            )
+
+    def set_transfer_points(self, idx: int, points: Sequence[Tuple[int, Sequence[float]]]) -> None:
+        d: Dict[int, Tuple[float, ...]] = {}
+        for k, v in points:
+            d.setdefault(int(k), tuple(float(x) for x in v))  # std::map::insert keeps the first
+        self.transfer_points[idx] = d
+
+    def add_clipping(self, px, py, pz, nx, ny, nz) -> bool:
+        ok = check(lib().isaac_hip_add_clipping(self._ctx, px, py, pz, nx, ny, nz), self._ctx, "add_clipping") == 1
+        if ok:
+            self.clipping.append(((px, py, pz), (nx, ny, nz)))
+        return ok
+
+    def edit_clipping(self, nr, px, py, pz, nx, ny, nz) -> bool:
+        ok = check(lib().isaac_hip_edit_clipping(self._ctx, nr, px, py, pz, nx, ny, nz), self._ctx, "edit_clipping") == 1
+        if ok and nr < len(self.clipping):
+            self.clipping[nr] = ((px, py, pz), (nx, ny, nz))
+        return ok
+
+    def remove_clipping(self, nr: int) -> None:
+        check(lib().isaac_hip_remove_clipping(self._ctx, nr), self._ctx, "remove_clipping")
+        if nr < len(self.clipping):
+            del self.clipping[nr]
+
+    def update_position(self, position) -> None:
+        self.position = tuple(int(v) for v in position)
+        arr = (ctypes.c_int32 * 3)(*self.position)
+        check(lib().isaac_hip_update_position(self._ctx, arr), self._ctx, "update_position")
+
+    def update_local_size(self, local_size) -> None:
+        self.local_size = tuple(int(v) for v in local_size)
+        arr = (ctypes.c_int64 * 3)(*self.local_size)
+        check(lib().isaac_hip_update_local_size(self._ctx, arr), self._ctx, "update_local_size")
+
+    def update_local_particle_size(self, size) -> None:
+        self.local_particle_size = tuple(int(v) for v in size)
+        arr = (ctypes.c_int64 * 3)(*self.local_particle_size)
+        check(lib().isaac_hip_update_local_particle_size(self._ctx, arr), self._ctx, "update_local_particle_size")
+
+    # camera steering (the "rotation/position/distance" JSON keys, lib/isaac.hpp:1440-1513)
+    def set_rotation_absolute(self, rotation: np.ndarray) -> None:
+        self.rotation = np.asarray(rotation, np.float64)
+        self._update_modelview()
+
+    def rotate_relative(self, relative: np.ndarray) -> None:
+        self.rotation = np.asarray(relative, np.float64) @ self.rotation
+        self._update_modelview()
+
+    def rotate_axis(self, axis, degrees: float) -> None:
+        from .scenes import rotation_axis
+
+        self.rotation = rotation_axis(axis, degrees) @ self.rotation
+        self._update_modelview()
+
+    def set_position_absolute(self, look_at) -> None:
+        self.look_at = np.asarray(look_at, np.float64)
+        self._update_modelview()
+
+    def move_relative(self, translation) -> None:
+        self.look_at = self.look_at + self.rotation.T @ np.asarray(translation, np.float64)
+        self._update_modelview()
+
+    def set_distance(self, distance: float) -> None:
+        self.distance = float(distance)
+        self._update_modelview()
+
+    def distance_relative(self, delta: float) -> None:
+        self.distance += float(delta)
+        self._update_modelview()
+
+    def _update_modelview(self) -> None:
+        self.modelview = modelview(self.look_at, self.rotation, self.distance)
+
+    def _push_state(self) -> None:
+        c = self._ctx
+        L = lib()
+        for i in range(self.combined_source_list_size):
+            check(L.isaac_hip_set_weight(c, i, float(self.source_weight[i])), c, "set_weight")
+            check(L.isaac_hip_set_iso_threshold(c, i, float(self.iso_threshold[i])), c, "set_iso")
+        check(L.isaac_hip_set_step(c, float(self.step)), c, "set_step")
+        check(L.isaac_hip_set_interpolation(c, int(bool(self.interpolation))), c, "set_interpolation")
+        check(L.isaac_hip_set_dither_mode(c, int(self.dither_mode)), c, "set_dither_mode")
+        ao = self.ambient_occlusion
+        check(L.isaac_hip_set_ao(c, int(bool(ao["isEnabled"])), float(ao["weight"])), c, "set_ao")
+        check(L.isaac_hip_set_render_mode(c, int(self.render_mode)), c, "set_render_mode")
+
+    def _bind_sources(self, pointer=None) -> None:
+        c = self._ctx
+        nv = len(self.volume_sources) + len(self.field_sources)
+        for i, s in enumerate(self.volume_sources + self.field_sources):
+            enabled = self.source_weight[i] != 0.0 or self.iso_threshold[i] != 0.0
+            s.update(enabled, pointer)
+            d = s.descriptor()
+            check(lib().isaac_hip_set_volume_source(c, i, ctypes.byref(d)), c, "set_volume_source")
+        for j, s in enumerate(self.particle_sources):
+            s.update(self.source_weight[nv + j] != 0.0, pointer)
+            d = s.descriptor()
+            check(lib().isaac_hip_set_particle_source(c, j, ctypes.byref(d)), c, "set_particle_source")
+
+    # ---------------------------------------------------------------- per frame
+    def update_buffers(self, pointer=None) -> None:
+        """doVisualization's buffer section (lib/isaac.hpp:1805-1948)."""
+        self._push_state()
+        self._bind_sources(pointer)
+        check(lib().isaac_hip_update_buffers(self._ctx), self._ctx, "update_buffers")
+
+    def draw(self, out=None, viewport=None, projection=None, modelview_=None):
+        """drawCallBack (lib/isaac.hpp:2140-2543).  `out`: None -> host numpy
+        RGBA8 (H, W, 4); a CUDA uint8 tensor -> device copy; False -> no copy."""
+        self._push_state()
+        W, H = self.framebuffer_size
+        proj = np.ascontiguousarray(projection if projection is not None else self.projection, np.float64)
+        mv = np.ascontiguousarray(modelview_ if modelview_ is not None else self.modelview, np.float64)
+        bg = np.asarray(self.background_color, np.float32)
+        vp = np.asarray(viewport if viewport is not None else (0, 0, W, H), np.int32)
+        result = None
+        if out is None:
+            result = np.zeros((H, W, 4), np.uint8)
+            ptr, dev = result.ctypes.data, 0
+        elif out is False:
+            ptr, dev = None, 0
+        else:
+            ptr, dev = out.data_ptr(), 1
+        check(
+            lib().isaac_hip_render(self._ctx, proj.ctypes.data, mv.ctypes.data, bg.ctypes.data, vp.ctypes.data, ptr, dev),
+            self._ctx,
+            "render",
+        )
+        return result if out is None else out
+
+    def do_visualization(self, meta_targets=None, pointer=None, redraw: bool = True, out=None):
+        """One frame: buffers (when redrawing) + draw; returns the framebuffer."""
+        if redraw:
+            self.update_buffers(pointer)
+        return self.draw(out=out)
+
+    def synchronize(self) -> None:
+        check(lib().isaac_hip_synchronize(self._ctx), self._ctx, "synchronize")
+
+    def timings(self) -> Dict[str, float]:
+        t = _lib.Timings()
+        check(lib().isaac_hip_get_timings(self._ctx, ctypes.byref(t)), self._ctx, "get_timings")
+        self.kernel_time, self.copy_time = t.kernel_us, t.copy_us
+        self.buffer_time, self.optimization_buffer_time = t.buffer_us, t.optimization_buffer_us
+        return {"kernel_us": t.kernel_us, "copy_us": t.copy_us, "buffer_us": t.buffer_us, "optimization_buffer_us": t.optimization_buffer_us}
+
+    def set_counting(self, on: bool) -> None:
+        check(lib().isaac_hip_set_counting(self._ctx, int(on)), self._ctx, "set_counting")
+
+    def counters(self) -> Dict[str, int]:
+        out = np.zeros(8, np.uint64)
+        check(lib().isaac_hip_get_counters(self._ctx, out.ctypes.data), self._ctx, "get_counters")
+        return dict(zip(_lib.CNT_NAMES, [int(v) for v in out]))
+
+    # ---------------------------------------------------------------- introspection (tests)
+    def gbuffer(self):
+        """Copies of the G-buffer: colour (H,W,4) u8, depth (H,W), normal (H,W,3), ao (H,W)."""
+        import torch
+
+        W, H = self.framebuffer_size
+        ptrs = [ctypes.c_void_p() for _ in range(4)]
+        check(lib().isaac_hip_get_gbuffer(self._ctx, *[ctypes.byref(p) for p in ptrs]), self._ctx, "get_gbuffer")
+        self.synchronize()
+        out = []
+        for p, shape, dt in zip(ptrs, [(H, W, 4), (H, W), (H, W, 3), (H, W)], [np.uint8, np.float32, np.float32, np.float32]):
+            a = np.zeros(shape, dt)
+            _hip_d2h(a, p.value)
+            out.append(a)
+        return out
+
+    def combined_textures(self):
+        X, Y, Z = self.local_size
+        pv, pi = ctypes.c_void_p(), ctypes.c_void_p()
+        check(lib().isaac_hip_get_combined_textures(self._ctx, ctypes.byref(pv), ctypes.byref(pi)), self._ctx, "textures")
+        self.synchronize()
+        vol = np.zeros((Z, Y, X, 4), np.uint8)
+        iso = np.zeros((Z, Y, X, 4), np.uint8)
+        _hip_d2h(vol, pv.value)
+        _hip_d2h(iso, pi.value)
+        return vol, iso
+
+    def cell_list(self, idx: int):
+        gx, gy, gz = self.local_particle_size
+        nc = gx * gy * gz
+        n = int(self.particle_sources[idx].pos.shape[0])
+        pc, po, pp = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        check(lib().isaac_hip_get_cell_list(self._ctx, idx, ctypes.byref(pc), ctypes.byref(po), ctypes.byref(pp)), self._ctx, "cell_list")
+        self.synchronize()
+        counts = np.zeros(nc, np.int32)
+        offs = np.zeros(nc + 1, np.int32)
+        perm = np.zeros(n, np.int32)
+        _hip_d2h(counts, pc.value)
+        _hip_d2h(offs, po.value)
+        _hip_d2h(perm, pp.value)
+        return counts, offs, perm
+
+
+_hip = None
+
+
+def _hip_d2h(arr: np.ndarray, dev_ptr: int) -> None:
+    global _hip
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    rc = _hip.hipMemcpy(arr.ctypes.data, ctypes.c_void_p(dev_ptr), arr.nbytes, 2)  # hipMemcpyDeviceToHost
+    if rc != 0:
+        raise _lib.IsaacHipError(f"hipMemcpy D2H failed ({rc})")
+
+
+# --------------------------------------------------------------------------- scene -> visualization
+def from_scene(spec, device: int = 0) -> IsaacVisualization:
+    """Build a visualization for a scenes.SceneSpec with its sources uploaded to `device`."""
+    import torch
+
+    dev = torch.device("cuda", device)
+    vols = [VolumeSource(v.name, torch.from_numpy(np.ascontiguousarray(v.data)).to(dev), v.persistent) for v in spec.volumes]
+    parts = []
+    for p in spec.particles:
+        rad = torch.from_numpy(p.radius).to(dev) if p.radius is not None else None
+        parts.append(
+            ParticleSource(p.name, torch.from_numpy(p.pos).to(dev), torch.from_numpy(p.attr).to(dev), rad, p.constant_radius)
+        )
+    torch.cuda.synchronize(dev)
+    vis = IsaacVisualization(
+        spec.name, spec.framebuffer, spec.global_size, spec.local_size, spec.particle_grid, spec.position,
+        vols, [], parts, spec.scale, spec.transfer_size, device,
+    )
+    apply_scene(vis, spec)
+    return vis
+
+
+def apply_scene(vis: IsaacVisualization, spec) -> None:
+    srcs = list(spec.volumes) + list(spec.particles)
+    vis.functions = [s.chain for s in srcs]
+    vis.update_functions()
+    for i, s in enumerate(srcs):
+        vis.set_transfer_points(i, s.tf_points)
+        vis.source_weight[i] = s.weight
+        vis.iso_threshold[i] = getattr(s, "iso", 0.0) if i < len(spec.volumes) else 0.0
+    vis.update_transfer()
+    vis.step = spec.step
+    vis.interpolation = spec.interpolation
+    vis.dither_mode = spec.dither_mode
+    vis.render_mode = spec.render_mode
+    vis.ambient_occlusion = {"isEnabled": spec.ao_enabled, "weight": spec.ao_weight}
+    vis.background_color = list(spec.background)
+    while len(vis.clipping):
+        vis.remove_clipping(0)
+    for pos, nrm in spec.clipping:
+        vis.add_clipping(*pos, *nrm)
+    cam = spec.camera
+    vis.look_at = np.asarray(cam.look_at, np.float64)
+    vis.rotation = np.asarray(cam.rotation, np.float64)
+    vis.distance = float(cam.distance)
+    vis._update_modelview()

@@ -1,0 +1,417 @@
+"""CPU ORACLE — TEST INFRASTRUCTURE ONLY.
+
+ctypes front end of oracle/liboracle.so (the C++ restatement in
+oracle/isaac_oracle.hpp) plus independent Python restatements of the
+host-side steps of the reference's render path:
+
+  build_lut      updateTransfer            lib/isaac.hpp:1244-1266
+  parse_chain    updateFunctions parser    lib/isaac.hpp:1079-1187
+  perspective    DefaultController          lib/isaac/isaac_controllers.hpp:33-37 (glm perspectiveRH_NO)
+  modelview      updateModelview            lib/isaac.hpp:2878-2889
+  clip_normal    editClipping               lib/isaac.hpp:958-986
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
+this module; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import re
+import subprocess
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+MAX_SOURCES = 16
+MAX_FUNCTORS = 3
+MAX_CLIPPING = 16
+FUNCTORS = ["idem", "add", "mul", "length", "pow", "sum"]  # lib/isaac/isaac_functors.hpp:185-208
+Z_NEAR = float(np.float32(0.002))  # ISAAC_Z_NEAR is a float literal (lib/isaac/isaac_defines.hpp:53)
+Z_FAR = 100.0
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", HERE])
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.oracle_render_f32.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        _lib.oracle_render_f64.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        _lib.oracle_apply_chain.restype = ctypes.c_float
+        _lib.oracle_ray.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_void_p]
+        _lib.oracle_perspective.argtypes = [ctypes.c_double] * 4 + [ctypes.c_void_p]
+    return _lib
+
+
+# --------------------------------------------------------------------------- host-side restatements
+def build_lut(points: Sequence[Tuple[int, Sequence[float]]], T: int) -> np.ndarray:
+    """updateTransfer: segments [p_k, p_{k+1}) of (c_k*(w-j) + c_{k+1}*j)/w in
+    float32; std::map insert keeps the first value of a duplicate key; entries
+    never written stay zero."""
+    d = {}
+    for k, rgba in points:
+        k = int(k) & 0xFFFFFFFF
+        if k not in d:
+            d[k] = np.asarray(rgba, dtype=np.float32)
+    keys = sorted(d)
+    lut = np.zeros((T, 4), dtype=np.float32)
+    for a, b in zip(keys, keys[1:]):
+        width = b - a
+        n = max(0, min(width, T - a))
+        if n == 0:
+            continue
+        j = np.arange(n, dtype=np.int64)
+        wj = (width - j).astype(np.float32)[:, None]
+        jj = j.astype(np.float32)[:, None]
+        lut[a : a + n] = (d[a][None, :] * wj + d[b][None, :] * jj) / np.float32(width)
+    return lut
+
+
+_FLOAT_PREFIX = re.compile(
+    r"^[ \t\n\v\f\r]*[+-]?(?:(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?|inf(?:inity)?|nan)", re.IGNORECASE
+)
+
+
+def _stof(s: str) -> float:
+    m = _FLOAT_PREFIX.match(s)
+    if not m:
+        raise ValueError(s)
+    v = float(m.group(0))
+    if math.isfinite(v) and abs(v) > float(np.finfo(np.float32).max):
+        raise OverflowError(s)
+    return float(np.float32(v))
+
+
+def parse_chain(src: str) -> Tuple[List[int], List[List[float]], int]:
+    """updateFunctions' parser for one source: returns (bytecode[3],
+    params[3][4], errorCode)."""
+    bytecode = [0, 0, 0]
+    params = [[0.0] * 4 for _ in range(MAX_FUNCTORS)]
+    error = 0
+    source = src
+    elem = 0
+    again = True
+    while again:
+        pos = source.find("|")
+        if pos < 0:
+            again = False
+            token = source
+        else:
+            token = source[:pos]
+            source = source[pos + 1 :]
+        if elem >= MAX_FUNCTORS:
+            error = 1
+            break
+        token = "".join(ch for ch in token if ch not in " \t\n\v\f\r")
+        t_begin = token.find("(")
+        if t_begin < 0:
+            params[elem] = [0.0] * 4
+        else:
+            t_end = token.find(")")
+            if t_end < 0:
+                error = -1
+                break
+            if t_end - t_begin == 1:
+                params[elem] = [0.0] * 4
+            else:
+                parameters = token[t_begin + 1 : t_end]
+                arr = params[elem]
+                p_elem = 0
+                p_again = True
+                while p_again:
+                    ppos = parameters.find(",")
+                    if ppos < 0:
+                        p_again = False
+                        par = parameters
+                    else:
+                        par = parameters[:ppos]
+                        parameters = parameters[ppos + 1 :]
+                    if p_elem >= 4:
+                        error = 2
+                        break
+                    try:
+                        arr[p_elem] = _stof(par)
+                    except (ValueError, OverflowError):
+                        error = -2
+                        p_elem += 1
+                        break
+                    p_elem += 1
+                for k in range(p_elem, 4):
+                    arr[k] = arr[k - 1]
+        if t_begin >= 0:
+            token = token[:t_begin]
+        if token in FUNCTORS:
+            bytecode[elem] = FUNCTORS.index(token)
+        else:
+            error = -2
+            break
+        elem += 1
+    for k in range(elem, MAX_FUNCTORS):
+        bytecode[k] = 0
+        params[k] = [0.0] * 4
+    return bytecode, params, error
+
+
+def perspective(fovy: float, aspect: float, zn: float, zf: float) -> np.ndarray:
+    """glm perspectiveRH_NO, column-major 16 doubles."""
+    t = math.tan(fovy / 2.0)
+    m = np.zeros(16)
+    m[0] = 1.0 / (aspect * t)
+    m[5] = 1.0 / t
+    m[10] = -(zf + zn) / (zf - zn)
+    m[11] = -1.0
+    m[14] = -(2.0 * zf * zn) / (zf - zn)
+    return m
+
+
+def default_projection(w: int, h: int) -> np.ndarray:
+    # DefaultController passes 45.0 to glm::perspective (radians in glm 0.9.9)
+    return perspective(45.0, float(w) / float(h), Z_NEAR, Z_FAR)
+
+
+def modelview(look_at, rotation: np.ndarray, distance: float) -> np.ndarray:
+    """distanceM * rotationM * translationM (lib/isaac.hpp:2878-2889), returned
+    column-major (glm value_ptr order).  `rotation` is the math matrix R."""
+    T = np.eye(4)
+    T[:3, 3] = np.asarray(look_at, dtype=np.float64)
+    R = np.eye(4)
+    R[:3, :3] = np.asarray(rotation, dtype=np.float64)
+    D = np.eye(4)
+    D[2, 3] = distance
+    M = D @ R @ T
+    return M.T.reshape(16).copy()
+
+
+def clip_normal(n: Sequence[float], scale: Sequence[float]) -> Optional[np.ndarray]:
+    f = np.float32
+    ns = np.array([f(n[0]) * f(scale[0]), f(n[1]) * f(scale[1]), f(n[2]) * f(scale[2])], dtype=np.float32)
+    l = np.sqrt(f(f(ns[0] * ns[0]) + f(ns[1] * ns[1])) + f(ns[2] * ns[2]))
+    if l == 0:
+        return None
+    return (ns / l).astype(np.float32)
+
+
+# --------------------------------------------------------------------------- ctypes structs
+class OracleScene(ctypes.Structure):
+    _fields_ = [
+        ("global_size", ctypes.c_int64 * 3),
+        ("local_size", ctypes.c_int64 * 3),
+        ("particle_grid", ctypes.c_int64 * 3),
+        ("position", ctypes.c_int32 * 3),
+        ("scale", ctypes.c_float * 3),
+        ("fb_w", ctypes.c_int32),
+        ("fb_h", ctypes.c_int32),
+        ("viewport", ctypes.c_int32 * 4),
+        ("transfer_size", ctypes.c_int32),
+        ("n_volume", ctypes.c_int32),
+        ("n_particle", ctypes.c_int32),
+        ("vol_data", ctypes.c_void_p * MAX_SOURCES),
+        ("vol_dim", ctypes.c_int32 * MAX_SOURCES),
+        ("vol_stride", (ctypes.c_int64 * 3) * MAX_SOURCES),
+        ("part_pos", ctypes.c_void_p * MAX_SOURCES),
+        ("part_attr", ctypes.c_void_p * MAX_SOURCES),
+        ("part_dim", ctypes.c_int32 * MAX_SOURCES),
+        ("part_radius", ctypes.c_void_p * MAX_SOURCES),
+        ("part_const_radius", ctypes.c_float * MAX_SOURCES),
+        ("part_count", ctypes.c_int64 * MAX_SOURCES),
+        ("lut", ctypes.c_void_p * (2 * MAX_SOURCES)),
+        ("bytecode", (ctypes.c_int32 * MAX_FUNCTORS) * (2 * MAX_SOURCES)),
+        ("params", ((ctypes.c_float * 4) * MAX_FUNCTORS) * (2 * MAX_SOURCES)),
+        ("weight", ctypes.c_float * (2 * MAX_SOURCES)),
+        ("iso", ctypes.c_float * (2 * MAX_SOURCES)),
+        ("step", ctypes.c_float),
+        ("interpolation", ctypes.c_int32),
+        ("dither_mode", ctypes.c_int32),
+        ("ao_enabled", ctypes.c_int32),
+        ("ao_weight", ctypes.c_float),
+        ("render_mode", ctypes.c_int32),
+        ("n_clip", ctypes.c_int32),
+        ("clip_pos", (ctypes.c_float * 3) * MAX_CLIPPING),
+        ("clip_normal", (ctypes.c_float * 3) * MAX_CLIPPING),
+        ("proj", ctypes.c_double * 16),
+        ("mv", ctypes.c_double * 16),
+        ("bg", ctypes.c_float * 4),
+        ("threads", ctypes.c_int32),
+    ]
+
+
+class OracleOutputs(ctypes.Structure):
+    _fields_ = [
+        ("rgba", ctypes.c_void_p),
+        ("solid", ctypes.c_void_p),
+        ("pre", ctypes.c_void_p),
+        ("pre_depth", ctypes.c_void_p),
+        ("depth", ctypes.c_void_p),
+        ("normal", ctypes.c_void_p),
+        ("ao", ctypes.c_void_p),
+        ("vol_tex", ctypes.c_void_p),
+        ("iso_tex", ctypes.c_void_p),
+        ("counters", ctypes.c_uint64 * 8),
+        ("seconds_classify", ctypes.c_double),
+        ("seconds_render", ctypes.c_double),
+        ("seconds_bin", ctypes.c_double),
+    ]
+
+
+COUNTER_NAMES = ["vol_samples", "iso_samples", "iso_hits", "cells_visited", "sphere_tests", "vol_rays", "iso_rays", "part_rays"]
+
+
+class PreparedScene:
+    """A SceneSpec converted to the oracle's flat struct (keeps numpy buffers alive)."""
+
+    def __init__(self, spec, proj: Optional[np.ndarray] = None, mv: Optional[np.ndarray] = None, threads: int = 0):
+        self.spec = spec
+        self.keep = []
+        sc = OracleScene()
+        sc.global_size[:] = spec.global_size
+        sc.local_size[:] = spec.local_size
+        sc.particle_grid[:] = spec.particle_grid
+        sc.position[:] = spec.position
+        sc.scale[:] = spec.scale
+        W, H = spec.framebuffer
+        sc.fb_w, sc.fb_h = W, H
+        sc.viewport[:] = spec.vp()
+        T = spec.transfer_size
+        sc.transfer_size = T
+        sc.n_volume = len(spec.volumes)
+        sc.n_particle = len(spec.particles)
+        nv = len(spec.volumes)
+        sources = list(spec.volumes) + list(spec.particles)
+        for s, v in enumerate(spec.volumes):
+            data = np.ascontiguousarray(v.data, dtype=np.float32)
+            self.keep.append(data)
+            sc.vol_data[s] = data.ctypes.data
+            sc.vol_dim[s] = v.feature_dim
+            X, Y, Z = spec.local_size
+            sc.vol_stride[s][0], sc.vol_stride[s][1], sc.vol_stride[s][2] = 1, X, X * Y
+        for s, p in enumerate(spec.particles):
+            pos = np.ascontiguousarray(p.pos, dtype=np.float32)
+            attr = np.ascontiguousarray(p.attr, dtype=np.float32)
+            self.keep += [pos, attr]
+            sc.part_pos[s] = pos.ctypes.data
+            sc.part_attr[s] = attr.ctypes.data
+            sc.part_dim[s] = p.feature_dim
+            if p.radius is not None:
+                r = np.ascontiguousarray(p.radius, dtype=np.float32)
+                self.keep.append(r)
+                sc.part_radius[s] = r.ctypes.data
+            else:
+                sc.part_radius[s] = None
+            sc.part_const_radius[s] = p.constant_radius
+            sc.part_count[s] = pos.shape[0]
+        self.luts = []
+        for s, src in enumerate(sources):
+            lut = build_lut(src.tf_points, T)
+            self.luts.append(lut)
+            sc.lut[s] = lut.ctypes.data
+            bc, pr, err = parse_chain(src.chain)
+            for k in range(MAX_FUNCTORS):
+                sc.bytecode[s][k] = bc[k]
+                for c in range(4):
+                    sc.params[s][k][c] = pr[k][c]
+            sc.weight[s] = src.weight
+            sc.iso[s] = getattr(src, "iso", 0.0) if s < nv else 0.0
+        sc.step = max(spec.step, float(np.float32(0.01)))
+        sc.interpolation = int(spec.interpolation)
+        sc.dither_mode = spec.dither_mode
+        sc.ao_enabled = int(spec.ao_enabled)
+        sc.ao_weight = spec.ao_weight
+        sc.render_mode = spec.render_mode
+        n = 0
+        for pos, nrm in spec.clipping:
+            cn = clip_normal(nrm, spec.scale)
+            if cn is None or n >= MAX_CLIPPING:
+                continue
+            sc.clip_pos[n][:] = pos
+            sc.clip_normal[n][:] = cn
+            n += 1
+        sc.n_clip = n
+        cam = spec.camera
+        self.proj = default_projection(W, H) if proj is None else np.asarray(proj, dtype=np.float64)
+        self.mv = modelview(cam.look_at, cam.rotation, cam.distance) if mv is None else np.asarray(mv, dtype=np.float64)
+        sc.proj[:] = self.proj.tolist()
+        sc.mv[:] = self.mv.tolist()
+        sc.bg[:] = spec.background
+        sc.threads = threads
+        self.sc = sc
+
+
+class OracleResult:
+    pass
+
+
+def render(spec, precision: int = 32, threads: int = 0, want_textures: bool = False, prepared: Optional[PreparedScene] = None):
+    """Render one frame (classification + binning + drawCallBack) on the CPU."""
+    ps = prepared or PreparedScene(spec, threads=threads)
+    ps.sc.threads = threads
+    W, H = spec.framebuffer
+    X, Y, Z = spec.local_size
+    res = OracleResult()
+    res.rgba = np.zeros((H, W, 4), np.uint8)
+    res.solid = np.zeros((H, W, 4), np.uint8)
+    res.pre = np.zeros((H, W, 4), np.uint8)
+    res.pre_depth = np.zeros((H, W), np.float32)
+    res.depth = np.zeros((H, W), np.float32)
+    res.normal = np.zeros((H, W, 3), np.float32)
+    res.ao = np.zeros((H, W), np.float32)
+    out = OracleOutputs()
+    out.rgba = res.rgba.ctypes.data
+    out.solid = res.solid.ctypes.data
+    out.pre = res.pre.ctypes.data
+    out.pre_depth = res.pre_depth.ctypes.data
+    out.depth = res.depth.ctypes.data
+    out.normal = res.normal.ctypes.data
+    out.ao = res.ao.ctypes.data
+    if want_textures:
+        res.vol_tex = np.zeros((Z, Y, X, 4), np.uint8)
+        res.iso_tex = np.zeros((Z, Y, X, 4), np.uint8)
+        out.vol_tex = res.vol_tex.ctypes.data
+        out.iso_tex = res.iso_tex.ctypes.data
+    fn = lib().oracle_render_f32 if precision == 32 else lib().oracle_render_f64
+    rc = fn(ctypes.byref(ps.sc), ctypes.byref(out))
+    if rc != 0:
+        raise RuntimeError(f"oracle render failed rc={rc}")
+    res.counters = dict(zip(COUNTER_NAMES, [int(c) for c in out.counters]))
+    res.seconds = {"classify": out.seconds_classify, "bin": out.seconds_bin, "render": out.seconds_render}
+    res.prepared = ps
+    return res
+
+
+def bin_particles(pos: np.ndarray, local_size, grid):
+    pos = np.ascontiguousarray(pos, dtype=np.float32)
+    n = pos.shape[0]
+    ncell = int(grid[0] * grid[1] * grid[2])
+    cell_of = np.zeros(n, np.int32)
+    counts = np.zeros(ncell, np.int32)
+    offsets = np.zeros(ncell + 1, np.int64)
+    perm = np.zeros(n, np.int64)
+    rel = np.zeros((n, 3), np.float32)
+    L = (ctypes.c_int64 * 3)(*local_size)
+    G = (ctypes.c_int64 * 3)(*grid)
+    lib().oracle_bin_particles(
+        ctypes.c_void_p(pos.ctypes.data), ctypes.c_int64(n), L, G,
+        ctypes.c_void_p(cell_of.ctypes.data), ctypes.c_void_p(counts.ctypes.data),
+        ctypes.c_void_p(offsets.ctypes.data), ctypes.c_void_p(perm.ctypes.data), ctypes.c_void_p(rel.ctypes.data),
+    )
+    return cell_of, counts, offsets, perm, rel
+
+
+def psnr(a: np.ndarray, b: np.ndarray, peak: float = 255.0) -> float:
+    d = a.astype(np.float64) - b.astype(np.float64)
+    mse = float(np.mean(d * d))
+    if mse == 0.0:
+        return float("inf")
+    return 10.0 * math.log10(peak * peak / mse)

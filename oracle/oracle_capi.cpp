@@ -1,0 +1,335 @@
+// CPU ORACLE C entry points — TEST INFRASTRUCTURE ONLY (loaded by tests/, the
+// smoke check and the benchmark's CPU-baseline leg through ctypes).
+#include "isaac_oracle.hpp"
+
+#include <chrono>
+#include <omp.h>
+
+using namespace isaac_oracle;
+
+
+// Flat scene description shared with oracle/oracle.py (ctypes mirror).
+struct OracleScene
+{
+    int64_t global_size[3];
+    int64_t local_size[3];
+    int64_t particle_grid[3];
+    int32_t position[3];
+    float scale[3];
+    int32_t fb_w, fb_h;
+    int32_t viewport[4];
+    int32_t transfer_size;
+    int32_t n_volume;
+    int32_t n_particle;
+    const float* vol_data[MAX_SOURCES];
+    int32_t vol_dim[MAX_SOURCES];
+    int64_t vol_stride[MAX_SOURCES][3];
+    const float* part_pos[MAX_SOURCES];
+    const float* part_attr[MAX_SOURCES];
+    int32_t part_dim[MAX_SOURCES];
+    const float* part_radius[MAX_SOURCES];
+    float part_const_radius[MAX_SOURCES];
+    int64_t part_count[MAX_SOURCES];
+    const float* lut[2 * MAX_SOURCES];
+    int32_t bytecode[2 * MAX_SOURCES][MAX_FUNCTORS];
+    float params[2 * MAX_SOURCES][MAX_FUNCTORS][4];
+    float weight[2 * MAX_SOURCES];
+    float iso[2 * MAX_SOURCES];
+    float step;
+    int32_t interpolation;
+    int32_t dither_mode;
+    int32_t ao_enabled;
+    float ao_weight;
+    int32_t render_mode;
+    int32_t n_clip;
+    float clip_pos[MAX_CLIPPING][3];
+    float clip_normal[MAX_CLIPPING][3];
+    double proj[16];
+    double mv[16];
+    float bg[4];
+    int32_t threads; // 0 = OpenMP default
+};
+
+struct OracleOutputs
+{
+    uint8_t* rgba; // W*H*4 final framebuffer
+    uint8_t* solid; // W*H*4 after shading, before volume
+    uint8_t* pre; // W*H*4 after particles + iso, before shading
+    float* pre_depth; // W*H depth before shading
+    float* depth; // W*H (as float even for the FP64 oracle)
+    float* normal; // W*H*3
+    float* ao; // W*H
+    uint8_t* vol_tex; // local voxels * 4
+    uint8_t* iso_tex; // local voxels * 4
+    uint64_t counters[8]; // volSamples isoSamples isoHits cellsVisited sphereTests volRays isoRays partRays
+    double seconds_classify;
+    double seconds_render;
+    double seconds_bin;
+};
+
+
+namespace
+{
+    State make_state(const OracleScene* sc)
+    {
+        State st;
+        st.scale[0] = sc->scale[0];
+        st.scale[1] = sc->scale[1];
+        st.scale[2] = sc->scale[2];
+        st.sizes = make_sizes(sc->global_size, sc->local_size, sc->particle_grid, sc->position, sc->scale);
+        st.fbW = sc->fb_w;
+        st.fbH = sc->fb_h;
+        for(int i = 0; i < 4; i++)
+            st.viewport[i] = sc->viewport[i];
+        st.T = sc->transfer_size;
+        const int nv = sc->n_volume, np = sc->n_particle;
+        st.vol.resize(nv);
+        for(int s = 0; s < nv; s++)
+        {
+            st.vol[s].data = sc->vol_data[s];
+            st.vol[s].dim = sc->vol_dim[s];
+            for(int k = 0; k < 3; k++)
+                st.vol[s].stride[k] = sc->vol_stride[s][k];
+        }
+        st.part.resize(np);
+        for(int s = 0; s < np; s++)
+        {
+            st.part[s].pos = sc->part_pos[s];
+            st.part[s].attr = sc->part_attr[s];
+            st.part[s].dim = sc->part_dim[s];
+            st.part[s].radius = sc->part_radius[s];
+            st.part[s].const_radius = sc->part_const_radius[s];
+            st.part[s].count = sc->part_count[s];
+        }
+        st.lut.resize(nv + np);
+        st.chain.resize(nv + np);
+        st.weight.resize(nv + np);
+        st.iso.resize(nv + np);
+        for(int s = 0; s < nv + np; s++)
+        {
+            st.lut[s] = sc->lut[s];
+            for(int k = 0; k < MAX_FUNCTORS; k++)
+            {
+                st.chain[s].bytecode[k] = sc->bytecode[s][k];
+                for(int c = 0; c < 4; c++)
+                    st.chain[s].params[k][c] = sc->params[s][k][c];
+            }
+            st.weight[s] = sc->weight[s];
+            st.iso[s] = sc->iso[s];
+        }
+        st.step = sc->step;
+        st.interpolation = sc->interpolation != 0;
+        st.ditherMode = sc->dither_mode;
+        st.aoEnabled = sc->ao_enabled != 0;
+        st.aoWeight = sc->ao_weight;
+        st.renderMode = sc->render_mode;
+        st.clip.resize(sc->n_clip);
+        for(int i = 0; i < sc->n_clip; i++)
+            for(int k = 0; k < 3; k++)
+            {
+                st.clip[i].position[k] = sc->clip_pos[i][k];
+                st.clip[i].normal[k] = sc->clip_normal[i][k];
+            }
+        for(int i = 0; i < 16; i++)
+        {
+            st.proj[i] = sc->proj[i];
+            st.mv[i] = sc->mv[i];
+        }
+        for(int i = 0; i < 4; i++)
+            st.bg[i] = sc->bg[i];
+        return st;
+    }
+
+    template<class R>
+    int render_impl(const OracleScene* sc, OracleOutputs* out)
+    {
+        if(sc->threads > 0)
+            omp_set_num_threads(sc->threads);
+        State st = make_state(sc);
+        using clk = std::chrono::steady_clock;
+        auto t0 = clk::now();
+        Tex3B4 volTex, isoTex;
+        classify<R>(st, volTex, isoTex);
+        auto t1 = clk::now();
+        std::vector<CellList> lists;
+        for(auto& ps : st.part)
+            lists.push_back(bin_particles(ps, st.sizes));
+        auto t2 = clk::now();
+        FrameOut<R> fo;
+        render<R>(st, volTex, isoTex, lists, fo);
+        auto t3 = clk::now();
+        out->seconds_classify = std::chrono::duration<double>(t1 - t0).count();
+        out->seconds_bin = std::chrono::duration<double>(t2 - t1).count();
+        out->seconds_render = std::chrono::duration<double>(t3 - t2).count();
+        const size_t N = size_t(st.fbW) * st.fbH;
+        if(out->rgba)
+            std::memcpy(out->rgba, fo.g.color.data(), N * 4);
+        if(out->solid)
+            std::memcpy(out->solid, fo.solid.data(), N * 4);
+        if(out->pre)
+            std::memcpy(out->pre, fo.pre.data(), N * 4);
+        for(size_t i = 0; i < N; i++)
+        {
+            if(out->depth)
+                out->depth[i] = float(fo.g.depth[i]);
+            if(out->pre_depth)
+                out->pre_depth[i] = float(fo.preDepth[i]);
+            if(out->ao)
+                out->ao[i] = float(fo.g.ao[i]);
+            if(out->normal)
+            {
+                out->normal[3 * i + 0] = float(fo.g.normal[i].x);
+                out->normal[3 * i + 1] = float(fo.g.normal[i].y);
+                out->normal[3 * i + 2] = float(fo.g.normal[i].z);
+            }
+        }
+        if(out->vol_tex)
+            std::memcpy(out->vol_tex, volTex.data.data(), volTex.data.size() * 4);
+        if(out->iso_tex)
+            std::memcpy(out->iso_tex, isoTex.data.data(), isoTex.data.size() * 4);
+        const Counters& c = fo.counters;
+        uint64_t cv[8]
+            = {c.volSamples, c.isoSamples, c.isoHits, c.cellsVisited, c.sphereTests, c.volRays, c.isoRays, c.partRays};
+        for(int i = 0; i < 8; i++)
+            out->counters[i] = cv[i];
+        return 0;
+    }
+
+} // namespace
+
+extern "C"
+{
+    int oracle_render_f32(const OracleScene* sc, OracleOutputs* out)
+    {
+        return render_impl<float>(sc, out);
+    }
+
+    int oracle_render_f64(const OracleScene* sc, OracleOutputs* out)
+    {
+        return render_impl<double>(sc, out);
+    }
+
+    // Stable counting-sort cell list for one particle source.
+    int oracle_bin_particles(
+        const float* pos,
+        int64_t count,
+        const int64_t local_size[3],
+        const int64_t grid[3],
+        int32_t* cell_of,
+        int32_t* counts,
+        int64_t* offsets,
+        int64_t* perm,
+        float* rel)
+    {
+        int64_t g[3] = {local_size[0], local_size[1], local_size[2]};
+        int32_t p0[3] = {0, 0, 0};
+        float s1[3] = {1, 1, 1};
+        Sizes sz = make_sizes(g, local_size, grid, p0, s1);
+        ParticleSource ps;
+        ps.pos = pos;
+        ps.count = count;
+        CellList cl = bin_particles(ps, sz);
+        const int64_t ncell = grid[0] * grid[1] * grid[2];
+        std::memcpy(cell_of, cl.cellOf.data(), sizeof(int32_t) * count);
+        std::memcpy(counts, cl.count.data(), sizeof(int32_t) * ncell);
+        std::memcpy(offsets, cl.offset.data(), sizeof(int64_t) * (ncell + 1));
+        std::memcpy(perm, cl.perm.data(), sizeof(int64_t) * count);
+        std::memcpy(rel, cl.rel.data(), sizeof(float) * 3 * count);
+        return 0;
+    }
+
+    // Stage-level entry points used by known-answer tests.
+    void oracle_inverse_f32(const float* m, float* out)
+    {
+        M4<float> a;
+        for(int c = 0; c < 4; c++)
+            for(int r = 0; r < 4; r++)
+                a.m[c][r] = m[c * 4 + r];
+        M4<float> b = mat_inverse(a);
+        for(int c = 0; c < 4; c++)
+            for(int r = 0; r < 4; r++)
+                out[c * 4 + r] = b.m[c][r];
+    }
+
+    void oracle_inverse_f64(const double* m, double* out)
+    {
+        M4<double> a;
+        for(int c = 0; c < 4; c++)
+            for(int r = 0; r < 4; r++)
+                a.m[c][r] = m[c * 4 + r];
+        M4<double> b = mat_inverse(a);
+        for(int c = 0; c < 4; c++)
+            for(int r = 0; r < 4; r++)
+                out[c * 4 + r] = b.m[c][r];
+    }
+
+    void oracle_perspective(double fovy, double aspect, double zn, double zf, double* out)
+    {
+        perspective(fovy, aspect, zn, zf, out);
+    }
+
+    float oracle_apply_chain(const int32_t bytecode[3], const float params[12], const float* v, int dim)
+    {
+        Chain c;
+        for(int k = 0; k < 3; k++)
+        {
+            c.bytecode[k] = bytecode[k];
+            for(int i = 0; i < 4; i++)
+                c.params[k][i] = params[k * 4 + i];
+        }
+        return apply_chain<float>(c, v, dim);
+    }
+
+    void oracle_dither(int mode, int x, int y, int z, const float c[4], uint8_t out[4])
+    {
+        B4 b = dither<float>(mode, x, y, z, V4<float>{c[0], c[1], c[2], c[3]});
+        out[0] = b.x;
+        out[1] = b.y;
+        out[2] = b.z;
+        out[3] = b.w;
+    }
+
+    // Trilinear/nearest sample of a byte4 texture (normalised to [0,1]).
+    void oracle_sample_b4(const uint8_t* tex, int64_t X, int64_t Y, int64_t Z, const float p[3], int linear, float out[4])
+    {
+        Tex3B4 t;
+        t.X = X;
+        t.Y = Y;
+        t.Z = Z;
+        t.data.resize(X * Y * Z);
+        std::memcpy(t.data.data(), tex, X * Y * Z * 4);
+        V4<float> r = sample_b4<float>(t, V3<float>{p[0], p[1], p[2]}, linear != 0);
+        out[0] = r.x;
+        out[1] = r.y;
+        out[2] = r.z;
+        out[3] = r.w;
+    }
+
+    // pixelToRay + clipRay for one pixel; returns 1 if the ray hits the local box.
+    int oracle_ray(const OracleScene* sc, int precision, float px, float py, double out[14])
+    {
+        State st = make_state(sc);
+        if(precision == 64)
+        {
+            Frame<double> fr = make_frame<double>(st);
+            Ray<double> r = pixel_to_ray(fr.invMVP, st.sizes, double(px), double(py), double(st.fbW), double(st.fbH));
+            int hit = clip_ray(r, st.sizes, st.clip) ? 1 : 0;
+            double o[14] = {r.dir.x, r.dir.y, r.dir.z, r.start.x, r.start.y, r.start.z, r.startDepth, r.endDepth,
+                            double(r.isClipped), r.clippingNormal.x, r.clippingNormal.y, r.clippingNormal.z, 0, 0};
+            std::memcpy(out, o, sizeof(o));
+            return hit;
+        }
+        Frame<float> fr = make_frame<float>(st);
+        Ray<float> r = pixel_to_ray(fr.invMVP, st.sizes, px, py, float(st.fbW), float(st.fbH));
+        int hit = clip_ray(r, st.sizes, st.clip) ? 1 : 0;
+        double o[14] = {r.dir.x, r.dir.y, r.dir.z, r.start.x, r.start.y, r.start.z, r.startDepth, r.endDepth,
+                        double(r.isClipped), r.clippingNormal.x, r.clippingNormal.y, r.clippingNormal.z, 0, 0};
+        std::memcpy(out, o, sizeof(o));
+        return hit;
+    }
+
+    int oracle_max_threads()
+    {
+        return omp_get_max_threads();
+    }
+}
