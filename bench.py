@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Throughput of one rank's render call on one MI355X.
+
+Workload (the metric's configuration): a 256^3 local volume with three sources
+-- float32 density (seeded Gaussian mixture + 3-octave value noise), float32x3
+curl-noise vector field shown through the `length` functor, 2,000,000 uniform
+particles on a 64^3 cell grid -- iso-surface at threshold 0.5, one clipping
+plane, SSAO, trilinear sampling at step 0.5, 1920x1080.  A step is one
+doVisualization(redraw): particle cell lists + classification + the draw
+callback, with the camera alternating between the default pose and a
+volume-filling pose.  Inputs are resident in HBM before timing starts and the
+framebuffer stays on the device (no PCIe in the timed region).
+
+Metric: trilinear samples per second (volume + iso march samples actually
+executed, after early ray termination), with frames/s alongside.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD-published
+
+
+def kernel_bytes(counts, spec, n_vox, n_particles, n_visible):
+    """Algorithmic bytes per launch of each kernel (read + write), from the
+    executed work counts of one frame."""
+    W, H = spec.framebuffer
+    npx = W * H
+    src_bytes = sum(4 * v.feature_dim for v in spec.volumes)
+    return {
+        # 16 B of sources in, two RGBA8 texels out per voxel
+        "classify": n_vox * (src_bytes + 8),
+        # pos 12 + attr 4 in; cell id 4 + record 16 written and read back; perm 4 written and read back;
+        # visible render record 16 written
+        "binning": n_particles * (12 + 4 + 2 * 4 + 2 * 16 + 2 * 4) + n_visible * 16,
+        # iso: 8 RGBA8 taps per sample + 7 samples at a hit; DDA: offset pair per cell, 16 B per sphere
+        # test; G-buffer writes colour 4 + normal 12 + depth 4
+        "solid": 32 * counts["iso_samples"] + 7 * 32 * counts["iso_hits"] + 8 * counts["cells_visited"]
+        + 16 * counts["sphere_tests"] + 20 * npx,
+        # 49 depth taps + ao write
+        "ssao": npx * (49 * 4 + 4),
+        # 8 RGBA8 taps per sample; G-buffer read colour 4 + normal 12 + depth 4 + ao 4, colour write 4
+        "volume": 32 * counts["vol_samples"] + 28 * npx,
+    }
+
+
+def cpu_baseline(spec, threads):
+    from oracle import oracle
+
+    ps = oracle.PreparedScene(spec, threads=threads)
+    t0 = time.perf_counter()
+    res = oracle.render(spec, 32, threads=threads, prepared=ps)
+    wall = time.perf_counter() - t0
+    sec = res.seconds["classify"] + res.seconds["bin"] + res.seconds["render"]
+    samples = res.counters["vol_samples"] + res.counters["iso_samples"]
+    return {
+        "value": samples / sec,
+        "unit": "samples/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "one full 1920x1080 frame at the default pose (classification + cell lists + draw), "
+        "FP32 C++ restatement of the reference CPU path, OpenMP",
+        "frame_s": sec,
+        "fps": 1.0 / sec,
+        "wall_s_incl_setup": wall,
+        "samples": samples,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=256, help="local volume side (the workload is 256)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            f"bench.py --gpus {args.gpus}: this build renders one rank's sub-volume on one GPU; "
+            "multi-GPU sort-last compositing is not part of it, so only --gpus 1 is measured.",
+            file=sys.stderr,
+        )
+        return 2
+
+    import numpy as np
+    import torch
+
+    from isaac_amd import scenes
+    from isaac_amd import visualization as V
+
+    n = args.size
+    spec = scenes.prod_scene(n=n)
+    poses = [scenes.DEFAULT_POSE, scenes.FILL_POSE]
+    vis = V.from_scene(spec, 0)
+    out = torch.empty((spec.framebuffer[1], spec.framebuffer[0], 4), dtype=torch.uint8, device="cuda")
+    n_vox = n ** 3
+    n_particles = int(spec.particles[0].pos.shape[0])
+
+    def set_pose(p):
+        vis.look_at = np.asarray(p.look_at, np.float64)
+        vis.rotation = np.asarray(p.rotation, np.float64)
+        vis.distance = float(p.distance)
+        vis._update_modelview()
+
+    # executed work per pose (counting kernels, untimed)
+    counts = []
+    vis.set_counting(True)
+    for p in poses:
+        set_pose(p)
+        vis.update_buffers()
+        vis.draw(out=out)
+        counts.append(vis.counters())
+    vis.set_counting(False)
+    n_visible = int(np.count_nonzero(spec.particles[0].attr[:, 0] >= 0.995))
+
+    for k in range(args.warmup):
+        set_pose(poses[k % 2])
+        vis.update_buffers()
+        vis.draw(out=out)
+    vis.synchronize()
+    torch.cuda.synchronize()
+
+    per = {"classify": [], "binning": [], "solid": [], "ssao": [], "volume": []}
+    total_samples = 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        pi = k % 2
+        set_pose(poses[pi])
+        vis.update_buffers()
+        vis.draw(out=out)
+        t = vis.timings()  # waits for the step: each frame completes before the next (as the reference)
+        per["classify"].append(t["optimization_buffer_us"])
+        per["binning"].append(t["binning_us"])
+        per["solid"].append(t["solid_us"])
+        per["ssao"].append(t["ssao_us"])
+        per["volume"].append(t["volume_us"])
+        total_samples += counts[pi]["vol_samples"] + counts[pi]["iso_samples"]
+    vis.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+
+    ms_per_step = wall / args.steps * 1e3
+    value = total_samples / wall
+    avg_us = {k: float(np.mean(v)) for k, v in per.items()}
+    avg_counts = {k: float(np.mean([c[k] for c in counts])) for k in counts[0]}
+    kb = kernel_bytes(avg_counts, spec, n_vox, n_particles, n_visible)
+    dom = max(avg_us, key=lambda k: avg_us[k])
+    achieved = kb[dom] / (avg_us[dom] * 1e-6) / 1e9
+    line = {
+        "metric": "samples/s + fps @256^3x3,1080p",
+        "value": value,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Gaussian mixture + value-noise density, curl-noise vector field, uniform particles)",
+        "config": {
+            "workload": f"{n}^3 local volume x 3 sources (f32 density, f32x3 curl field via length, "
+            f"{n_particles} particles on a {spec.particle_grid[0]}^3 cell grid), iso 0.5, 1 clipping plane, "
+            "SSAO, trilinear step 0.5, 1920x1080, poses default/fill alternating",
+            "frame": "cell lists + classification + draw per step",
+        },
+        "fps": args.steps / wall,
+        "samples_per_frame": {"default": counts[0]["vol_samples"] + counts[0]["iso_samples"],
+                              "fill": counts[1]["vol_samples"] + counts[1]["iso_samples"]},
+        "kernel_ms": {k: v / 1e3 for k, v in avg_us.items()},
+        "kernel_algorithmic_GBs": {k: kb[k] / (avg_us[k] * 1e-6) / 1e9 if avg_us[k] > 0 else None for k in kb},
+        "roofline": {
+            "kernel": dom,
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": kb[dom],
+        },
+    }
+    vis.close()
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        spec0 = spec.with_camera(poses[0])
+        line["cpu_baseline"] = cpu_baseline(spec0, threads)
+        line["gpu_vs_cpu_samples"] = value / line["cpu_baseline"]["value"]
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

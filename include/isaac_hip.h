@@ -121,6 +121,12 @@ extern "C"
         double copy_us; /* framebuffer copy */
         double buffer_us; /* source update + particle binning */
         double optimization_buffer_us; /* classification */
+        /* per kernel (GPU time between events on the context stream) */
+        double source_copy_us; /* non-persistent source textures */
+        double binning_us; /* particle cell lists */
+        double solid_us; /* clear + particles + iso */
+        double ssao_us; /* SSAO */
+        double volume_us; /* shading + volume */
     } isaac_hip_timings;
 
     /* ---------------------------------------------------------------- lifetime */
@@ -200,8 +206,10 @@ extern "C"
         const void** depth,
         const void** normal,
         const void** ao);
-    /* device pointers of the combined RGBA8 textures, x-fastest local voxels */
-    int isaac_hip_get_combined_textures(const isaac_hip_ctx* ctx, const void** volume, const void** iso);
+    /* device pointer of the combined textures (lib/isaac.hpp:646-647): one texel per
+     * local voxel, x fastest; each texel is *texel_bytes (8) bytes: the combined volume
+     * RGBA8 followed by the combined iso RGBA8 */
+    int isaac_hip_get_combined_textures(const isaac_hip_ctx* ctx, const void** texels, int32_t* texel_bytes);
     /* device pointers of a particle source's exact cell list: per-cell count
      * (int32) and exclusive offset (int32, ncells+1), and the stable
      * permutation of particle indices (int32) */
@@ -232,6 +240,9 @@ extern "C"
         float rgba[8]);
     /* library version string */
     const char* isaac_hip_version(void);
+    /* blocking copy of `bytes` from device memory to host memory on the context
+     * stream (reads back G-buffer / texture / cell-list pointers) */
+    int isaac_hip_copy_to_host(isaac_hip_ctx* ctx, void* dst, const void* src_device, size_t bytes);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,434 @@
+// isaac_mi355x.hpp — header-only C++17 drop-in for one rank's
+// IsaacVisualization (lib/isaac.hpp:58-3036), forwarding to the gfx950 C-ABI
+// in isaac_hip.h.  Link with -lisaac_hip.
+//
+// What stays the same as the reference interface:
+//   - the constructor arguments (name, master, server, sizes, position,
+//     source lists, scale; lib/isaac.hpp:565-583); the alpaka host/acc/stream
+//     triple becomes a device index and an optional hipStream_t (void*)
+//   - source concepts: featureDim, guardSize, persistent, getName(),
+//     update(bool enabled, void* pointer) (example/example.cpp:39-228)
+//   - steering state and setters: functions[i].source + updateFunctions(),
+//     transfer points + updateTransfer(), sourceWeight, isoThreshold, step,
+//     interpolation, ditherMode, renderMode, ambientOcclusion,
+//     add/edit/removeClipping, updatePosition/LocalSize/LocalParticleSize
+//   - doVisualization(metaTargets, pointer, redraw) and the public timing
+//     members kernelTime, copyTime, bufferTime, optimizationBufferTime
+// What differs, because a precompiled GPU library cannot call a user's
+// device-side operator[]:
+//   - a volume source also provides `isaac::VolumeView view() const` (device
+//     pointer at local voxel (0,0,0) plus element strides);
+//   - a particle source provides `isaac::ParticleView view() const` (raw
+//     particles; the library builds the per-cell lists that getIterator(cell)
+//     returned in the reference);
+//   - source lists are std::tuple instead of boost::fusion::list;
+//   - there is no server, JSON, MPI or IceT: doVisualization renders the
+//     local sub-volume and keeps the RGBA8 frame in `image()`.
+#pragma once
+
+#include "isaac_hip.h"
+
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <utility>
+#include <vector>
+
+namespace isaac
+{
+    using isaac_float = float;
+    using isaac_double = double;
+    using isaac_int = int32_t;
+    using isaac_uint = uint32_t;
+
+    struct isaac_size2
+    {
+        int64_t x, y;
+    };
+    struct isaac_size3
+    {
+        int64_t x, y, z;
+    };
+    struct isaac_int3
+    {
+        int32_t x, y, z;
+    };
+    struct isaac_float3
+    {
+        float x, y, z;
+    };
+    struct isaac_float4
+    {
+        float x, y, z, w;
+    };
+
+    enum IsaacVisualizationMetaEnum
+    {
+        META_MERGE = 0,
+        META_MASTER = 1
+    };
+
+    struct VolumeView
+    {
+        const float* data; // device pointer at local voxel (0,0,0)
+        int64_t stride[3]; // element strides x, y, z
+    };
+
+    struct ParticleView
+    {
+        const float* position; // device, count x 3, local voxel coordinates
+        const float* attribute; // device, count x featureDim
+        const float* radius; // device, count, or nullptr
+        float constantRadius; // getRadius() when radius == nullptr
+        int64_t count;
+    };
+
+    struct FunctionsStruct
+    {
+        std::string source;
+        isaac_int errorCode = 0;
+    };
+
+    struct AOParams
+    {
+        isaac_float weight = 0.5f;
+        bool isEnabled = false;
+    };
+
+    // DefaultController (lib/isaac/isaac_controllers.hpp:23-43)
+    struct DefaultController
+    {
+        static const int passCount = 1;
+        static void projection(int32_t w, int32_t h, double out[16])
+        {
+            isaac_hip_default_projection(w, h, out);
+        }
+    };
+
+    class IsaacError : public std::runtime_error
+    {
+    public:
+        using std::runtime_error::runtime_error;
+    };
+
+    template<typename T_VolumeSourceList, typename T_FieldSourceList, typename T_ParticleList, int T_transferSize, typename T_Controller = DefaultController>
+    class IsaacVisualization
+    {
+        static constexpr int vSourceListSize = int(std::tuple_size<T_VolumeSourceList>::value);
+        static constexpr int fSourceListSize = int(std::tuple_size<T_FieldSourceList>::value);
+        static constexpr int pSourceListSize = int(std::tuple_size<T_ParticleList>::value);
+
+    public:
+        static constexpr int volumeFieldSourceListSize = vSourceListSize + fSourceListSize;
+        static constexpr int combinedSourceListSize = volumeFieldSourceListSize + pSourceListSize;
+
+        IsaacVisualization(
+            int device,
+            void* stream,
+            const std::string name,
+            const isaac_int master,
+            const std::string serverUrl,
+            const isaac_uint serverPort,
+            const isaac_size2 framebufferSize,
+            const isaac_size3 globalSize,
+            const isaac_size3 localSize,
+            const isaac_size3 localParticleSize,
+            const isaac_int3 position,
+            T_VolumeSourceList& volumeSources,
+            T_FieldSourceList& fieldSources,
+            T_ParticleList& particleSources,
+            isaac_float3 scale)
+            : name(name)
+            , master(master)
+            , framebufferSize(framebufferSize)
+            , volumeSources(volumeSources)
+            , fieldSources(fieldSources)
+            , particleSources(particleSources)
+        {
+            (void) serverUrl;
+            (void) serverPort;
+            isaac_hip_config cfg{};
+            cfg.framebuffer_size[0] = int32_t(framebufferSize.x);
+            cfg.framebuffer_size[1] = int32_t(framebufferSize.y);
+            const int64_t g[3] = {globalSize.x, globalSize.y, globalSize.z};
+            const int64_t l[3] = {localSize.x, localSize.y, localSize.z};
+            const int64_t p[3] = {localParticleSize.x, localParticleSize.y, localParticleSize.z};
+            for(int i = 0; i < 3; i++)
+            {
+                cfg.global_size[i] = g[i];
+                cfg.local_size[i] = l[i];
+                cfg.local_particle_size[i] = p[i];
+            }
+            cfg.position[0] = position.x;
+            cfg.position[1] = position.y;
+            cfg.position[2] = position.z;
+            cfg.scale[0] = scale.x;
+            cfg.scale[1] = scale.y;
+            cfg.scale[2] = scale.z;
+            cfg.transfer_size = T_transferSize;
+            cfg.n_volume_sources = vSourceListSize;
+            cfg.n_field_sources = fSourceListSize;
+            cfg.n_particle_sources = pSourceListSize;
+            int rc = isaac_hip_create(&cfg, device, &ctx);
+            if(rc != 0)
+                throw IsaacError("isaac_hip_create failed (" + std::to_string(rc) + ")");
+            if(stream)
+                check(isaac_hip_set_stream(ctx, stream), "set_stream");
+            // defaults (lib/isaac.hpp:608-609,696-699,720-794,2987-2988)
+            for(int i = 0; i < combinedSourceListSize; i++)
+            {
+                functions[i].source = (i >= vSourceListSize && i < volumeFieldSourceListSize) ? "length" : "idem";
+                sourceWeight[i] = 2.0f;
+                isoThreshold[i] = 0.0f;
+                uint32_t pos[2];
+                float rgba[8];
+                isaac_hip_default_transfer(i, combinedSourceListSize, i >= volumeFieldSourceListSize, T_transferSize, pos, rgba);
+                transferPoints[i][pos[0]] = isaac_float4{rgba[0], rgba[1], rgba[2], rgba[3]};
+                transferPoints[i][pos[1]] = isaac_float4{rgba[4], rgba[5], rgba[6], rgba[7]};
+            }
+            T_Controller::projection(int32_t(framebufferSize.x), int32_t(framebufferSize.y), projection);
+            rotation = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+            lookAt = {0, 0, 0};
+            distance = -4.5;
+            updateModelview();
+            updateFunctions();
+            updateTransfer();
+            frame.resize(size_t(framebufferSize.x) * size_t(framebufferSize.y) * 4);
+        }
+
+        ~IsaacVisualization()
+        {
+            isaac_hip_destroy(ctx);
+        }
+
+        IsaacVisualization(const IsaacVisualization&) = delete;
+        IsaacVisualization& operator=(const IsaacVisualization&) = delete;
+
+        // no server: always succeeds (lib/isaac.hpp:1276-1295 returns -1 only on connection failure)
+        int init(int communicatorBehaviour = 0)
+        {
+            (void) communicatorBehaviour;
+            return 0;
+        }
+
+        void updateFunctions()
+        {
+            for(int i = 0; i < combinedSourceListSize; i++)
+            {
+                int32_t err = 0;
+                check(isaac_hip_set_function(ctx, i, functions[i].source.c_str(), &err), "set_function");
+                functions[i].errorCode = err;
+            }
+        }
+
+        void updateTransfer()
+        {
+            for(int i = 0; i < combinedSourceListSize; i++)
+            {
+                std::vector<uint32_t> pos;
+                std::vector<float> rgba;
+                for(const auto& kv : transferPoints[i])
+                {
+                    pos.push_back(kv.first);
+                    rgba.insert(rgba.end(), {kv.second.x, kv.second.y, kv.second.z, kv.second.w});
+                }
+                check(isaac_hip_set_transfer_points(ctx, i, int(pos.size()), pos.data(), rgba.data()), "set_transfer");
+            }
+        }
+
+        bool editClipping(isaac_uint nr, isaac_float px, isaac_float py, isaac_float pz, isaac_float nx, isaac_float ny, isaac_float nz)
+        {
+            return isaac_hip_edit_clipping(ctx, nr, px, py, pz, nx, ny, nz) == 1;
+        }
+
+        void addClipping(isaac_float px, isaac_float py, isaac_float pz, isaac_float nx, isaac_float ny, isaac_float nz)
+        {
+            isaac_hip_add_clipping(ctx, px, py, pz, nx, ny, nz);
+        }
+
+        void removeClipping(isaac_uint nr)
+        {
+            isaac_hip_remove_clipping(ctx, nr);
+        }
+
+        void updatePosition(const isaac_int3 position)
+        {
+            const int32_t p[3] = {position.x, position.y, position.z};
+            check(isaac_hip_update_position(ctx, p), "update_position");
+        }
+
+        void updateLocalSize(const isaac_size3 localSize)
+        {
+            const int64_t l[3] = {localSize.x, localSize.y, localSize.z};
+            check(isaac_hip_update_local_size(ctx, l), "update_local_size");
+        }
+
+        void updateLocalParticleSize(const isaac_size3 localParticleSize)
+        {
+            const int64_t l[3] = {localParticleSize.x, localParticleSize.y, localParticleSize.z};
+            check(isaac_hip_update_local_particle_size(ctx, l), "update_local_particle_size");
+        }
+
+        // camera steering (the JSON keys of lib/isaac.hpp:1440-1513)
+        void setRotation(const std::array<double, 9>& columnMajor)
+        {
+            rotation = columnMajor;
+            updateModelview();
+        }
+        void setLookAt(double x, double y, double z)
+        {
+            lookAt = {x, y, z};
+            updateModelview();
+        }
+        void setDistance(double d)
+        {
+            distance = d;
+            updateModelview();
+        }
+
+        // doVisualization (lib/isaac.hpp:1298-2104) for this rank: buffers when
+        // redrawing, then the draw callback; the frame lands in image()
+        void doVisualization(IsaacVisualizationMetaEnum metaTargets = META_MASTER, void* pointer = nullptr, bool redraw = true)
+        {
+            (void) metaTargets;
+            pushState();
+            if(redraw)
+            {
+                bindSources(pointer);
+                check(isaac_hip_update_buffers(ctx), "update_buffers");
+            }
+            const int32_t vp[4] = {0, 0, int32_t(framebufferSize.x), int32_t(framebufferSize.y)};
+            check(isaac_hip_render(ctx, projection, modelview, backgroundColor, vp, frame.data(), 0), "render");
+            isaac_hip_timings t{};
+            isaac_hip_get_timings(ctx, &t);
+            kernelTime = uint64_t(t.kernel_us);
+            copyTime = uint64_t(t.copy_us);
+            bufferTime = uint64_t(t.buffer_us);
+            optimizationBufferTime = uint64_t(t.optimization_buffer_us);
+        }
+
+        // RGBA8, premultiplied, row 0 = bottom (the IceT image of lib/isaac.hpp:2533-2539)
+        const std::vector<uint8_t>& image() const
+        {
+            return frame;
+        }
+
+        isaac_hip_ctx* handle() const
+        {
+            return ctx;
+        }
+
+        // steering state, reference names
+        FunctionsStruct functions[combinedSourceListSize > 0 ? combinedSourceListSize : 1];
+        std::map<isaac_uint, isaac_float4> transferPoints[combinedSourceListSize > 0 ? combinedSourceListSize : 1];
+        isaac_float sourceWeight[combinedSourceListSize > 0 ? combinedSourceListSize : 1];
+        isaac_float isoThreshold[combinedSourceListSize > 0 ? combinedSourceListSize : 1];
+        isaac_float step = 0.5f;
+        bool interpolation = false;
+        isaac_int ditherMode = 1;
+        isaac_int renderMode = 0;
+        AOParams ambientOcclusion;
+        float backgroundColor[4] = {0, 0, 0, 1};
+        double projection[16];
+        double modelview[16];
+        std::array<double, 3> lookAt;
+        std::array<double, 9> rotation;
+        double distance;
+        // per-phase microseconds (lib/isaac.hpp:2129-2137)
+        uint64_t kernelTime = 0, copyTime = 0, bufferTime = 0, optimizationBufferTime = 0;
+
+    private:
+        void check(int rc, const char* what)
+        {
+            if(rc < 0)
+                throw IsaacError(std::string(what) + ": " + isaac_hip_last_error(ctx));
+        }
+
+        void updateModelview()
+        {
+            isaac_hip_modelview(lookAt.data(), rotation.data(), distance, modelview);
+        }
+
+        void pushState()
+        {
+            for(int i = 0; i < combinedSourceListSize; i++)
+            {
+                check(isaac_hip_set_weight(ctx, i, sourceWeight[i]), "set_weight");
+                check(isaac_hip_set_iso_threshold(ctx, i, isoThreshold[i]), "set_iso_threshold");
+            }
+            check(isaac_hip_set_step(ctx, step), "set_step");
+            check(isaac_hip_set_interpolation(ctx, interpolation ? 1 : 0), "set_interpolation");
+            check(isaac_hip_set_dither_mode(ctx, ditherMode), "set_dither_mode");
+            check(isaac_hip_set_ao(ctx, ambientOcclusion.isEnabled ? 1 : 0, ambientOcclusion.weight), "set_ao");
+            check(isaac_hip_set_render_mode(ctx, renderMode), "set_render_mode");
+        }
+
+        template<typename T_Source>
+        void bindVolume(int idx, T_Source& s, void* pointer)
+        {
+            const bool enabled = sourceWeight[idx] != 0.0f || isoThreshold[idx] != 0.0f;
+            s.update(enabled, pointer);
+            const VolumeView v = s.view();
+            isaac_hip_volume_desc d{};
+            d.data = v.data;
+            d.dtype = ISAAC_HIP_F32;
+            d.feature_dim = int32_t(T_Source::featureDim);
+            for(int k = 0; k < 3; k++)
+                d.stride[k] = v.stride[k];
+            d.guard_size = int32_t(T_Source::guardSize);
+            d.persistent = T_Source::persistent ? 1 : 0;
+            check(isaac_hip_set_volume_source(ctx, idx, &d), "set_volume_source");
+        }
+
+        template<typename T_Source>
+        void bindParticles(int idx, T_Source& s, void* pointer)
+        {
+            s.update(sourceWeight[volumeFieldSourceListSize + idx] != 0.0f, pointer);
+            const ParticleView v = s.view();
+            isaac_hip_particle_desc d{};
+            d.position = v.position;
+            d.attribute = v.attribute;
+            d.feature_dim = int32_t(T_Source::featureDim);
+            d.radius = v.radius;
+            d.constant_radius = v.constantRadius;
+            d.count = v.count;
+            check(isaac_hip_set_particle_source(ctx, idx, &d), "set_particle_source");
+        }
+
+        template<std::size_t... I>
+        void bindVolumes(void* pointer, std::index_sequence<I...>)
+        {
+            (bindVolume(int(I), std::get<I>(volumeSources), pointer), ...);
+        }
+        template<std::size_t... I>
+        void bindFields(void* pointer, std::index_sequence<I...>)
+        {
+            (bindVolume(vSourceListSize + int(I), std::get<I>(fieldSources), pointer), ...);
+        }
+        template<std::size_t... I>
+        void bindParticleList(void* pointer, std::index_sequence<I...>)
+        {
+            (bindParticles(int(I), std::get<I>(particleSources), pointer), ...);
+        }
+
+        void bindSources(void* pointer)
+        {
+            bindVolumes(pointer, std::make_index_sequence<vSourceListSize>{});
+            bindFields(pointer, std::make_index_sequence<fSourceListSize>{});
+            bindParticleList(pointer, std::make_index_sequence<pSourceListSize>{});
+        }
+
+        std::string name;
+        isaac_int master;
+        isaac_size2 framebufferSize;
+        T_VolumeSourceList& volumeSources;
+        T_FieldSourceList& fieldSources;
+        T_ParticleList& particleSources;
+        isaac_hip_ctx* ctx = nullptr;
+        std::vector<uint8_t> frame;
+    };
+} // namespace isaac

@@ -68,6 +68,11 @@ class Timings(ctypes.Structure):
         ("copy_us", ctypes.c_double),
         ("buffer_us", ctypes.c_double),
         ("optimization_buffer_us", ctypes.c_double),
+        ("source_copy_us", ctypes.c_double),
+        ("binning_us", ctypes.c_double),
+        ("solid_us", ctypes.c_double),
+        ("ssao_us", ctypes.c_double),
+        ("volume_us", ctypes.c_double),
     ]
 
 
@@ -116,6 +121,7 @@ _SIGS = {
     "isaac_hip_build_transfer_lut": (ctypes.c_int, [I32, I32, P, P, P]),
     "isaac_hip_default_transfer": (ctypes.c_int, [I32, I32, I32, I32, P, P]),
     "isaac_hip_version": (ctypes.c_char_p, []),
+    "isaac_hip_copy_to_host": (ctypes.c_int, [P, P, P, ctypes.c_size_t]),
 }
 
 
@@ -135,6 +141,14 @@ def lib() -> ctypes.CDLL:
             f"{LIB_PATH} is missing: build the gfx950 library first (python -m isaac_amd.build); "
             "there is no fallback renderer"
         )
+    # PyTorch-ROCm wheels carry their own libamdhip64/libhsa-runtime64.  Loading
+    # torch first makes this library's HIP runtime bind to the HSA runtime that
+    # torch already loaded (same soname libhsa-runtime64.so.1), so the process
+    # has one HSA runtime and torch's device buffers are valid here.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     l = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in _SIGS.items():
         fn = getattr(l, name)

@@ -84,8 +84,10 @@ def build_shim_example() -> str:
     if not os.path.exists(SHIM_SRC):
         return ""
     cxx = shutil.which("g++") or "c++"
-    cmd = [cxx, "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), SHIM_SRC, "-o", SHIM_BIN,
-           "-L" + HERE, "-lisaac_hip", "-Wl,-rpath," + HERE]
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROOT, "include"),
+           "-I" + os.path.join(rocm, "include"), SHIM_SRC, "-o", SHIM_BIN,
+           "-L" + HERE, "-lisaac_hip", "-Wl,-rpath," + HERE, "-L" + os.path.join(rocm, "lib"), "-lamdhip64"]
     subprocess.check_call(cmd)
     return SHIM_BIN
 

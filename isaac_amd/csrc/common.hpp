@@ -57,8 +57,7 @@ namespace isaac_hip
         int32_t nLut; // LUTs staged in LDS (0 = read from global)
         uint32_t divX_mul, divX_shift, divXY_mul, divXY_shift;
         const float4* luts; // nsrc * T
-        uint32_t* volTex;
-        uint32_t* isoTex;
+        uint2* tex; // per voxel: combined volume RGBA8 (.x), combined iso RGBA8 (.y)
         int64_t nvox;
         ClassifySource src[kMaxSources];
     };
@@ -84,8 +83,7 @@ namespace isaac_hip
         int32_t linear;
         // textures
         int32_t X, Y, Z;
-        const uint32_t* volTex;
-        const uint32_t* isoTex;
+        const uint2* tex; // interleaved combined textures, x fastest
         int32_t isoEnabled;
         int32_t volumeEnabled;
         float minSize;
@@ -103,6 +101,7 @@ namespace isaac_hip
         float* depth;
         float* normal; // 3 floats per pixel
         float* ao;
+        float4* volColor; // premultiplied volume colour per pixel; w < 0: the volume pass did not blend
         unsigned long long* counters;
     };
 

@@ -337,18 +337,20 @@ struct isaac_hip_ctx
     int64_t gsScaled[3], lsScaled[3], maxGsScaled;
     int32_t posScaled[3];
     // device buffers
-    uint32_t* volTex = nullptr;
-    uint32_t* isoTex = nullptr;
+    uint2* tex = nullptr; // per voxel: combined volume RGBA8, combined iso RGBA8
     size_t texVox = 0;
     uint32_t* color = nullptr;
     float* depth = nullptr;
     float* normal = nullptr;
     float* ao = nullptr;
+    float4* volColor = nullptr;
     unsigned long long* counters = nullptr;
     bool counting = false;
     bool classified = false;
-    hipEvent_t ev[8] = {};
-    bool evRecorded[4] = {false, false, false, false};
+    // events: 0 update start, 1 after source copies, 2 after binning, 3 after
+    // classification; 4 draw start, 5 after solid, 6 after SSAO, 7 after volume, 8 after copy
+    hipEvent_t ev[12] = {};
+    bool evRecorded[2] = {false, false};
 };
 
 namespace
@@ -388,15 +390,11 @@ namespace
     int alloc_textures(isaac_hip_ctx* c)
     {
         const size_t nvox = size_t(c->local[0]) * size_t(c->local[1]) * size_t(c->local[2]);
-        if(c->volTex)
-            (void) hipFree(c->volTex);
-        if(c->isoTex)
-            (void) hipFree(c->isoTex);
-        c->volTex = c->isoTex = nullptr;
-        HIPCHK(c, hipMalloc(&c->volTex, std::max<size_t>(nvox, 1) * 4));
-        HIPCHK(c, hipMalloc(&c->isoTex, std::max<size_t>(nvox, 1) * 4));
-        HIPCHK(c, hipMemsetAsync(c->volTex, 0, std::max<size_t>(nvox, 1) * 4, c->stream));
-        HIPCHK(c, hipMemsetAsync(c->isoTex, 0, std::max<size_t>(nvox, 1) * 4, c->stream));
+        if(c->tex)
+            (void) hipFree(c->tex);
+        c->tex = nullptr;
+        HIPCHK(c, hipMalloc(&c->tex, std::max<size_t>(nvox, 1) * 8));
+        HIPCHK(c, hipMemsetAsync(c->tex, 0, std::max<size_t>(nvox, 1) * 8, c->stream));
         c->texVox = nvox;
         c->classified = false;
         return 0;
@@ -646,6 +644,7 @@ extern "C"
         if((e = hipMalloc(&c->lutDev, size_t(std::max(c->nsrc, 1)) * T * sizeof(float4))) != hipSuccess
            || (e = hipMalloc(&c->color, npx * 4)) != hipSuccess || (e = hipMalloc(&c->depth, npx * 4)) != hipSuccess
            || (e = hipMalloc(&c->normal, npx * 12)) != hipSuccess || (e = hipMalloc(&c->ao, npx * 4)) != hipSuccess
+           || (e = hipMalloc(&c->volColor, npx * 16)) != hipSuccess
            || (e = hipMalloc(&c->counters, 64 * sizeof(unsigned long long))) != hipSuccess)
         {
             isaac_hip_destroy(c);
@@ -679,7 +678,7 @@ extern "C"
                 (void) hipFree(v.npTex);
         for(auto& p : c->parts)
             free_part(p);
-        void* bufs[] = {c->lutDev, c->volTex, c->isoTex, c->color, c->depth, c->normal, c->ao, c->counters};
+        void* bufs[] = {c->lutDev, c->tex, c->color, c->depth, c->normal, c->ao, c->volColor, c->counters};
         for(void* b : bufs)
             if(b)
                 (void) hipFree(b);
@@ -975,6 +974,7 @@ extern "C"
                     (const float*) v.desc.data, v.desc.stride, v.desc.feature_dim, c->chain[s], int(c->local[0] + 2 * g),
                     int(c->local[1] + 2 * g), int(c->local[2] + 2 * g), g, v.npTex, c->stream));
         }
+        HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
         // particle cell lists
         for(int s = 0; s < c->np; s++)
         {
@@ -1002,7 +1002,7 @@ extern "C"
             P.filter = 1;
             HIPCHK(c, launch_bin(P, p.B, c->stream));
         }
-        HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+        HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
         // classification (lib/isaac.hpp:1916-1947)
         ClassifyParams P{};
         P.X = int32_t(c->local[0]);
@@ -1021,8 +1021,7 @@ extern "C"
         make_fastdiv(uint32_t(P.X), P.divX_mul, P.divX_shift);
         make_fastdiv(uint32_t(P.X * P.Y), P.divXY_mul, P.divXY_shift);
         P.luts = c->lutDev;
-        P.volTex = c->volTex;
-        P.isoTex = c->isoTex;
+        P.tex = c->tex;
         const int maxLds = int((64 * 1024) / (size_t(T) * sizeof(float4)));
         P.nLut = std::min(nvf, std::max(0, maxLds));
         for(int s = 0; s < nvf; s++)
@@ -1063,7 +1062,7 @@ extern "C"
             }
         }
         HIPCHK(c, launch_classify(P, c->stream));
-        HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+        HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
         c->evRecorded[0] = true;
         c->classified = true;
         return 0;
@@ -1137,8 +1136,7 @@ extern "C"
         F.X = int32_t(c->local[0]);
         F.Y = int32_t(c->local[1]);
         F.Z = int32_t(c->local[2]);
-        F.volTex = c->volTex;
-        F.isoTex = c->isoTex;
+        F.tex = c->tex;
         const int nvf = c->nv + c->nf;
         F.isoEnabled = 0;
         for(int s = 0; s < nvf; s++)
@@ -1169,6 +1167,7 @@ extern "C"
         F.depth = c->depth;
         F.normal = c->normal;
         F.ao = c->ao;
+        F.volColor = c->volColor;
         F.counters = c->counters;
         // particles
         ParticleParams PP{};
@@ -1199,15 +1198,15 @@ extern "C"
         }
         if(c->counting)
             HIPCHK(c, launch_fill_u32(reinterpret_cast<uint32_t*>(c->counters), 0u, 16, c->stream));
-        HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-        HIPCHK(c, launch_frame(F, PP, c->counting, c->stream));
         HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+        HIPCHK(c, launch_frame(F, PP, c->counting, c->stream, c->ev[5], c->ev[6]));
+        HIPCHK(c, hipEventRecord(c->ev[7], c->stream));
         if(out_rgba)
         {
             const size_t bytes = size_t(W) * size_t(H) * 4;
             HIPCHK(c, hipMemcpyAsync(out_rgba, c->color, bytes, out_is_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
         }
-        HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
+        HIPCHK(c, hipEventRecord(c->ev[8], c->stream));
         c->evRecorded[1] = true;
         if(out_rgba && !out_is_device)
             HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1230,14 +1229,28 @@ extern "C"
         *out = isaac_hip_timings{};
         if(c->evRecorded[0])
         {
-            out->buffer_us = elapsed_us(c->ev[0], c->ev[1]);
-            out->optimization_buffer_us = elapsed_us(c->ev[1], c->ev[2]);
+            out->source_copy_us = elapsed_us(c->ev[0], c->ev[1]);
+            out->binning_us = elapsed_us(c->ev[1], c->ev[2]);
+            out->buffer_us = elapsed_us(c->ev[0], c->ev[2]);
+            out->optimization_buffer_us = elapsed_us(c->ev[2], c->ev[3]);
         }
         if(c->evRecorded[1])
         {
-            out->kernel_us = elapsed_us(c->ev[3], c->ev[4]);
-            out->copy_us = elapsed_us(c->ev[4], c->ev[5]);
+            out->solid_us = elapsed_us(c->ev[4], c->ev[5]);
+            out->ssao_us = elapsed_us(c->ev[5], c->ev[6]);
+            out->volume_us = elapsed_us(c->ev[6], c->ev[7]);
+            out->kernel_us = elapsed_us(c->ev[4], c->ev[7]);
+            out->copy_us = elapsed_us(c->ev[7], c->ev[8]);
         }
+        return 0;
+    }
+
+    int isaac_hip_copy_to_host(isaac_hip_ctx* c, void* dst, const void* src, size_t bytes)
+    {
+        if(!c || (!dst && bytes) || (!src && bytes))
+            return ISAAC_HIP_ERR_ARG;
+        HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
         return 0;
     }
 
@@ -1276,14 +1289,14 @@ extern "C"
         return 0;
     }
 
-    int isaac_hip_get_combined_textures(const isaac_hip_ctx* c, const void** volume, const void** iso)
+    int isaac_hip_get_combined_textures(const isaac_hip_ctx* c, const void** texels, int32_t* texel_bytes)
     {
         if(!c)
             return ISAAC_HIP_ERR_ARG;
-        if(volume)
-            *volume = c->volTex;
-        if(iso)
-            *iso = c->isoTex;
+        if(texels)
+            *texels = c->tex;
+        if(texel_bytes)
+            *texel_bytes = 8;
         return 0;
     }
 

@@ -17,7 +17,15 @@ namespace isaac_hip
         int guard,
         float* dst,
         hipStream_t s);
-    hipError_t launch_frame(const FrameParams& F, const ParticleParams& PP, bool counting, hipStream_t s);
+    // solid pass, SSAO (when enabled), shading + volume; afterSolid / afterSsao are
+    // recorded between them (per-kernel GPU time)
+    hipError_t launch_frame(
+        const FrameParams& F,
+        const ParticleParams& PP,
+        bool counting,
+        hipStream_t s,
+        hipEvent_t afterSolid,
+        hipEvent_t afterSsao);
 
     // ---- particle cell lists
     struct BinParams

@@ -1,12 +1,13 @@
 // gfx950 kernels of the per-rank render path.
 //
 //   classify_kernel        MergeToCombinedTextureKernel  lib/isaac/isaac_common_kernel.hpp:553-615,690-758
-//   solid_kernel           ClearBufferKernel + ParticleRenderKernel + CombinedIsoRenderKernel, fused per pixel
+//   march_kernel           ClearBufferKernel + ParticleRenderKernel + CombinedIsoRenderKernel +
+//                          the march of CombinedVolumeRenderKernel, fused per pixel
 //                          lib/isaac/isaac_common_kernel.hpp:195-217, lib/isaac/isaac_particle_kernel.hpp:117-260,
-//                          lib/isaac/isaac_iso_kernel.hpp:25-128
+//                          lib/isaac/isaac_iso_kernel.hpp:25-128, lib/isaac/isaac_volume_kernel.hpp:23-82
 //   ssao_kernel            SSAOKernel (live 7x7 part)     lib/isaac/isaac_ssao_kernel.hpp:161-182
-//   shade_volume_kernel    ShadingKernel + CombinedVolumeRenderKernel, fused per pixel
-//                          lib/isaac/isaac_common_kernel.hpp:220-320, lib/isaac/isaac_volume_kernel.hpp:23-88
+//   shade_kernel           ShadingKernel + the volume pass's final blend
+//                          lib/isaac/isaac_common_kernel.hpp:220-320, lib/isaac/isaac_volume_kernel.hpp:83-86
 //
 // The fusions keep the reference's quantisation points: every stage that the
 // reference stores as RGBA8 in the G-buffer is rounded to RGBA8 in registers
@@ -178,12 +179,13 @@ namespace isaac_hip
             {
                 vr = vg = vb = va = 0.0f;
             }
-            P.volTex[idx] = dither_b4(P.ditherMode, x, y, z, vr, vg, vb, va);
+            const uint32_t vol = dither_b4(P.ditherMode, x, y, z, vr, vg, vb, va);
             uint32_t q0 = (uint32_t) roundf(gclamp(ir, 0.0f, 1.0f) * 255.0f);
             uint32_t q1 = (uint32_t) roundf(gclamp(ig, 0.0f, 1.0f) * 255.0f);
             uint32_t q2 = (uint32_t) roundf(gclamp(ib, 0.0f, 1.0f) * 255.0f);
             uint32_t q3 = (uint32_t) roundf(gclamp(ia, 0.0f, 1.0f) * 255.0f);
-            P.isoTex[idx] = q0 | (q1 << 8) | (q2 << 16) | (q3 << 24);
+            // one 8-byte texel per voxel: combined volume RGBA8, combined iso RGBA8
+            P.tex[idx] = make_uint2(vol, q0 | (q1 << 8) | (q2 << 16) | (q3 << 24));
         }
     }
 
@@ -321,92 +323,89 @@ namespace isaac_hip
     }
 
     // ------------------------------------------------------------------ texture sampling
+    // x / 255 in binary32 as x*r plus one fma residual correction, r = RN(1/255).
+    // For every float x in [0, 256] this equals the IEEE quotient bit for bit
+    // (tests/test_div255.py checks all 1,132,462,081 inputs); trilinear results
+    // of RGBA8 texels always lie in [0, 255].  Three instructions instead of the
+    // correctly rounded division sequence.
+    __device__ inline float div255(float x)
+    {
+        const float r = 1.0f / 255.0f;
+        const float q0 = x * r;
+        const float e = __builtin_fmaf(-q0, 255.0f, x);
+        return __builtin_fmaf(e, r, q0);
+    }
+
+    // The 8 trilinear taps of one sample position (or the single NEAREST tap),
+    // each an 8-byte texel: .x combined volume RGBA8, .y combined iso RGBA8.
+    // Tap k holds voxel (x0 + (k>>2), y0 + ((k>>1)&1), z0 + (k&1)), clamped.
+    struct Taps
+    {
+        uint2 t[8];
+        float fx, fy, fz;
+    };
+
+    // Sampler<LINEAR|NEAREST, CLAMP> addressing (lib/isaac/isaac_texture.hpp:182-259).  Every
+    // coordinate is clamped into the texture, so a fetch at any position is in bounds.
+    template<bool LINEAR>
+    __device__ inline void fetch_taps(const uint2* __restrict__ tex, int X, int Y, int Z, float px, float py, float pz, Taps& T)
+    {
+        if(!LINEAR)
+        {
+            const uint32_t x = iclamp(int(px), 0, X - 1), y = iclamp(int(py), 0, Y - 1), z = iclamp(int(pz), 0, Z - 1);
+            T.t[0] = tex[x + uint32_t(X) * (y + uint32_t(Y) * z)];
+            T.fx = T.fy = T.fz = 0.0f;
+            return;
+        }
+        const float cx = px - 0.5f, cy = py - 0.5f, cz = pz - 0.5f;
+        const float flx = floorf(cx), fly = floorf(cy), flz = floorf(cz);
+        T.fx = cx - flx;
+        T.fy = cy - fly;
+        T.fz = cz - flz;
+        const int bx = int(flx), by = int(fly), bz = int(flz);
+        const uint32_t x0 = iclamp(bx, 0, X - 1), x1 = iclamp(bx + 1, 0, X - 1);
+        const uint32_t y0 = iclamp(by, 0, Y - 1), y1 = iclamp(by + 1, 0, Y - 1);
+        const uint32_t z0 = iclamp(bz, 0, Z - 1), z1 = iclamp(bz + 1, 0, Z - 1);
+        const uint32_t r00 = uint32_t(X) * (y0 + uint32_t(Y) * z0), r01 = uint32_t(X) * (y0 + uint32_t(Y) * z1);
+        const uint32_t r10 = uint32_t(X) * (y1 + uint32_t(Y) * z0), r11 = uint32_t(X) * (y1 + uint32_t(Y) * z1);
+        T.t[0] = tex[x0 + r00];
+        T.t[1] = tex[x0 + r01];
+        T.t[2] = tex[x0 + r10];
+        T.t[3] = tex[x0 + r11];
+        T.t[4] = tex[x1 + r00];
+        T.t[5] = tex[x1 + r01];
+        T.t[6] = tex[x1 + r10];
+        T.t[7] = tex[x1 + r11];
+    }
+
+    // one normalised channel: trilinear of the bytes (z innermost, x outermost,
+    // lib/isaac/isaac_helper.hpp:124-144), then / 255
+    template<bool LINEAR, bool ISO>
+    __device__ inline float channel(const Taps& T, int sh)
+    {
+        auto d = [&](int k) { return float(((ISO ? T.t[k].y : T.t[k].x) >> sh) & 0xffu); };
+        if(!LINEAR)
+            return div255(d(0));
+        const float oz = 1.0f - T.fz, oy = 1.0f - T.fy, ox = 1.0f - T.fx;
+        const float a0 = d(0) * oz + d(1) * T.fz, a1 = d(2) * oz + d(3) * T.fz;
+        const float A = a0 * oy + a1 * T.fy;
+        const float b0 = d(4) * oz + d(5) * T.fz, b1 = d(6) * oz + d(7) * T.fz;
+        const float B = b0 * oy + b1 * T.fy;
+        return div255(A * ox + B * T.fx);
+    }
+
+    template<bool LINEAR>
+    __device__ inline float iso_alpha_at(const FrameParams& F, float px, float py, float pz)
+    {
+        Taps T;
+        fetch_taps<LINEAR>(F.tex, F.X, F.Y, F.Z, px, py, pz, T);
+        return channel<LINEAR, true>(T, 24);
+    }
+
     struct F4
     {
         float x, y, z, w;
     };
-
-    __device__ inline F4 unpack4(uint32_t v)
-    {
-        return F4{float(v & 0xffu), float((v >> 8) & 0xffu), float((v >> 16) & 0xffu), float(v >> 24)};
-    }
-
-    // Sampler<LINEAR|NEAREST, CLAMP, normalized> on RGBA8 (lib/isaac/isaac_texture.hpp:135-306)
-    template<bool LINEAR>
-    __device__ inline F4 sample4(const uint32_t* __restrict__ tex, int X, int Y, int Z, float px, float py, float pz)
-    {
-        F4 r;
-        if(!LINEAR)
-        {
-            const int x = iclamp(int(px), 0, X - 1), y = iclamp(int(py), 0, Y - 1), z = iclamp(int(pz), 0, Z - 1);
-            r = unpack4(tex[uint32_t(x) + uint32_t(X) * (uint32_t(y) + uint32_t(Y) * uint32_t(z))]);
-        }
-        else
-        {
-            const float cx = px - 0.5f, cy = py - 0.5f, cz = pz - 0.5f;
-            const float flx = floorf(cx), fly = floorf(cy), flz = floorf(cz);
-            const float fx = cx - flx, fy = cy - fly, fz = cz - flz;
-            const int bx = int(flx), by = int(fly), bz = int(flz);
-            const uint32_t x0 = iclamp(bx, 0, X - 1), x1 = iclamp(bx + 1, 0, X - 1);
-            const uint32_t y0 = iclamp(by, 0, Y - 1), y1 = iclamp(by + 1, 0, Y - 1);
-            const uint32_t z0 = iclamp(bz, 0, Z - 1), z1 = iclamp(bz + 1, 0, Z - 1);
-            const uint32_t r00 = uint32_t(X) * (y0 + uint32_t(Y) * z0), r01 = uint32_t(X) * (y0 + uint32_t(Y) * z1);
-            const uint32_t r10 = uint32_t(X) * (y1 + uint32_t(Y) * z0), r11 = uint32_t(X) * (y1 + uint32_t(Y) * z1);
-            const uint32_t t000 = tex[x0 + r00], t001 = tex[x0 + r01], t010 = tex[x0 + r10], t011 = tex[x0 + r11];
-            const uint32_t t100 = tex[x1 + r00], t101 = tex[x1 + r01], t110 = tex[x1 + r10], t111 = tex[x1 + r11];
-            const float oz = 1.0f - fz, oy = 1.0f - fy, ox = 1.0f - fx;
-            auto lin = [](float o, float t, float a, float b) { return a * o + b * t; };
-            auto tri = [&](int sh) -> float
-            {
-                const float d000 = float((t000 >> sh) & 0xffu), d001 = float((t001 >> sh) & 0xffu);
-                const float d010 = float((t010 >> sh) & 0xffu), d011 = float((t011 >> sh) & 0xffu);
-                const float d100 = float((t100 >> sh) & 0xffu), d101 = float((t101 >> sh) & 0xffu);
-                const float d110 = float((t110 >> sh) & 0xffu), d111 = float((t111 >> sh) & 0xffu);
-                const float a0 = lin(oz, fz, d000, d001), a1 = lin(oz, fz, d010, d011);
-                const float A = lin(oy, fy, a0, a1);
-                const float b0 = lin(oz, fz, d100, d101), b1 = lin(oz, fz, d110, d111);
-                const float B = lin(oy, fy, b0, b1);
-                return lin(ox, fx, A, B);
-            };
-            r = F4{tri(0), tri(8), tri(16), tri(24)};
-        }
-        return F4{r.x / 255.0f, r.y / 255.0f, r.z / 255.0f, r.w / 255.0f};
-    }
-
-    // alpha channel only (the iso march); identical arithmetic on channel 3
-    template<bool LINEAR>
-    __device__ inline float sample_alpha(const uint32_t* __restrict__ tex, int X, int Y, int Z, float px, float py, float pz)
-    {
-        float r;
-        if(!LINEAR)
-        {
-            const int x = iclamp(int(px), 0, X - 1), y = iclamp(int(py), 0, Y - 1), z = iclamp(int(pz), 0, Z - 1);
-            r = float(tex[uint32_t(x) + uint32_t(X) * (uint32_t(y) + uint32_t(Y) * uint32_t(z))] >> 24);
-        }
-        else
-        {
-            const float cx = px - 0.5f, cy = py - 0.5f, cz = pz - 0.5f;
-            const float flx = floorf(cx), fly = floorf(cy), flz = floorf(cz);
-            const float fx = cx - flx, fy = cy - fly, fz = cz - flz;
-            const int bx = int(flx), by = int(fly), bz = int(flz);
-            const uint32_t x0 = iclamp(bx, 0, X - 1), x1 = iclamp(bx + 1, 0, X - 1);
-            const uint32_t y0 = iclamp(by, 0, Y - 1), y1 = iclamp(by + 1, 0, Y - 1);
-            const uint32_t z0 = iclamp(bz, 0, Z - 1), z1 = iclamp(bz + 1, 0, Z - 1);
-            const uint32_t r00 = uint32_t(X) * (y0 + uint32_t(Y) * z0), r01 = uint32_t(X) * (y0 + uint32_t(Y) * z1);
-            const uint32_t r10 = uint32_t(X) * (y1 + uint32_t(Y) * z0), r11 = uint32_t(X) * (y1 + uint32_t(Y) * z1);
-            const float d000 = float(tex[x0 + r00] >> 24), d001 = float(tex[x0 + r01] >> 24);
-            const float d010 = float(tex[x0 + r10] >> 24), d011 = float(tex[x0 + r11] >> 24);
-            const float d100 = float(tex[x1 + r00] >> 24), d101 = float(tex[x1 + r01] >> 24);
-            const float d110 = float(tex[x1 + r10] >> 24), d111 = float(tex[x1 + r11] >> 24);
-            const float oz = 1.0f - fz, oy = 1.0f - fy, ox = 1.0f - fx;
-            const float a0 = d000 * oz + d001 * fz, a1 = d010 * oz + d011 * fz;
-            const float A = a0 * oy + a1 * fy;
-            const float b0 = d100 * oz + d101 * fz, b1 = d110 * oz + d111 * fz;
-            const float B = b0 * oy + b1 * fy;
-            r = A * ox + B * fx;
-        }
-        return r / 255.0f;
-    }
 
     // ------------------------------------------------------------------ pixel tiling
     // workgroup -> 16x16 tile, XCD-banded; returns false for threads outside the viewport
@@ -442,19 +441,32 @@ namespace isaac_hip
         __syncthreads();
     }
 
-    // ------------------------------------------------------------------ solid pass
+    // ------------------------------------------------------------------ fused march
+    // One pass per pixel: clear, particle DDA, then ONE traversal of the ray that
+    // serves both the iso march (CombinedIsoRenderKernel) and the volume march
+    // (CombinedVolumeRenderKernel).  Both reference kernels sample the same
+    // positions startU + stepVec*i from the same startSteps; the volume stops at
+    // floor(min(clipEnd, depth)/step) where depth is the particle/iso depth, so
+    // the iso decision at step i is made before the volume consumes step i.  The
+    // premultiplied volume colour goes to volColor and is blended over the shaded
+    // solid colour by shade_kernel after SSAO, which keeps the reference's pass
+    // order (particles, iso, SSAO, shading, volume) for every pixel.
+    // The taps of step i+1 are fetched before step i is composited (clamped
+    // addresses, so a fetch past the last step stays in bounds).
     template<bool LINEAR, bool COUNT>
-    __global__ __launch_bounds__(256) void solid_kernel(const FrameParams F, const ParticleParams PP)
+    __global__ __launch_bounds__(256) void march_kernel(const FrameParams F, const ParticleParams PP)
     {
         int px, py;
         const bool valid = tile_pixel(F, px, py);
         unsigned long long nCells = 0, nTests = 0, nIso = 0, nIsoHits = 0, nIsoRays = 0, nPartRays = 0;
+        unsigned long long nVol = 0, nVolRays = 0;
         if(valid)
         {
             const size_t p = size_t(py) * size_t(F.W) + size_t(px);
             uint32_t color = F.bgClear;
             F3 normal = f3(0.f, 0.f, 0.f);
             float depth = FLT_MAX;
+            float4 vcol = make_float4(0.f, 0.f, 0.f, -1.f);
             Ray ray = pixel_to_ray(F, px, py);
             const bool inside = clip_ray(F, ray);
             // ---- particles (ParticleRenderKernel): 3D-DDA over the particle cell grid
@@ -556,61 +568,121 @@ namespace isaac_hip
                     depth = pdepth + ray.startDepth;
                 }
             }
-            // ---- iso (CombinedIsoRenderKernel)
-            if(inside && F.isoEnabled)
+            // ---- iso + volume, one traversal
+            const int mode = F.renderMode;
+            // ShadingKernel zeroes the depth in these modes, so the volume pass never blends there
+            const bool depthZeroed = mode == 2 || (mode >= 4 && mode <= 8);
+            if(inside)
             {
-                const float endDepth = gmin(ray.endDepth, depth);
-                if(endDepth > ray.startDepth)
+                const float endAfterParticles = gmin(ray.endDepth, depth);
+                bool isoOn = F.isoEnabled && endAfterParticles > ray.startDepth;
+                bool volBlend = F.volumeEnabled && !depthZeroed && endAfterParticles > ray.startDepth;
+                if(isoOn || volBlend)
                 {
-                    nIsoRays++;
                     const F3 sc = f3(F.scale[0], F.scale[1], F.scale[2]);
                     const float stepU = F.step * (len3(ray.dir) / len3(divv(ray.dir, sc)));
+                    const float factor = stepU / F.minSize * 2.0f * F.totalWeightAll;
                     const int startSteps = int(ceilf(ray.startDepth / stepU));
-                    const int endSteps = int(floorf(endDepth / stepU));
+                    const int endIso = int(floorf(endAfterParticles / stepU));
+                    int endVol = endIso;
                     const F3 stepVec = divv(muls(ray.dir, stepU), sc);
                     const F3 startU = divv(ray.start, sc);
+                    bool volOn = volBlend;
+                    if(isoOn)
+                        nIsoRays++;
+                    float cr = 0.f, cg = 0.f, cb = 0.f, ca = 0.f;
                     float oldValue = 0.0f;
-                    for(int i = startSteps; i <= endSteps; i++)
+                    int i = startSteps;
+                    int last = isoOn ? endIso : endVol;
+                    if(i <= last)
                     {
-                        const float fi = float(i);
-                        const float qx = startU.x + stepVec.x * fi;
-                        const float qy = startU.y + stepVec.y * fi;
-                        const float qz = startU.z + stepVec.z * fi;
-                        const float value = sample_alpha<LINEAR>(F.isoTex, F.X, F.Y, F.Z, qx, qy, qz);
-                        if(COUNT)
-                            nIso++;
-                        const float tmp = oldValue;
-                        oldValue = value;
-                        if(value < 0.5f)
-                            continue;
-                        const bool first = ray.isClipped && i == startSteps;
-                        const float tt = fi * stepU;
-                        float d;
-                        if(first)
-                            d = ray.startDepth;
-                        else
-                            d = tt + stepU * (0.5f - tmp) / (value - tmp);
-                        const F3 pu = divv(add(ray.start, muls(ray.dir, d)), sc);
-                        F4 hc = sample4<LINEAR>(F.isoTex, F.X, F.Y, F.Z, pu.x, pu.y, pu.z);
-                        F3 g;
-                        g.x = sample_alpha<LINEAR>(F.isoTex, F.X, F.Y, F.Z, pu.x + 1.0f, pu.y, pu.z)
-                            - sample_alpha<LINEAR>(F.isoTex, F.X, F.Y, F.Z, pu.x - 1.0f, pu.y, pu.z);
-                        g.y = sample_alpha<LINEAR>(F.isoTex, F.X, F.Y, F.Z, pu.x, pu.y + 1.0f, pu.z)
-                            - sample_alpha<LINEAR>(F.isoTex, F.X, F.Y, F.Z, pu.x, pu.y - 1.0f, pu.z);
-                        g.z = sample_alpha<LINEAR>(F.isoTex, F.X, F.Y, F.Z, pu.x, pu.y, pu.z + 1.0f)
-                            - sample_alpha<LINEAR>(F.isoTex, F.X, F.Y, F.Z, pu.x, pu.y, pu.z - 1.0f);
-                        float gl = len3(g);
-                        if(first || gl == 0.0f)
+                        Taps cur;
+                        fetch_taps<LINEAR>(F.tex, F.X, F.Y, F.Z, startU.x + stepVec.x * float(i), startU.y + stepVec.y * float(i),
+                                           startU.z + stepVec.z * float(i), cur);
+                        for(;;)
                         {
-                            g = ray.clipN;
-                            gl = 1.0f;
+                            Taps nxt;
+                            const bool ahead = i + 1 <= last;
+                            if(ahead)
+                            {
+                                const float fn = float(i + 1);
+                                fetch_taps<LINEAR>(F.tex, F.X, F.Y, F.Z, startU.x + stepVec.x * fn, startU.y + stepVec.y * fn,
+                                                   startU.z + stepVec.z * fn, nxt);
+                            }
+                            if(isoOn)
+                            {
+                                const float value = channel<LINEAR, true>(cur, 24);
+                                if(COUNT)
+                                    nIso++;
+                                const float tmp = oldValue;
+                                oldValue = value;
+                                if(value >= 0.5f)
+                                {
+                                    const bool first = ray.isClipped && i == startSteps;
+                                    const float tt = float(i) * stepU;
+                                    float d;
+                                    if(first)
+                                        d = ray.startDepth;
+                                    else
+                                        d = tt + stepU * (0.5f - tmp) / (value - tmp);
+                                    const F3 pu = divv(add(ray.start, muls(ray.dir, d)), sc);
+                                    Taps h;
+                                    fetch_taps<LINEAR>(F.tex, F.X, F.Y, F.Z, pu.x, pu.y, pu.z, h);
+                                    const float hr = channel<LINEAR, true>(h, 0), hg = channel<LINEAR, true>(h, 8);
+                                    const float hb = channel<LINEAR, true>(h, 16);
+                                    F3 g;
+                                    g.x = iso_alpha_at<LINEAR>(F, pu.x + 1.0f, pu.y, pu.z) - iso_alpha_at<LINEAR>(F, pu.x - 1.0f, pu.y, pu.z);
+                                    g.y = iso_alpha_at<LINEAR>(F, pu.x, pu.y + 1.0f, pu.z) - iso_alpha_at<LINEAR>(F, pu.x, pu.y - 1.0f, pu.z);
+                                    g.z = iso_alpha_at<LINEAR>(F, pu.x, pu.y, pu.z + 1.0f) - iso_alpha_at<LINEAR>(F, pu.x, pu.y, pu.z - 1.0f);
+                                    float gl = len3(g);
+                                    if(first || gl == 0.0f)
+                                    {
+                                        g = ray.clipN;
+                                        gl = 1.0f;
+                                    }
+                                    normal = f3(-g.x / gl, -g.y / gl, -g.z / gl);
+                                    color = to_byte4(hr, hg, hb, 1.0f);
+                                    depth = d;
+                                    if(COUNT)
+                                        nIsoHits++;
+                                    isoOn = false;
+                                    // the volume pass ends at min(clip end, iso depth)
+                                    const float ve = gmin(ray.endDepth, d);
+                                    if(!(ve > ray.startDepth))
+                                        volBlend = volOn = false;
+                                    else
+                                        endVol = int(floorf(ve / stepU));
+                                }
+                            }
+                            if(volOn && i <= endVol)
+                            {
+                                if(COUNT)
+                                    nVol++;
+                                const float vr = channel<LINEAR, false>(cur, 0) * factor;
+                                const float vg = channel<LINEAR, false>(cur, 8) * factor;
+                                const float vb = channel<LINEAR, false>(cur, 16) * factor;
+                                const float va = channel<LINEAR, false>(cur, 24) * factor;
+                                const float w = 1.0f - ca;
+                                cr = cr + w * vr;
+                                cg = cg + w * vg;
+                                cb = cb + w * vb;
+                                ca = ca + w * va;
+                                if(ca > 0.99f)
+                                    volOn = false;
+                            }
+                            last = isoOn ? endIso : -0x7fffffff;
+                            if(volOn && endVol > last)
+                                last = endVol;
+                            if(i + 1 > last)
+                                break;
+                            cur = nxt;
+                            i++;
                         }
-                        normal = f3(-g.x / gl, -g.y / gl, -g.z / gl);
-                        color = to_byte4(hc.x, hc.y, hc.z, 1.0f);
-                        depth = d;
-                        if(COUNT)
-                            nIsoHits++;
-                        break;
+                    }
+                    if(volBlend)
+                    {
+                        nVolRays++;
+                        vcol = make_float4(cr, cg, cb, ca);
                     }
                 }
             }
@@ -619,15 +691,18 @@ namespace isaac_hip
             F.normal[3 * p + 0] = normal.x;
             F.normal[3 * p + 1] = normal.y;
             F.normal[3 * p + 2] = normal.z;
+            F.volColor[p] = vcol;
             if(!F.aoEnabled)
                 F.ao[p] = 0.0f;
         }
         if(COUNT)
         {
+            block_count_add(F.counters + 0, nVol);
             block_count_add(F.counters + 1, nIso);
             block_count_add(F.counters + 2, nIsoHits);
             block_count_add(F.counters + 3, nCells);
             block_count_add(F.counters + 4, nTests);
+            block_count_add(F.counters + 5, nVolRays);
             block_count_add(F.counters + 6, nIsoRays);
             block_count_add(F.counters + 7, nPartRays);
         }
@@ -656,132 +731,93 @@ namespace isaac_hip
         F.ao[size_t(py) * F.W + px] = gclamp(occ / 42.0f, 0.0f, 1.0f);
     }
 
-    // ------------------------------------------------------------------ shading + volume
-    template<bool LINEAR, bool COUNT>
-    __global__ __launch_bounds__(256) void shade_volume_kernel(const FrameParams F)
+    // ------------------------------------------------------------------ shading + blend
+    // ShadingKernel (lib/isaac/isaac_common_kernel.hpp:220-320), then the volume
+    // pass's final blend color + (1 - color.a) * solid
+    // (lib/isaac/isaac_volume_kernel.hpp:83-86) for pixels whose volume pass ran.
+    __global__ __launch_bounds__(256) void shade_kernel(const FrameParams F)
     {
         int px, py;
-        const bool valid = tile_pixel(F, px, py);
-        unsigned long long nSamples = 0, nRays = 0;
-        if(valid)
+        if(!tile_pixel(F, px, py))
+            return;
+        const size_t p = size_t(py) * size_t(F.W) + size_t(px);
+        const uint32_t cb = F.color[p];
+        F4 color = F4{div255(float(cb & 0xffu)), div255(float((cb >> 8) & 0xffu)), div255(float((cb >> 16) & 0xffu)),
+                      div255(float(cb >> 24))};
+        const F3 normal = f3(F.normal[3 * p + 0], F.normal[3 * p + 1], F.normal[3 * p + 2]);
+        float depth = F.depth[p];
+        const float ao = F.aoEnabled ? F.ao[p] : 0.0f;
+        const float aoStrength = 1.0f - ao;
+        const int mode = F.renderMode;
+        uint32_t solid = cb;
+        bool depthChanged = false;
+        if(mode == 7)
+            color = F4{F.rankHue[0], F.rankHue[1], F.rankHue[2], color.w};
+        if(mode < 3 || mode == 7)
         {
-            const size_t p = size_t(py) * size_t(F.W) + size_t(px);
-            const uint32_t cb = F.color[p];
-            F4 color = F4{float(cb & 0xffu) / 255.0f, float((cb >> 8) & 0xffu) / 255.0f,
-                          float((cb >> 16) & 0xffu) / 255.0f, float(cb >> 24) / 255.0f};
-            const F3 normal = f3(F.normal[3 * p + 0], F.normal[3 * p + 1], F.normal[3 * p + 2]);
-            float depth = F.depth[p];
-            const float ao = F.aoEnabled ? F.ao[p] : 0.0f;
-            const float aoStrength = 1.0f - ao;
-            const int mode = F.renderMode;
-            Ray ray = pixel_to_ray(F, px, py);
-            uint32_t solid = cb;
-            bool depthChanged = false;
-            if(mode == 7)
-                color = F4{F.rankHue[0], F.rankHue[1], F.rankHue[2], color.w};
-            if(mode < 3 || mode == 7)
+            const Ray ray = pixel_to_ray(F, px, py);
+            const F3 lightDir = f3(-ray.dir.x, -ray.dir.y, -ray.dir.z);
+            float lightFactor = fabsf(dot3(normal, lightDir));
+            const F3 half = normalize3(add(lightDir, lightDir));
+            float spec = dot3(normal, half);
+            const float s2 = spec * spec;
+            spec = s2 * s2;
+            spec = spec * 0.5f;
+            if(mode == 1)
+                spec = 0.0f;
+            const float w = F.aoWeight;
+            const float aoF = (1.0f - w) + w * aoStrength;
+            lightFactor = lightFactor + aoF;
+            lightFactor = lightFactor * 0.5f;
+            solid = to_byte4(gmin(color.x * lightFactor + spec, 1.0f), gmin(color.y * lightFactor + spec, 1.0f),
+                             gmin(color.z * lightFactor + spec, 1.0f), color.w);
+            if(mode == 2 || mode == 7)
             {
-                const F3 lightDir = f3(-ray.dir.x, -ray.dir.y, -ray.dir.z);
-                float lightFactor = fabsf(dot3(normal, lightDir));
-                const F3 half = normalize3(add(lightDir, lightDir));
-                float spec = dot3(normal, half);
-                const float s2 = spec * spec;
-                spec = s2 * s2;
-                spec = spec * 0.5f;
-                if(mode == 1)
-                    spec = 0.0f;
-                const float w = F.aoWeight;
-                const float aoF = (1.0f - w) + w * aoStrength;
-                lightFactor = lightFactor + aoF;
-                lightFactor = lightFactor * 0.5f;
-                solid = to_byte4(gmin(color.x * lightFactor + spec, 1.0f), gmin(color.y * lightFactor + spec, 1.0f),
-                                 gmin(color.z * lightFactor + spec, 1.0f), color.w);
-                if(mode == 2 || mode == 7)
-                {
-                    depth = 0.0f;
-                    depthChanged = true;
-                }
-            }
-            else if(mode == 3)
-                solid = to_byte4(F.bg[0], F.bg[1], F.bg[2], color.w);
-            else if(mode == 4)
-            {
-                solid = to_byte4(normal.x * 0.5f + 0.5f, normal.y * 0.5f + 0.5f, normal.z * 0.5f + 0.5f, color.w);
                 depth = 0.0f;
                 depthChanged = true;
             }
-            else if(mode == 5)
-            {
-                const float d = depth / F.maxGlobalSizeScaled;
-                solid = to_byte4(d, d, d, color.w);
-                depth = 0.0f;
-                depthChanged = true;
-            }
-            else if(mode == 6)
-            {
-                const float w = F.aoWeight;
-                const float aoF = (1.0f - w) + w * aoStrength;
-                solid = to_byte4(aoF, aoF, aoF, color.w);
-                depth = 0.0f;
-                depthChanged = true;
-            }
-            else if(mode == 8)
-            {
-                solid = to_byte4(F.rankHue[0], F.rankHue[1], F.rankHue[2], 1.0f);
-                depth = 0.0f;
-                depthChanged = true;
-            }
-            uint32_t out = solid;
-            if(F.volumeEnabled && clip_ray(F, ray))
-            {
-                const float endDepth = gmin(ray.endDepth, depth);
-                if(endDepth > ray.startDepth)
-                {
-                    nRays++;
-                    const F3 sc = f3(F.scale[0], F.scale[1], F.scale[2]);
-                    const float stepU = F.step * (len3(ray.dir) / len3(divv(ray.dir, sc)));
-                    const float factor = stepU / F.minSize * 2.0f * F.totalWeightAll;
-                    const int startSteps = int(ceilf(ray.startDepth / stepU));
-                    const int endSteps = int(floorf(endDepth / stepU));
-                    const F3 stepVec = divv(muls(ray.dir, stepU), sc);
-                    const F3 startU = divv(ray.start, sc);
-                    float cr = 0.f, cg = 0.f, cbl = 0.f, ca = 0.f;
-                    for(int i = startSteps; i <= endSteps; i++)
-                    {
-                        const float fi = float(i);
-                        const float qx = startU.x + stepVec.x * fi;
-                        const float qy = startU.y + stepVec.y * fi;
-                        const float qz = startU.z + stepVec.z * fi;
-                        F4 v = sample4<LINEAR>(F.volTex, F.X, F.Y, F.Z, qx, qy, qz);
-                        if(COUNT)
-                            nSamples++;
-                        v.x = v.x * factor;
-                        v.y = v.y * factor;
-                        v.z = v.z * factor;
-                        v.w = v.w * factor;
-                        const float w = 1.0f - ca;
-                        cr = cr + w * v.x;
-                        cg = cg + w * v.y;
-                        cbl = cbl + w * v.z;
-                        ca = ca + w * v.w;
-                        if(ca > 0.99f)
-                            break;
-                    }
-                    const float sr = float(solid & 0xffu) / 255.0f, sg = float((solid >> 8) & 0xffu) / 255.0f;
-                    const float sb = float((solid >> 16) & 0xffu) / 255.0f, sa = float(solid >> 24) / 255.0f;
-                    const float om = 1.0f - ca;
-                    out = to_byte4(cr + om * sr, cg + om * sg, cbl + om * sb, ca + om * sa);
-                }
-            }
-            F.color[p] = out;
-            if(depthChanged)
-                F.depth[p] = depth;
         }
-        if(COUNT)
+        else if(mode == 3)
+            solid = to_byte4(F.bg[0], F.bg[1], F.bg[2], color.w);
+        else if(mode == 4)
         {
-            block_count_add(F.counters + 0, nSamples);
-            block_count_add(F.counters + 5, nRays);
+            solid = to_byte4(normal.x * 0.5f + 0.5f, normal.y * 0.5f + 0.5f, normal.z * 0.5f + 0.5f, color.w);
+            depth = 0.0f;
+            depthChanged = true;
         }
+        else if(mode == 5)
+        {
+            const float d = depth / F.maxGlobalSizeScaled;
+            solid = to_byte4(d, d, d, color.w);
+            depth = 0.0f;
+            depthChanged = true;
+        }
+        else if(mode == 6)
+        {
+            const float w = F.aoWeight;
+            const float aoF = (1.0f - w) + w * aoStrength;
+            solid = to_byte4(aoF, aoF, aoF, color.w);
+            depth = 0.0f;
+            depthChanged = true;
+        }
+        else if(mode == 8)
+        {
+            solid = to_byte4(F.rankHue[0], F.rankHue[1], F.rankHue[2], 1.0f);
+            depth = 0.0f;
+            depthChanged = true;
+        }
+        uint32_t out = solid;
+        const float4 v = F.volColor[p];
+        if(v.w >= 0.0f)
+        {
+            const float sr = div255(float(solid & 0xffu)), sg = div255(float((solid >> 8) & 0xffu));
+            const float sb = div255(float((solid >> 16) & 0xffu)), sa = div255(float(solid >> 24));
+            const float om = 1.0f - v.w;
+            out = to_byte4(v.x + om * sr, v.y + om * sg, v.z + om * sb, v.w + om * sa);
+        }
+        F.color[p] = out;
+        if(depthChanged)
+            F.depth[p] = depth;
     }
 
     // ------------------------------------------------------------------ launchers
@@ -821,42 +857,41 @@ namespace isaac_hip
         return hipGetLastError();
     }
 
-    hipError_t launch_frame(const FrameParams& F, const ParticleParams& PP, bool counting, hipStream_t s)
+    hipError_t launch_frame(
+        const FrameParams& F,
+        const ParticleParams& PP,
+        bool counting,
+        hipStream_t s,
+        hipEvent_t afterMarch,
+        hipEvent_t afterSsao)
     {
         const unsigned nb = unsigned(F.tilesX) * unsigned(F.tilesY);
         if(nb == 0)
+        {
+            (void) hipEventRecord(afterMarch, s);
+            (void) hipEventRecord(afterSsao, s);
             return hipSuccess;
+        }
         const dim3 grid(nb), block(256);
         if(F.linear)
         {
             if(counting)
-                hipLaunchKernelGGL((solid_kernel<true, true>), grid, block, 0, s, F, PP);
+                hipLaunchKernelGGL((march_kernel<true, true>), grid, block, 0, s, F, PP);
             else
-                hipLaunchKernelGGL((solid_kernel<true, false>), grid, block, 0, s, F, PP);
+                hipLaunchKernelGGL((march_kernel<true, false>), grid, block, 0, s, F, PP);
         }
         else
         {
             if(counting)
-                hipLaunchKernelGGL((solid_kernel<false, true>), grid, block, 0, s, F, PP);
+                hipLaunchKernelGGL((march_kernel<false, true>), grid, block, 0, s, F, PP);
             else
-                hipLaunchKernelGGL((solid_kernel<false, false>), grid, block, 0, s, F, PP);
+                hipLaunchKernelGGL((march_kernel<false, false>), grid, block, 0, s, F, PP);
         }
+        (void) hipEventRecord(afterMarch, s);
         if(F.aoEnabled)
             hipLaunchKernelGGL(ssao_kernel, grid, block, 0, s, F);
-        if(F.linear)
-        {
-            if(counting)
-                hipLaunchKernelGGL((shade_volume_kernel<true, true>), grid, block, 0, s, F);
-            else
-                hipLaunchKernelGGL((shade_volume_kernel<true, false>), grid, block, 0, s, F);
-        }
-        else
-        {
-            if(counting)
-                hipLaunchKernelGGL((shade_volume_kernel<false, true>), grid, block, 0, s, F);
-            else
-                hipLaunchKernelGGL((shade_volume_kernel<false, false>), grid, block, 0, s, F);
-        }
+        (void) hipEventRecord(afterSsao, s);
+        hipLaunchKernelGGL(shade_kernel, grid, block, 0, s, F);
         return hipGetLastError();
     }
 } // namespace isaac_hip

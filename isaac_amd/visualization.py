@@ -199,6 +199,7 @@ class IsaacVisualization:
         self.projection = default_projection(*self.framebuffer_size)
         self.modelview = modelview(self.look_at, self.rotation, self.distance)
         self.clipping: List[Tuple[Tuple[float, float, float], Tuple[float, float, float]]] = []
+        self.viewport: Optional[Tuple[int, int, int, int]] = None  # IceT readback viewport (x, y, w, h)
         self.kernel_time = self.copy_time = self.buffer_time = self.optimization_buffer_time = 0.0
         self._sources_bound = False
         self.update_functions()
@@ -350,7 +351,9 @@ class IsaacVisualization:
         proj = np.ascontiguousarray(projection if projection is not None else self.projection, np.float64)
         mv = np.ascontiguousarray(modelview_ if modelview_ is not None else self.modelview, np.float64)
         bg = np.asarray(self.background_color, np.float32)
-        vp = np.asarray(viewport if viewport is not None else (0, 0, W, H), np.int32)
+        if viewport is None:
+            viewport = self.viewport if self.viewport is not None else (0, 0, W, H)
+        vp = np.asarray(viewport, np.int32)
         result = None
         if out is None:
             result = np.zeros((H, W, 4), np.uint8)
@@ -380,7 +383,7 @@ class IsaacVisualization:
         check(lib().isaac_hip_get_timings(self._ctx, ctypes.byref(t)), self._ctx, "get_timings")
         self.kernel_time, self.copy_time = t.kernel_us, t.copy_us
         self.buffer_time, self.optimization_buffer_time = t.buffer_us, t.optimization_buffer_us
-        return {"kernel_us": t.kernel_us, "copy_us": t.copy_us, "buffer_us": t.buffer_us, "optimization_buffer_us": t.optimization_buffer_us}
+        return {name: float(getattr(t, name)) for name, _ in _lib.Timings._fields_}
 
     def set_counting(self, on: bool) -> None:
         check(lib().isaac_hip_set_counting(self._ctx, int(on)), self._ctx, "set_counting")
@@ -402,20 +405,19 @@ class IsaacVisualization:
         out = []
         for p, shape, dt in zip(ptrs, [(H, W, 4), (H, W), (H, W, 3), (H, W)], [np.uint8, np.float32, np.float32, np.float32]):
             a = np.zeros(shape, dt)
-            _hip_d2h(a, p.value)
+            self._d2h(a, p.value)
             out.append(a)
         return out
 
     def combined_textures(self):
+        """(volume, iso) combined RGBA8 textures, each (Z, Y, X, 4) uint8."""
         X, Y, Z = self.local_size
-        pv, pi = ctypes.c_void_p(), ctypes.c_void_p()
-        check(lib().isaac_hip_get_combined_textures(self._ctx, ctypes.byref(pv), ctypes.byref(pi)), self._ctx, "textures")
+        pt, nb = ctypes.c_void_p(), ctypes.c_int32()
+        check(lib().isaac_hip_get_combined_textures(self._ctx, ctypes.byref(pt), ctypes.byref(nb)), self._ctx, "textures")
         self.synchronize()
-        vol = np.zeros((Z, Y, X, 4), np.uint8)
-        iso = np.zeros((Z, Y, X, 4), np.uint8)
-        _hip_d2h(vol, pv.value)
-        _hip_d2h(iso, pi.value)
-        return vol, iso
+        tex = np.zeros((Z, Y, X, int(nb.value)), np.uint8)
+        self._d2h(tex, pt.value)
+        return np.ascontiguousarray(tex[..., 0:4]), np.ascontiguousarray(tex[..., 4:8])
 
     def cell_list(self, idx: int):
         gx, gy, gz = self.local_particle_size
@@ -427,23 +429,13 @@ class IsaacVisualization:
         counts = np.zeros(nc, np.int32)
         offs = np.zeros(nc + 1, np.int32)
         perm = np.zeros(n, np.int32)
-        _hip_d2h(counts, pc.value)
-        _hip_d2h(offs, po.value)
-        _hip_d2h(perm, pp.value)
+        self._d2h(counts, pc.value)
+        self._d2h(offs, po.value)
+        self._d2h(perm, pp.value)
         return counts, offs, perm
 
-
-_hip = None
-
-
-def _hip_d2h(arr: np.ndarray, dev_ptr: int) -> None:
-    global _hip
-    if _hip is None:
-        _hip = ctypes.CDLL("libamdhip64.so")
-        _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
-    rc = _hip.hipMemcpy(arr.ctypes.data, ctypes.c_void_p(dev_ptr), arr.nbytes, 2)  # hipMemcpyDeviceToHost
-    if rc != 0:
-        raise _lib.IsaacHipError(f"hipMemcpy D2H failed ({rc})")
+    def _d2h(self, arr: np.ndarray, dev_ptr: int) -> None:
+        check(lib().isaac_hip_copy_to_host(self._ctx, arr.ctypes.data, ctypes.c_void_p(dev_ptr), arr.nbytes), self._ctx, "copy_to_host")
 
 
 # --------------------------------------------------------------------------- scene -> visualization
@@ -483,6 +475,7 @@ def apply_scene(vis: IsaacVisualization, spec) -> None:
     vis.render_mode = spec.render_mode
     vis.ambient_occlusion = {"isEnabled": spec.ao_enabled, "weight": spec.ao_weight}
     vis.background_color = list(spec.background)
+    vis.viewport = spec.viewport
     while len(vis.clipping):
         vis.remove_clipping(0)
     for pos, nrm in spec.clipping:
