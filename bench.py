@@ -41,14 +41,15 @@ def kernel_bytes(counts, spec, n_vox, n_particles, n_visible):
         # pos 12 + attr 4 in; cell id 4 + record 16 written and read back; perm 4 written and read back;
         # visible render record 16 written
         "binning": n_particles * (12 + 4 + 2 * 4 + 2 * 16 + 2 * 4) + n_visible * 16,
-        # iso: 8 RGBA8 taps per sample + 7 samples at a hit; DDA: offset pair per cell, 16 B per sphere
-        # test; G-buffer writes colour 4 + normal 12 + depth 4
-        "solid": 32 * counts["iso_samples"] + 7 * 32 * counts["iso_hits"] + 8 * counts["cells_visited"]
-        + 16 * counts["sphere_tests"] + 20 * npx,
+        # fused march: 8 RGBA8 taps per volume sample and per iso sample, 7 iso samples at a hit;
+        # DDA: offset pair per cell, 16 B per sphere test; G-buffer writes colour 4 + normal 12 + depth 4
+        # + volume colour 16
+        "march": 32 * (counts["vol_samples"] + counts["iso_samples"]) + 7 * 32 * counts["iso_hits"]
+        + 8 * counts["cells_visited"] + 16 * counts["sphere_tests"] + 36 * npx,
         # 49 depth taps + ao write
         "ssao": npx * (49 * 4 + 4),
-        # 8 RGBA8 taps per sample; G-buffer read colour 4 + normal 12 + depth 4 + ao 4, colour write 4
-        "volume": 32 * counts["vol_samples"] + 28 * npx,
+        # G-buffer read colour 4 + normal 12 + depth 4 + ao 4 + volume colour 16, colour write 4
+        "shade": 44 * npx,
     }
 
 
@@ -83,6 +84,9 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=256, help="local volume side (the workload is 256)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--prefetch", type=int, default=0, help="march kernel: fetch step i+1 before compositing step i")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run one plain device copy of this size first (scales profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -98,10 +102,19 @@ def main() -> int:
     from isaac_amd import scenes
     from isaac_amd import visualization as V
 
+    if args.copy_calibration_mib:
+        nbytes = args.copy_calibration_mib << 20
+        a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
     n = args.size
     spec = scenes.prod_scene(n=n)
     poses = [scenes.DEFAULT_POSE, scenes.FILL_POSE]
     vis = V.from_scene(spec, 0)
+    vis.set_tuning(vis.TUNE_MARCH_PREFETCH, args.prefetch)
     out = torch.empty((spec.framebuffer[1], spec.framebuffer[0], 4), dtype=torch.uint8, device="cuda")
     n_vox = n ** 3
     n_particles = int(spec.particles[0].pos.shape[0])
@@ -130,7 +143,7 @@ def main() -> int:
     vis.synchronize()
     torch.cuda.synchronize()
 
-    per = {"classify": [], "binning": [], "solid": [], "ssao": [], "volume": []}
+    per = {"classify": [], "binning": [], "march": [], "ssao": [], "shade": []}
     total_samples = 0
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -141,9 +154,9 @@ def main() -> int:
         t = vis.timings()  # waits for the step: each frame completes before the next (as the reference)
         per["classify"].append(t["optimization_buffer_us"])
         per["binning"].append(t["binning_us"])
-        per["solid"].append(t["solid_us"])
+        per["march"].append(t["march_us"])
         per["ssao"].append(t["ssao_us"])
-        per["volume"].append(t["volume_us"])
+        per["shade"].append(t["shade_us"])
         total_samples += counts[pi]["vol_samples"] + counts[pi]["iso_samples"]
     vis.synchronize()
     torch.cuda.synchronize()

@@ -124,9 +124,9 @@ extern "C"
         /* per kernel (GPU time between events on the context stream) */
         double source_copy_us; /* non-persistent source textures */
         double binning_us; /* particle cell lists */
-        double solid_us; /* clear + particles + iso */
+        double march_us; /* clear + particles + iso + volume march (one traversal) */
         double ssao_us; /* SSAO */
-        double volume_us; /* shading + volume */
+        double shade_us; /* shading + volume blend */
     } isaac_hip_timings;
 
     /* ---------------------------------------------------------------- lifetime */
@@ -194,6 +194,53 @@ extern "C"
         uint8_t* out_rgba,
         int out_is_device);
     int isaac_hip_synchronize(isaac_hip_ctx* ctx);
+
+    /* ---------------------------------------------------------------- tuning
+     * Launch-schedule knobs with no counterpart in the reference (results do not
+     * change, only speed); used by the autotuner. */
+    enum isaac_hip_tuning_key
+    {
+        ISAAC_HIP_TUNE_MARCH_PREFETCH = 0 /* 1: fetch step i+1 before compositing step i (default 0) */
+    };
+    int isaac_hip_set_tuning(isaac_hip_ctx* ctx, int32_t key, int32_t value);
+
+    /* ---------------------------------------------------------------- render forms
+     * No counterpart in the reference (its build picks the optimized renderer;
+     * the legacy post-classification renderer, lib/isaac/isaac_volume_kernel.hpp:91-632,
+     * is compiled out by default).
+     *   COMBINED  the reference's optimized renderer: classification into the
+     *             combined RGBA8 volume / iso textures, then the march (default).
+     *   BRICKS    each volume/field source re-laid into 8^3 bricks (one-voxel apron)
+     *             in brick_dtype (ISAAC_HIP_F32, F16 or BF16; 16-bit forms store
+     *             v / s with one FP32 scale s = max |v| per brick and component);
+     *             the march interpolates the raw field and applies the functor
+     *             chain and transfer function per sample (post-classification),
+     *             accumulating in FP32, with the camera inverse and ray/box/plane
+     *             setup in FP64.  At most 4 volume/field sources may be enabled.
+     * Takes effect at the next isaac_hip_update_buffers. */
+    enum isaac_hip_form
+    {
+        ISAAC_HIP_FORM_COMBINED = 0,
+        ISAAC_HIP_FORM_BRICKS = 1
+    };
+    int isaac_hip_set_render_form(isaac_hip_ctx* ctx, int32_t form, int32_t brick_dtype);
+    /* device pointers of volume/field source i's bricks: *pairs = nbricks * feature_dim
+     * blocks of 648 pairs (brick index = (bz*nby + by)*nbx + bx, x fastest; pair
+     * (lz*9 + ly)*8 + lx holds (v[x], v[x+1]) at voxel (8bx+lx, 8by+ly, 8bz+lz),
+     * coordinates clamped into the local box), each pair *pair_bytes (8 or 4) bytes;
+     * *scales = nbricks * feature_dim floats.  Valid after update_buffers in BRICKS form. */
+    int isaac_hip_get_bricks(
+        const isaac_hip_ctx* ctx,
+        int source,
+        const void** pairs,
+        const void** scales,
+        int64_t* nbricks,
+        int32_t* pair_bytes);
+    /* When enabled, render also writes the final colour before RGBA8 quantisation
+     * (float RGBA per pixel, clamped to [0,1]) to a device buffer whose pointer
+     * isaac_hip_get_float_image returns (precision studies against an FP64 render). */
+    int isaac_hip_set_float_output(isaac_hip_ctx* ctx, int enabled);
+    int isaac_hip_get_float_image(const isaac_hip_ctx* ctx, const void** rgba_f32);
 
     /* ---------------------------------------------------------------- introspection */
     int isaac_hip_get_timings(const isaac_hip_ctx* ctx, isaac_hip_timings* out);

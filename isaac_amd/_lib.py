@@ -70,9 +70,9 @@ class Timings(ctypes.Structure):
         ("optimization_buffer_us", ctypes.c_double),
         ("source_copy_us", ctypes.c_double),
         ("binning_us", ctypes.c_double),
-        ("solid_us", ctypes.c_double),
+        ("march_us", ctypes.c_double),
         ("ssao_us", ctypes.c_double),
-        ("volume_us", ctypes.c_double),
+        ("shade_us", ctypes.c_double),
     ]
 
 
@@ -111,6 +111,11 @@ _SIGS = {
     "isaac_hip_synchronize": (ctypes.c_int, [P]),
     "isaac_hip_get_timings": (ctypes.c_int, [P, P]),
     "isaac_hip_set_counting": (ctypes.c_int, [P, ctypes.c_int]),
+    "isaac_hip_set_tuning": (ctypes.c_int, [P, I32, I32]),
+    "isaac_hip_set_render_form": (ctypes.c_int, [P, I32, I32]),
+    "isaac_hip_get_bricks": (ctypes.c_int, [P, ctypes.c_int, P, P, P, P]),
+    "isaac_hip_set_float_output": (ctypes.c_int, [P, ctypes.c_int]),
+    "isaac_hip_get_float_image": (ctypes.c_int, [P, P]),
     "isaac_hip_get_counters": (ctypes.c_int, [P, P]),
     "isaac_hip_get_gbuffer": (ctypes.c_int, [P, P, P, P, P]),
     "isaac_hip_get_combined_textures": (ctypes.c_int, [P, P, P]),

@@ -6,6 +6,7 @@
 // reference's CPU path.
 #pragma once
 
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -81,6 +82,7 @@ namespace isaac_hip
         float scale[3];
         float step;
         int32_t linear;
+        int32_t prefetch; // fetch the next step's taps before compositing the current one
         // textures
         int32_t X, Y, Z;
         const uint2* tex; // interleaved combined textures, x fastest
@@ -102,7 +104,46 @@ namespace isaac_hip
         float* normal; // 3 floats per pixel
         float* ao;
         float4* volColor; // premultiplied volume colour per pixel; w < 0: the volume pass did not blend
+        float4* outFloat; // optional: final colour before RGBA8 quantisation (precision studies)
         unsigned long long* counters;
+    };
+
+    // ------------------------------------------------------------------ precision form (field bricks)
+    // A source re-laid into 8^3 bricks with a one-voxel apron on the + side, stored
+    // as x-pairs (v[x], v[x+1]) for x in 0..7, y, z in 0..8 (648 pairs per brick and
+    // component), so the 8 trilinear taps of any sample come from one brick with 4
+    // aligned pair loads.  Pairs are float2 (F32) or two 16-bit values (F16, BF16)
+    // holding v / scale, with one FP32 scale per brick and component.
+    constexpr int kBrick = 8;
+    constexpr int kBrickPairs = 8 * 9 * 9;
+    constexpr int kMaxBrickSources = 4;
+    constexpr int kDtF32 = 0, kDtF16 = 1, kDtBf16 = 2; // values of isaac_hip_dtype
+
+    struct BrickSource
+    {
+        const void* data; // [brick][component][648 pairs]
+        const float* scale; // [brick][component]
+        int32_t dim;
+        int32_t active; // weight + iso > 0
+        float weight;
+        float iso;
+        ChainDesc chain;
+        int32_t lutSlot;
+    };
+
+    struct PostParams
+    {
+        int32_t nsrc;
+        int32_t dtype; // ISAAC_HIP_F32 / F16 / BF16
+        int32_t nbx, nby, nbz;
+        float totalWeight; // sum of volume-source weights (classification normaliser)
+        int32_t T;
+        int32_t nLut; // LUTs staged in LDS
+        const float4* luts;
+        double invd[16]; // inverse MVP in binary64 (ray setup in FP64)
+        double maxSized; // maxGlobalSizeScaled * 0.5
+        double cposd[kMaxClipping][3]; // clipping plane positions, scaled and offset, binary64
+        BrickSource src[kMaxBrickSources];
     };
 
     struct ParticleSourceDev

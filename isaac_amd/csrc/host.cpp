@@ -22,18 +22,21 @@ namespace
     const char* kFunctorNames[6] = {"idem", "add", "mul", "length", "pow", "sum"}; // lib/isaac/isaac_functors.hpp:185-208
 
     // ------------------------------------------------------------------ host math (glm op order)
-    struct M4
+    template<class R>
+    struct M4T
     {
-        float m[4][4];
+        R m[4][4];
     };
+    using M4 = M4T<float>;
 
-    M4 mat_mul(const M4& a, const M4& b)
+    template<class R>
+    M4T<R> mat_mul(const M4T<R>& a, const M4T<R>& b)
     {
-        M4 r;
+        M4T<R> r;
         for(int c = 0; c < 4; c++)
             for(int row = 0; row < 4; row++)
             {
-                float s = a.m[0][row] * b.m[c][0];
+                R s = a.m[0][row] * b.m[c][0];
                 s = s + a.m[1][row] * b.m[c][1];
                 s = s + a.m[2][row] * b.m[c][2];
                 s = s + a.m[3][row] * b.m[c][3];
@@ -42,51 +45,53 @@ namespace
         return r;
     }
 
-    // glm compute_inverse<4,4> in binary32 (lib/isaac.hpp:2177 calls glm::inverse)
-    M4 mat_inverse(const M4& M)
+    // glm compute_inverse<4,4> (lib/isaac.hpp:2177 calls glm::inverse); binary32 for the
+    // combined form, binary64 for the ray setup of the brick form
+    template<class R>
+    M4T<R> mat_inverse(const M4T<R>& M)
     {
         auto m = [&](int c, int r) { return M.m[c][r]; };
-        const float c00 = m(2, 2) * m(3, 3) - m(3, 2) * m(2, 3);
-        const float c02 = m(1, 2) * m(3, 3) - m(3, 2) * m(1, 3);
-        const float c03 = m(1, 2) * m(2, 3) - m(2, 2) * m(1, 3);
-        const float c04 = m(2, 1) * m(3, 3) - m(3, 1) * m(2, 3);
-        const float c06 = m(1, 1) * m(3, 3) - m(3, 1) * m(1, 3);
-        const float c07 = m(1, 1) * m(2, 3) - m(2, 1) * m(1, 3);
-        const float c08 = m(2, 1) * m(3, 2) - m(3, 1) * m(2, 2);
-        const float c10 = m(1, 1) * m(3, 2) - m(3, 1) * m(1, 2);
-        const float c11 = m(1, 1) * m(2, 2) - m(2, 1) * m(1, 2);
-        const float c12 = m(2, 0) * m(3, 3) - m(3, 0) * m(2, 3);
-        const float c14 = m(1, 0) * m(3, 3) - m(3, 0) * m(1, 3);
-        const float c15 = m(1, 0) * m(2, 3) - m(2, 0) * m(1, 3);
-        const float c16 = m(2, 0) * m(3, 2) - m(3, 0) * m(2, 2);
-        const float c18 = m(1, 0) * m(3, 2) - m(3, 0) * m(1, 2);
-        const float c19 = m(1, 0) * m(2, 2) - m(2, 0) * m(1, 2);
-        const float c20 = m(2, 0) * m(3, 1) - m(3, 0) * m(2, 1);
-        const float c22 = m(1, 0) * m(3, 1) - m(3, 0) * m(1, 1);
-        const float c23 = m(1, 0) * m(2, 1) - m(2, 0) * m(1, 1);
-        const float f0[4] = {c00, c00, c02, c03}, f1[4] = {c04, c04, c06, c07}, f2[4] = {c08, c08, c10, c11};
-        const float f3_[4] = {c12, c12, c14, c15}, f4[4] = {c16, c16, c18, c19}, f5[4] = {c20, c20, c22, c23};
-        const float v0[4] = {m(1, 0), m(0, 0), m(0, 0), m(0, 0)};
-        const float v1[4] = {m(1, 1), m(0, 1), m(0, 1), m(0, 1)};
-        const float v2[4] = {m(1, 2), m(0, 2), m(0, 2), m(0, 2)};
-        const float v3[4] = {m(1, 3), m(0, 3), m(0, 3), m(0, 3)};
-        float col[4][4];
-        const float sa[4] = {1.f, -1.f, 1.f, -1.f}, sb[4] = {-1.f, 1.f, -1.f, 1.f};
+        const R c00 = m(2, 2) * m(3, 3) - m(3, 2) * m(2, 3);
+        const R c02 = m(1, 2) * m(3, 3) - m(3, 2) * m(1, 3);
+        const R c03 = m(1, 2) * m(2, 3) - m(2, 2) * m(1, 3);
+        const R c04 = m(2, 1) * m(3, 3) - m(3, 1) * m(2, 3);
+        const R c06 = m(1, 1) * m(3, 3) - m(3, 1) * m(1, 3);
+        const R c07 = m(1, 1) * m(2, 3) - m(2, 1) * m(1, 3);
+        const R c08 = m(2, 1) * m(3, 2) - m(3, 1) * m(2, 2);
+        const R c10 = m(1, 1) * m(3, 2) - m(3, 1) * m(1, 2);
+        const R c11 = m(1, 1) * m(2, 2) - m(2, 1) * m(1, 2);
+        const R c12 = m(2, 0) * m(3, 3) - m(3, 0) * m(2, 3);
+        const R c14 = m(1, 0) * m(3, 3) - m(3, 0) * m(1, 3);
+        const R c15 = m(1, 0) * m(2, 3) - m(2, 0) * m(1, 3);
+        const R c16 = m(2, 0) * m(3, 2) - m(3, 0) * m(2, 2);
+        const R c18 = m(1, 0) * m(3, 2) - m(3, 0) * m(1, 2);
+        const R c19 = m(1, 0) * m(2, 2) - m(2, 0) * m(1, 2);
+        const R c20 = m(2, 0) * m(3, 1) - m(3, 0) * m(2, 1);
+        const R c22 = m(1, 0) * m(3, 1) - m(3, 0) * m(1, 1);
+        const R c23 = m(1, 0) * m(2, 1) - m(2, 0) * m(1, 1);
+        const R f0[4] = {c00, c00, c02, c03}, f1[4] = {c04, c04, c06, c07}, f2[4] = {c08, c08, c10, c11};
+        const R f3_[4] = {c12, c12, c14, c15}, f4[4] = {c16, c16, c18, c19}, f5[4] = {c20, c20, c22, c23};
+        const R v0[4] = {m(1, 0), m(0, 0), m(0, 0), m(0, 0)};
+        const R v1[4] = {m(1, 1), m(0, 1), m(0, 1), m(0, 1)};
+        const R v2[4] = {m(1, 2), m(0, 2), m(0, 2), m(0, 2)};
+        const R v3[4] = {m(1, 3), m(0, 3), m(0, 3), m(0, 3)};
+        R col[4][4];
+        const R sa[4] = {R(1), R(-1), R(1), R(-1)}, sb[4] = {R(-1), R(1), R(-1), R(1)};
         for(int k = 0; k < 4; k++)
         {
-            const float i0 = v1[k] * f0[k] - v2[k] * f1[k] + v3[k] * f2[k];
-            const float i1 = v0[k] * f0[k] - v2[k] * f3_[k] + v3[k] * f4[k];
-            const float i2 = v0[k] * f1[k] - v1[k] * f3_[k] + v3[k] * f5[k];
-            const float i3 = v0[k] * f2[k] - v1[k] * f4[k] + v2[k] * f5[k];
+            const R i0 = v1[k] * f0[k] - v2[k] * f1[k] + v3[k] * f2[k];
+            const R i1 = v0[k] * f0[k] - v2[k] * f3_[k] + v3[k] * f4[k];
+            const R i2 = v0[k] * f1[k] - v1[k] * f3_[k] + v3[k] * f5[k];
+            const R i3 = v0[k] * f2[k] - v1[k] * f4[k] + v2[k] * f5[k];
             col[0][k] = i0 * sa[k];
             col[1][k] = i1 * sb[k];
             col[2][k] = i2 * sa[k];
             col[3][k] = i3 * sb[k];
         }
-        const float d0[4] = {M.m[0][0] * col[0][0], M.m[0][1] * col[1][0], M.m[0][2] * col[2][0], M.m[0][3] * col[3][0]};
-        const float d1 = (d0[0] + d0[1]) + (d0[2] + d0[3]);
-        const float inv = 1.0f / d1;
-        M4 r;
+        const R d0[4] = {M.m[0][0] * col[0][0], M.m[0][1] * col[1][0], M.m[0][2] * col[2][0], M.m[0][3] * col[3][0]};
+        const R d1 = (d0[0] + d0[1]) + (d0[2] + d0[3]);
+        const R inv = R(1) / d1;
+        M4T<R> r;
         for(int c = 0; c < 4; c++)
             for(int k = 0; k < 4; k++)
                 r.m[c][k] = col[c][k] * inv;
@@ -292,6 +297,12 @@ namespace
         bool set = false;
         float* npTex = nullptr; // non-persistent functor-applied texture (with guard)
         size_t npBytes = 0;
+        void* brick = nullptr; // brick form: pairs
+        float* brickScale = nullptr;
+        size_t brickBytes = 0, scaleCount = 0;
+        int64_t nbricks = 0;
+        int brickDim = 0;
+        bool bricked = false; // bricks hold the source as of the last update_buffers
     };
 
     struct PartSlot
@@ -347,6 +358,12 @@ struct isaac_hip_ctx
     unsigned long long* counters = nullptr;
     bool counting = false;
     bool classified = false;
+    int tunePrefetch = 0;
+    int form = ISAAC_HIP_FORM_COMBINED;
+    int brickDtype = ISAAC_HIP_F32;
+    int builtForm = -1; // form prepared by the last update_buffers
+    bool floatOut = false;
+    float4* outFloat = nullptr;
     // events: 0 update start, 1 after source copies, 2 after binning, 3 after
     // classification; 4 draw start, 5 after solid, 6 after SSAO, 7 after volume, 8 after copy
     hipEvent_t ev[12] = {};
@@ -414,6 +431,33 @@ namespace
         return 0;
     }
 
+    int ensure_bricks(isaac_hip_ctx* c, VolSlot& v, int dim)
+    {
+        const int64_t nbx = (c->local[0] + 7) / 8, nby = (c->local[1] + 7) / 8, nbz = (c->local[2] + 7) / 8;
+        const int64_t nb = nbx * nby * nbz;
+        const size_t pairBytes = c->brickDtype == ISAAC_HIP_F32 ? 8 : 4;
+        const size_t bytes = size_t(nb) * size_t(dim) * size_t(kBrickPairs) * pairBytes;
+        const size_t ns = size_t(nb) * size_t(dim);
+        if(uint64_t(ns) * uint64_t(kBrickPairs) >= (uint64_t(1) << 32))
+            return fail(c, ISAAC_HIP_ERR_UNSUPPORTED, "brick form: more than 2^32 pairs per source (32-bit pair index)");
+        if(v.brick && v.brickBytes == bytes && v.scaleCount == ns)
+            return 0;
+        if(v.brick)
+            (void) hipFree(v.brick);
+        if(v.brickScale)
+            (void) hipFree(v.brickScale);
+        v.brick = nullptr;
+        v.brickScale = nullptr;
+        v.brickBytes = v.scaleCount = 0;
+        HIPCHK(c, hipMalloc(&v.brick, std::max<size_t>(bytes, 8)));
+        HIPCHK(c, hipMalloc(&v.brickScale, std::max<size_t>(ns, 1) * 4));
+        v.brickBytes = bytes;
+        v.scaleCount = ns;
+        v.nbricks = nb;
+        v.brickDim = dim;
+        return 0;
+    }
+
     void free_part(PartSlot& p)
     {
         for(void*& b : p.blocks)
@@ -434,23 +478,23 @@ namespace
         if(p.capN >= n && p.capCells == nc && (p.B.recRadius != nullptr) == needRad)
             return 0;
         free_part(p);
-        size_t sizes[12] = {size_t(n) * 4, size_t(n) * 16, size_t(nc) * 4, size_t(nc) * 4, size_t(nc + 1) * 4,
-                            size_t(nc + 1) * 4, size_t(nc) * 4, size_t(n) * 4, size_t(n) * 16,
-                            needRad ? size_t(n) * 4 : 0, bin_scan_workspace_ints(nc) * 4, 0};
+        const size_t sizes[11] = {size_t(n) * 4, size_t(n) * 4, size_t(n) * 16, size_t(nc) * 8, size_t(nc) * 4,
+                                  size_t(nc + 1) * 4, size_t(nc + 1) * 4, size_t(n) * 4, size_t(n) * 16,
+                                  needRad ? size_t(n) * 4 : 0, bin_scan_workspace_elems(nc) * 8};
         for(int i = 0; i < 11; i++)
             if(sizes[i])
                 HIPCHK(c, hipMalloc(&p.blocks[i], sizes[i]));
         p.B.cellOf = (int32_t*) p.blocks[0];
-        p.B.rel = (float4*) p.blocks[1];
-        p.B.count = (int32_t*) p.blocks[2];
-        p.B.vcount = (int32_t*) p.blocks[3];
-        p.B.offset = (int32_t*) p.blocks[4];
-        p.B.voffset = (int32_t*) p.blocks[5];
-        p.B.cursor = (int32_t*) p.blocks[6];
+        p.B.rank = (uint32_t*) p.blocks[1];
+        p.B.rel = (float4*) p.blocks[2];
+        p.B.count64 = (unsigned long long*) p.blocks[3];
+        p.B.count = (int32_t*) p.blocks[4];
+        p.B.offset = (int32_t*) p.blocks[5];
+        p.B.voffset = (int32_t*) p.blocks[6];
         p.B.perm = (int32_t*) p.blocks[7];
         p.B.recs = (float4*) p.blocks[8];
         p.B.recRadius = (float*) p.blocks[9];
-        p.B.scanTmp = (int32_t*) p.blocks[10];
+        p.B.scanTmp = (unsigned long long*) p.blocks[10];
         p.capN = n;
         p.capCells = nc;
         return 0;
@@ -674,11 +718,17 @@ extern "C"
         if(c->stream)
             (void) hipStreamSynchronize(c->stream);
         for(auto& v : c->vols)
+        {
             if(v.npTex)
                 (void) hipFree(v.npTex);
+            if(v.brick)
+                (void) hipFree(v.brick);
+            if(v.brickScale)
+                (void) hipFree(v.brickScale);
+        }
         for(auto& p : c->parts)
             free_part(p);
-        void* bufs[] = {c->lutDev, c->tex, c->color, c->depth, c->normal, c->ao, c->volColor, c->counters};
+        void* bufs[] = {c->lutDev, c->tex, c->color, c->depth, c->normal, c->ao, c->volColor, c->counters, c->outFloat};
         for(void* b : bufs)
             if(b)
                 (void) hipFree(b);
@@ -1003,6 +1053,52 @@ extern "C"
             HIPCHK(c, launch_bin(P, p.B, c->stream));
         }
         HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+        if(c->form == ISAAC_HIP_FORM_BRICKS)
+        {
+            // brick form: re-lay every enabled volume/field source (the raw
+            // source when persistent, its functor-applied texture otherwise)
+            int active = 0;
+            for(int s = 0; s < nvf; s++)
+            {
+                VolSlot& v = c->vols[s];
+                v.bricked = false;
+                if(!v.set || c->weight[s] + c->iso[s] <= 0.0f)
+                    continue;
+                if(++active > kMaxBrickSources)
+                    return fail(c, ISAAC_HIP_ERR_UNSUPPORTED, "brick form: at most 4 enabled volume/field sources");
+                const float* src;
+                int64_t st[3];
+                int dim;
+                if(v.desc.persistent)
+                {
+                    src = (const float*) v.desc.data;
+                    dim = v.desc.feature_dim;
+                    for(int k = 0; k < 3; k++)
+                        st[k] = v.desc.stride[k];
+                }
+                else
+                {
+                    const int64_t g = v.desc.guard_size;
+                    const int64_t gX = c->local[0] + 2 * g, gY = c->local[1] + 2 * g;
+                    src = v.npTex + (g + g * gX + g * gX * gY);
+                    dim = 1;
+                    st[0] = 1;
+                    st[1] = gX;
+                    st[2] = gX * gY;
+                }
+                int rc = ensure_bricks(c, v, dim);
+                if(rc)
+                    return rc;
+                HIPCHK(c, launch_bricks(src, st, dim, int(c->local[0]), int(c->local[1]), int(c->local[2]), c->brickDtype,
+                                        v.brickScale, v.brick, c->stream));
+                v.bricked = true;
+            }
+            HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+            c->evRecorded[0] = true;
+            c->classified = true;
+            c->builtForm = ISAAC_HIP_FORM_BRICKS;
+            return 0;
+        }
         // classification (lib/isaac.hpp:1916-1947)
         ClassifyParams P{};
         P.X = int32_t(c->local[0]);
@@ -1065,6 +1161,7 @@ extern "C"
         HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
         c->evRecorded[0] = true;
         c->classified = true;
+        c->builtForm = ISAAC_HIP_FORM_COMBINED;
         return 0;
     }
 
@@ -1082,6 +1179,8 @@ extern "C"
         HIPCHK(c, hipSetDevice(c->device));
         if(!c->classified)
             return fail(c, ISAAC_HIP_ERR_STATE, "isaac_hip_update_buffers must run before the first render");
+        if(c->builtForm != c->form)
+            return fail(c, ISAAC_HIP_ERR_STATE, "render form changed; call isaac_hip_update_buffers");
         const int W = c->cfg.framebuffer_size[0], H = c->cfg.framebuffer_size[1];
         FrameParams F{};
         // inverse MVP in binary32 (lib/isaac.hpp:2174-2178)
@@ -1133,6 +1232,7 @@ extern "C"
             }
         F.step = c->step;
         F.linear = c->interpolation ? 1 : 0;
+        F.prefetch = c->tunePrefetch;
         F.X = int32_t(c->local[0]);
         F.Y = int32_t(c->local[1]);
         F.Z = int32_t(c->local[2]);
@@ -1196,10 +1296,66 @@ extern "C"
             S.weight = c->weight[gi];
             S.lut = c->lutDev + size_t(gi) * T;
         }
+        F.outFloat = c->floatOut ? c->outFloat : nullptr;
+        PostParams Q{};
+        if(c->form == ISAAC_HIP_FORM_BRICKS)
+        {
+            // binary64 camera inverse for the ray setup
+            M4T<double> Pd, MVd;
+            for(int col = 0; col < 4; col++)
+                for(int r = 0; r < 4; r++)
+                {
+                    Pd.m[col][r] = proj[col * 4 + r];
+                    MVd.m[col][r] = mv[col * 4 + r];
+                }
+            const M4T<double> invd = mat_inverse(mat_mul(Pd, MVd));
+            for(int col = 0; col < 4; col++)
+                for(int r = 0; r < 4; r++)
+                    Q.invd[col * 4 + r] = invd.m[col][r];
+            Q.maxSized = double(c->maxGsScaled) * 0.5;
+            for(int i = 0; i < c->nclip; i++)
+                for(int k = 0; k < 3; k++)
+                    Q.cposd[i][k] = double(c->clipPos[i][k]) * double(c->gsScaled[k]) * 0.5 + double(F.off[k]);
+            Q.dtype = c->brickDtype;
+            Q.nbx = int32_t((c->local[0] + 7) / 8);
+            Q.nby = int32_t((c->local[1] + 7) / 8);
+            Q.nbz = int32_t((c->local[2] + 7) / 8);
+            Q.T = T;
+            Q.luts = c->lutDev;
+            float tw = 0.0f;
+            for(int s = 0; s < nvf; s++)
+                tw += c->weight[s];
+            Q.totalWeight = tw;
+            int maxSlot = -1;
+            for(int s = 0; s < nvf; s++)
+            {
+                const VolSlot& v = c->vols[s];
+                if(!v.set || c->weight[s] + c->iso[s] <= 0.0f)
+                    continue;
+                if(!v.bricked)
+                    return fail(c, ISAAC_HIP_ERR_STATE, "source " + std::to_string(s) + " enabled after update_buffers; call it again");
+                BrickSource& B = Q.src[Q.nsrc++];
+                B.data = v.brick;
+                B.scale = v.brickScale;
+                B.dim = v.brickDim;
+                B.active = 1;
+                B.weight = c->weight[s];
+                B.iso = c->iso[s];
+                B.chain = v.desc.persistent ? c->chain[s] : ChainDesc{};
+                B.lutSlot = s;
+                maxSlot = s;
+            }
+            // stage LUT slots 0..maxSlot in LDS when they fit in 64 KiB
+            const size_t need = size_t(maxSlot + 1) * size_t(T) * sizeof(float4);
+            Q.nLut = (maxSlot >= 0 && need <= 64 * 1024) ? maxSlot + 1 : 0;
+        }
         if(c->counting)
             HIPCHK(c, launch_fill_u32(reinterpret_cast<uint32_t*>(c->counters), 0u, 16, c->stream));
         HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
-        HIPCHK(c, launch_frame(F, PP, c->counting, c->stream, c->ev[5], c->ev[6]));
+        if(c->form == ISAAC_HIP_FORM_BRICKS)
+            HIPCHK(c, launch_brick_frame(F, Q, PP, c->counting, c->stream, c->ev[5], c->ev[6]));
+        else
+            HIPCHK(c, launch_frame(F, PP, c->counting, c->stream, c->ev[5], c->ev[6]));
         HIPCHK(c, hipEventRecord(c->ev[7], c->stream));
         if(out_rgba)
         {
@@ -1236,9 +1392,9 @@ extern "C"
         }
         if(c->evRecorded[1])
         {
-            out->solid_us = elapsed_us(c->ev[4], c->ev[5]);
+            out->march_us = elapsed_us(c->ev[4], c->ev[5]);
             out->ssao_us = elapsed_us(c->ev[5], c->ev[6]);
-            out->volume_us = elapsed_us(c->ev[6], c->ev[7]);
+            out->shade_us = elapsed_us(c->ev[6], c->ev[7]);
             out->kernel_us = elapsed_us(c->ev[4], c->ev[7]);
             out->copy_us = elapsed_us(c->ev[7], c->ev[8]);
         }
@@ -1251,6 +1407,77 @@ extern "C"
             return ISAAC_HIP_ERR_ARG;
         HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
+        return 0;
+    }
+
+    int isaac_hip_set_tuning(isaac_hip_ctx* c, int32_t key, int32_t value)
+    {
+        if(!c)
+            return ISAAC_HIP_ERR_ARG;
+        switch(key)
+        {
+        case ISAAC_HIP_TUNE_MARCH_PREFETCH:
+            c->tunePrefetch = value != 0;
+            return 0;
+        default:
+            return fail(c, ISAAC_HIP_ERR_ARG, "unknown tuning key");
+        }
+    }
+
+    int isaac_hip_set_render_form(isaac_hip_ctx* c, int32_t form, int32_t brick_dtype)
+    {
+        if(!c)
+            return ISAAC_HIP_ERR_ARG;
+        if(form != ISAAC_HIP_FORM_COMBINED && form != ISAAC_HIP_FORM_BRICKS)
+            return fail(c, ISAAC_HIP_ERR_ARG, "unknown render form");
+        if(brick_dtype != ISAAC_HIP_F32 && brick_dtype != ISAAC_HIP_F16 && brick_dtype != ISAAC_HIP_BF16)
+            return fail(c, ISAAC_HIP_ERR_ARG, "brick dtype must be ISAAC_HIP_F32, ISAAC_HIP_F16 or ISAAC_HIP_BF16");
+        c->form = form;
+        c->brickDtype = brick_dtype;
+        return 0;
+    }
+
+    int isaac_hip_get_bricks(
+        const isaac_hip_ctx* c,
+        int source,
+        const void** pairs,
+        const void** scales,
+        int64_t* nbricks,
+        int32_t* pair_bytes)
+    {
+        if(!c || source < 0 || source >= c->nv + c->nf)
+            return ISAAC_HIP_ERR_ARG;
+        const VolSlot& v = c->vols[source];
+        if(pairs)
+            *pairs = v.bricked ? v.brick : nullptr;
+        if(scales)
+            *scales = v.bricked ? v.brickScale : nullptr;
+        if(nbricks)
+            *nbricks = v.bricked ? v.nbricks : 0;
+        if(pair_bytes)
+            *pair_bytes = c->brickDtype == ISAAC_HIP_F32 ? 8 : 4;
+        return 0;
+    }
+
+    int isaac_hip_set_float_output(isaac_hip_ctx* c, int enabled)
+    {
+        if(!c)
+            return ISAAC_HIP_ERR_ARG;
+        if(enabled && !c->outFloat)
+        {
+            const size_t npx = size_t(c->cfg.framebuffer_size[0]) * size_t(c->cfg.framebuffer_size[1]);
+            HIPCHK(c, hipMalloc(&c->outFloat, npx * sizeof(float4)));
+            HIPCHK(c, launch_fill_u32(reinterpret_cast<uint32_t*>(c->outFloat), 0u, npx * 4, c->stream));
+        }
+        c->floatOut = enabled != 0;
+        return 0;
+    }
+
+    int isaac_hip_get_float_image(const isaac_hip_ctx* c, const void** rgba_f32)
+    {
+        if(!c || !rgba_f32)
+            return ISAAC_HIP_ERR_ARG;
+        *rgba_f32 = c->floatOut ? c->outFloat : nullptr;
         return 0;
     }
 

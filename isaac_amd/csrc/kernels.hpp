@@ -27,6 +27,29 @@ namespace isaac_hip
         hipEvent_t afterSolid,
         hipEvent_t afterSsao);
 
+    // ---- precision form: field bricks and the post-classification march
+    // src: local voxel (0,0,0) of an AoS float source (dim components, element strides);
+    // scale: nbricks*dim floats; out: nbricks*dim*648 pairs (float2 or 2 x 16 bit)
+    hipError_t launch_bricks(
+        const float* src,
+        const int64_t stride[3],
+        int dim,
+        int X,
+        int Y,
+        int Z,
+        int dtype,
+        float* scale,
+        void* out,
+        hipStream_t s);
+    hipError_t launch_brick_frame(
+        const FrameParams& F,
+        const PostParams& Q,
+        const ParticleParams& PP,
+        bool counting,
+        hipStream_t s,
+        hipEvent_t afterMarch,
+        hipEvent_t afterSsao);
+
     // ---- particle cell lists
     struct BinParams
     {
@@ -46,19 +69,19 @@ namespace isaac_hip
     struct BinBuffers
     {
         int32_t* cellOf; // n
+        uint32_t* rank; // n: arrival rank of the particle inside its cell
         float4* rel; // n: rel.xyz, lut index (or -1 if invisible) in .w
-        int32_t* count; // ncells
-        int32_t* vcount; // ncells
+        unsigned long long* count64; // ncells: all particles (low 32 bits), visible ones (high 32 bits)
+        int32_t* count; // ncells (all particles)
         int32_t* offset; // ncells + 1
         int32_t* voffset; // ncells + 1
-        int32_t* cursor; // ncells
         int32_t* perm; // n
         float4* recs; // visible records in cell order
         float* recRadius; // per record radius or nullptr
-        int32_t* scanTmp; // scan workspace (>= 2 * ceil(ncells/1024) + 64 ints)
+        unsigned long long* scanTmp; // scan workspace (bin_scan_workspace_elems entries)
     };
 
-    size_t bin_scan_workspace_ints(int64_t ncells);
+    size_t bin_scan_workspace_elems(int64_t ncells);
     hipError_t launch_bin(const BinParams& P, const BinBuffers& B, hipStream_t s);
     hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s);
 } // namespace isaac_hip

@@ -6,13 +6,17 @@
 // library builds those lists itself from raw particles:
 //
 //   1. bin_count   per particle: cell = clamp(floor(p / cw)), rel = p / cw - cell,
-//                  TF index of the functor-applied attribute, atomic per-cell counts
-//                  (all particles, and those whose TF alpha >= 0.5)
-//   2. scan        exclusive prefix sums of both count arrays (3-phase block scan)
-//   3. bin_scatter atomic slot claim inside each cell (arbitrary order)
-//   4. bin_sort    per cell: sort the claimed indices ascending, which makes the
-//                  permutation equal to a stable counting sort by cell (bit-exact),
-//                  then write the render records of the visible particles in that order
+//                  TF index of the functor-applied attribute; ONE 64-bit atomicAdd per
+//                  particle counts it (low 32 bits) and, when its TF alpha is >= 0.5,
+//                  counts it as visible (high 32 bits).  The returned low half is the
+//                  particle's arrival rank inside its cell.
+//   2. scan        exclusive prefix sum of the packed 64-bit counts: the low half of
+//                  the sum is the cell offset, the high half the visible-record offset
+//                  (the low half cannot carry into the high half below 2^32 particles)
+//   3. bin_scatter perm[offset[cell] + rank] = i, plain stores
+//   4. bin_sort    per cell: sort the cell's particle indices ascending, which makes
+//                  the permutation equal to a stable counting sort by cell
+//                  (bit-exact), then write the visible render records in that order
 //
 // A particle whose TF alpha is below 0.5 can never produce a hit in the render
 // kernel (the alpha test follows the geometric test and changes nothing when
@@ -92,23 +96,22 @@ namespace isaac_hip
             const bool visible = !P.filter || P.lut[li].w >= 0.5f;
             B.cellOf[i] = c;
             B.rel[i] = make_float4(qx - float(cx), qy - float(cy), qz - float(cz), __int_as_float(visible ? li : -1));
-            atomicAdd(&B.count[c], 1);
-            if(visible)
-                atomicAdd(&B.vcount[c], 1);
+            const unsigned long long inc = 1ull | (visible ? (1ull << 32) : 0ull);
+            B.rank[i] = uint32_t(atomicAdd(&B.count64[c], inc));
         }
     }
 
-    // ---- exclusive scan of int32 counts into offsets[0..n] (offsets[n] = total)
+    // ---- exclusive scan of packed 64-bit counts (3-phase, 1024-wide blocks)
     constexpr int kScanBlock = 1024;
 
-    __device__ inline int block_inclusive_scan(int v, int* sh)
+    __device__ inline unsigned long long block_inclusive_scan(unsigned long long v, unsigned long long* sh)
     {
         const int tid = threadIdx.x;
         sh[tid] = v;
         __syncthreads();
         for(int off = 1; off < kScanBlock; off <<= 1)
         {
-            int t = (tid >= off) ? sh[tid - off] : 0;
+            unsigned long long t = (tid >= off) ? sh[tid - off] : 0ull;
             __syncthreads();
             sh[tid] += t;
             __syncthreads();
@@ -116,26 +119,26 @@ namespace isaac_hip
         return sh[tid];
     }
 
-    __global__ __launch_bounds__(kScanBlock) void scan_reduce_kernel(const int32_t* in, int64_t n, int32_t* blockSums)
+    __global__ __launch_bounds__(kScanBlock) void scan_reduce_kernel(const unsigned long long* in, int64_t n, unsigned long long* blockSums)
     {
-        __shared__ int sh[kScanBlock];
+        __shared__ unsigned long long sh[kScanBlock];
         const int64_t i = int64_t(blockIdx.x) * kScanBlock + threadIdx.x;
-        int v = i < n ? in[i] : 0;
-        int s = block_inclusive_scan(v, sh);
+        const unsigned long long v = i < n ? in[i] : 0ull;
+        const unsigned long long s = block_inclusive_scan(v, sh);
         if(threadIdx.x == kScanBlock - 1)
             blockSums[blockIdx.x] = s;
     }
 
     // single workgroup: exclusive scan of the block sums, in place
-    __global__ __launch_bounds__(kScanBlock) void scan_blocksums_kernel(int32_t* blockSums, int nb)
+    __global__ __launch_bounds__(kScanBlock) void scan_blocksums_kernel(unsigned long long* blockSums, int nb)
     {
-        __shared__ int sh[kScanBlock];
-        int carry = 0;
+        __shared__ unsigned long long sh[kScanBlock];
+        unsigned long long carry = 0;
         for(int base = 0; base < nb; base += kScanBlock)
         {
             const int i = base + threadIdx.x;
-            int v = i < nb ? blockSums[i] : 0;
-            int s = block_inclusive_scan(v, sh);
+            const unsigned long long v = i < nb ? blockSums[i] : 0ull;
+            const unsigned long long s = block_inclusive_scan(v, sh);
             if(i < nb)
                 blockSums[i] = carry + s - v;
             __syncthreads();
@@ -144,31 +147,39 @@ namespace isaac_hip
         }
     }
 
+    // writes per-cell counts, cell offsets and visible-record offsets (ncells + 1 each)
     __global__ __launch_bounds__(kScanBlock) void scan_apply_kernel(
-        const int32_t* in,
+        const unsigned long long* in,
         int64_t n,
-        const int32_t* blockSums,
-        int32_t* out)
+        const unsigned long long* blockSums,
+        int32_t* count,
+        int32_t* offset,
+        int32_t* voffset)
     {
-        __shared__ int sh[kScanBlock];
+        __shared__ unsigned long long sh[kScanBlock];
         const int64_t i = int64_t(blockIdx.x) * kScanBlock + threadIdx.x;
-        int v = i < n ? in[i] : 0;
-        int s = block_inclusive_scan(v, sh);
-        const int base = blockSums[blockIdx.x];
+        const unsigned long long v = i < n ? in[i] : 0ull;
+        const unsigned long long s = block_inclusive_scan(v, sh);
+        const unsigned long long base = blockSums[blockIdx.x];
         if(i < n)
-            out[i] = base + s - v;
+        {
+            const unsigned long long ex = base + s - v;
+            count[i] = int32_t(uint32_t(v));
+            offset[i] = int32_t(uint32_t(ex));
+            voffset[i] = int32_t(uint32_t(ex >> 32));
+        }
         if(i == n - 1)
-            out[n] = base + s;
+        {
+            const unsigned long long tot = base + s;
+            offset[n] = int32_t(uint32_t(tot));
+            voffset[n] = int32_t(uint32_t(tot >> 32));
+        }
     }
 
     __global__ __launch_bounds__(256) void bin_scatter_kernel(const BinParams P, const BinBuffers B)
     {
         for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < P.n; i += int64_t(gridDim.x) * blockDim.x)
-        {
-            const int c = B.cellOf[i];
-            const int slot = B.offset[c] + atomicAdd(&B.cursor[c], 1);
-            B.perm[slot] = int32_t(i);
-        }
+            B.perm[B.offset[B.cellOf[i]] + int32_t(B.rank[i])] = int32_t(i);
     }
 
     // one thread per cell: ascending sort of the cell's particle indices, then
@@ -247,39 +258,27 @@ namespace isaac_hip
         return hipGetLastError();
     }
 
-    size_t bin_scan_workspace_ints(int64_t ncells)
+    size_t bin_scan_workspace_elems(int64_t ncells)
     {
         return size_t((ncells + kScanBlock - 1) / kScanBlock) + 64;
-    }
-
-    static hipError_t launch_scan(const int32_t* in, int64_t n, int32_t* out, int32_t* tmp, hipStream_t s)
-    {
-        const int nb = int((n + kScanBlock - 1) / kScanBlock);
-        hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(kScanBlock), 0, s, in, n, tmp);
-        hipLaunchKernelGGL(scan_blocksums_kernel, dim3(1), dim3(kScanBlock), 0, s, tmp, nb);
-        hipLaunchKernelGGL(scan_apply_kernel, dim3(nb), dim3(kScanBlock), 0, s, in, n, tmp, out);
-        return hipGetLastError();
     }
 
     hipError_t launch_bin(const BinParams& P, const BinBuffers& B, hipStream_t s)
     {
         const int64_t ncells = int64_t(P.grid[0]) * P.grid[1] * P.grid[2];
         hipError_t e;
-        if((e = launch_fill_u32(reinterpret_cast<uint32_t*>(B.count), 0u, size_t(ncells), s)) != hipSuccess)
-            return e;
-        if((e = launch_fill_u32(reinterpret_cast<uint32_t*>(B.vcount), 0u, size_t(ncells), s)) != hipSuccess)
-            return e;
-        if((e = launch_fill_u32(reinterpret_cast<uint32_t*>(B.cursor), 0u, size_t(ncells), s)) != hipSuccess)
+        if((e = launch_fill_u32(reinterpret_cast<uint32_t*>(B.count64), 0u, size_t(ncells) * 2, s)) != hipSuccess)
             return e;
         int64_t pb = (P.n + 255) / 256;
         if(pb > 8192)
             pb = 8192;
         if(P.n > 0)
             hipLaunchKernelGGL(bin_count_kernel, dim3(unsigned(pb)), dim3(256), 0, s, P, B);
-        if((e = launch_scan(B.count, ncells, B.offset, B.scanTmp, s)) != hipSuccess)
-            return e;
-        if((e = launch_scan(B.vcount, ncells, B.voffset, B.scanTmp, s)) != hipSuccess)
-            return e;
+        const int nb = int((ncells + kScanBlock - 1) / kScanBlock);
+        hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(kScanBlock), 0, s, B.count64, ncells, B.scanTmp);
+        hipLaunchKernelGGL(scan_blocksums_kernel, dim3(1), dim3(kScanBlock), 0, s, B.scanTmp, nb);
+        hipLaunchKernelGGL(scan_apply_kernel, dim3(nb), dim3(kScanBlock), 0, s, B.count64, ncells, B.scanTmp, B.count,
+                           B.offset, B.voffset);
         if(P.n > 0)
             hipLaunchKernelGGL(bin_scatter_kernel, dim3(unsigned(pb)), dim3(256), 0, s, P, B);
         int64_t cb = (ncells + 255) / 256;

@@ -441,6 +441,116 @@ namespace isaac_hip
         __syncthreads();
     }
 
+    // ------------------------------------------------------------------ particles
+    // ParticleRenderKernel (lib/isaac/isaac_particle_kernel.hpp:117-260): 3D-DDA over
+    // the particle cell grid from the clipped ray start; the first cell holding a
+    // visible sphere hit ends the walk (hit point of the nearest sphere in it).
+    template<bool COUNT>
+    __device__ inline void particle_dda(
+        const ParticleParams& PP,
+        const Ray& ray,
+        uint32_t& color,
+        float& depth,
+        F3& normal,
+        unsigned long long& nCells,
+        unsigned long long& nTests)
+    {
+        const F3 ps = f3(PP.pscale[0], PP.pscale[1], PP.pscale[2]);
+        const int ds[3] = {int(gsign(ray.dir.x)), int(gsign(ray.dir.y)), int(gsign(ray.dir.z))};
+        const F3 cur = add(ray.start, muls(ray.dir, ray.startDepth));
+        const F3 cdiv = divv(cur, ps);
+        uint32_t cell[3];
+        cell[0] = uint32_t(iclamp(int(cdiv.x), 0, PP.grid[0] - 1));
+        cell[1] = uint32_t(iclamp(int(cdiv.y), 0, PP.grid[1] - 1));
+        cell[2] = uint32_t(iclamp(int(cdiv.z), 0, PP.grid[2] - 1));
+        const float rayLength = ray.endDepth - ray.startDepth;
+        float tested = 0.0f;
+        const float curA[3] = {cur.x, cur.y, cur.z};
+        const float dA[3] = {ray.dir.x, ray.dir.y, ray.dir.z};
+        const float psA[3] = {ps.x, ps.y, ps.z};
+        float t[3], dt[3];
+#pragma unroll
+        for(int i = 0; i < 3; i++)
+        {
+            t[i] = ((float(cell[i]) + float(ds[i] > 0 ? ds[i] : 0)) * psA[i] - curA[i]) / dA[i];
+            dt[i] = psA[i] / dA[i] * float(ds[i]);
+            if(dA[i] == 0.0f)
+                t[i] = FLT_MAX;
+        }
+        bool hit = false;
+        float pdepth = FLT_MAX;
+        float4 pcol = make_float4(0.f, 0.f, 0.f, 0.f);
+        F3 pn = f3(0.f, 0.f, 0.f);
+        while(cell[0] < uint32_t(PP.grid[0]) && cell[1] < uint32_t(PP.grid[1]) && cell[2] < uint32_t(PP.grid[2])
+              && !hit && tested <= rayLength)
+        {
+            if(COUNT)
+                nCells++;
+            const int cid = int(cell[0] + uint32_t(PP.grid[0]) * (cell[1] + uint32_t(PP.grid[1]) * cell[2]));
+            const F3 cf = f3(float(cell[0]), float(cell[1]), float(cell[2]));
+            for(int s = 0; s < PP.nsrc; s++)
+            {
+                const ParticleSourceDev& S = PP.src[s];
+                const int k0 = S.voff[cid], k1 = S.voff[cid + 1];
+                for(int k = k0; k < k1; k++)
+                {
+                    if(COUNT)
+                        nTests++;
+                    const float4 rec = S.recs[k];
+                    const F3 ppos = mul(add(f3(rec.x, rec.y, rec.z), cf), ps);
+                    const F3 L = sub(ppos, cur);
+                    const float radius = (S.radius ? S.radius[k] : S.constRadius) * S.weight;
+                    const float r2 = radius * radius;
+                    const float tca = dot3(L, ray.dir);
+                    const float d2 = dot3(L, L) - tca * tca;
+                    if(d2 <= r2)
+                    {
+                        const float thc = sqrtf(r2 - d2);
+                        const float t0 = tca - thc;
+                        const float t1 = tca + thc;
+                        if(t1 >= 0.0f && t0 < pdepth)
+                        {
+                            const float4 val = S.lut[__float_as_int(rec.w)];
+                            if(val.w >= 0.5f)
+                            {
+                                pcol = val;
+                                pdepth = t0;
+                                hit = true;
+                                pn = sub(add(cur, muls(ray.dir, t0)), ppos);
+                                if(t0 < 0.0f && ray.isClipped)
+                                    pn = f3(-ray.clipN.x, -ray.clipN.y, -ray.clipN.z);
+                            }
+                        }
+                    }
+                }
+            }
+            if(t[0] < t[1] && t[0] < t[2])
+            {
+                cell[0] += uint32_t(ds[0]);
+                tested = t[0];
+                t[0] = t[0] + dt[0];
+            }
+            else if(t[1] < t[0] && t[1] < t[2])
+            {
+                cell[1] += uint32_t(ds[1]);
+                tested = t[1];
+                t[1] = t[1] + dt[1];
+            }
+            else
+            {
+                cell[2] += uint32_t(ds[2]);
+                tested = t[2];
+                t[2] = t[2] + dt[2];
+            }
+        }
+        if(hit)
+        {
+            normal = normalize3(pn);
+            color = to_byte4(pcol.x, pcol.y, pcol.z, 1.0f);
+            depth = pdepth + ray.startDepth;
+        }
+    }
+
     // ------------------------------------------------------------------ fused march
     // One pass per pixel: clear, particle DDA, then ONE traversal of the ray that
     // serves both the iso march (CombinedIsoRenderKernel) and the volume march
@@ -453,7 +563,7 @@ namespace isaac_hip
     // order (particles, iso, SSAO, shading, volume) for every pixel.
     // The taps of step i+1 are fetched before step i is composited (clamped
     // addresses, so a fetch past the last step stays in bounds).
-    template<bool LINEAR, bool COUNT>
+    template<bool LINEAR, bool COUNT, bool PREFETCH>
     __global__ __launch_bounds__(256) void march_kernel(const FrameParams F, const ParticleParams PP)
     {
         int px, py;
@@ -469,104 +579,11 @@ namespace isaac_hip
             float4 vcol = make_float4(0.f, 0.f, 0.f, -1.f);
             Ray ray = pixel_to_ray(F, px, py);
             const bool inside = clip_ray(F, ray);
-            // ---- particles (ParticleRenderKernel): 3D-DDA over the particle cell grid
+            // ---- particles
             if(inside && PP.nsrc > 0)
             {
                 nPartRays++;
-                const F3 ps = f3(PP.pscale[0], PP.pscale[1], PP.pscale[2]);
-                const int ds[3] = {int(gsign(ray.dir.x)), int(gsign(ray.dir.y)), int(gsign(ray.dir.z))};
-                const F3 cur = add(ray.start, muls(ray.dir, ray.startDepth));
-                const F3 cdiv = divv(cur, ps);
-                uint32_t cell[3];
-                cell[0] = uint32_t(iclamp(int(cdiv.x), 0, PP.grid[0] - 1));
-                cell[1] = uint32_t(iclamp(int(cdiv.y), 0, PP.grid[1] - 1));
-                cell[2] = uint32_t(iclamp(int(cdiv.z), 0, PP.grid[2] - 1));
-                const float rayLength = ray.endDepth - ray.startDepth;
-                float tested = 0.0f;
-                const float curA[3] = {cur.x, cur.y, cur.z};
-                const float dA[3] = {ray.dir.x, ray.dir.y, ray.dir.z};
-                const float psA[3] = {ps.x, ps.y, ps.z};
-                float t[3], dt[3];
-#pragma unroll
-                for(int i = 0; i < 3; i++)
-                {
-                    t[i] = ((float(cell[i]) + float(ds[i] > 0 ? ds[i] : 0)) * psA[i] - curA[i]) / dA[i];
-                    dt[i] = psA[i] / dA[i] * float(ds[i]);
-                    if(dA[i] == 0.0f)
-                        t[i] = FLT_MAX;
-                }
-                bool hit = false;
-                float pdepth = FLT_MAX;
-                float4 pcol = make_float4(0.f, 0.f, 0.f, 0.f);
-                F3 pn = f3(0.f, 0.f, 0.f);
-                while(cell[0] < uint32_t(PP.grid[0]) && cell[1] < uint32_t(PP.grid[1]) && cell[2] < uint32_t(PP.grid[2])
-                      && !hit && tested <= rayLength)
-                {
-                    if(COUNT)
-                        nCells++;
-                    const int cid = int(cell[0] + uint32_t(PP.grid[0]) * (cell[1] + uint32_t(PP.grid[1]) * cell[2]));
-                    const F3 cf = f3(float(cell[0]), float(cell[1]), float(cell[2]));
-                    for(int s = 0; s < PP.nsrc; s++)
-                    {
-                        const ParticleSourceDev& S = PP.src[s];
-                        const int k0 = S.voff[cid], k1 = S.voff[cid + 1];
-                        for(int k = k0; k < k1; k++)
-                        {
-                            if(COUNT)
-                                nTests++;
-                            const float4 rec = S.recs[k];
-                            const F3 ppos = mul(add(f3(rec.x, rec.y, rec.z), cf), ps);
-                            const F3 L = sub(ppos, cur);
-                            const float radius = (S.radius ? S.radius[k] : S.constRadius) * S.weight;
-                            const float r2 = radius * radius;
-                            const float tca = dot3(L, ray.dir);
-                            const float d2 = dot3(L, L) - tca * tca;
-                            if(d2 <= r2)
-                            {
-                                const float thc = sqrtf(r2 - d2);
-                                const float t0 = tca - thc;
-                                const float t1 = tca + thc;
-                                if(t1 >= 0.0f && t0 < pdepth)
-                                {
-                                    const float4 val = S.lut[__float_as_int(rec.w)];
-                                    if(val.w >= 0.5f)
-                                    {
-                                        pcol = val;
-                                        pdepth = t0;
-                                        hit = true;
-                                        pn = sub(add(cur, muls(ray.dir, t0)), ppos);
-                                        if(t0 < 0.0f && ray.isClipped)
-                                            pn = f3(-ray.clipN.x, -ray.clipN.y, -ray.clipN.z);
-                                    }
-                                }
-                            }
-                        }
-                    }
-                    if(t[0] < t[1] && t[0] < t[2])
-                    {
-                        cell[0] += uint32_t(ds[0]);
-                        tested = t[0];
-                        t[0] = t[0] + dt[0];
-                    }
-                    else if(t[1] < t[0] && t[1] < t[2])
-                    {
-                        cell[1] += uint32_t(ds[1]);
-                        tested = t[1];
-                        t[1] = t[1] + dt[1];
-                    }
-                    else
-                    {
-                        cell[2] += uint32_t(ds[2]);
-                        tested = t[2];
-                        t[2] = t[2] + dt[2];
-                    }
-                }
-                if(hit)
-                {
-                    normal = normalize3(pn);
-                    color = to_byte4(pcol.x, pcol.y, pcol.z, 1.0f);
-                    depth = pdepth + ray.startDepth;
-                }
+                particle_dda<COUNT>(PP, ray, color, depth, normal, nCells, nTests);
             }
             // ---- iso + volume, one traversal
             const int mode = F.renderMode;
@@ -592,6 +609,7 @@ namespace isaac_hip
                         nIsoRays++;
                     float cr = 0.f, cg = 0.f, cb = 0.f, ca = 0.f;
                     float oldValue = 0.0f;
+                    bool isoHit = false, isoFirst = false;
                     int i = startSteps;
                     int last = isoOn ? endIso : endVol;
                     if(i <= last)
@@ -602,7 +620,7 @@ namespace isaac_hip
                         for(;;)
                         {
                             Taps nxt;
-                            const bool ahead = i + 1 <= last;
+                            const bool ahead = PREFETCH && i + 1 <= last;
                             if(ahead)
                             {
                                 const float fn = float(i + 1);
@@ -625,23 +643,9 @@ namespace isaac_hip
                                         d = ray.startDepth;
                                     else
                                         d = tt + stepU * (0.5f - tmp) / (value - tmp);
-                                    const F3 pu = divv(add(ray.start, muls(ray.dir, d)), sc);
-                                    Taps h;
-                                    fetch_taps<LINEAR>(F.tex, F.X, F.Y, F.Z, pu.x, pu.y, pu.z, h);
-                                    const float hr = channel<LINEAR, true>(h, 0), hg = channel<LINEAR, true>(h, 8);
-                                    const float hb = channel<LINEAR, true>(h, 16);
-                                    F3 g;
-                                    g.x = iso_alpha_at<LINEAR>(F, pu.x + 1.0f, pu.y, pu.z) - iso_alpha_at<LINEAR>(F, pu.x - 1.0f, pu.y, pu.z);
-                                    g.y = iso_alpha_at<LINEAR>(F, pu.x, pu.y + 1.0f, pu.z) - iso_alpha_at<LINEAR>(F, pu.x, pu.y - 1.0f, pu.z);
-                                    g.z = iso_alpha_at<LINEAR>(F, pu.x, pu.y, pu.z + 1.0f) - iso_alpha_at<LINEAR>(F, pu.x, pu.y, pu.z - 1.0f);
-                                    float gl = len3(g);
-                                    if(first || gl == 0.0f)
-                                    {
-                                        g = ray.clipN;
-                                        gl = 1.0f;
-                                    }
-                                    normal = f3(-g.x / gl, -g.y / gl, -g.z / gl);
-                                    color = to_byte4(hr, hg, hb, 1.0f);
+                                    // colour and gradient at the hit are sampled after the loop
+                                    isoHit = true;
+                                    isoFirst = first;
                                     depth = d;
                                     if(COUNT)
                                         nIsoHits++;
@@ -675,9 +679,35 @@ namespace isaac_hip
                                 last = endVol;
                             if(i + 1 > last)
                                 break;
-                            cur = nxt;
                             i++;
+                            if(PREFETCH)
+                                cur = nxt;
+                            else
+                                fetch_taps<LINEAR>(F.tex, F.X, F.Y, F.Z, startU.x + stepVec.x * float(i),
+                                                   startU.y + stepVec.y * float(i), startU.z + stepVec.z * float(i), cur);
                         }
+                    }
+                    if(isoHit)
+                    {
+                        // hit colour and central-difference gradient of the iso alpha
+                        // (lib/isaac/isaac_iso_kernel.hpp:97-119)
+                        const F3 pu = divv(add(ray.start, muls(ray.dir, depth)), sc);
+                        Taps h;
+                        fetch_taps<LINEAR>(F.tex, F.X, F.Y, F.Z, pu.x, pu.y, pu.z, h);
+                        const float hr = channel<LINEAR, true>(h, 0), hg = channel<LINEAR, true>(h, 8);
+                        const float hb = channel<LINEAR, true>(h, 16);
+                        F3 g;
+                        g.x = iso_alpha_at<LINEAR>(F, pu.x + 1.0f, pu.y, pu.z) - iso_alpha_at<LINEAR>(F, pu.x - 1.0f, pu.y, pu.z);
+                        g.y = iso_alpha_at<LINEAR>(F, pu.x, pu.y + 1.0f, pu.z) - iso_alpha_at<LINEAR>(F, pu.x, pu.y - 1.0f, pu.z);
+                        g.z = iso_alpha_at<LINEAR>(F, pu.x, pu.y, pu.z + 1.0f) - iso_alpha_at<LINEAR>(F, pu.x, pu.y, pu.z - 1.0f);
+                        float gl = len3(g);
+                        if(isoFirst || gl == 0.0f)
+                        {
+                            g = ray.clipN;
+                            gl = 1.0f;
+                        }
+                        normal = f3(-g.x / gl, -g.y / gl, -g.z / gl);
+                        color = to_byte4(hr, hg, hb, 1.0f);
                     }
                     if(volBlend)
                     {
@@ -808,16 +838,593 @@ namespace isaac_hip
         }
         uint32_t out = solid;
         const float4 v = F.volColor[p];
+        const float sr = div255(float(solid & 0xffu)), sg = div255(float((solid >> 8) & 0xffu));
+        const float sb = div255(float((solid >> 16) & 0xffu)), sa = div255(float(solid >> 24));
+        float4 fo = make_float4(sr, sg, sb, sa);
         if(v.w >= 0.0f)
         {
-            const float sr = div255(float(solid & 0xffu)), sg = div255(float((solid >> 8) & 0xffu));
-            const float sb = div255(float((solid >> 16) & 0xffu)), sa = div255(float(solid >> 24));
             const float om = 1.0f - v.w;
-            out = to_byte4(v.x + om * sr, v.y + om * sg, v.z + om * sb, v.w + om * sa);
+            fo = make_float4(v.x + om * sr, v.y + om * sg, v.z + om * sb, v.w + om * sa);
+            out = to_byte4(fo.x, fo.y, fo.z, fo.w);
         }
+        if(F.outFloat)
+            F.outFloat[p] = make_float4(gclamp(fo.x, 0.f, 1.f), gclamp(fo.y, 0.f, 1.f), gclamp(fo.z, 0.f, 1.f), gclamp(fo.w, 0.f, 1.f));
         F.color[p] = out;
         if(depthChanged)
             F.depth[p] = depth;
+    }
+
+    // ------------------------------------------------------------------ precision form
+    // Field bricks (FP32 / FP16 / BF16 + FP32 scale) rendered with
+    // post-classification: interpolate the raw field, then functor chain and
+    // transfer function per sample (the semantics of the reference's legacy
+    // renderer, lib/isaac/isaac_volume_kernel.hpp:101-168), blended across
+    // sources as the classification pass does (lib/isaac/isaac_common_kernel.hpp:574-612)
+    // but without the RGBA8 quantisation.  The ray setup (pixelToRay + clipRay)
+    // runs in binary64; interpolation and compositing accumulate in binary32.
+
+    struct RayD
+    {
+        double dir[3], start[3];
+        double startDepth, endDepth;
+        bool isClipped;
+        double clipN[3];
+    };
+
+    __device__ inline bool ray_setup_d(const FrameParams& F, const PostParams& Q, int px, int py, RayD& r)
+    {
+        const double vx = double(px) / double(F.fbW) * 2.0 - 1.0;
+        const double vy = double(py) / double(F.fbH) * 2.0 - 1.0;
+        double s[4], e[4];
+        for(int k = 0; k < 4; k++)
+        {
+            const double s0 = Q.invd[0 * 4 + k] * vx + Q.invd[1 * 4 + k] * vy;
+            s[k] = s0 + (Q.invd[3 * 4 + k] - Q.invd[2 * 4 + k]);
+            e[k] = s0 + (Q.invd[3 * 4 + k] + Q.invd[2 * 4 + k]);
+        }
+        double d[3], len2 = 0.0;
+        for(int k = 0; k < 3; k++)
+        {
+            r.start[k] = s[k] / s[3] * Q.maxSized;
+            d[k] = e[k] / e[3] * Q.maxSized - r.start[k];
+            len2 += d[k] * d[k];
+        }
+        const double il = 1.0 / sqrt(len2);
+        double bmin[3], bmax[3];
+        for(int k = 0; k < 3; k++)
+        {
+            r.dir[k] = d[k] * il;
+            r.start[k] += double(F.off[k]);
+            bmin[k] = (double(F.bbMin[k]) - r.start[k]) / r.dir[k];
+            bmax[k] = (double(F.bbMax[k]) - r.start[k]) / r.dir[k];
+            if(bmax[k] < bmin[k])
+            {
+                const double t = bmax[k];
+                bmax[k] = bmin[k];
+                bmin[k] = t;
+            }
+            r.clipN[k] = 0.0;
+        }
+        r.isClipped = false;
+        r.startDepth = fmax(bmin[0], fmax(bmin[1], bmin[2]));
+        r.endDepth = fmin(bmax[0], fmin(bmax[1], bmax[2]));
+        for(int k = 0; k < 3; k++)
+        {
+            const double sgn = (r.dir[k] > 0.0) ? 1.0 : ((r.dir[k] < 0.0) ? -1.0 : 0.0);
+            if(bmin[k] == r.startDepth && ((F.outerLow[k] && sgn + 1.0 != 0.0) || (F.outerHigh[k] && sgn - 1.0 != 0.0)))
+            {
+                r.isClipped = true;
+                r.clipN[k] = sgn;
+            }
+        }
+        for(int i = 0; i < F.nclip; i++)
+        {
+            double dn = 0.0, pn = 0.0, sn = 0.0;
+            for(int k = 0; k < 3; k++)
+            {
+                dn += r.dir[k] * double(F.cnorm[i][k]);
+                pn += Q.cposd[i][k] * double(F.cnorm[i][k]);
+                sn += r.start[k] * double(F.cnorm[i][k]);
+            }
+            const double t = (pn - sn) / dn;
+            if(dn > 0.0)
+            {
+                if(r.endDepth < t)
+                    return false;
+                if(r.startDepth <= t)
+                {
+                    for(int k = 0; k < 3; k++)
+                        r.clipN[k] = double(F.cnorm[i][k]);
+                    r.isClipped = true;
+                    r.startDepth = t;
+                }
+            }
+            else
+            {
+                if(r.startDepth > t)
+                    return false;
+                if(r.endDepth > t)
+                    r.endDepth = t;
+            }
+        }
+        r.startDepth = fmax(r.startDepth, double(kZNear));
+        return !(r.startDepth > r.endDepth || isinf(r.startDepth) || isinf(r.endDepth));
+    }
+
+    __device__ inline float half_lo(uint32_t v)
+    {
+        return __half2float(__ushort_as_half((unsigned short) (v & 0xffffu)));
+    }
+    __device__ inline float half_hi(uint32_t v)
+    {
+        return __half2float(__ushort_as_half((unsigned short) (v >> 16)));
+    }
+
+    // pair (v[x], v[x+1]) of a brick at pair index idx, still divided by the brick scale
+    template<int DT>
+    __device__ inline void load_pair(const void* data, uint32_t idx, float& a, float& b)
+    {
+        if(DT == kDtF32)
+        {
+            const float2 v = reinterpret_cast<const float2*>(data)[idx];
+            a = v.x;
+            b = v.y;
+        }
+        else
+        {
+            const uint32_t v = reinterpret_cast<const uint32_t*>(data)[idx];
+            if(DT == kDtF16)
+            {
+                a = half_lo(v);
+                b = half_hi(v);
+            }
+            else
+            {
+                a = __uint_as_float(v << 16);
+                b = __uint_as_float(v & 0xffff0000u);
+            }
+        }
+    }
+
+    // raw sample of every component (Sampler LINEAR|NEAREST, CLAMP addressing,
+    // lib/isaac/isaac_texture.hpp:182-259).  The taps of a sample all come from the
+    // brick holding its clamped (x0, y0, z0) tap; x1 = x0 (clamped at the edge)
+    // duplicates the first element of each pair.  Component c of a brick sits
+    // c * 648 pairs after component 0.
+    template<int DT, bool LINEAR>
+    __device__ inline void brick_sample(
+        const PostParams& Q, const BrickSource& S, int X, int Y, int Z, float px, float py, float pz, float v[4])
+    {
+        if(!LINEAR)
+        {
+            const uint32_t x = iclamp(int(px), 0, X - 1), y = iclamp(int(py), 0, Y - 1), z = iclamp(int(pz), 0, Z - 1);
+            const uint32_t brick = ((z >> 3) * uint32_t(Q.nby) + (y >> 3)) * uint32_t(Q.nbx) + (x >> 3);
+            const uint32_t bc0 = brick * uint32_t(S.dim);
+            const uint32_t idx = bc0 * kBrickPairs + ((z & 7) * 9 + (y & 7)) * 8 + (x & 7);
+            for(int c = 0; c < S.dim; c++)
+            {
+                float a, b;
+                load_pair<DT>(S.data, idx + uint32_t(c) * kBrickPairs, a, b);
+                v[c] = a * S.scale[bc0 + c];
+            }
+            return;
+        }
+        const float cx = px - 0.5f, cy = py - 0.5f, cz = pz - 0.5f;
+        const float flx = floorf(cx), fly = floorf(cy), flz = floorf(cz);
+        const float fx = cx - flx, fy = cy - fly, fz = cz - flz;
+        const int bx = int(flx), by = int(fly), bz = int(flz);
+        const uint32_t x0 = iclamp(bx, 0, X - 1), x1 = iclamp(bx + 1, 0, X - 1);
+        const uint32_t y0 = iclamp(by, 0, Y - 1), y1 = iclamp(by + 1, 0, Y - 1);
+        const uint32_t z0 = iclamp(bz, 0, Z - 1), z1 = iclamp(bz + 1, 0, Z - 1);
+        const uint32_t Bx = x0 >> 3, By = y0 >> 3, Bz = z0 >> 3;
+        const uint32_t lx = x0 - (Bx << 3);
+        const uint32_t ly0 = y0 - (By << 3), ly1 = y1 - (By << 3);
+        const uint32_t lz0 = z0 - (Bz << 3), lz1 = z1 - (Bz << 3);
+        const uint32_t bc0 = ((Bz * uint32_t(Q.nby) + By) * uint32_t(Q.nbx) + Bx) * uint32_t(S.dim);
+        const uint32_t i00 = bc0 * kBrickPairs + lx + (lz0 * 9 + ly0) * 8, i10 = bc0 * kBrickPairs + lx + (lz0 * 9 + ly1) * 8;
+        const uint32_t i01 = bc0 * kBrickPairs + lx + (lz1 * 9 + ly0) * 8, i11 = bc0 * kBrickPairs + lx + (lz1 * 9 + ly1) * 8;
+        const bool dup = x1 == x0;
+        const float oz = 1.0f - fz, oy = 1.0f - fy, ox = 1.0f - fx;
+        for(int c = 0; c < S.dim; c++)
+        {
+            const uint32_t co = uint32_t(c) * kBrickPairs;
+            float d000, d100, d010, d110, d001, d101, d011, d111;
+            load_pair<DT>(S.data, i00 + co, d000, d100);
+            load_pair<DT>(S.data, i10 + co, d010, d110);
+            load_pair<DT>(S.data, i01 + co, d001, d101);
+            load_pair<DT>(S.data, i11 + co, d011, d111);
+            if(dup)
+            {
+                d100 = d000;
+                d110 = d010;
+                d101 = d001;
+                d111 = d011;
+            }
+            const float a0 = d000 * oz + d001 * fz, a1 = d010 * oz + d011 * fz;
+            const float A = a0 * oy + a1 * fy;
+            const float b0 = d100 * oz + d101 * fz, b1 = d110 * oz + d111 * fz;
+            const float B = b0 * oy + b1 * fy;
+            v[c] = (A * ox + B * fx) * S.scale[bc0 + c];
+        }
+    }
+
+    struct PostSample
+    {
+        float r, g, b, a; // premultiplied, normalised volume colour
+        float ir, ig, ib, ia; // iso colour and alpha (alpha / threshold * 0.5, maximum over sources)
+    };
+
+    template<int DT, bool LINEAR>
+    __device__ inline PostSample post_classify(const PostParams& Q, const float4* lut, int X, int Y, int Z, float px, float py, float pz)
+    {
+        PostSample o{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        for(int s = 0; s < Q.nsrc; s++)
+        {
+            const BrickSource& S = Q.src[s];
+            if(!S.active)
+                continue;
+            float v[4] = {0.f, 0.f, 0.f, 0.f};
+            brick_sample<DT, LINEAR>(Q, S, X, Y, Z, px, py, pz, v);
+            const float tv = apply_chain(S.chain, v[0], v[1], v[2], v[3], S.dim);
+            const float4 col = lut[S.lutSlot * Q.T + tf_index(tv, Q.T)];
+            if(S.weight > 0.0f)
+            {
+                const float a = col.w * S.weight;
+                o.r = o.r + col.x * a;
+                o.g = o.g + col.y * a;
+                o.b = o.b + col.z * a;
+                o.a = o.a + a;
+            }
+            if(S.iso > 0.0f)
+            {
+                const float a = col.w / S.iso * 0.5f;
+                if(o.ia < a)
+                {
+                    o.ir = col.x;
+                    o.ig = col.y;
+                    o.ib = col.z;
+                    o.ia = a;
+                }
+            }
+        }
+        if(Q.totalWeight != 0.0f)
+        {
+            o.r = o.r / Q.totalWeight;
+            o.g = o.g / Q.totalWeight;
+            o.b = o.b / Q.totalWeight;
+            o.a = o.a / Q.totalWeight;
+        }
+        else
+            o.r = o.g = o.b = o.a = 0.0f;
+        return o;
+    }
+
+    // Post-classification march over field bricks.  Same pass structure as
+    // march_kernel (clear, particle DDA, one traversal serving iso and volume,
+    // G-buffer + premultiplied volume colour out); sample positions
+    // startU + stepVec*i with startSteps / end steps taken from the binary64 ray.
+    template<int DT, bool LINEAR, bool COUNT>
+    __global__ __launch_bounds__(256) void brick_march_kernel(const FrameParams F, const PostParams Q, const ParticleParams PP)
+    {
+        extern __shared__ float4 lds_lut[];
+        for(int i = threadIdx.x; i < Q.nLut * Q.T; i += blockDim.x)
+            lds_lut[i] = Q.luts[i];
+        __syncthreads();
+        const float4* lut = Q.nLut > 0 ? lds_lut : Q.luts;
+        int px, py;
+        const bool valid = tile_pixel(F, px, py);
+        unsigned long long nVol = 0, nIso = 0, nIsoHits = 0, nVolRays = 0, nIsoRays = 0;
+        unsigned long long nCells = 0, nTests = 0, nPartRays = 0;
+        if(valid)
+        {
+            const size_t p = size_t(py) * size_t(F.W) + size_t(px);
+            uint32_t color = F.bgClear;
+            F3 normal = f3(0.f, 0.f, 0.f);
+            float depth = FLT_MAX;
+            float4 vcol = make_float4(0.f, 0.f, 0.f, -1.f);
+            RayD rd;
+            const bool inside = ray_setup_d(F, Q, px, py, rd);
+            Ray ray;
+            ray.dir = f3(float(rd.dir[0]), float(rd.dir[1]), float(rd.dir[2]));
+            ray.start = f3(float(rd.start[0]), float(rd.start[1]), float(rd.start[2]));
+            ray.startDepth = float(rd.startDepth);
+            ray.endDepth = float(rd.endDepth);
+            ray.isClipped = rd.isClipped;
+            ray.clipN = f3(float(rd.clipN[0]), float(rd.clipN[1]), float(rd.clipN[2]));
+            if(inside && PP.nsrc > 0)
+            {
+                nPartRays++;
+                particle_dda<COUNT>(PP, ray, color, depth, normal, nCells, nTests);
+            }
+            const int mode = F.renderMode;
+            const bool depthZeroed = mode == 2 || (mode >= 4 && mode <= 8);
+            if(inside)
+            {
+                const double endD = fmin(rd.endDepth, double(depth));
+                bool isoOn = F.isoEnabled && endD > rd.startDepth;
+                bool volBlend = F.volumeEnabled && !depthZeroed && endD > rd.startDepth;
+                if(isoOn || volBlend)
+                {
+                    const F3 sc = f3(F.scale[0], F.scale[1], F.scale[2]);
+                    const double ex = rd.dir[0] / double(sc.x), ey = rd.dir[1] / double(sc.y), ez = rd.dir[2] / double(sc.z);
+                    const double stepUd = double(F.step)
+                        * (sqrt(rd.dir[0] * rd.dir[0] + rd.dir[1] * rd.dir[1] + rd.dir[2] * rd.dir[2]) / sqrt(ex * ex + ey * ey + ez * ez));
+                    const float stepU = float(stepUd);
+                    const float factor = stepU / F.minSize * 2.0f * F.totalWeightAll;
+                    const int startSteps = int(ceil(rd.startDepth / stepUd));
+                    const int endIso = int(floor(endD / stepUd));
+                    int endVol = endIso;
+                    const F3 stepVec = divv(muls(ray.dir, stepU), sc);
+                    const F3 startU = divv(ray.start, sc);
+                    bool volOn = volBlend;
+                    if(isoOn)
+                        nIsoRays++;
+                    float cr = 0.f, cg = 0.f, cb = 0.f, ca = 0.f;
+                    float oldValue = 0.0f;
+                    bool isoHit = false, isoFirst = false;
+                    for(int i = startSteps;; i++)
+                    {
+                        int last = isoOn ? endIso : -0x7fffffff;
+                        if(volOn && endVol > last)
+                            last = endVol;
+                        if(i > last)
+                            break;
+                        const float fi = float(i);
+                        const PostSample sm = post_classify<DT, LINEAR>(
+                            Q, lut, F.X, F.Y, F.Z, startU.x + stepVec.x * fi, startU.y + stepVec.y * fi, startU.z + stepVec.z * fi);
+                        if(isoOn)
+                        {
+                            const float value = gmin(sm.ia, 1.0f);
+                            if(COUNT)
+                                nIso++;
+                            const float tmp = oldValue;
+                            oldValue = value;
+                            if(value >= 0.5f)
+                            {
+                                isoFirst = ray.isClipped && i == startSteps;
+                                depth = isoFirst ? ray.startDepth : fi * stepU + stepU * (0.5f - tmp) / (value - tmp);
+                                isoHit = true;
+                                isoOn = false;
+                                if(COUNT)
+                                    nIsoHits++;
+                                const double ve = fmin(rd.endDepth, double(depth));
+                                if(!(ve > rd.startDepth))
+                                    volBlend = volOn = false;
+                                else
+                                    endVol = int(floor(ve / stepUd));
+                            }
+                        }
+                        if(volOn && i <= endVol)
+                        {
+                            if(COUNT)
+                                nVol++;
+                            const float w = 1.0f - ca;
+                            cr = cr + w * (sm.r * factor);
+                            cg = cg + w * (sm.g * factor);
+                            cb = cb + w * (sm.b * factor);
+                            ca = ca + w * (sm.a * factor);
+                            if(ca > 0.99f)
+                                volOn = false;
+                        }
+                    }
+                    if(isoHit)
+                    {
+                        const F3 pu = divv(add(ray.start, muls(ray.dir, depth)), sc);
+                        const PostSample h = post_classify<DT, LINEAR>(Q, lut, F.X, F.Y, F.Z, pu.x, pu.y, pu.z);
+                        auto ia = [&](float x, float y, float z) {
+                            return gmin(post_classify<DT, LINEAR>(Q, lut, F.X, F.Y, F.Z, x, y, z).ia, 1.0f);
+                        };
+                        F3 g;
+                        g.x = ia(pu.x + 1.0f, pu.y, pu.z) - ia(pu.x - 1.0f, pu.y, pu.z);
+                        g.y = ia(pu.x, pu.y + 1.0f, pu.z) - ia(pu.x, pu.y - 1.0f, pu.z);
+                        g.z = ia(pu.x, pu.y, pu.z + 1.0f) - ia(pu.x, pu.y, pu.z - 1.0f);
+                        float gl = len3(g);
+                        if(isoFirst || gl == 0.0f)
+                        {
+                            g = ray.clipN;
+                            gl = 1.0f;
+                        }
+                        normal = f3(-g.x / gl, -g.y / gl, -g.z / gl);
+                        color = to_byte4(h.ir, h.ig, h.ib, 1.0f);
+                    }
+                    if(volBlend)
+                    {
+                        nVolRays++;
+                        vcol = make_float4(cr, cg, cb, ca);
+                    }
+                }
+            }
+            F.color[p] = color;
+            F.depth[p] = depth;
+            F.normal[3 * p + 0] = normal.x;
+            F.normal[3 * p + 1] = normal.y;
+            F.normal[3 * p + 2] = normal.z;
+            F.volColor[p] = vcol;
+            if(!F.aoEnabled)
+                F.ao[p] = 0.0f;
+        }
+        if(COUNT)
+        {
+            block_count_add(F.counters + 0, nVol);
+            block_count_add(F.counters + 1, nIso);
+            block_count_add(F.counters + 2, nIsoHits);
+            block_count_add(F.counters + 3, nCells);
+            block_count_add(F.counters + 4, nTests);
+            block_count_add(F.counters + 5, nVolRays);
+            block_count_add(F.counters + 6, nIsoRays);
+            block_count_add(F.counters + 7, nPartRays);
+        }
+    }
+
+    // per brick and component: scale = max |v| over the 9^3 region (1 if all zero)
+    __global__ __launch_bounds__(256) void brick_scale_kernel(
+        const float* src,
+        int64_t sx,
+        int64_t sy,
+        int64_t sz,
+        int dim,
+        int X,
+        int Y,
+        int Z,
+        int nbx,
+        int nby,
+        float* scale)
+    {
+        __shared__ float red[256];
+        const int bc = blockIdx.x;
+        const int brick = bc / dim, c = bc % dim;
+        const int Bx = brick % nbx, By = (brick / nbx) % nby, Bz = brick / (nbx * nby);
+        float m = 0.0f;
+        for(int k = threadIdx.x; k < 729; k += 256)
+        {
+            const int lx = k % 9, ly = (k / 9) % 9, lz = k / 81;
+            const int x = min(Bx * 8 + lx, X - 1), y = min(By * 8 + ly, Y - 1), z = min(Bz * 8 + lz, Z - 1);
+            m = fmaxf(m, fabsf(src[(int64_t(x) * sx + int64_t(y) * sy + int64_t(z) * sz) * dim + c]));
+        }
+        red[threadIdx.x] = m;
+        __syncthreads();
+        for(int off = 128; off > 0; off >>= 1)
+        {
+            if(int(threadIdx.x) < off)
+                red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + off]);
+            __syncthreads();
+        }
+        if(threadIdx.x == 0)
+            scale[bc] = red[0] > 0.0f ? red[0] : 1.0f;
+    }
+
+    __device__ inline uint32_t to_bf16_rne(float f)
+    {
+        const uint32_t u = __float_as_uint(f);
+        if((u & 0x7f800000u) == 0x7f800000u)
+            return (u >> 16) | ((u & 0xffffu) ? 0x40u : 0u); // inf / nan
+        return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+    }
+
+    template<int DT>
+    __global__ __launch_bounds__(256) void brick_encode_kernel(
+        const float* src,
+        int64_t sx,
+        int64_t sy,
+        int64_t sz,
+        int dim,
+        int X,
+        int Y,
+        int Z,
+        int nbx,
+        int nby,
+        int64_t nbricks,
+        const float* scale,
+        void* out)
+    {
+        const int64_t total = nbricks * dim * kBrickPairs;
+        for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += int64_t(gridDim.x) * blockDim.x)
+        {
+            const int64_t bc = i / kBrickPairs;
+            const int pi = int(i - bc * kBrickPairs);
+            const int64_t brick = bc / dim;
+            const int c = int(bc - brick * dim);
+            const int lx = pi % 8, ly = (pi / 8) % 9, lz = pi / 72;
+            const int Bx = int(brick % nbx), By = int((brick / nbx) % nby), Bz = int(brick / (int64_t(nbx) * nby));
+            const int x0 = min(Bx * 8 + lx, X - 1), x1 = min(Bx * 8 + lx + 1, X - 1);
+            const int y = min(By * 8 + ly, Y - 1), z = min(Bz * 8 + lz, Z - 1);
+            const int64_t row = int64_t(y) * sy + int64_t(z) * sz;
+            const float v0 = src[(int64_t(x0) * sx + row) * dim + c];
+            const float v1 = src[(int64_t(x1) * sx + row) * dim + c];
+            if(DT == kDtF32)
+                reinterpret_cast<float2*>(out)[i] = make_float2(v0, v1);
+            else
+            {
+                const float s = scale[bc];
+                const float q0 = v0 / s, q1 = v1 / s;
+                uint32_t w;
+                if(DT == kDtF16)
+                    w = uint32_t(__half_as_ushort(__float2half_rn(q0))) | (uint32_t(__half_as_ushort(__float2half_rn(q1))) << 16);
+                else
+                    w = to_bf16_rne(q0) | (to_bf16_rne(q1) << 16);
+                reinterpret_cast<uint32_t*>(out)[i] = w;
+            }
+        }
+    }
+
+    hipError_t launch_bricks(
+        const float* src,
+        const int64_t stride[3],
+        int dim,
+        int X,
+        int Y,
+        int Z,
+        int dtype,
+        float* scale,
+        void* out,
+        hipStream_t s)
+    {
+        const int nbx = (X + 7) / 8, nby = (Y + 7) / 8, nbz = (Z + 7) / 8;
+        const int64_t nbricks = int64_t(nbx) * nby * nbz;
+        if(dtype == kDtF32)
+        {
+            const hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(scale), 0x3f800000u, size_t(nbricks * dim), s);
+            if(e != hipSuccess)
+                return e;
+        }
+        else
+            hipLaunchKernelGGL(brick_scale_kernel, dim3(unsigned(nbricks * dim)), dim3(256), 0, s, src, stride[0], stride[1],
+                               stride[2], dim, X, Y, Z, nbx, nby, scale);
+        int64_t blocks = (nbricks * dim * kBrickPairs + 255) / 256;
+        if(blocks > 16384)
+            blocks = 16384;
+        if(dtype == kDtF32)
+            hipLaunchKernelGGL((brick_encode_kernel<kDtF32>), dim3(unsigned(blocks)), dim3(256), 0, s, src, stride[0],
+                               stride[1], stride[2], dim, X, Y, Z, nbx, nby, nbricks, scale, out);
+        else if(dtype == kDtF16)
+            hipLaunchKernelGGL((brick_encode_kernel<kDtF16>), dim3(unsigned(blocks)), dim3(256), 0, s, src, stride[0],
+                               stride[1], stride[2], dim, X, Y, Z, nbx, nby, nbricks, scale, out);
+        else
+            hipLaunchKernelGGL((brick_encode_kernel<kDtBf16>), dim3(unsigned(blocks)), dim3(256), 0, s, src, stride[0],
+                               stride[1], stride[2], dim, X, Y, Z, nbx, nby, nbricks, scale, out);
+        return hipGetLastError();
+    }
+
+    hipError_t launch_brick_frame(
+        const FrameParams& F,
+        const PostParams& Q,
+        const ParticleParams& PP,
+        bool counting,
+        hipStream_t s,
+        hipEvent_t afterMarch,
+        hipEvent_t afterSsao)
+    {
+        const unsigned nb = unsigned(F.tilesX) * unsigned(F.tilesY);
+        if(nb == 0)
+        {
+            (void) hipEventRecord(afterMarch, s);
+            (void) hipEventRecord(afterSsao, s);
+            return hipSuccess;
+        }
+        const dim3 grid(nb), block(256);
+        const size_t lds = size_t(Q.nLut) * Q.T * sizeof(float4);
+        const int v = (Q.dtype * 4) | (F.linear ? 2 : 0) | (counting ? 1 : 0);
+        switch(v)
+        {
+        case 0: hipLaunchKernelGGL((brick_march_kernel<0, false, false>), grid, block, lds, s, F, Q, PP); break;
+        case 1: hipLaunchKernelGGL((brick_march_kernel<0, false, true>), grid, block, lds, s, F, Q, PP); break;
+        case 2: hipLaunchKernelGGL((brick_march_kernel<0, true, false>), grid, block, lds, s, F, Q, PP); break;
+        case 3: hipLaunchKernelGGL((brick_march_kernel<0, true, true>), grid, block, lds, s, F, Q, PP); break;
+        case 4: hipLaunchKernelGGL((brick_march_kernel<1, false, false>), grid, block, lds, s, F, Q, PP); break;
+        case 5: hipLaunchKernelGGL((brick_march_kernel<1, false, true>), grid, block, lds, s, F, Q, PP); break;
+        case 6: hipLaunchKernelGGL((brick_march_kernel<1, true, false>), grid, block, lds, s, F, Q, PP); break;
+        case 7: hipLaunchKernelGGL((brick_march_kernel<1, true, true>), grid, block, lds, s, F, Q, PP); break;
+        case 8: hipLaunchKernelGGL((brick_march_kernel<2, false, false>), grid, block, lds, s, F, Q, PP); break;
+        case 9: hipLaunchKernelGGL((brick_march_kernel<2, false, true>), grid, block, lds, s, F, Q, PP); break;
+        case 10: hipLaunchKernelGGL((brick_march_kernel<2, true, false>), grid, block, lds, s, F, Q, PP); break;
+        default: hipLaunchKernelGGL((brick_march_kernel<2, true, true>), grid, block, lds, s, F, Q, PP); break;
+        }
+        (void) hipEventRecord(afterMarch, s);
+        if(F.aoEnabled)
+            hipLaunchKernelGGL(ssao_kernel, grid, block, 0, s, F);
+        (void) hipEventRecord(afterSsao, s);
+        hipLaunchKernelGGL(shade_kernel, grid, block, 0, s, F);
+        return hipGetLastError();
     }
 
     // ------------------------------------------------------------------ launchers
@@ -873,19 +1480,17 @@ namespace isaac_hip
             return hipSuccess;
         }
         const dim3 grid(nb), block(256);
-        if(F.linear)
+        const int v = (F.linear ? 4 : 0) | (counting ? 2 : 0) | (F.prefetch ? 1 : 0);
+        switch(v)
         {
-            if(counting)
-                hipLaunchKernelGGL((march_kernel<true, true>), grid, block, 0, s, F, PP);
-            else
-                hipLaunchKernelGGL((march_kernel<true, false>), grid, block, 0, s, F, PP);
-        }
-        else
-        {
-            if(counting)
-                hipLaunchKernelGGL((march_kernel<false, true>), grid, block, 0, s, F, PP);
-            else
-                hipLaunchKernelGGL((march_kernel<false, false>), grid, block, 0, s, F, PP);
+        case 0: hipLaunchKernelGGL((march_kernel<false, false, false>), grid, block, 0, s, F, PP); break;
+        case 1: hipLaunchKernelGGL((march_kernel<false, false, true>), grid, block, 0, s, F, PP); break;
+        case 2: hipLaunchKernelGGL((march_kernel<false, true, false>), grid, block, 0, s, F, PP); break;
+        case 3: hipLaunchKernelGGL((march_kernel<false, true, true>), grid, block, 0, s, F, PP); break;
+        case 4: hipLaunchKernelGGL((march_kernel<true, false, false>), grid, block, 0, s, F, PP); break;
+        case 5: hipLaunchKernelGGL((march_kernel<true, false, true>), grid, block, 0, s, F, PP); break;
+        case 6: hipLaunchKernelGGL((march_kernel<true, true, false>), grid, block, 0, s, F, PP); break;
+        default: hipLaunchKernelGGL((march_kernel<true, true, true>), grid, block, 0, s, F, PP); break;
         }
         (void) hipEventRecord(afterMarch, s);
         if(F.aoEnabled)

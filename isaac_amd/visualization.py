@@ -16,7 +16,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import numpy as np
 
 from . import _lib
-from ._lib import check, lib
+from ._lib import IsaacHipError, check, lib
 
 ISAAC_MAX_FUNCTORS = 3
 
@@ -384,6 +384,57 @@ class IsaacVisualization:
         self.kernel_time, self.copy_time = t.kernel_us, t.copy_us
         self.buffer_time, self.optimization_buffer_time = t.buffer_us, t.optimization_buffer_us
         return {name: float(getattr(t, name)) for name, _ in _lib.Timings._fields_}
+
+    TUNE_MARCH_PREFETCH = 0
+
+    def set_tuning(self, key: int, value: int) -> None:
+        check(lib().isaac_hip_set_tuning(self._ctx, int(key), int(value)), self._ctx, "set_tuning")
+
+    # render forms (no counterpart in the reference; include/isaac_hip.h)
+    FORM_COMBINED = 0
+    FORM_BRICKS = 1
+    F32, F16, BF16 = 0, 1, 2
+
+    def set_render_form(self, form: int, brick_dtype: int = 0) -> None:
+        """COMBINED: the reference's optimized renderer; BRICKS: post-classification
+        march over 8^3 field bricks stored in brick_dtype (F32/F16/BF16 + FP32 scale).
+        Takes effect at the next update_buffers."""
+        check(lib().isaac_hip_set_render_form(self._ctx, int(form), int(brick_dtype)), self._ctx, "set_render_form")
+
+    def bricks(self, source: int):
+        """Host copies of a source's bricks: (pairs, scales) with pairs shaped
+        (nbricks, dim, 9, 9, 8, 2) as float32 (F32) or uint16 bit patterns (F16/BF16)
+        and scales (nbricks, dim)."""
+        pp, ps = ctypes.c_void_p(), ctypes.c_void_p()
+        nb, pb = ctypes.c_int64(), ctypes.c_int32()
+        check(lib().isaac_hip_get_bricks(self._ctx, int(source), ctypes.byref(pp), ctypes.byref(ps), ctypes.byref(nb),
+                                         ctypes.byref(pb)), self._ctx, "get_bricks")
+        if not pp.value:
+            raise IsaacHipError(f"source {source} has no bricks (render form BRICKS, enabled, after update_buffers)")
+        src = (self.volume_sources + self.field_sources)[source]
+        dim = int(src.feature_dim) if src.persistent else 1  # non-persistent: the functor-applied scalar
+        self.synchronize()
+        dt = np.float32 if pb.value == 8 else np.uint16
+        pairs = np.zeros((nb.value, dim, 9, 9, 8, 2), dt)
+        scales = np.zeros((nb.value, dim), np.float32)
+        self._d2h(pairs, pp.value)
+        self._d2h(scales, ps.value)
+        return pairs, scales
+
+    def set_float_output(self, on: bool) -> None:
+        check(lib().isaac_hip_set_float_output(self._ctx, int(on)), self._ctx, "set_float_output")
+
+    def float_image(self) -> np.ndarray:
+        """Final colour before RGBA8 quantisation, (H, W, 4) float32 (needs set_float_output)."""
+        p = ctypes.c_void_p()
+        check(lib().isaac_hip_get_float_image(self._ctx, ctypes.byref(p)), self._ctx, "get_float_image")
+        if not p.value:
+            raise IsaacHipError("float output is off (set_float_output(True) before draw)")
+        W, H = self.framebuffer_size
+        self.synchronize()
+        a = np.zeros((H, W, 4), np.float32)
+        self._d2h(a, p.value)
+        return a
 
     def set_counting(self, on: bool) -> None:
         check(lib().isaac_hip_set_counting(self._ctx, int(on)), self._ctx, "set_counting")

@@ -348,11 +348,11 @@ namespace isaac_oracle
     // propagating through the chain (lib/isaac/isaac_functor_chain.hpp:96-108,
     // lib/isaac/isaac_functors.hpp:42-183).
     template<class R>
-    inline R apply_chain(const Chain& c, const float* v_in, int dim)
+    inline R apply_chain_vals(const Chain& c, const R* v_in, int dim)
     {
         R v[4];
         for(int i = 0; i < dim; i++)
-            v[i] = R(v_in[i]);
+            v[i] = v_in[i];
         int n = dim;
         for(int k = 0; k < MAX_FUNCTORS; k++)
         {
@@ -402,6 +402,15 @@ namespace isaac_oracle
             }
         }
         return v[0];
+    }
+
+    template<class R>
+    inline R apply_chain(const Chain& c, const float* v_in, int dim)
+    {
+        R v[4];
+        for(int i = 0; i < dim; i++)
+            v[i] = R(v_in[i]);
+        return apply_chain_vals<R>(c, v, dim);
     }
 
     // TF lookup index: clamp(round(v*T), 0, T-1) (lib/isaac/isaac_common_kernel.hpp:590-592)
@@ -810,6 +819,136 @@ namespace isaac_oracle
             });
     }
 
+    // The walk of one ray (ParticleRenderKernel, lib/isaac/isaac_particle_kernel.hpp:117-260):
+    // 3D-DDA over the particle cell grid from the clipped ray start; the walk
+    // ends in the first cell that yields a visible hit.  On a hit returns true with
+    // the RGBA8 colour, the normal and the depth (from the ray origin).
+    template<class R>
+    inline bool particle_ray(
+        const State& st,
+        const std::vector<CellList>& lists,
+        const Ray<R>& ray,
+        B4& colorOut,
+        V3<R>& normalOut,
+        R& depthOut,
+        uint64_t& myCells,
+        uint64_t& myTests)
+    {
+        const int nv = (int) st.vol.size();
+        const int np = (int) st.part.size();
+        const Sizes& sz = st.sizes;
+        // scale = localSizeScaled / localParticleSize (lib/isaac.hpp:2312)
+        V3<R> pscale;
+        for(int i = 0; i < 3; i++)
+            pscale[i] = R(float(sz.localSizeScaled[i]) / float(sz.localParticleSize[i]));
+        const int64_t gx = sz.localParticleSize[0], gy = sz.localParticleSize[1], gz = sz.localParticleSize[2];
+        V4<R> pColor{0, 0, 0, 0};
+        R depth = std::numeric_limits<R>::max();
+        bool hit = false;
+        int32_t ds[3] = {(int32_t) gsign(ray.dir.x), (int32_t) gsign(ray.dir.y), (int32_t) gsign(ray.dir.z)};
+        V3<R> cur = ray.start + ray.dir * ray.startDepth;
+        uint32_t cell[3];
+        V3<R> cdiv = cur / pscale;
+        for(int i = 0; i < 3; i++)
+        {
+            int32_t c = (int32_t) cdiv[i];
+            int32_t hi = (int32_t) (sz.localParticleSize[i] - 1);
+            c = std::min(std::max(c, int32_t(0)), hi);
+            cell[i] = (uint32_t) c;
+        }
+        R rayLength = ray.endDepth - ray.startDepth;
+        R tested = R(0);
+        V3<R> t, dt;
+        for(int i = 0; i < 3; i++)
+        {
+            t[i] = ((R(cell[i]) + R(std::max(ds[i], 0))) * pscale[i] - cur[i]) / ray.dir[i];
+            dt[i] = pscale[i] / ray.dir[i] * R(ds[i]);
+        }
+        for(int i = 0; i < 3; i++)
+            if(ray.dir[i] == R(0))
+                t[i] = std::numeric_limits<R>::max();
+        V3<R> normal{0, 0, 0};
+        while(cell[0] < (uint64_t) gx && cell[1] < (uint64_t) gy && cell[2] < (uint64_t) gz && !hit && tested <= rayLength)
+        {
+            myCells++;
+            const int64_t cid = cell[0] + cell[1] * gx + cell[2] * gx * gy;
+            for(int s = 0; s < np; s++)
+            {
+                const float w = st.weight[nv + s];
+                if(w == 0.0f)
+                    continue;
+                const ParticleSource& ps = st.part[s];
+                const CellList& cl = lists[s];
+                for(int64_t k = cl.offset[cid]; k < cl.offset[cid + 1]; k++)
+                {
+                    myTests++;
+                    const int64_t pi = cl.perm[k];
+                    V3<R> rel{R(cl.rel[3 * k + 0]), R(cl.rel[3 * k + 1]), R(cl.rel[3 * k + 2])};
+                    V3<R> ppos = (rel + V3<R>{R(cell[0]), R(cell[1]), R(cell[2])}) * pscale;
+                    V3<R> L = ppos - cur;
+                    R radius = R(ps.radius ? ps.radius[pi] : ps.const_radius) * R(w);
+                    R r2 = radius * radius;
+                    R tca = dot(L, ray.dir);
+                    R d2 = dot(L, L) - tca * tca;
+                    if(d2 <= r2)
+                    {
+                        R thc = std::sqrt(r2 - d2);
+                        R t0 = tca - thc;
+                        R t1 = tca + thc;
+                        if(t1 >= R(0) && t0 < depth)
+                        {
+                            R v = apply_chain<R>(st.chain[nv + s], ps.attr + pi * ps.dim, ps.dim);
+                            V4<R> val = lut_at<R>(st.lut[nv + s], tf_index(v, st.T));
+                            if(val.w >= R(0.5))
+                            {
+                                pColor = val;
+                                depth = t0;
+                                hit = true;
+                                normal = cur + t0 * ray.dir - ppos;
+                                if(t0 < R(0) && ray.isClipped)
+                                    normal = -ray.clippingNormal;
+                            }
+                        }
+                    }
+                }
+            }
+            if(t.x < t.y && t.x < t.z)
+            {
+                cell[0] += (uint32_t) ds[0];
+                tested = t.x;
+                t.x = t.x + dt.x;
+            }
+            else if(t.y < t.x && t.y < t.z)
+            {
+                cell[1] += (uint32_t) ds[1];
+                tested = t.y;
+                t.y = t.y + dt.y;
+            }
+            else
+            {
+                cell[2] += (uint32_t) ds[2];
+                tested = t.z;
+                t.z = t.z + dt.z;
+            }
+        }
+        if(!hit)
+            return false;
+        pColor.w = R(1);
+        colorOut = to_byte(pColor);
+        normalOut = normalize(normal);
+        depthOut = depth + ray.startDepth;
+        return true;
+    }
+
+    inline bool any_particles(const State& st)
+    {
+        const int nv = (int) st.vol.size();
+        bool any = false;
+        for(size_t s = 0; s < st.part.size(); s++)
+            any |= st.weight[nv + s] != 0.0f;
+        return any;
+    }
+
     // ParticleRenderKernel + MergeParticleSourceIterator (lib/isaac/isaac_particle_kernel.hpp:32-260)
     template<class R>
     inline void pass_particles(
@@ -819,19 +958,9 @@ namespace isaac_oracle
         GBuffer<R>& g,
         Counters& cnt)
     {
-        const int nv = (int) st.vol.size();
-        const int np = (int) st.part.size();
-        bool any = false;
-        for(int s = 0; s < np; s++)
-            any |= st.weight[nv + s] != 0.0f;
-        if(!any)
+        if(!any_particles(st))
             return;
         const Sizes& sz = st.sizes;
-        // scale = localSizeScaled / localParticleSize (lib/isaac.hpp:2312)
-        V3<R> pscale;
-        for(int i = 0; i < 3; i++)
-            pscale[i] = R(float(sz.localSizeScaled[i]) / float(sz.localParticleSize[i]));
-        const int64_t gx = sz.localParticleSize[0], gy = sz.localParticleSize[1], gz = sz.localParticleSize[2];
         uint64_t cells = 0, tests = 0, rays = 0;
         for_viewport<R>(
             st,
@@ -842,97 +971,11 @@ namespace isaac_oracle
                 if(!clip_ray(ray, sz, st.clip))
                     return;
                 ray.endDepth = gmin(ray.endDepth, g.depth[p]);
-                V4<R> pColor{0, 0, 0, 0};
-                R depth = std::numeric_limits<R>::max();
-                bool hit = false;
-                int32_t ds[3] = {(int32_t) gsign(ray.dir.x), (int32_t) gsign(ray.dir.y), (int32_t) gsign(ray.dir.z)};
-                V3<R> cur = ray.start + ray.dir * ray.startDepth;
-                uint32_t cell[3];
-                V3<R> cdiv = cur / pscale;
-                for(int i = 0; i < 3; i++)
-                {
-                    int32_t c = (int32_t) cdiv[i];
-                    int32_t hi = (int32_t) (sz.localParticleSize[i] - 1);
-                    c = std::min(std::max(c, int32_t(0)), hi);
-                    cell[i] = (uint32_t) c;
-                }
-                R rayLength = ray.endDepth - ray.startDepth;
-                R tested = R(0);
-                V3<R> t, dt;
-                for(int i = 0; i < 3; i++)
-                {
-                    t[i] = ((R(cell[i]) + R(std::max(ds[i], 0))) * pscale[i] - cur[i]) / ray.dir[i];
-                    dt[i] = pscale[i] / ray.dir[i] * R(ds[i]);
-                }
-                for(int i = 0; i < 3; i++)
-                    if(ray.dir[i] == R(0))
-                        t[i] = std::numeric_limits<R>::max();
-                V3<R> normal{0, 0, 0};
                 uint64_t myCells = 0, myTests = 0;
-                while(cell[0] < (uint64_t) gx && cell[1] < (uint64_t) gy && cell[2] < (uint64_t) gz && !hit
-                      && tested <= rayLength)
-                {
-                    myCells++;
-                    const int64_t cid = cell[0] + cell[1] * gx + cell[2] * gx * gy;
-                    for(int s = 0; s < np; s++)
-                    {
-                        const float w = st.weight[nv + s];
-                        if(w == 0.0f)
-                            continue;
-                        const ParticleSource& ps = st.part[s];
-                        const CellList& cl = lists[s];
-                        for(int64_t k = cl.offset[cid]; k < cl.offset[cid + 1]; k++)
-                        {
-                            myTests++;
-                            const int64_t pi = cl.perm[k];
-                            V3<R> rel{R(cl.rel[3 * k + 0]), R(cl.rel[3 * k + 1]), R(cl.rel[3 * k + 2])};
-                            V3<R> ppos = (rel + V3<R>{R(cell[0]), R(cell[1]), R(cell[2])}) * pscale;
-                            V3<R> L = ppos - cur;
-                            R radius = R(ps.radius ? ps.radius[pi] : ps.const_radius) * R(w);
-                            R r2 = radius * radius;
-                            R tca = dot(L, ray.dir);
-                            R d2 = dot(L, L) - tca * tca;
-                            if(d2 <= r2)
-                            {
-                                R thc = std::sqrt(r2 - d2);
-                                R t0 = tca - thc;
-                                R t1 = tca + thc;
-                                if(t1 >= R(0) && t0 < depth)
-                                {
-                                    R v = apply_chain<R>(st.chain[nv + s], ps.attr + pi * ps.dim, ps.dim);
-                                    V4<R> val = lut_at<R>(st.lut[nv + s], tf_index(v, st.T));
-                                    if(val.w >= R(0.5))
-                                    {
-                                        pColor = val;
-                                        depth = t0;
-                                        hit = true;
-                                        normal = cur + t0 * ray.dir - ppos;
-                                        if(t0 < R(0) && ray.isClipped)
-                                            normal = -ray.clippingNormal;
-                                    }
-                                }
-                            }
-                        }
-                    }
-                    if(t.x < t.y && t.x < t.z)
-                    {
-                        cell[0] += (uint32_t) ds[0];
-                        tested = t.x;
-                        t.x = t.x + dt.x;
-                    }
-                    else if(t.y < t.x && t.y < t.z)
-                    {
-                        cell[1] += (uint32_t) ds[1];
-                        tested = t.y;
-                        t.y = t.y + dt.y;
-                    }
-                    else
-                    {
-                        cell[2] += (uint32_t) ds[2];
-                        tested = t.z;
-                        t.z = t.z + dt.z;
-                    }
-                }
+                B4 col;
+                V3<R> n;
+                R d;
+                const bool hit = particle_ray(st, lists, ray, col, n, d, myCells, myTests);
 #pragma omp atomic
                 cells += myCells;
 #pragma omp atomic
@@ -941,11 +984,9 @@ namespace isaac_oracle
                 rays += 1;
                 if(!hit)
                     return;
-                normal = normalize(normal);
-                pColor.w = R(1);
-                g.color[p] = to_byte(pColor);
-                g.normal[p] = normal;
-                g.depth[p] = depth + ray.startDepth;
+                g.color[p] = col;
+                g.normal[p] = n;
+                g.depth[p] = d;
             });
         cnt.cellsVisited += cells;
         cnt.sphereTests += tests;

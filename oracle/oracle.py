@@ -245,6 +245,20 @@ class OracleScene(ctypes.Structure):
         ("mv", ctypes.c_double * 16),
         ("bg", ctypes.c_float * 4),
         ("threads", ctypes.c_int32),
+        ("vol_nonpersistent", ctypes.c_int32 * MAX_SOURCES),
+    ]
+
+
+class OraclePostOutputs(ctypes.Structure):
+    _fields_ = [
+        ("rgba", ctypes.c_void_p),
+        ("rgba_f", ctypes.c_void_p),
+        ("pre", ctypes.c_void_p),
+        ("depth", ctypes.c_void_p),
+        ("normal", ctypes.c_void_p),
+        ("ao", ctypes.c_void_p),
+        ("counters", ctypes.c_uint64 * 8),
+        ("seconds", ctypes.c_double),
     ]
 
 
@@ -297,6 +311,7 @@ class PreparedScene:
             sc.vol_dim[s] = v.feature_dim
             X, Y, Z = spec.local_size
             sc.vol_stride[s][0], sc.vol_stride[s][1], sc.vol_stride[s][2] = 1, X, X * Y
+            sc.vol_nonpersistent[s] = 0 if v.persistent else 1
         for s, p in enumerate(spec.particles):
             pos = np.ascontiguousarray(p.pos, dtype=np.float32)
             attr = np.ascontiguousarray(p.attr, dtype=np.float32)
@@ -388,6 +403,84 @@ def render(spec, precision: int = 32, threads: int = 0, want_textures: bool = Fa
     res.seconds = {"classify": out.seconds_classify, "bin": out.seconds_bin, "render": out.seconds_render}
     res.prepared = ps
     return res
+
+
+BRICK_RAW, BRICK_F32, BRICK_F16, BRICK_BF16 = -1, 0, 1, 2
+
+
+def render_post(spec, precision: int = 64, dtype: int = BRICK_RAW, stride: int = 1, threads: int = 0,
+                prepared: Optional[PreparedScene] = None):
+    """One frame of the brick (post-classification) form on the CPU.
+
+    precision 32: ray setup in binary64, sampling/compositing in binary32 (the
+    library's arithmetic); 64: everything in binary64.  dtype BRICK_RAW samples the
+    float sources (the FP64 reference image), BRICK_F32/F16/BF16 the bricks.
+    stride > 1 renders only pixels with x % stride == 0 and y % stride == 0."""
+    ps = prepared or PreparedScene(spec, threads=threads)
+    ps.sc.threads = threads
+    W, H = spec.framebuffer
+    res = OracleResult()
+    res.rgba = np.zeros((H, W, 4), np.uint8)
+    res.rgba_f = np.zeros((H, W, 4), np.float32)
+    res.pre = np.zeros((H, W, 4), np.uint8)
+    res.depth = np.zeros((H, W), np.float32)
+    res.normal = np.zeros((H, W, 3), np.float32)
+    res.ao = np.zeros((H, W), np.float32)
+    out = OraclePostOutputs()
+    out.rgba = res.rgba.ctypes.data
+    out.rgba_f = res.rgba_f.ctypes.data
+    out.pre = res.pre.ctypes.data
+    out.depth = res.depth.ctypes.data
+    out.normal = res.normal.ctypes.data
+    out.ao = res.ao.ctypes.data
+    rc = lib().oracle_render_post(ctypes.byref(ps.sc), int(precision), int(dtype), int(stride), ctypes.byref(out))
+    if rc != 0:
+        raise RuntimeError(f"oracle post render failed rc={rc}")
+    res.counters = dict(zip(COUNTER_NAMES, [int(c) for c in out.counters]))
+    res.seconds = out.seconds
+    res.prepared = ps
+    return res
+
+
+def brick_scales(data: np.ndarray, dtype: int) -> np.ndarray:
+    """Per-brick, per-component scales of a (Z, Y, X[, C]) float32 source."""
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    Z, Y, X = data.shape[:3]
+    dim = 1 if data.ndim == 3 else data.shape[3]
+    nb = ((X + 7) // 8) * ((Y + 7) // 8) * ((Z + 7) // 8)
+    out = np.zeros((nb, dim), np.float32)
+    st = (ctypes.c_int64 * 3)(1, X, X * Y)
+    loc = (ctypes.c_int64 * 3)(X, Y, Z)
+    lib().oracle_brick_scales(ctypes.c_void_p(data.ctypes.data), st, dim, loc, int(dtype), ctypes.c_void_p(out.ctypes.data))
+    return out
+
+
+def f32_to_f16_bits(x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.zeros(x.shape, np.uint16)
+    lib().oracle_f32_to_f16(ctypes.c_void_p(x.ctypes.data), ctypes.c_void_p(out.ctypes.data), ctypes.c_int64(x.size))
+    return out
+
+
+def f16_bits_to_f32(h: np.ndarray) -> np.ndarray:
+    h = np.ascontiguousarray(h, dtype=np.uint16)
+    out = np.zeros(h.shape, np.float32)
+    lib().oracle_f16_to_f32(ctypes.c_void_p(h.ctypes.data), ctypes.c_void_p(out.ctypes.data), ctypes.c_int64(h.size))
+    return out
+
+
+def f32_to_bf16_bits(x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.zeros(x.shape, np.uint16)
+    lib().oracle_f32_to_bf16(ctypes.c_void_p(x.ctypes.data), ctypes.c_void_p(out.ctypes.data), ctypes.c_int64(x.size))
+    return out
+
+
+def psnr_float(a: np.ndarray, b: np.ndarray) -> float:
+    """PSNR of two [0, 1] float images (peak 1)."""
+    d = a.astype(np.float64) - b.astype(np.float64)
+    mse = float(np.mean(d * d))
+    return float("inf") if mse == 0.0 else 10.0 * math.log10(1.0 / mse)
 
 
 def bin_particles(pos: np.ndarray, local_size, grid):

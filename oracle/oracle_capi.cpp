@@ -1,6 +1,6 @@
 // CPU ORACLE C entry points — TEST INFRASTRUCTURE ONLY (loaded by tests/, the
 // smoke check and the benchmark's CPU-baseline leg through ctypes).
-#include "isaac_oracle.hpp"
+#include "post_oracle.hpp"
 
 #include <chrono>
 #include <omp.h>
@@ -48,6 +48,19 @@ struct OracleScene
     double mv[16];
     float bg[4];
     int32_t threads; // 0 = OpenMP default
+    int32_t vol_nonpersistent[MAX_SOURCES]; // brick form: 1 = sample the functor-applied field
+};
+
+struct OraclePostOutputs
+{
+    uint8_t* rgba; // W*H*4
+    float* rgba_f; // W*H*4 final colour before quantisation, clamped
+    uint8_t* pre; // W*H*4 after particles + iso, before shading
+    float* depth; // W*H
+    float* normal; // W*H*3
+    float* ao; // W*H
+    uint64_t counters[8];
+    double seconds;
 };
 
 struct OracleOutputs
@@ -195,6 +208,96 @@ namespace
         return 0;
     }
 
+    template<class R>
+    int render_post_impl(const OracleScene* sc, int dtype, int stride, OraclePostOutputs* out)
+    {
+        if(sc->threads > 0)
+            omp_set_num_threads(sc->threads);
+        if(stride < 1)
+            return -1;
+        State st = make_state(sc);
+        using clk = std::chrono::steady_clock;
+        auto t0 = clk::now();
+        PostScene ps;
+        ps.st = &st;
+        const int nv = (int) st.vol.size();
+        std::vector<std::vector<float>> applied(nv);
+        for(int s = 0; s < nv; s++)
+        {
+            const VolumeSource& v = st.vol[s];
+            const bool np = sc->vol_nonpersistent[s] != 0;
+            ps.persistent.push_back(!np);
+            if(!(st.weight[s] + st.iso[s] > 0.0f))
+            {
+                ps.fields.emplace_back();
+                continue;
+            }
+            if(!np)
+            {
+                ps.fields.push_back(make_brick_field(v.data, v.stride, v.dim, st.sizes, dtype));
+                continue;
+            }
+            // non-persistent: the functor-applied float field (UpdatePersistendTextureKernel,
+            // lib/isaac/isaac_common_kernel.hpp:432-451)
+            const int64_t X = st.sizes.localSize[0], Y = st.sizes.localSize[1], Z = st.sizes.localSize[2];
+            std::vector<float>& a = applied[s];
+            a.resize(size_t(X * Y * Z));
+#pragma omp parallel for schedule(static)
+            for(int64_t z = 0; z < Z; z++)
+                for(int64_t y = 0; y < Y; y++)
+                    for(int64_t x = 0; x < X; x++)
+                    {
+                        const float* e = v.data + (x * v.stride[0] + y * v.stride[1] + z * v.stride[2]) * v.dim;
+                        a[size_t((z * Y + y) * X + x)] = apply_chain<float>(st.chain[s], e, v.dim);
+                    }
+            const int64_t st1[3] = {1, X, X * Y};
+            ps.fields.push_back(make_brick_field(a.data(), st1, 1, st.sizes, dtype));
+        }
+        std::vector<CellList> lists;
+        for(auto& p : st.part)
+            lists.push_back(bin_particles(p, st.sizes));
+        PostOut<R> po;
+        try
+        {
+            render_post<R>(ps, lists, stride, po);
+        }
+        catch(const std::exception&)
+        {
+            return -2;
+        }
+        out->seconds = std::chrono::duration<double>(clk::now() - t0).count();
+        const size_t N = size_t(st.fbW) * st.fbH;
+        if(out->rgba)
+            std::memcpy(out->rgba, po.g.color.data(), N * 4);
+        if(out->pre)
+            std::memcpy(out->pre, po.pre.data(), N * 4);
+        for(size_t i = 0; i < N; i++)
+        {
+            if(out->rgba_f)
+            {
+                out->rgba_f[4 * i + 0] = float(po.finalColor[i].x);
+                out->rgba_f[4 * i + 1] = float(po.finalColor[i].y);
+                out->rgba_f[4 * i + 2] = float(po.finalColor[i].z);
+                out->rgba_f[4 * i + 3] = float(po.finalColor[i].w);
+            }
+            if(out->depth)
+                out->depth[i] = float(po.g.depth[i]);
+            if(out->ao)
+                out->ao[i] = float(po.g.ao[i]);
+            if(out->normal)
+            {
+                out->normal[3 * i + 0] = float(po.g.normal[i].x);
+                out->normal[3 * i + 1] = float(po.g.normal[i].y);
+                out->normal[3 * i + 2] = float(po.g.normal[i].z);
+            }
+        }
+        const Counters& c = po.counters;
+        uint64_t cv[8]
+            = {c.volSamples, c.isoSamples, c.isoHits, c.cellsVisited, c.sphereTests, c.volRays, c.isoRays, c.partRays};
+        for(int i = 0; i < 8; i++)
+            out->counters[i] = cv[i];
+        return 0;
+    }
 } // namespace
 
 extern "C"
@@ -326,6 +429,45 @@ extern "C"
                         double(r.isClipped), r.clippingNormal.x, r.clippingNormal.y, r.clippingNormal.z, 0, 0};
         std::memcpy(out, o, sizeof(o));
         return hit;
+    }
+
+    // Brick form (post-classification) frame.  precision 32: ray setup in binary64,
+    // sampling and compositing in binary32 (the library's arithmetic); 64: all binary64.
+    // dtype -1 samples the float sources directly, 0/1/2 the F32/F16/BF16 bricks.
+    int oracle_render_post(const OracleScene* sc, int precision, int dtype, int stride, OraclePostOutputs* out)
+    {
+        if(dtype < BRICK_RAW || dtype > BRICK_BF16)
+            return -1;
+        return precision == 64 ? render_post_impl<double>(sc, dtype, stride, out) : render_post_impl<float>(sc, dtype, stride, out);
+    }
+
+    // per-brick, per-component scales of one source (nbricks*dim floats)
+    int oracle_brick_scales(const float* data, const int64_t stride[3], int dim, const int64_t local[3], int dtype, float* out)
+    {
+        Sizes sz{};
+        for(int k = 0; k < 3; k++)
+            sz.localSize[k] = local[k];
+        BrickField f = make_brick_field(data, stride, dim, sz, dtype);
+        std::memcpy(out, f.scale.data(), f.scale.size() * sizeof(float));
+        return 0;
+    }
+
+    void oracle_f32_to_f16(const float* in, uint16_t* out, int64_t n)
+    {
+        for(int64_t i = 0; i < n; i++)
+            out[i] = f32_to_f16_rne(in[i]);
+    }
+
+    void oracle_f16_to_f32(const uint16_t* in, float* out, int64_t n)
+    {
+        for(int64_t i = 0; i < n; i++)
+            out[i] = f16_to_f32(in[i]);
+    }
+
+    void oracle_f32_to_bf16(const float* in, uint16_t* out, int64_t n)
+    {
+        for(int64_t i = 0; i < n; i++)
+            out[i] = f32_to_bf16_rne(in[i]);
     }
 
     int oracle_max_threads()
