@@ -12,7 +12,9 @@ volume-filling pose.  Inputs are resident in HBM before timing starts and the
 framebuffer stays on the device (no PCIe in the timed region).
 
 Metric: trilinear samples per second (volume + iso march samples actually
-executed, after early ray termination), with frames/s alongside.
+executed, after early ray termination), with frames/s alongside.  The same run
+also times the brick render forms (FP32 / FP16 / BF16 field bricks,
+post-classification) on the same workload and reports them under "forms".
 """
 from __future__ import annotations
 
@@ -27,30 +29,47 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD-published
+# measured DRAM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a
+# plain copy of known size); written from the profile summaries in profiles/
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+FORMS = {"combined": (0, 0), "bricks-f32": (1, 0), "bricks-f16": (1, 1), "bricks-bf16": (1, 2)}
+VALUE_BYTES = {0: 4, 1: 2, 2: 2}  # bytes per stored brick value
 
 
-def kernel_bytes(counts, spec, n_vox, n_particles, n_visible):
+def kernel_bytes(counts, spec, n_vox, n_particles, n_visible, form="combined"):
     """Algorithmic bytes per launch of each kernel (read + write), from the
     executed work counts of one frame."""
     W, H = spec.framebuffer
     npx = W * H
     src_bytes = sum(4 * v.feature_dim for v in spec.volumes)
-    return {
-        # 16 B of sources in, two RGBA8 texels out per voxel
-        "classify": n_vox * (src_bytes + 8),
-        # pos 12 + attr 4 in; cell id 4 + record 16 written and read back; perm 4 written and read back;
-        # visible render record 16 written
-        "binning": n_particles * (12 + 4 + 2 * 4 + 2 * 16 + 2 * 4) + n_visible * 16,
-        # fused march: 8 RGBA8 taps per volume sample and per iso sample, 7 iso samples at a hit;
-        # DDA: offset pair per cell, 16 B per sphere test; G-buffer writes colour 4 + normal 12 + depth 4
-        # + volume colour 16
-        "march": 32 * (counts["vol_samples"] + counts["iso_samples"]) + 7 * 32 * counts["iso_hits"]
-        + 8 * counts["cells_visited"] + 16 * counts["sphere_tests"] + 36 * npx,
+    comps = sum(v.feature_dim for v in spec.volumes)
+    out = {
         # 49 depth taps + ao write
         "ssao": npx * (49 * 4 + 4),
         # G-buffer read colour 4 + normal 12 + depth 4 + ao 4 + volume colour 16, colour write 4
         "shade": 44 * npx,
+        # pos 12 + attr 4 in; cell id 4 + record 16 written and read back; perm 4 written and read back;
+        # visible render record 16 written
+        "binning": n_particles * (12 + 4 + 2 * 4 + 2 * 16 + 2 * 4) + n_visible * 16,
     }
+    particles = 8 * counts["cells_visited"] + 16 * counts["sphere_tests"]
+    if form == "combined":
+        # 16 B of sources in, two RGBA8 texels out per voxel
+        out["classify"] = n_vox * (src_bytes + 8)
+        # fused march: 8 RGBA8 taps per volume sample and per iso sample, 7 iso samples at a hit;
+        # G-buffer writes colour 4 + normal 12 + depth 4 + volume colour 16
+        out["march"] = (32 * (counts["vol_samples"] + counts["iso_samples"]) + 7 * 32 * counts["iso_hits"]
+                        + particles + 36 * npx)
+    else:
+        vb = VALUE_BYTES[FORMS[form][1]]
+        # brick build: sources in; 648 pairs of 2 values per 512 voxels and component out, + scales
+        out["classify"] = n_vox * src_bytes + (n_vox // 512) * comps * (648 * 2 * vb + 4)
+        # one classification per step: 8 taps x stored bytes per component (+ the brick scale
+        # for 16-bit forms); 7 more at an iso hit
+        per = comps * (8 * vb + (4 if vb < 4 else 0))
+        out["march"] = per * counts["march_steps"] + 7 * per * counts["iso_hits"] + particles + 36 * npx
+    return out
 
 
 def cpu_baseline(spec, threads):
@@ -76,12 +95,63 @@ def cpu_baseline(spec, threads):
     }
 
 
+def measure(vis, poses, out, steps, warmup):
+    """Counting frames per pose (untimed), warmup, then `steps` timed frames."""
+    import numpy as np
+    import torch
+
+    def set_pose(p):
+        vis.look_at = np.asarray(p.look_at, np.float64)
+        vis.rotation = np.asarray(p.rotation, np.float64)
+        vis.distance = float(p.distance)
+        vis._update_modelview()
+
+    counts = []
+    vis.set_counting(True)
+    for p in poses:
+        set_pose(p)
+        vis.update_buffers()
+        vis.draw(out=out)
+        counts.append(vis.counters())
+    vis.set_counting(False)
+    for k in range(warmup):
+        set_pose(poses[k % 2])
+        vis.update_buffers()
+        vis.draw(out=out)
+    vis.synchronize()
+    torch.cuda.synchronize()
+    per = {"classify": [], "binning": [], "march": [], "ssao": [], "shade": []}
+    total_samples = 0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        pi = k % 2
+        set_pose(poses[pi])
+        vis.update_buffers()
+        vis.draw(out=out)
+        t = vis.timings()  # waits for the step: each frame completes before the next (as the reference)
+        per["classify"].append(t["optimization_buffer_us"])
+        per["binning"].append(t["binning_us"])
+        per["march"].append(t["march_us"])
+        per["ssao"].append(t["ssao_us"])
+        per["shade"].append(t["shade_us"])
+        total_samples += counts[pi]["vol_samples"] + counts[pi]["iso_samples"]
+    vis.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    avg_us = {k: float(np.mean(v)) for k, v in per.items()}
+    avg_counts = {k: float(np.mean([c[k] for c in counts])) for k in counts[0]}
+    return wall, total_samples, avg_us, avg_counts, counts
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=256, help="local volume side (the workload is 256)")
+    ap.add_argument("--form", default="combined", choices=sorted(FORMS), help="render form of the headline line")
+    ap.add_argument("--no-forms", action="store_true", help="skip timing the other render forms")
+    ap.add_argument("--form-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--prefetch", type=int, default=0, help="march kernel: fetch step i+1 before compositing step i")
@@ -118,57 +188,25 @@ def main() -> int:
     out = torch.empty((spec.framebuffer[1], spec.framebuffer[0], 4), dtype=torch.uint8, device="cuda")
     n_vox = n ** 3
     n_particles = int(spec.particles[0].pos.shape[0])
-
-    def set_pose(p):
-        vis.look_at = np.asarray(p.look_at, np.float64)
-        vis.rotation = np.asarray(p.rotation, np.float64)
-        vis.distance = float(p.distance)
-        vis._update_modelview()
-
-    # executed work per pose (counting kernels, untimed)
-    counts = []
-    vis.set_counting(True)
-    for p in poses:
-        set_pose(p)
-        vis.update_buffers()
-        vis.draw(out=out)
-        counts.append(vis.counters())
-    vis.set_counting(False)
     n_visible = int(np.count_nonzero(spec.particles[0].attr[:, 0] >= 0.995))
 
-    for k in range(args.warmup):
-        set_pose(poses[k % 2])
-        vis.update_buffers()
-        vis.draw(out=out)
-    vis.synchronize()
-    torch.cuda.synchronize()
-
-    per = {"classify": [], "binning": [], "march": [], "ssao": [], "shade": []}
-    total_samples = 0
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        pi = k % 2
-        set_pose(poses[pi])
-        vis.update_buffers()
-        vis.draw(out=out)
-        t = vis.timings()  # waits for the step: each frame completes before the next (as the reference)
-        per["classify"].append(t["optimization_buffer_us"])
-        per["binning"].append(t["binning_us"])
-        per["march"].append(t["march_us"])
-        per["ssao"].append(t["ssao_us"])
-        per["shade"].append(t["shade_us"])
-        total_samples += counts[pi]["vol_samples"] + counts[pi]["iso_samples"]
-    vis.synchronize()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-
+    form, dt = FORMS[args.form]
+    vis.set_render_form(form, dt)
+    wall, total_samples, avg_us, avg_counts, counts = measure(vis, poses, out, args.steps, args.warmup)
     ms_per_step = wall / args.steps * 1e3
     value = total_samples / wall
-    avg_us = {k: float(np.mean(v)) for k, v in per.items()}
-    avg_counts = {k: float(np.mean([c[k] for c in counts])) for k in counts[0]}
-    kb = kernel_bytes(avg_counts, spec, n_vox, n_particles, n_visible)
+    kb = kernel_bytes(avg_counts, spec, n_vox, n_particles, n_visible, args.form)
     dom = max(avg_us, key=lambda k: avg_us[k])
     achieved = kb[dom] / (avg_us[dom] * 1e-6) / 1e9
+    traffic = None
+    traffic_src = None
+    if os.path.exists(TRAFFIC_FILE) and args.size == 256:
+        with open(TRAFFIC_FILE) as f:
+            tf = json.load(f)
+        entry = tf.get("kernels", {}).get(f"{args.form}:{dom}")
+        if entry:
+            traffic = entry["bytes_per_launch"]
+            traffic_src = tf.get("source")
     line = {
         "metric": "samples/s + fps @256^3x3,1080p",
         "value": value,
@@ -180,13 +218,14 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if args.form in ("combined", "bricks-f32") else args.form.split("-")[1],
         "data": "synthetic (seeded Gaussian mixture + value-noise density, curl-noise vector field, uniform particles)",
         "config": {
             "workload": f"{n}^3 local volume x 3 sources (f32 density, f32x3 curl field via length, "
             f"{n_particles} particles on a {spec.particle_grid[0]}^3 cell grid), iso 0.5, 1 clipping plane, "
             "SSAO, trilinear step 0.5, 1920x1080, poses default/fill alternating",
             "frame": "cell lists + classification + draw per step",
+            "form": args.form,
         },
         "fps": args.steps / wall,
         "samples_per_frame": {"default": counts[0]["vol_samples"] + counts[0]["iso_samples"],
@@ -200,10 +239,29 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "algorithmic_bytes_per_launch": kb[dom],
         },
     }
+    if not args.no_forms:
+        forms = {}
+        for name in ("combined", "bricks-f32", "bricks-f16", "bricks-bf16"):
+            if name == args.form:
+                continue
+            f, d = FORMS[name]
+            vis.set_render_form(f, d)
+            w2, s2, us2, c2, _ = measure(vis, poses, out, args.form_steps, 2)
+            kb2 = kernel_bytes(c2, spec, n_vox, n_particles, n_visible, name)
+            forms[name] = {
+                "ms_per_step": w2 / args.form_steps * 1e3,
+                "fps": args.form_steps / w2,
+                "samples_per_s": s2 / w2,
+                "kernel_ms": {k: v / 1e3 for k, v in us2.items()},
+                "march_algorithmic_GBs": kb2["march"] / (us2["march"] * 1e-6) / 1e9,
+                "march_steps_per_frame": c2["march_steps"],
+            }
+        line["forms"] = forms
     vis.close()
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))

@@ -63,7 +63,8 @@ extern "C"
         ISAAC_HIP_CNT_VOL_RAYS = 5,
         ISAAC_HIP_CNT_ISO_RAYS = 6,
         ISAAC_HIP_CNT_PART_RAYS = 7,
-        ISAAC_HIP_CNT_COUNT = 8
+        ISAAC_HIP_CNT_MARCH_STEPS = 8, /* fused iso+volume traversal steps (one gather of the taps each) */
+        ISAAC_HIP_CNT_COUNT = 9
     };
 
     typedef struct isaac_hip_ctx isaac_hip_ctx;

@@ -569,7 +569,7 @@ namespace isaac_hip
         int px, py;
         const bool valid = tile_pixel(F, px, py);
         unsigned long long nCells = 0, nTests = 0, nIso = 0, nIsoHits = 0, nIsoRays = 0, nPartRays = 0;
-        unsigned long long nVol = 0, nVolRays = 0;
+        unsigned long long nVol = 0, nVolRays = 0, nSteps = 0;
         if(valid)
         {
             const size_t p = size_t(py) * size_t(F.W) + size_t(px);
@@ -619,6 +619,8 @@ namespace isaac_hip
                                            startU.z + stepVec.z * float(i), cur);
                         for(;;)
                         {
+                            if(COUNT)
+                                nSteps++;
                             Taps nxt;
                             const bool ahead = PREFETCH && i + 1 <= last;
                             if(ahead)
@@ -735,6 +737,7 @@ namespace isaac_hip
             block_count_add(F.counters + 5, nVolRays);
             block_count_add(F.counters + 6, nIsoRays);
             block_count_add(F.counters + 7, nPartRays);
+            block_count_add(F.counters + 8, nSteps);
         }
     }
 
@@ -1114,7 +1117,7 @@ namespace isaac_hip
         int px, py;
         const bool valid = tile_pixel(F, px, py);
         unsigned long long nVol = 0, nIso = 0, nIsoHits = 0, nVolRays = 0, nIsoRays = 0;
-        unsigned long long nCells = 0, nTests = 0, nPartRays = 0;
+        unsigned long long nCells = 0, nTests = 0, nPartRays = 0, nSteps = 0;
         if(valid)
         {
             const size_t p = size_t(py) * size_t(F.W) + size_t(px);
@@ -1169,6 +1172,8 @@ namespace isaac_hip
                             last = endVol;
                         if(i > last)
                             break;
+                        if(COUNT)
+                            nSteps++;
                         const float fi = float(i);
                         const PostSample sm = post_classify<DT, LINEAR>(
                             Q, lut, F.X, F.Y, F.Z, startU.x + stepVec.x * fi, startU.y + stepVec.y * fi, startU.z + stepVec.z * fi);
@@ -1253,6 +1258,7 @@ namespace isaac_hip
             block_count_add(F.counters + 5, nVolRays);
             block_count_add(F.counters + 6, nIsoRays);
             block_count_add(F.counters + 7, nPartRays);
+            block_count_add(F.counters + 8, nSteps);
         }
     }
 
@@ -1384,6 +1390,8 @@ namespace isaac_hip
                                stride[1], stride[2], dim, X, Y, Z, nbx, nby, nbricks, scale, out);
         return hipGetLastError();
     }
+
+    static_assert(sizeof(FrameParams) + sizeof(PostParams) + sizeof(ParticleParams) <= 4096, "kernel argument block");
 
     hipError_t launch_brick_frame(
         const FrameParams& F,

@@ -261,6 +261,13 @@ def small_scene(n: int = 64, fb: Tuple[int, int] = (512, 512), seed: int = 1) ->
     )
 
 
+def precision_scene(n: int = 512, fb: Tuple[int, int] = (3840, 2160), step: float = 0.25, seed: int = 1) -> SceneSpec:
+    """The small scene's single Gaussian-mixture source at n^3 (sigma scaled with
+    n), 3840x2160, trilinear, step 0.25: the workload of the FP16/BF16 brick study."""
+    spec = small_scene(n=n, fb=fb, seed=seed)
+    return dataclasses.replace(spec, name=f"precision{n}", step=step)
+
+
 def prod_scene(
     n: int = 256,
     fb: Tuple[int, int] = (1920, 1080),

@@ -440,7 +440,7 @@ class IsaacVisualization:
         check(lib().isaac_hip_set_counting(self._ctx, int(on)), self._ctx, "set_counting")
 
     def counters(self) -> Dict[str, int]:
-        out = np.zeros(8, np.uint64)
+        out = np.zeros(len(_lib.CNT_NAMES), np.uint64)
         check(lib().isaac_hip_get_counters(self._ctx, out.ctypes.data), self._ctx, "get_counters")
         return dict(zip(_lib.CNT_NAMES, [int(v) for v in out]))
 

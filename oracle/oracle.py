@@ -455,6 +455,20 @@ def brick_scales(data: np.ndarray, dtype: int) -> np.ndarray:
     return out
 
 
+def sample_field(data: np.ndarray, dtype: int, pos: np.ndarray, linear: bool = True, precision: int = 64) -> np.ndarray:
+    """Brick-form samples (n, 4) of a (Z, Y, X[, C]) float32 source at positions (n, 3)."""
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    Z, Y, X = data.shape[:3]
+    dim = 1 if data.ndim == 3 else data.shape[3]
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    out = np.zeros((pos.shape[0], 4), np.float64)
+    st = (ctypes.c_int64 * 3)(1, X, X * Y)
+    loc = (ctypes.c_int64 * 3)(X, Y, Z)
+    lib().oracle_sample_field(ctypes.c_void_p(data.ctypes.data), st, dim, loc, int(dtype), int(precision), int(linear),
+                              ctypes.c_int64(pos.shape[0]), ctypes.c_void_p(pos.ctypes.data), ctypes.c_void_p(out.ctypes.data))
+    return out
+
+
 def f32_to_f16_bits(x: np.ndarray) -> np.ndarray:
     x = np.ascontiguousarray(x, dtype=np.float32)
     out = np.zeros(x.shape, np.uint16)

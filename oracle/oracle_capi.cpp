@@ -452,6 +452,43 @@ extern "C"
         return 0;
     }
 
+    // Brick-form samples of one source at n positions (voxel units, x y z), all components
+    int oracle_sample_field(
+        const float* data,
+        const int64_t stride[3],
+        int dim,
+        const int64_t local[3],
+        int dtype,
+        int precision,
+        int linear,
+        int64_t n,
+        const double* pos,
+        double* out)
+    {
+        Sizes sz{};
+        for(int k = 0; k < 3; k++)
+            sz.localSize[k] = local[k];
+        BrickField f = make_brick_field(data, stride, dim, sz, dtype);
+        for(int64_t i = 0; i < n; i++)
+        {
+            if(precision == 64)
+            {
+                double v[4] = {0, 0, 0, 0};
+                sample_field<double>(f, V3<double>{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]}, linear != 0, v);
+                for(int c = 0; c < 4; c++)
+                    out[4 * i + c] = v[c];
+            }
+            else
+            {
+                float v[4] = {0, 0, 0, 0};
+                sample_field<float>(f, V3<float>{float(pos[3 * i]), float(pos[3 * i + 1]), float(pos[3 * i + 2])}, linear != 0, v);
+                for(int c = 0; c < 4; c++)
+                    out[4 * i + c] = v[c];
+            }
+        }
+        return 0;
+    }
+
     void oracle_f32_to_f16(const float* in, uint16_t* out, int64_t n)
     {
         for(int64_t i = 0; i < n; i++)
