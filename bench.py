@@ -155,6 +155,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--prefetch", type=int, default=0, help="march kernel: fetch step i+1 before compositing step i")
+    ap.add_argument("--march-waves", type=int, default=0, help="march occupancy target (0, 6, 8)")
+    ap.add_argument("--no-tune", action="store_true", help="keep --prefetch / --march-waves instead of autotuning")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain device copy of this size first (scales profiler byte counters)")
     args = ap.parse_args()
@@ -190,8 +192,15 @@ def main() -> int:
     n_particles = int(spec.particles[0].pos.shape[0])
     n_visible = int(np.count_nonzero(spec.particles[0].attr[:, 0] >= 0.995))
 
+    vis.set_tuning(vis.TUNE_MARCH_WAVES, args.march_waves)
     form, dt = FORMS[args.form]
     vis.set_render_form(form, dt)
+    tuning = None
+    if not args.no_tune and args.form == "combined":
+        from isaac_amd import autotune
+
+        vis.update_buffers()
+        tuning = autotune.tune_march(vis, poses)
     wall, total_samples, avg_us, avg_counts, counts = measure(vis, poses, out, args.steps, args.warmup)
     ms_per_step = wall / args.steps * 1e3
     value = total_samples / wall
@@ -232,6 +241,7 @@ def main() -> int:
                               "fill": counts[1]["vol_samples"] + counts[1]["iso_samples"]},
         "kernel_ms": {k: v / 1e3 for k, v in avg_us.items()},
         "kernel_algorithmic_GBs": {k: kb[k] / (avg_us[k] * 1e-6) / 1e9 if avg_us[k] > 0 else None for k in kb},
+        "tuning": tuning,
         "roofline": {
             "kernel": dom,
             "bound": "hbm",

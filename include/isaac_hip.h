@@ -201,7 +201,8 @@ extern "C"
      * change, only speed); used by the autotuner. */
     enum isaac_hip_tuning_key
     {
-        ISAAC_HIP_TUNE_MARCH_PREFETCH = 0 /* 1: fetch step i+1 before compositing step i (default 0) */
+        ISAAC_HIP_TUNE_MARCH_PREFETCH = 0, /* 1: fetch step i+1 before compositing step i (default 0) */
+        ISAAC_HIP_TUNE_MARCH_WAVES = 1 /* march occupancy target: 0 (compiler default), 6 or 8 waves per SIMD */
     };
     int isaac_hip_set_tuning(isaac_hip_ctx* ctx, int32_t key, int32_t value);
 
@@ -242,6 +243,46 @@ extern "C"
      * isaac_hip_get_float_image returns (precision studies against an FP64 render). */
     int isaac_hip_set_float_output(isaac_hip_ctx* ctx, int enabled);
     int isaac_hip_get_float_image(const isaac_hip_ctx* ctx, const void** rgba_f32);
+
+    /* ---------------------------------------------------------------- learned occlusion
+     * No counterpart in the reference.  A U-Net over the G-buffer (input: depth /
+     * maxGlobalSizeScaled, background flag, normal; 3x3 convolutions with 16/32/64/64
+     * channels over four levels, FP16 weights and activations, FP32 accumulation on the
+     * matrix cores; 1x1 linear head) that stands in for the SSAO pass
+     * (SSAOKernel, lib/isaac/isaac_ssao_kernel.hpp:161-182) when the occlusion mode is
+     * LEARNED.  It is trained on the GPU from crops of frames rendered with the plain
+     * pass: the loss is sum((net - ao_plain)^2), the optimiser Adam.
+     * Parameters: one flat float vector; per convolution layer (e0a e0b e1a e1b e2a e2b
+     * e3a e3b d2 d1 d0) the weights [cout][9][cin] (tap = ky*3 + kx) then the bias
+     * [cout]; last the head's 16 weights and its bias. */
+    enum isaac_hip_ao_mode
+    {
+        ISAAC_HIP_AO_PLAIN = 0, /* the reference's 7x7 depth-comparison pass (default) */
+        ISAAC_HIP_AO_LEARNED = 1
+    };
+    int isaac_hip_set_ao_mode(isaac_hip_ctx* ctx, int32_t mode);
+    int64_t isaac_hip_aonet_param_count(void);
+    int isaac_hip_aonet_set_params(isaac_hip_ctx* ctx, const float* params, int64_t n); /* host; resets Adam */
+    int isaac_hip_aonet_get_params(isaac_hip_ctx* ctx, float* params, int64_t n);
+    /* training batch: square crops (side a multiple of 8, at most 64 crops) of the
+     * current G-buffer, whose ao must hold the plain pass's result (render with
+     * ambient occlusion on and mode PLAIN first); xy = n (x0, y0) pairs (host) */
+    int isaac_hip_aonet_clear_batch(isaac_hip_ctx* ctx);
+    int isaac_hip_aonet_add_crops(isaac_hip_ctx* ctx, int32_t n, const int32_t* xy, int32_t side);
+    /* one Adam step over the batch; *loss = mean squared error before the step */
+    int isaac_hip_aonet_train_step(isaac_hip_ctx* ctx, float lr, float* loss);
+    /* checks: loss and gradient (host arrays, n*h*w*8 features, n*h*w targets, h and
+     * w multiples of 8) and the raw network output; both overwrite the batch */
+    int isaac_hip_aonet_loss_grad(
+        isaac_hip_ctx* ctx,
+        int32_t n,
+        int32_t h,
+        int32_t w,
+        const float* features,
+        const float* target,
+        float* loss,
+        float* grad);
+    int isaac_hip_aonet_forward(isaac_hip_ctx* ctx, int32_t n, int32_t h, int32_t w, const float* features, float* out);
 
     /* ---------------------------------------------------------------- introspection */
     int isaac_hip_get_timings(const isaac_hip_ctx* ctx, isaac_hip_timings* out);

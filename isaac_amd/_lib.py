@@ -117,6 +117,15 @@ _SIGS = {
     "isaac_hip_get_bricks": (ctypes.c_int, [P, ctypes.c_int, P, P, P, P]),
     "isaac_hip_set_float_output": (ctypes.c_int, [P, ctypes.c_int]),
     "isaac_hip_get_float_image": (ctypes.c_int, [P, P]),
+    "isaac_hip_set_ao_mode": (ctypes.c_int, [P, I32]),
+    "isaac_hip_aonet_param_count": (ctypes.c_int64, []),
+    "isaac_hip_aonet_set_params": (ctypes.c_int, [P, P, ctypes.c_int64]),
+    "isaac_hip_aonet_get_params": (ctypes.c_int, [P, P, ctypes.c_int64]),
+    "isaac_hip_aonet_clear_batch": (ctypes.c_int, [P]),
+    "isaac_hip_aonet_add_crops": (ctypes.c_int, [P, I32, P, I32]),
+    "isaac_hip_aonet_train_step": (ctypes.c_int, [P, F, P]),
+    "isaac_hip_aonet_loss_grad": (ctypes.c_int, [P, I32, I32, I32, P, P, P, P]),
+    "isaac_hip_aonet_forward": (ctypes.c_int, [P, I32, I32, I32, P, P]),
     "isaac_hip_get_counters": (ctypes.c_int, [P, P]),
     "isaac_hip_get_gbuffer": (ctypes.c_int, [P, P, P, P, P]),
     "isaac_hip_get_combined_textures": (ctypes.c_int, [P, P, P]),

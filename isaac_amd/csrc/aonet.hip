@@ -1,0 +1,675 @@
+// Learned occlusion pass: U-Net kernels on gfx950 (forward, backward, Adam).
+//
+// Convolutions are implicit GEMMs on v_mfma_f32_16x16x16_f16.  For one
+// 16x16x16 step, lane l of the wave supplies
+//   A (16 x 16, rows = output channels):  row l % 16, k = 4*(l / 16) .. +3
+//   B (16 x 16, cols = pixels):           col l % 16, k = 4*(l / 16) .. +3
+// and receives D rows 4*(l / 16) .. +3 of column l % 16.  With k = tap*Cin + ci
+// and Cin a multiple of 4, the four k of a lane are four consecutive channels of
+// one tap: one 8-byte load each for A (weights) and B (input patch in LDS).
+#include "aonet.hpp"
+
+#include <cfloat>
+
+namespace isaac_hip
+{
+    typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+    typedef float f4 __attribute__((ext_vector_type(4)));
+
+    __device__ inline f4 mfma16(h4 a, h4 b, f4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+    }
+
+    // ------------------------------------------------------------------ convolution
+    // Workgroup = 64 consecutive pixels of one image row, four waves of 16 pixels,
+    // all output channels.  The 3 x 66 x Cin input patch is staged in LDS once.
+    template<int COUT, bool FWD>
+    __global__ __launch_bounds__(256) void conv3x3_kernel(const ConvArgs a)
+    {
+        extern __shared__ _Float16 patch[];
+        const int x0 = blockIdx.x * 64, y = blockIdx.y, n = blockIdx.z;
+        const int cin = a.cin;
+        const int units = 3 * 66 * cin / 4;
+        for(int u = threadIdx.x; u < units; u += 256)
+        {
+            const int e = u * 4;
+            const int pix = e / cin, c = e - pix * cin;
+            const int r = pix / 66, col = pix - r * 66;
+            const int yy = y + r - 1, xx = x0 + col - 1;
+            h4 v = {0, 0, 0, 0};
+            if(yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_)
+                v = *reinterpret_cast<const h4*>(a.x + ((size_t(n) * a.h + yy) * a.w_ + xx) * cin + c);
+            *reinterpret_cast<h4*>(patch + e) = v;
+        }
+        __syncthreads();
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int pl = wave * 16 + (lane & 15);
+        const int kq = 4 * (lane >> 4);
+        f4 acc[COUT / 16];
+#pragma unroll
+        for(int cb = 0; cb < COUT / 16; cb++)
+            acc[cb] = f4{0.f, 0.f, 0.f, 0.f};
+        const _Float16* wrow = a.w + size_t(lane & 15) * a.kp;
+        for(int kc = 0; kc < a.kp; kc += 16)
+        {
+            const int kk = kc + kq;
+            const int tap = kk / cin, ci = kk - tap * cin;
+            h4 b = {0, 0, 0, 0};
+            if(tap < 9)
+            {
+                const int ky = tap / 3, kx = tap - ky * 3;
+                b = *reinterpret_cast<const h4*>(patch + (ky * 66 + pl + kx) * cin + ci);
+            }
+#pragma unroll
+            for(int cb = 0; cb < COUT / 16; cb++)
+            {
+                const h4 w = *reinterpret_cast<const h4*>(wrow + size_t(cb * 16) * a.kp + kk);
+                acc[cb] = mfma16(w, b, acc[cb]);
+            }
+        }
+        const int x = x0 + pl;
+        if(x >= a.w_)
+            return;
+        const size_t p = (size_t(n) * a.h + y) * a.w_ + x;
+#pragma unroll
+        for(int cb = 0; cb < COUT / 16; cb++)
+        {
+            const int co = cb * 16 + 4 * (lane >> 4);
+            if(FWD)
+            {
+                h4 o;
+#pragma unroll
+                for(int r = 0; r < 4; r++)
+                {
+                    const float v = acc[cb][r] + a.bias[co + r];
+                    o[r] = _Float16(v > 0.0f ? v : 0.0f);
+                }
+                *reinterpret_cast<h4*>(reinterpret_cast<_Float16*>(a.y) + p * COUT + co) = o;
+            }
+            else
+            {
+                float4* dst = reinterpret_cast<float4*>(reinterpret_cast<float*>(a.y) + p * COUT + co);
+                float4 d = *dst;
+                d.x += acc[cb][0];
+                d.y += acc[cb][1];
+                d.z += acc[cb][2];
+                d.w += acc[cb][3];
+                *dst = d;
+            }
+        }
+    }
+
+    template<int COUT, bool FWD>
+    hipError_t conv_launch(const ConvArgs& a, hipStream_t s)
+    {
+        const dim3 grid(unsigned((a.w_ + 63) / 64), unsigned(a.h), unsigned(a.n));
+        const size_t lds = size_t(3 * 66) * a.cin * sizeof(_Float16);
+        hipLaunchKernelGGL((conv3x3_kernel<COUT, FWD>), grid, dim3(256), lds, s, a);
+        return hipGetLastError();
+    }
+
+    hipError_t launch_conv3x3(const ConvArgs& a, int cout, bool fwd, hipStream_t s)
+    {
+        if(a.cin % 4 != 0 || a.kp % 16 != 0 || a.kp < 9 * a.cin || 3 * 66 * a.cin * 2 > 64 * 1024)
+            return hipErrorInvalidValue;
+        if(a.n == 0 || a.h == 0 || a.w_ == 0)
+            return hipSuccess;
+        switch(cout)
+        {
+        case 16: return fwd ? conv_launch<16, true>(a, s) : conv_launch<16, false>(a, s);
+        case 32: return fwd ? conv_launch<32, true>(a, s) : conv_launch<32, false>(a, s);
+        case 48: return fwd ? conv_launch<48, true>(a, s) : conv_launch<48, false>(a, s);
+        case 64: return fwd ? conv_launch<64, true>(a, s) : conv_launch<64, false>(a, s);
+        case 96: return fwd ? conv_launch<96, true>(a, s) : conv_launch<96, false>(a, s);
+        case 128: return fwd ? conv_launch<128, true>(a, s) : conv_launch<128, false>(a, s);
+        default: return hipErrorInvalidValue;
+        }
+    }
+
+    // ------------------------------------------------------------------ weight gradient
+    // GEMM dW (cout x kp) = dZ^T (cout x P) . col(X) (P x kp) over the pixels P.
+    // Workgroup (tile group g, pixel split q): waves own up to 4 16x16 tiles of dW each
+    // (tile t = (cb, kb), t = g*16 + wave*4 + j) and accumulate over the 64-pixel row
+    // segments [q*segs, (q+1)*segs); one FP32 atomic per tile element at the end.
+    __global__ __launch_bounds__(256) void conv_dw_kernel(
+        const _Float16* __restrict__ x,
+        const _Float16* __restrict__ dz,
+        float* __restrict__ dw,
+        int n,
+        int h,
+        int w,
+        int cin,
+        int cout,
+        int kdim,
+        int segsPerBlock)
+    {
+        extern __shared__ _Float16 lds[];
+        _Float16* patch = lds; // [3][66][cin]
+        _Float16* dzt = lds + 3 * 66 * cin; // [64][cout]
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int ncb = cout / 16, nkb = (kdim + 15) / 16, ntiles = ncb * nkb;
+        const int segW = (w + 63) / 64;
+        const int64_t nseg = int64_t(n) * h * segW;
+        int tcb[4], tkb[4];
+        bool own[4];
+        for(int j = 0; j < 4; j++)
+        {
+            const int t = blockIdx.x * 16 + wave * 4 + j;
+            own[j] = t < ntiles;
+            tcb[j] = own[j] ? t / nkb : 0;
+            tkb[j] = own[j] ? t - tcb[j] * nkb : 0;
+        }
+        f4 acc[4];
+        for(int j = 0; j < 4; j++)
+            acc[j] = f4{0.f, 0.f, 0.f, 0.f};
+        const int64_t s0 = int64_t(blockIdx.y) * segsPerBlock;
+        const int64_t s1 = s0 + segsPerBlock < nseg ? s0 + segsPerBlock : nseg;
+        for(int64_t sg = s0; sg < s1; sg++)
+        {
+            const int xs = int(sg % segW) * 64;
+            const int64_t row = sg / segW;
+            const int y = int(row % h), img = int(row / h);
+            __syncthreads();
+            const int units = 3 * 66 * cin / 4;
+            for(int u = threadIdx.x; u < units; u += 256)
+            {
+                const int e = u * 4;
+                const int pix = e / cin, c = e - pix * cin;
+                const int r = pix / 66, col = pix - r * 66;
+                const int yy = y + r - 1, xx = xs + col - 1;
+                h4 v = {0, 0, 0, 0};
+                if(yy >= 0 && yy < h && xx >= 0 && xx < w)
+                    v = *reinterpret_cast<const h4*>(x + ((size_t(img) * h + yy) * w + xx) * cin + c);
+                *reinterpret_cast<h4*>(patch + e) = v;
+            }
+            const int dunits = 64 * cout / 4;
+            for(int u = threadIdx.x; u < dunits; u += 256)
+            {
+                const int e = u * 4;
+                const int pix = e / cout, c = e - pix * cout;
+                h4 v = {0, 0, 0, 0};
+                if(xs + pix < w)
+                    v = *reinterpret_cast<const h4*>(dz + ((size_t(img) * h + y) * w + xs + pix) * cout + c);
+                *reinterpret_cast<h4*>(dzt + e) = v;
+            }
+            __syncthreads();
+            for(int sub = 0; sub < 4; sub++)
+            {
+                const int pb = sub * 16 + 4 * (lane >> 4);
+                for(int j = 0; j < 4; j++)
+                {
+                    if(!own[j])
+                        continue;
+                    const int co = tcb[j] * 16 + (lane & 15);
+                    h4 av;
+                    for(int e = 0; e < 4; e++)
+                        av[e] = dzt[(pb + e) * cout + co];
+                    const int k = tkb[j] * 16 + (lane & 15);
+                    const int tap = k / cin, ci = k - tap * cin;
+                    h4 bv = {0, 0, 0, 0};
+                    if(tap < 9)
+                    {
+                        const int ky = tap / 3, kx = tap - ky * 3;
+                        for(int e = 0; e < 4; e++)
+                            bv[e] = patch[(ky * 66 + pb + e + kx) * cin + ci];
+                    }
+                    acc[j] = mfma16(av, bv, acc[j]);
+                }
+            }
+        }
+        for(int j = 0; j < 4; j++)
+        {
+            if(!own[j])
+                continue;
+            const int k = tkb[j] * 16 + (lane & 15);
+            if(k >= kdim)
+                continue;
+            for(int r = 0; r < 4; r++)
+            {
+                const int co = tcb[j] * 16 + 4 * (lane >> 4) + r;
+                if(acc[j][r] != 0.0f)
+                    atomicAdd(dw + size_t(co) * kdim + k, acc[j][r]);
+            }
+        }
+    }
+
+    hipError_t launch_conv3x3_dw(
+        const _Float16* x,
+        const _Float16* dz,
+        float* dw,
+        int n,
+        int h,
+        int w,
+        int cin,
+        int cout,
+        hipStream_t s)
+    {
+        if(cin % 4 != 0 || cout % 16 != 0)
+            return hipErrorInvalidValue;
+        const int kdim = 9 * cin;
+        const int64_t nseg = int64_t(n) * h * ((w + 63) / 64);
+        if(nseg == 0)
+            return hipSuccess;
+        const int ntiles = (cout / 16) * ((kdim + 15) / 16);
+        const int groups = (ntiles + 15) / 16;
+        // about 2048 pixels per workgroup, at least 256 workgroups in total when possible
+        int64_t segs = 32;
+        while(segs > 1 && groups * ((nseg + segs - 1) / segs) < 256)
+            segs /= 2;
+        const dim3 grid(unsigned(groups), unsigned((nseg + segs - 1) / segs));
+        const size_t lds = (size_t(3 * 66) * cin + size_t(64) * cout) * sizeof(_Float16);
+        if(lds > 64 * 1024)
+            return hipErrorInvalidValue;
+        hipLaunchKernelGGL(conv_dw_kernel, grid, dim3(256), lds, s, x, dz, dw, n, h, w, cin, cout, kdim, int(segs));
+        return hipGetLastError();
+    }
+
+    // ------------------------------------------------------------------ elementwise / reshaping
+    __global__ __launch_bounds__(256) void bias_grad_kernel(const _Float16* __restrict__ dz, float* db, int64_t npx, int c)
+    {
+        // thread t: channel t % c, pixel lane t / c; grid-stride over pixels
+        const int lanesPerBlock = 256 / c;
+        const int ch = threadIdx.x % c, pl = threadIdx.x / c;
+        float sum = 0.0f;
+        if(pl < lanesPerBlock)
+            for(int64_t p = int64_t(blockIdx.x) * lanesPerBlock + pl; p < npx; p += int64_t(gridDim.x) * lanesPerBlock)
+                sum += float(dz[p * c + ch]);
+        __shared__ float red[256];
+        red[threadIdx.x] = pl < lanesPerBlock ? sum : 0.0f;
+        __syncthreads();
+        if(threadIdx.x < c)
+        {
+            float t = 0.0f;
+            for(int i = 0; i < lanesPerBlock; i++)
+                t += red[i * c + threadIdx.x];
+            atomicAdd(db + threadIdx.x, t);
+        }
+    }
+
+    hipError_t launch_bias_grad(const _Float16* dz, float* db, int64_t npx, int c, hipStream_t s)
+    {
+        if(c > 256 || 256 % c != 0)
+            return hipErrorInvalidValue;
+        if(npx == 0)
+            return hipSuccess;
+        const int lanes = 256 / c;
+        int64_t blocks = (npx + lanes * 64 - 1) / (lanes * 64);
+        if(blocks > 1024)
+            blocks = 1024;
+        hipLaunchKernelGGL(bias_grad_kernel, dim3(unsigned(blocks)), dim3(256), 0, s, dz, db, npx, c);
+        return hipGetLastError();
+    }
+
+    __global__ __launch_bounds__(256) void relu_bwd_kernel(const float* dy, const _Float16* y, _Float16* dz, int64_t count)
+    {
+        for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x)
+            dz[i] = _Float16(float(y[i]) > 0.0f ? dy[i] : 0.0f);
+    }
+
+    static unsigned grid_for(int64_t count)
+    {
+        int64_t b = (count + 255) / 256;
+        return unsigned(b > 8192 ? 8192 : (b < 1 ? 1 : b));
+    }
+
+    hipError_t launch_relu_bwd(const float* dy, const _Float16* y, _Float16* dz, int64_t count, hipStream_t s)
+    {
+        if(count == 0)
+            return hipSuccess;
+        hipLaunchKernelGGL(relu_bwd_kernel, dim3(grid_for(count)), dim3(256), 0, s, dy, y, dz, count);
+        return hipGetLastError();
+    }
+
+    // output element i = ((n*ho + yo)*wo + xo)*c + ch
+    __global__ __launch_bounds__(256) void pool2_kernel(const _Float16* x, _Float16* y, int n, int h, int w, int c)
+    {
+        const int ho = h / 2, wo = w / 2;
+        const int64_t count = int64_t(n) * ho * wo * c;
+        for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x)
+        {
+            const int ch = int(i % c);
+            const int64_t q = i / c;
+            const int xo = int(q % wo), yo = int((q / wo) % ho);
+            const int64_t img = q / (int64_t(wo) * ho);
+            const _Float16* b = x + ((img * h + 2 * yo) * w + 2 * xo) * c + ch;
+            const float s = (float(b[0]) + float(b[c])) + (float(b[size_t(w) * c]) + float(b[size_t(w) * c + c]));
+            y[i] = _Float16(s * 0.25f);
+        }
+    }
+
+    hipError_t launch_pool2(const _Float16* x, _Float16* y, int n, int h, int w, int c, hipStream_t s)
+    {
+        const int64_t count = int64_t(n) * (h / 2) * (w / 2) * c;
+        if(count == 0)
+            return hipSuccess;
+        hipLaunchKernelGGL(pool2_kernel, dim3(grid_for(count)), dim3(256), 0, s, x, y, n, h, w, c);
+        return hipGetLastError();
+    }
+
+    // dx (n, h, w, c) += dy(n, h/2, w/2, c) / 4; one thread per dx element
+    __global__ __launch_bounds__(256) void pool2_bwd_kernel(const float* dy, float* dx, int n, int h, int w, int c)
+    {
+        const int64_t count = int64_t(n) * h * w * c;
+        for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x)
+        {
+            const int ch = int(i % c);
+            const int64_t q = i / c;
+            const int xx = int(q % w), yy = int((q / w) % h);
+            const int64_t img = q / (int64_t(w) * h);
+            dx[i] += 0.25f * dy[((img * (h / 2) + yy / 2) * (w / 2) + xx / 2) * c + ch];
+        }
+    }
+
+    hipError_t launch_pool2_bwd(const float* dy, float* dx, int n, int h, int w, int c, hipStream_t s)
+    {
+        const int64_t count = int64_t(n) * h * w * c;
+        if(count == 0)
+            return hipSuccess;
+        hipLaunchKernelGGL(pool2_bwd_kernel, dim3(grid_for(count)), dim3(256), 0, s, dy, dx, n, h, w, c);
+        return hipGetLastError();
+    }
+
+    // y (n, h, w, ca+cs): channels [0, ca) from a (n, h/2, w/2, ca), [ca, ca+cs) from skip
+    __global__ __launch_bounds__(256) void upcat_kernel(const _Float16* a, const _Float16* skip, _Float16* y, int n, int h, int w, int ca, int cs)
+    {
+        const int ct = ca + cs;
+        const int64_t count = int64_t(n) * h * w * ct / 4;
+        for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x)
+        {
+            const int64_t e = i * 4;
+            const int ch = int(e % ct);
+            const int64_t q = e / ct;
+            const int xx = int(q % w), yy = int((q / w) % h);
+            const int64_t img = q / (int64_t(w) * h);
+            h4 v;
+            if(ch < ca)
+                v = *reinterpret_cast<const h4*>(a + ((img * (h / 2) + yy / 2) * (w / 2) + xx / 2) * ca + ch);
+            else
+                v = *reinterpret_cast<const h4*>(skip + q * cs + (ch - ca));
+            *reinterpret_cast<h4*>(y + e) = v;
+        }
+    }
+
+    hipError_t launch_upcat(const _Float16* a, const _Float16* skip, _Float16* y, int n, int h, int w, int ca, int cs, hipStream_t s)
+    {
+        if(ca % 4 || cs % 4 || h % 2 || w % 2)
+            return hipErrorInvalidValue;
+        const int64_t count = int64_t(n) * h * w * (ca + cs) / 4;
+        if(count == 0)
+            return hipSuccess;
+        hipLaunchKernelGGL(upcat_kernel, dim3(grid_for(count)), dim3(256), 0, s, a, skip, y, n, h, w, ca, cs);
+        return hipGetLastError();
+    }
+
+    // da (n, h/2, w/2, ca) += sum of the 4 upsampled positions; dskip (n, h, w, cs) += dy[.., ca:]
+    __global__ __launch_bounds__(256) void upcat_bwd_a_kernel(const float* dy, float* da, int n, int h, int w, int ca, int cs)
+    {
+        const int ct = ca + cs, ho = h / 2, wo = w / 2;
+        const int64_t count = int64_t(n) * ho * wo * ca;
+        for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x)
+        {
+            const int ch = int(i % ca);
+            const int64_t q = i / ca;
+            const int xo = int(q % wo), yo = int((q / wo) % ho);
+            const int64_t img = q / (int64_t(wo) * ho);
+            const float* b = dy + ((img * h + 2 * yo) * w + 2 * xo) * ct + ch;
+            da[i] += (b[0] + b[ct]) + (b[size_t(w) * ct] + b[size_t(w) * ct + ct]);
+        }
+    }
+
+    __global__ __launch_bounds__(256) void upcat_bwd_skip_kernel(const float* dy, float* dskip, int n, int h, int w, int ca, int cs)
+    {
+        const int ct = ca + cs;
+        const int64_t count = int64_t(n) * h * w * cs;
+        for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x)
+        {
+            const int ch = int(i % cs);
+            const int64_t q = i / cs;
+            dskip[i] += dy[q * ct + ca + ch];
+        }
+    }
+
+    hipError_t launch_upcat_bwd(const float* dy, float* da, float* dskip, int n, int h, int w, int ca, int cs, hipStream_t s)
+    {
+        const int64_t ca_count = int64_t(n) * (h / 2) * (w / 2) * ca;
+        const int64_t cs_count = int64_t(n) * h * w * cs;
+        if(ca_count)
+            hipLaunchKernelGGL(upcat_bwd_a_kernel, dim3(grid_for(ca_count)), dim3(256), 0, s, dy, da, n, h, w, ca, cs);
+        if(cs_count)
+            hipLaunchKernelGGL(upcat_bwd_skip_kernel, dim3(grid_for(cs_count)), dim3(256), 0, s, dy, dskip, n, h, w, ca, cs);
+        return hipGetLastError();
+    }
+
+    // ------------------------------------------------------------------ features and head
+    __global__ __launch_bounds__(256) void features_kernel(const FeatureArgs a)
+    {
+        const int64_t count = int64_t(a.n) * a.h * a.w;
+        for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x)
+        {
+            const int xx = int(i % a.w), yy = int((i / a.w) % a.h);
+            const int item = int(i / (int64_t(a.w) * a.h));
+            const int gx = xx + (a.origin ? a.origin[2 * item] : 0), gy = yy + (a.origin ? a.origin[2 * item + 1] : 0);
+            float d = FLT_MAX, nx = 0.f, ny = 0.f, nz = 0.f, t = 0.f;
+            if(gx >= 0 && gx < a.imgW && gy >= 0 && gy < a.imgH)
+            {
+                const size_t p = size_t(gy) * a.imgW + gx;
+                d = a.depth[p];
+                nx = a.normal[3 * p + 0];
+                ny = a.normal[3 * p + 1];
+                nz = a.normal[3 * p + 2];
+                if(a.ao)
+                    t = a.ao[p];
+            }
+            const bool bg = !(d < 3.0e38f);
+            h4 lo, hi;
+            lo[0] = _Float16(bg ? 0.0f : d * a.invScale);
+            lo[1] = _Float16(bg ? 1.0f : 0.0f);
+            lo[2] = _Float16(nx);
+            lo[3] = _Float16(ny);
+            hi[0] = _Float16(nz);
+            hi[1] = hi[2] = hi[3] = _Float16(0.0f);
+            *reinterpret_cast<h4*>(a.x + i * 8) = lo;
+            *reinterpret_cast<h4*>(a.x + i * 8 + 4) = hi;
+            if(a.target)
+                a.target[i] = t;
+        }
+    }
+
+    hipError_t launch_features(const FeatureArgs& a, hipStream_t s)
+    {
+        const int64_t count = int64_t(a.n) * a.h * a.w;
+        if(count == 0)
+            return hipSuccess;
+        hipLaunchKernelGGL(features_kernel, dim3(grid_for(count)), dim3(256), 0, s, a);
+        return hipGetLastError();
+    }
+
+    __device__ inline float head_dot(const _Float16* d0, const float* wh, int64_t p)
+    {
+        const h4* v = reinterpret_cast<const h4*>(d0 + p * 16);
+        float s = wh[16];
+#pragma unroll
+        for(int q = 0; q < 4; q++)
+        {
+            const h4 u = v[q];
+#pragma unroll
+            for(int r = 0; r < 4; r++)
+                s += float(u[r]) * wh[q * 4 + r];
+        }
+        return s;
+    }
+
+    __global__ __launch_bounds__(256) void head_infer_kernel(const _Float16* d0, const float* wh, float* ao, int h, int w, int imgW, int imgH)
+    {
+        const int64_t count = int64_t(imgW) * imgH;
+        for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x)
+        {
+            const int xx = int(i % imgW), yy = int(i / imgW);
+            const float v = head_dot(d0, wh, int64_t(yy) * w + xx);
+            ao[i] = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+        }
+    }
+
+    hipError_t launch_head_infer(const _Float16* d0, const float* wh, float* ao, int h, int w, int imgW, int imgH, hipStream_t s)
+    {
+        if(imgW > w || imgH > h)
+            return hipErrorInvalidValue;
+        const int64_t count = int64_t(imgW) * imgH;
+        if(count == 0)
+            return hipSuccess;
+        hipLaunchKernelGGL(head_infer_kernel, dim3(grid_for(count)), dim3(256), 0, s, d0, wh, ao, h, w, imgW, imgH);
+        return hipGetLastError();
+    }
+
+    __global__ __launch_bounds__(256) void head_train_kernel(
+        const _Float16* d0,
+        const float* wh,
+        const float* target,
+        float* loss,
+        float* gwh,
+        float* dd0,
+        float* out,
+        int64_t npx)
+    {
+        float gl = 0.0f, gw[17];
+        for(int c = 0; c < 17; c++)
+            gw[c] = 0.0f;
+        for(int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < npx; p += int64_t(gridDim.x) * blockDim.x)
+        {
+            const float o = head_dot(d0, wh, p);
+            if(out)
+                out[p] = o;
+            const float e = o - target[p];
+            const float dout = 2.0f * e;
+            gl += e * e;
+            const h4* v = reinterpret_cast<const h4*>(d0 + p * 16);
+            for(int q = 0; q < 4; q++)
+            {
+                const h4 u = v[q];
+                float4 o;
+                o.x = dout * wh[q * 4 + 0];
+                o.y = dout * wh[q * 4 + 1];
+                o.z = dout * wh[q * 4 + 2];
+                o.w = dout * wh[q * 4 + 3];
+                reinterpret_cast<float4*>(dd0 + p * 16)[q] = o;
+                for(int r = 0; r < 4; r++)
+                    gw[q * 4 + r] += dout * float(u[r]);
+            }
+            gw[16] += dout;
+        }
+        __shared__ float red[18][256];
+        red[0][threadIdx.x] = gl;
+        for(int c = 0; c < 17; c++)
+            red[c + 1][threadIdx.x] = gw[c];
+        __syncthreads();
+        for(int off = 128; off > 0; off >>= 1)
+        {
+            if(int(threadIdx.x) < off)
+                for(int c = 0; c < 18; c++)
+                    red[c][threadIdx.x] += red[c][threadIdx.x + off];
+            __syncthreads();
+        }
+        if(threadIdx.x == 0)
+            atomicAdd(loss, red[0][0]);
+        if(threadIdx.x >= 1 && threadIdx.x < 18)
+            atomicAdd(gwh + threadIdx.x - 1, red[threadIdx.x][0]);
+    }
+
+    hipError_t launch_head_train(
+        const _Float16* d0,
+        const float* wh,
+        const float* target,
+        float* loss,
+        float* gwh,
+        float* dd0,
+        float* out,
+        int64_t npx,
+        hipStream_t s)
+    {
+        if(npx == 0)
+            return hipSuccess;
+        int64_t blocks = (npx + 255) / 256;
+        if(blocks > 2048)
+            blocks = 2048;
+        hipLaunchKernelGGL(head_train_kernel, dim3(unsigned(blocks)), dim3(256), 0, s, d0, wh, target, loss, gwh, dd0, out, npx);
+        return hipGetLastError();
+    }
+
+    // ------------------------------------------------------------------ optimiser
+    __global__ __launch_bounds__(256) void adam_kernel(
+        float* w,
+        const float* g,
+        float* m,
+        float* v,
+        int64_t count,
+        float lr,
+        float b1,
+        float b2,
+        float eps,
+        float c1,
+        float c2,
+        float gscale)
+    {
+        for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x)
+        {
+            const float gi = g[i] * gscale;
+            const float mi = b1 * m[i] + (1.0f - b1) * gi;
+            const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+            m[i] = mi;
+            v[i] = vi;
+            w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        }
+    }
+
+    hipError_t launch_adam(
+        float* w,
+        const float* g,
+        float* m,
+        float* v,
+        int64_t count,
+        float lr,
+        float b1,
+        float b2,
+        float eps,
+        float c1,
+        float c2,
+        float gscale,
+        hipStream_t s)
+    {
+        if(count == 0)
+            return hipSuccess;
+        hipLaunchKernelGGL(adam_kernel, dim3(grid_for(count)), dim3(256), 0, s, w, g, m, v, count, lr, b1, b2, eps, c1, c2, gscale);
+        return hipGetLastError();
+    }
+
+    __global__ __launch_bounds__(256) void pack_weights_kernel(const float* master, _Float16* wf, _Float16* wb, int cin, int cout, int kp)
+    {
+        const int K = 9 * cin;
+        const int64_t nf = int64_t(cout) * kp;
+        const int64_t nb = int64_t(cin) * 9 * cout;
+        for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nf + nb; i += int64_t(gridDim.x) * blockDim.x)
+        {
+            if(i < nf)
+            {
+                const int co = int(i / kp), k = int(i - int64_t(co) * kp);
+                wf[i] = _Float16(k < K ? master[int64_t(co) * K + k] : 0.0f);
+            }
+            else
+            {
+                const int64_t j = i - nf; // [ci][t][co]
+                const int co = int(j % cout);
+                const int t = int((j / cout) % 9);
+                const int ci = int(j / (int64_t(cout) * 9));
+                wb[j] = _Float16(master[(int64_t(co) * 9 + (8 - t)) * cin + ci]);
+            }
+        }
+    }
+
+    hipError_t launch_pack_weights(const float* master, _Float16* wf, _Float16* wb, int cin, int cout, int kp, hipStream_t s)
+    {
+        const int64_t count = int64_t(cout) * kp + int64_t(cin) * 9 * cout;
+        hipLaunchKernelGGL(pack_weights_kernel, dim3(grid_for(count)), dim3(256), 0, s, master, wf, wb, cin, cout, kp);
+        return hipGetLastError();
+    }
+} // namespace isaac_hip

@@ -1,0 +1,111 @@
+// Kernels of the learned occlusion pass: a U-Net over the G-buffer whose 3x3
+// convolutions run as implicit GEMMs on the gfx950 matrix cores
+// (v_mfma_f32_16x16x16_f16: FP16 operands, FP32 accumulation).
+//
+// Tensors are channels-last: element (n, y, x, c) at ((n*H + y)*W + x)*C + c.
+// Convolution weights are [Cout][Kp] FP16 with k = tap*Cin + ci (tap = ky*3 + kx
+// reads pixel (y + ky - 1, x + kx - 1), zero padding), Kp = 9*Cin rounded up to
+// a multiple of 16 with zero columns.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace isaac_hip
+{
+    struct ConvArgs
+    {
+        const _Float16* x; // [n][h][w][cin], cin a multiple of 4
+        const _Float16* w; // [cout][kp]
+        const float* bias; // [cout] (forward only)
+        void* y; // forward: _Float16 [n][h][w][cout] = relu(conv + bias); backward data: float, +=
+        int32_t n, h, w_, cin, kp;
+    };
+
+    // cout in {16, 32, 48, 64, 96, 128}; relu_fp16_out: forward form, else FP32 accumulate
+    hipError_t launch_conv3x3(const ConvArgs& a, int cout, bool relu_fp16_out, hipStream_t s);
+
+    // dW[co][k] += sum_p dz[p][co] * col(x)[p][k] for k < 9*cin (FP32 atomics into dw,
+    // rows of 9*cin floats: the master weight layout [cout][9][cin])
+    hipError_t launch_conv3x3_dw(
+        const _Float16* x,
+        const _Float16* dz,
+        float* dw,
+        int n,
+        int h,
+        int w,
+        int cin,
+        int cout,
+        hipStream_t s);
+    // db[c] += sum_p dz[p][c]
+    hipError_t launch_bias_grad(const _Float16* dz, float* db, int64_t npx, int c, hipStream_t s);
+    // dz = half(dy * (y > 0))
+    hipError_t launch_relu_bwd(const float* dy, const _Float16* y, _Float16* dz, int64_t count, hipStream_t s);
+    // 2x2 average pool (h, w even) and its backward (dx += dy / 4)
+    hipError_t launch_pool2(const _Float16* x, _Float16* y, int n, int h, int w, int c, hipStream_t s);
+    hipError_t launch_pool2_bwd(const float* dy, float* dx, int n, int h, int w, int c, hipStream_t s);
+    // y = concat(nearest-upsample(a), skip) at the skip's size; backward splits dy
+    hipError_t launch_upcat(const _Float16* a, const _Float16* skip, _Float16* y, int n, int h, int w, int ca, int cs, hipStream_t s);
+    hipError_t launch_upcat_bwd(const float* dy, float* da, float* dskip, int n, int h, int w, int ca, int cs, hipStream_t s);
+
+    // network input from a G-buffer window: 8 channels (depth / scale or 0, background
+    // flag, normal xyz, 0, 0, 0); pixels outside the image are background.  With `ao`
+    // and `target` set, also copies the occlusion term as the training target.
+    struct FeatureArgs
+    {
+        const float* depth; // [imgH][imgW]
+        const float* normal; // [imgH][imgW][3]
+        const float* ao; // [imgH][imgW] or nullptr
+        int32_t imgW, imgH;
+        float invScale;
+        const int32_t* origin; // per batch item: x0, y0 of the window (device), or nullptr = (0, 0)
+        _Float16* x; // [n][h][w][8]
+        float* target; // [n][h][w] or nullptr
+        int32_t n, h, w;
+    };
+    hipError_t launch_features(const FeatureArgs& a, hipStream_t s);
+
+    // 1x1 head 16 -> 1.  Inference: ao[y][x] = clamp(out, 0, 1) for x < imgW, y < imgH.
+    hipError_t launch_head_infer(const _Float16* d0, const float* wh, float* ao, int h, int w, int imgW, int imgH, hipStream_t s);
+    // Training: out = d0 . wh + bh (wh[16] = bh); loss += sum (out - t)^2;
+    // gwh[0..15] += sum dout * d0, gwh[16] += sum dout; dd0 = dout * wh with
+    // dout = 2 (out - t); `out` (nullable) receives the raw outputs
+    hipError_t launch_head_train(
+        const _Float16* d0,
+        const float* wh,
+        const float* target,
+        float* loss,
+        float* gwh,
+        float* dd0,
+        float* out,
+        int64_t npx,
+        hipStream_t s);
+
+    // Adam over the flat FP32 parameter vector (gradients scaled by gscale first)
+    hipError_t launch_adam(
+        float* w,
+        const float* g,
+        float* m,
+        float* v,
+        int64_t count,
+        float lr,
+        float b1,
+        float b2,
+        float eps,
+        float c1,
+        float c2,
+        float gscale,
+        hipStream_t s);
+    // FP16 forward weights [cout][kp] and flipped-transposed backward weights
+    // [cin][9*cout] (w'[ci][t][co] = w[co][8-t][ci]) from FP32 master [cout][9][cin]
+    hipError_t launch_pack_weights(
+        const float* master,
+        _Float16* wf,
+        _Float16* wb,
+        int cin,
+        int cout,
+        int kp,
+        hipStream_t s);
+} // namespace isaac_hip

@@ -83,6 +83,7 @@ namespace isaac_hip
         float step;
         int32_t linear;
         int32_t prefetch; // fetch the next step's taps before compositing the current one
+        int32_t waves; // march occupancy target: 0 compiler default, 6 or 8 waves per SIMD
         // textures
         int32_t X, Y, Z;
         const uint2* tex; // interleaved combined textures, x fastest
@@ -127,8 +128,9 @@ namespace isaac_hip
         int32_t active; // weight + iso > 0
         float weight;
         float iso;
+        float isoScale; // 0.5 / iso: iso value = alpha * isoScale
         ChainDesc chain;
-        int32_t lutSlot;
+        int32_t lutSlot; // source index of its transfer function in luts
     };
 
     struct PostParams
@@ -136,10 +138,9 @@ namespace isaac_hip
         int32_t nsrc;
         int32_t dtype; // ISAAC_HIP_F32 / F16 / BF16
         int32_t nbx, nby, nbz;
-        float totalWeight; // sum of volume-source weights (classification normaliser)
+        float invTotalWeight; // 1 / sum of volume-source weights (0 if that sum is 0)
         int32_t T;
-        int32_t nLut; // LUTs staged in LDS
-        const float4* luts;
+        const float4* luts; // all sources' transfer functions, T entries each
         double invd[16]; // inverse MVP in binary64 (ray setup in FP64)
         double maxSized; // maxGlobalSizeScaled * 0.5
         double cposd[kMaxClipping][3]; // clipping plane positions, scaled and offset, binary64

@@ -1,0 +1,127 @@
+// Internal definition of the render context shared by the host translation
+// units of the library (host.cpp: rendering and steering; aonet_host.cpp: the
+// learned occlusion pass).  Not part of the C-ABI.
+#pragma once
+
+#include "isaac_hip.h"
+#include "kernels.hpp"
+
+#include <string>
+#include <vector>
+
+namespace isaac_hip
+{
+    struct AoNet; // learned occlusion network (aonet_host.cpp)
+
+    struct VolSlot
+    {
+        isaac_hip_volume_desc desc{};
+        bool set = false;
+        float* npTex = nullptr; // non-persistent functor-applied texture (with guard)
+        size_t npBytes = 0;
+        void* brick = nullptr; // brick form: pairs
+        float* brickScale = nullptr;
+        size_t brickBytes = 0, scaleCount = 0;
+        int64_t nbricks = 0;
+        int brickDim = 0;
+        bool bricked = false; // bricks hold the source as of the last update_buffers
+    };
+
+    struct PartSlot
+    {
+        isaac_hip_particle_desc desc{};
+        bool set = false;
+        int64_t capN = 0, capCells = 0;
+        BinBuffers B{};
+        void* blocks[12] = {};
+    };
+} // namespace isaac_hip
+
+struct isaac_hip_ctx
+{
+    using VolSlot = isaac_hip::VolSlot;
+    using PartSlot = isaac_hip::PartSlot;
+    isaac_hip_config cfg{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool ownStream = false;
+    std::string err;
+    int nv = 0, nf = 0, np = 0, nsrc = 0;
+    std::vector<VolSlot> vols;
+    std::vector<PartSlot> parts;
+    std::vector<isaac_hip::ChainDesc> chain;
+    std::vector<int32_t> chainErr;
+    std::vector<std::vector<float>> lutHost;
+    float4* lutDev = nullptr;
+    bool lutDirty = true;
+    std::vector<float> weight, iso;
+    float step = 0.5f;
+    bool interpolation = false;
+    int ditherMode = 1;
+    bool aoEnabled = false;
+    float aoWeight = 0.5f;
+    int renderMode = 0;
+    int nclip = 0;
+    float clipPos[isaac_hip::kMaxClipping][3];
+    float clipNormal[isaac_hip::kMaxClipping][3];
+    float clipSaved[isaac_hip::kMaxClipping][3];
+    // sizes
+    int64_t global[3], local[3], pgrid[3];
+    int32_t position[3];
+    float scale[3];
+    int64_t gsScaled[3], lsScaled[3], maxGsScaled;
+    int32_t posScaled[3];
+    // device buffers
+    uint2* tex = nullptr; // per voxel: combined volume RGBA8, combined iso RGBA8
+    size_t texVox = 0;
+    uint32_t* color = nullptr;
+    float* depth = nullptr;
+    float* normal = nullptr;
+    float* ao = nullptr;
+    float4* volColor = nullptr;
+    unsigned long long* counters = nullptr;
+    bool counting = false;
+    bool classified = false;
+    int tunePrefetch = 0;
+    int tuneWaves = 0;
+    int form = ISAAC_HIP_FORM_COMBINED;
+    int brickDtype = ISAAC_HIP_F32;
+    int builtForm = -1; // form prepared by the last update_buffers
+    bool floatOut = false;
+    float4* outFloat = nullptr;
+    int aoMode = ISAAC_HIP_AO_PLAIN;
+    isaac_hip::AoNet* aonet = nullptr;
+    // events: 0 update start, 1 after source copies, 2 after binning, 3 after
+    // classification (or brick build); 4 draw start, 5 after the march, 6 after the
+    // occlusion pass, 7 after shading, 8 after the framebuffer copy
+    hipEvent_t ev[12] = {};
+    bool evRecorded[2] = {false, false};
+};
+
+namespace isaac_hip
+{
+    inline int fail(isaac_hip_ctx* c, int code, const std::string& msg)
+    {
+        if(c)
+            c->err = msg;
+        return code;
+    }
+
+    // learned occlusion pass (aonet_host.cpp)
+    int aonet_occlusion(isaac_hip_ctx* c);
+    void aonet_destroy(isaac_hip_ctx* c);
+    int64_t aonet_param_count();
+
+    inline int hip_fail(isaac_hip_ctx* c, hipError_t e, const char* what)
+    {
+        return fail(c, ISAAC_HIP_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    }
+
+#define HIPCHK(ctx, call)                                                                                             \
+    do                                                                                                                \
+    {                                                                                                                 \
+        hipError_t e_ = (call);                                                                                       \
+        if(e_ != hipSuccess)                                                                                          \
+            return hip_fail(ctx, e_, #call);                                                                          \
+    } while(0)
+} // namespace isaac_hip

@@ -2,8 +2,7 @@
 // include/isaac_hip.h.  It owns the device buffers of one rank's render state
 // (the reference's IsaacVisualization members, lib/isaac.hpp:565-945) and
 // turns the steering setters into kernel parameter blocks.
-#include "isaac_hip.h"
-#include "kernels.hpp"
+#include "context.hpp"
 
 #include <algorithm>
 #include <cctype>
@@ -291,107 +290,11 @@ namespace
         }
     }
 
-    struct VolSlot
-    {
-        isaac_hip_volume_desc desc{};
-        bool set = false;
-        float* npTex = nullptr; // non-persistent functor-applied texture (with guard)
-        size_t npBytes = 0;
-        void* brick = nullptr; // brick form: pairs
-        float* brickScale = nullptr;
-        size_t brickBytes = 0, scaleCount = 0;
-        int64_t nbricks = 0;
-        int brickDim = 0;
-        bool bricked = false; // bricks hold the source as of the last update_buffers
-    };
-
-    struct PartSlot
-    {
-        isaac_hip_particle_desc desc{};
-        bool set = false;
-        int64_t capN = 0, capCells = 0;
-        BinBuffers B{};
-        void* blocks[12] = {};
-    };
 } // namespace
 
-struct isaac_hip_ctx
-{
-    isaac_hip_config cfg{};
-    int device = 0;
-    hipStream_t stream = nullptr;
-    bool ownStream = false;
-    std::string err;
-    int nv = 0, nf = 0, np = 0, nsrc = 0;
-    std::vector<VolSlot> vols;
-    std::vector<PartSlot> parts;
-    std::vector<ChainDesc> chain;
-    std::vector<int32_t> chainErr;
-    std::vector<std::vector<float>> lutHost;
-    float4* lutDev = nullptr;
-    bool lutDirty = true;
-    std::vector<float> weight, iso;
-    float step = 0.5f;
-    bool interpolation = false;
-    int ditherMode = 1;
-    bool aoEnabled = false;
-    float aoWeight = 0.5f;
-    int renderMode = 0;
-    int nclip = 0;
-    float clipPos[kMaxClipping][3];
-    float clipNormal[kMaxClipping][3];
-    float clipSaved[kMaxClipping][3];
-    // sizes
-    int64_t global[3], local[3], pgrid[3];
-    int32_t position[3];
-    float scale[3];
-    int64_t gsScaled[3], lsScaled[3], maxGsScaled;
-    int32_t posScaled[3];
-    // device buffers
-    uint2* tex = nullptr; // per voxel: combined volume RGBA8, combined iso RGBA8
-    size_t texVox = 0;
-    uint32_t* color = nullptr;
-    float* depth = nullptr;
-    float* normal = nullptr;
-    float* ao = nullptr;
-    float4* volColor = nullptr;
-    unsigned long long* counters = nullptr;
-    bool counting = false;
-    bool classified = false;
-    int tunePrefetch = 0;
-    int form = ISAAC_HIP_FORM_COMBINED;
-    int brickDtype = ISAAC_HIP_F32;
-    int builtForm = -1; // form prepared by the last update_buffers
-    bool floatOut = false;
-    float4* outFloat = nullptr;
-    // events: 0 update start, 1 after source copies, 2 after binning, 3 after
-    // classification; 4 draw start, 5 after solid, 6 after SSAO, 7 after volume, 8 after copy
-    hipEvent_t ev[12] = {};
-    bool evRecorded[2] = {false, false};
-};
 
 namespace
 {
-    int fail(isaac_hip_ctx* c, int code, const std::string& msg)
-    {
-        if(c)
-            c->err = msg;
-        return code;
-    }
-
-    int hip_fail(isaac_hip_ctx* c, hipError_t e, const char* what)
-    {
-        return fail(c, ISAAC_HIP_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
-    }
-
-#define HIPCHK(ctx, call)                                                                                             \
-    do                                                                                                                \
-    {                                                                                                                 \
-        hipError_t e_ = (call);                                                                                       \
-        if(e_ != hipSuccess)                                                                                          \
-            return hip_fail(ctx, e_, #call);                                                                          \
-    } while(0)
-
     void derive_sizes(isaac_hip_ctx* c)
     {
         // the ctor's float conversions (lib/isaac.hpp:659-661,796-797)
@@ -728,6 +631,7 @@ extern "C"
         }
         for(auto& p : c->parts)
             free_part(p);
+        aonet_destroy(c);
         void* bufs[] = {c->lutDev, c->tex, c->color, c->depth, c->normal, c->ao, c->volColor, c->counters, c->outFloat};
         for(void* b : bufs)
             if(b)
@@ -1233,6 +1137,7 @@ extern "C"
         F.step = c->step;
         F.linear = c->interpolation ? 1 : 0;
         F.prefetch = c->tunePrefetch;
+        F.waves = c->tuneWaves;
         F.X = int32_t(c->local[0]);
         F.Y = int32_t(c->local[1]);
         F.Z = int32_t(c->local[2]);
@@ -1325,8 +1230,7 @@ extern "C"
             float tw = 0.0f;
             for(int s = 0; s < nvf; s++)
                 tw += c->weight[s];
-            Q.totalWeight = tw;
-            int maxSlot = -1;
+            Q.invTotalWeight = tw != 0.0f ? 1.0f / tw : 0.0f;
             for(int s = 0; s < nvf; s++)
             {
                 const VolSlot& v = c->vols[s];
@@ -1341,21 +1245,35 @@ extern "C"
                 B.active = 1;
                 B.weight = c->weight[s];
                 B.iso = c->iso[s];
+                B.isoScale = B.iso > 0.0f ? 0.5f / B.iso : 0.0f;
                 B.chain = v.desc.persistent ? c->chain[s] : ChainDesc{};
                 B.lutSlot = s;
-                maxSlot = s;
             }
-            // stage LUT slots 0..maxSlot in LDS when they fit in 64 KiB
-            const size_t need = size_t(maxSlot + 1) * size_t(T) * sizeof(float4);
-            Q.nLut = (maxSlot >= 0 && need <= 64 * 1024) ? maxSlot + 1 : 0;
+            // the enabled sources' transfer functions are staged in LDS (64 KiB per workgroup)
+            if(size_t(Q.nsrc) * size_t(T) * sizeof(float4) > 64 * 1024)
+                return fail(c, ISAAC_HIP_ERR_UNSUPPORTED, "brick form: enabled sources x transfer size above 4096 entries");
         }
         if(c->counting)
-            HIPCHK(c, launch_fill_u32(reinterpret_cast<uint32_t*>(c->counters), 0u, 16, c->stream));
+            HIPCHK(c, launch_fill_u32(reinterpret_cast<uint32_t*>(c->counters), 0u, 2 * ISAAC_HIP_CNT_COUNT, c->stream));
         HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
         if(c->form == ISAAC_HIP_FORM_BRICKS)
-            HIPCHK(c, launch_brick_frame(F, Q, PP, c->counting, c->stream, c->ev[5], c->ev[6]));
+            HIPCHK(c, launch_brick_march(F, Q, PP, c->counting, c->stream));
         else
-            HIPCHK(c, launch_frame(F, PP, c->counting, c->stream, c->ev[5], c->ev[6]));
+            HIPCHK(c, launch_march(F, PP, c->counting, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
+        if(F.aoEnabled)
+        {
+            if(c->aoMode == ISAAC_HIP_AO_LEARNED)
+            {
+                int rc = aonet_occlusion(c);
+                if(rc)
+                    return rc;
+            }
+            else
+                HIPCHK(c, launch_ssao(F, c->stream));
+        }
+        HIPCHK(c, hipEventRecord(c->ev[6], c->stream));
+        HIPCHK(c, launch_shade(F, c->stream));
         HIPCHK(c, hipEventRecord(c->ev[7], c->stream));
         if(out_rgba)
         {
@@ -1418,6 +1336,11 @@ extern "C"
         {
         case ISAAC_HIP_TUNE_MARCH_PREFETCH:
             c->tunePrefetch = value != 0;
+            return 0;
+        case ISAAC_HIP_TUNE_MARCH_WAVES:
+            if(value != 0 && value != 6 && value != 8)
+                return fail(c, ISAAC_HIP_ERR_ARG, "march waves: 0, 6 or 8");
+            c->tuneWaves = value;
             return 0;
         default:
             return fail(c, ISAAC_HIP_ERR_ARG, "unknown tuning key");

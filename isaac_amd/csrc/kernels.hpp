@@ -17,15 +17,11 @@ namespace isaac_hip
         int guard,
         float* dst,
         hipStream_t s);
-    // solid pass, SSAO (when enabled), shading + volume; afterSolid / afterSsao are
-    // recorded between them (per-kernel GPU time)
-    hipError_t launch_frame(
-        const FrameParams& F,
-        const ParticleParams& PP,
-        bool counting,
-        hipStream_t s,
-        hipEvent_t afterSolid,
-        hipEvent_t afterSsao);
+    // the fused march (clear, particles, iso + volume), the SSAO pass and shading
+    // + volume blend; the host records its per-kernel events between them
+    hipError_t launch_march(const FrameParams& F, const ParticleParams& PP, bool counting, hipStream_t s);
+    hipError_t launch_ssao(const FrameParams& F, hipStream_t s);
+    hipError_t launch_shade(const FrameParams& F, hipStream_t s);
 
     // ---- precision form: field bricks and the post-classification march
     // src: local voxel (0,0,0) of an AoS float source (dim components, element strides);
@@ -41,14 +37,7 @@ namespace isaac_hip
         float* scale,
         void* out,
         hipStream_t s);
-    hipError_t launch_brick_frame(
-        const FrameParams& F,
-        const PostParams& Q,
-        const ParticleParams& PP,
-        bool counting,
-        hipStream_t s,
-        hipEvent_t afterMarch,
-        hipEvent_t afterSsao);
+    hipError_t launch_brick_march(const FrameParams& F, const PostParams& Q, const ParticleParams& PP, bool counting, hipStream_t s);
 
     // ---- particle cell lists
     struct BinParams

@@ -394,6 +394,20 @@ namespace isaac_hip
         return div255(A * ox + B * T.fx);
     }
 
+    // bitwise OR of the taps' volume (.x) or iso (.y) texels
+    template<bool LINEAR, bool ISO>
+    __device__ inline uint32_t taps_or(const Taps& T)
+    {
+        uint32_t o = ISO ? T.t[0].y : T.t[0].x;
+        if(LINEAR)
+        {
+#pragma unroll
+            for(int k = 1; k < 8; k++)
+                o |= ISO ? T.t[k].y : T.t[k].x;
+        }
+        return o;
+    }
+
     template<bool LINEAR>
     __device__ inline float iso_alpha_at(const FrameParams& F, float px, float py, float pz)
     {
@@ -564,7 +578,7 @@ namespace isaac_hip
     // The taps of step i+1 are fetched before step i is composited (clamped
     // addresses, so a fetch past the last step stays in bounds).
     template<bool LINEAR, bool COUNT, bool PREFETCH>
-    __global__ __launch_bounds__(256) void march_kernel(const FrameParams F, const ParticleParams PP)
+    __device__ __attribute__((always_inline)) inline void march_body(const FrameParams& F, const ParticleParams& PP)
     {
         int px, py;
         const bool valid = tile_pixel(F, px, py);
@@ -631,7 +645,8 @@ namespace isaac_hip
                             }
                             if(isoOn)
                             {
-                                const float value = channel<LINEAR, true>(cur, 24);
+                                // eight zero alpha bytes interpolate to exactly 0
+                                const float value = (taps_or<LINEAR, true>(cur) >> 24) ? channel<LINEAR, true>(cur, 24) : 0.0f;
                                 if(COUNT)
                                     nIso++;
                                 const float tmp = oldValue;
@@ -660,21 +675,26 @@ namespace isaac_hip
                                         endVol = int(floorf(ve / stepU));
                                 }
                             }
+                            // eight zero volume texels give the sample (0,0,0,0), which leaves
+                            // the accumulated colour unchanged: skip its arithmetic
                             if(volOn && i <= endVol)
                             {
                                 if(COUNT)
                                     nVol++;
-                                const float vr = channel<LINEAR, false>(cur, 0) * factor;
-                                const float vg = channel<LINEAR, false>(cur, 8) * factor;
-                                const float vb = channel<LINEAR, false>(cur, 16) * factor;
-                                const float va = channel<LINEAR, false>(cur, 24) * factor;
-                                const float w = 1.0f - ca;
-                                cr = cr + w * vr;
-                                cg = cg + w * vg;
-                                cb = cb + w * vb;
-                                ca = ca + w * va;
-                                if(ca > 0.99f)
-                                    volOn = false;
+                                if(taps_or<LINEAR, false>(cur) != 0u)
+                                {
+                                    const float vr = channel<LINEAR, false>(cur, 0) * factor;
+                                    const float vg = channel<LINEAR, false>(cur, 8) * factor;
+                                    const float vb = channel<LINEAR, false>(cur, 16) * factor;
+                                    const float va = channel<LINEAR, false>(cur, 24) * factor;
+                                    const float w = 1.0f - ca;
+                                    cr = cr + w * vr;
+                                    cg = cg + w * vg;
+                                    cb = cb + w * vb;
+                                    ca = ca + w * va;
+                                    if(ca > 0.99f)
+                                        volOn = false;
+                                }
                             }
                             last = isoOn ? endIso : -0x7fffffff;
                             if(volOn && endVol > last)
@@ -739,6 +759,25 @@ namespace isaac_hip
             block_count_add(F.counters + 7, nPartRays);
             block_count_add(F.counters + 8, nSteps);
         }
+    }
+
+    // The march at the compiler's register allocation, and at occupancy targets of
+    // 6 and 8 waves per SIMD (at most 80 / 64 VGPRs); the launcher picks one by the
+    // tuning key ISAAC_HIP_TUNE_MARCH_WAVES.
+    template<bool LINEAR, bool COUNT, bool PREFETCH>
+    __global__ __launch_bounds__(256) void march_kernel(const FrameParams F, const ParticleParams PP)
+    {
+        march_body<LINEAR, COUNT, PREFETCH>(F, PP);
+    }
+    template<bool LINEAR>
+    __global__ __launch_bounds__(256, 6) void march_kernel_w6(const FrameParams F, const ParticleParams PP)
+    {
+        march_body<LINEAR, false, false>(F, PP);
+    }
+    template<bool LINEAR>
+    __global__ __launch_bounds__(256, 8) void march_kernel_w8(const FrameParams F, const ParticleParams PP)
+    {
+        march_body<LINEAR, false, false>(F, PP);
     }
 
     // ------------------------------------------------------------------ SSAO
@@ -989,32 +1028,38 @@ namespace isaac_hip
         }
     }
 
-    // raw sample of every component (Sampler LINEAR|NEAREST, CLAMP addressing,
-    // lib/isaac/isaac_texture.hpp:182-259).  The taps of a sample all come from the
-    // brick holding its clamped (x0, y0, z0) tap; x1 = x0 (clamped at the edge)
-    // duplicates the first element of each pair.  Component c of a brick sits
-    // c * 648 pairs after component 0.
-    template<int DT, bool LINEAR>
-    __device__ inline void brick_sample(
-        const PostParams& Q, const BrickSource& S, int X, int Y, int Z, float px, float py, float pz, float v[4])
+    // Tap geometry of one sample position, shared by every source (all sources
+    // of a rank have the local box's brick grid).  Sampler LINEAR|NEAREST with
+    // CLAMP addressing (lib/isaac/isaac_texture.hpp:182-259): the taps of a sample
+    // all come from the brick holding its clamped (x0, y0, z0) tap; x1 = x0
+    // (clamped at the edge) duplicates the first element of each pair.
+    struct BrickTaps
     {
+        uint32_t brick; // brick index
+        uint32_t i00, i10, i01, i11; // pair index in a component block of rows (y0,z0) (y1,z0) (y0,z1) (y1,z1)
+        bool dup;
+        float fx, fy, fz;
+    };
+
+    template<bool LINEAR>
+    __device__ inline BrickTaps brick_taps(const PostParams& Q, int X, int Y, int Z, float px, float py, float pz)
+    {
+        BrickTaps t;
         if(!LINEAR)
         {
             const uint32_t x = iclamp(int(px), 0, X - 1), y = iclamp(int(py), 0, Y - 1), z = iclamp(int(pz), 0, Z - 1);
-            const uint32_t brick = ((z >> 3) * uint32_t(Q.nby) + (y >> 3)) * uint32_t(Q.nbx) + (x >> 3);
-            const uint32_t bc0 = brick * uint32_t(S.dim);
-            const uint32_t idx = bc0 * kBrickPairs + ((z & 7) * 9 + (y & 7)) * 8 + (x & 7);
-            for(int c = 0; c < S.dim; c++)
-            {
-                float a, b;
-                load_pair<DT>(S.data, idx + uint32_t(c) * kBrickPairs, a, b);
-                v[c] = a * S.scale[bc0 + c];
-            }
-            return;
+            t.brick = ((z >> 3) * uint32_t(Q.nby) + (y >> 3)) * uint32_t(Q.nbx) + (x >> 3);
+            t.i00 = ((z & 7) * 9 + (y & 7)) * 8 + (x & 7);
+            t.i10 = t.i01 = t.i11 = t.i00;
+            t.dup = true;
+            t.fx = t.fy = t.fz = 0.0f;
+            return t;
         }
         const float cx = px - 0.5f, cy = py - 0.5f, cz = pz - 0.5f;
         const float flx = floorf(cx), fly = floorf(cy), flz = floorf(cz);
-        const float fx = cx - flx, fy = cy - fly, fz = cz - flz;
+        t.fx = cx - flx;
+        t.fy = cy - fly;
+        t.fz = cz - flz;
         const int bx = int(flx), by = int(fly), bz = int(flz);
         const uint32_t x0 = iclamp(bx, 0, X - 1), x1 = iclamp(bx + 1, 0, X - 1);
         const uint32_t y0 = iclamp(by, 0, Y - 1), y1 = iclamp(by + 1, 0, Y - 1);
@@ -1023,31 +1068,52 @@ namespace isaac_hip
         const uint32_t lx = x0 - (Bx << 3);
         const uint32_t ly0 = y0 - (By << 3), ly1 = y1 - (By << 3);
         const uint32_t lz0 = z0 - (Bz << 3), lz1 = z1 - (Bz << 3);
-        const uint32_t bc0 = ((Bz * uint32_t(Q.nby) + By) * uint32_t(Q.nbx) + Bx) * uint32_t(S.dim);
-        const uint32_t i00 = bc0 * kBrickPairs + lx + (lz0 * 9 + ly0) * 8, i10 = bc0 * kBrickPairs + lx + (lz0 * 9 + ly1) * 8;
-        const uint32_t i01 = bc0 * kBrickPairs + lx + (lz1 * 9 + ly0) * 8, i11 = bc0 * kBrickPairs + lx + (lz1 * 9 + ly1) * 8;
-        const bool dup = x1 == x0;
-        const float oz = 1.0f - fz, oy = 1.0f - fy, ox = 1.0f - fx;
-        for(int c = 0; c < S.dim; c++)
+        t.brick = (Bz * uint32_t(Q.nby) + By) * uint32_t(Q.nbx) + Bx;
+        t.i00 = lx + (lz0 * 9 + ly0) * 8;
+        t.i10 = lx + (lz0 * 9 + ly1) * 8;
+        t.i01 = lx + (lz1 * 9 + ly0) * 8;
+        t.i11 = lx + (lz1 * 9 + ly1) * 8;
+        t.dup = x1 == x0;
+        return t;
+    }
+
+    // raw sample of every component of one source (v[c] for c < S.dim)
+    template<int DT, bool LINEAR>
+    __device__ inline void brick_sample(const BrickSource& S, const BrickTaps& t, float v[4])
+    {
+        const uint32_t bc0 = t.brick * uint32_t(S.dim);
+        const float oz = 1.0f - t.fz, oy = 1.0f - t.fy, ox = 1.0f - t.fx;
+#pragma unroll
+        for(int c = 0; c < 4; c++)
         {
-            const uint32_t co = uint32_t(c) * kBrickPairs;
+            if(c >= S.dim)
+                break;
+            const uint32_t base = (bc0 + uint32_t(c)) * kBrickPairs;
+            const float sc = S.scale[bc0 + c];
+            if(!LINEAR)
+            {
+                float a, b;
+                load_pair<DT>(S.data, base + t.i00, a, b);
+                v[c] = a * sc;
+                continue;
+            }
             float d000, d100, d010, d110, d001, d101, d011, d111;
-            load_pair<DT>(S.data, i00 + co, d000, d100);
-            load_pair<DT>(S.data, i10 + co, d010, d110);
-            load_pair<DT>(S.data, i01 + co, d001, d101);
-            load_pair<DT>(S.data, i11 + co, d011, d111);
-            if(dup)
+            load_pair<DT>(S.data, base + t.i00, d000, d100);
+            load_pair<DT>(S.data, base + t.i10, d010, d110);
+            load_pair<DT>(S.data, base + t.i01, d001, d101);
+            load_pair<DT>(S.data, base + t.i11, d011, d111);
+            if(t.dup)
             {
                 d100 = d000;
                 d110 = d010;
                 d101 = d001;
                 d111 = d011;
             }
-            const float a0 = d000 * oz + d001 * fz, a1 = d010 * oz + d011 * fz;
-            const float A = a0 * oy + a1 * fy;
-            const float b0 = d100 * oz + d101 * fz, b1 = d110 * oz + d111 * fz;
-            const float B = b0 * oy + b1 * fy;
-            v[c] = (A * ox + B * fx) * S.scale[bc0 + c];
+            const float a0 = d000 * oz + d001 * t.fz, a1 = d010 * oz + d011 * t.fz;
+            const float A = a0 * oy + a1 * t.fy;
+            const float b0 = d100 * oz + d101 * t.fz, b1 = d110 * oz + d111 * t.fz;
+            const float B = b0 * oy + b1 * t.fy;
+            v[c] = (A * ox + B * t.fx) * sc;
         }
     }
 
@@ -1058,18 +1124,21 @@ namespace isaac_hip
     };
 
     template<int DT, bool LINEAR>
-    __device__ inline PostSample post_classify(const PostParams& Q, const float4* lut, int X, int Y, int Z, float px, float py, float pz)
+    __device__ __attribute__((always_inline)) inline PostSample post_classify(const PostParams& Q, const float4* lut, int X, int Y, int Z, float px, float py, float pz)
     {
+        const BrickTaps t = brick_taps<LINEAR>(Q, X, Y, Z, px, py, pz);
         PostSample o{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        for(int s = 0; s < Q.nsrc; s++)
+        // unrolled over the source slots so every parameter-block index is constant
+#pragma unroll
+        for(int s = 0; s < kMaxBrickSources; s++)
         {
+            if(s >= Q.nsrc)
+                break;
             const BrickSource& S = Q.src[s];
-            if(!S.active)
-                continue;
             float v[4] = {0.f, 0.f, 0.f, 0.f};
-            brick_sample<DT, LINEAR>(Q, S, X, Y, Z, px, py, pz, v);
+            brick_sample<DT, LINEAR>(S, t, v);
             const float tv = apply_chain(S.chain, v[0], v[1], v[2], v[3], S.dim);
-            const float4 col = lut[S.lutSlot * Q.T + tf_index(tv, Q.T)];
+            const float4 col = lut[s * Q.T + tf_index(tv, Q.T)];
             if(S.weight > 0.0f)
             {
                 const float a = col.w * S.weight;
@@ -1080,7 +1149,7 @@ namespace isaac_hip
             }
             if(S.iso > 0.0f)
             {
-                const float a = col.w / S.iso * 0.5f;
+                const float a = col.w * S.isoScale;
                 if(o.ia < a)
                 {
                     o.ir = col.x;
@@ -1090,15 +1159,11 @@ namespace isaac_hip
                 }
             }
         }
-        if(Q.totalWeight != 0.0f)
-        {
-            o.r = o.r / Q.totalWeight;
-            o.g = o.g / Q.totalWeight;
-            o.b = o.b / Q.totalWeight;
-            o.a = o.a / Q.totalWeight;
-        }
-        else
-            o.r = o.g = o.b = o.a = 0.0f;
+        // invTotalWeight = 1 / (sum of volume-source weights), 0 when that sum is 0
+        o.r = o.r * Q.invTotalWeight;
+        o.g = o.g * Q.invTotalWeight;
+        o.b = o.b * Q.invTotalWeight;
+        o.a = o.a * Q.invTotalWeight;
         return o;
     }
 
@@ -1109,11 +1174,19 @@ namespace isaac_hip
     template<int DT, bool LINEAR, bool COUNT>
     __global__ __launch_bounds__(256) void brick_march_kernel(const FrameParams F, const PostParams Q, const ParticleParams PP)
     {
+        // the enabled sources' transfer functions, staged in LDS in source order
         extern __shared__ float4 lds_lut[];
-        for(int i = threadIdx.x; i < Q.nLut * Q.T; i += blockDim.x)
-            lds_lut[i] = Q.luts[i];
+#pragma unroll
+        for(int j = 0; j < kMaxBrickSources; j++)
+        {
+            if(j >= Q.nsrc)
+                break;
+            const float4* src = Q.luts + size_t(Q.src[j].lutSlot) * Q.T;
+            for(int i = threadIdx.x; i < Q.T; i += blockDim.x)
+                lds_lut[j * Q.T + i] = src[i];
+        }
         __syncthreads();
-        const float4* lut = Q.nLut > 0 ? lds_lut : Q.luts;
+        const float4* lut = lds_lut;
         int px, py;
         const bool valid = tile_pixel(F, px, py);
         unsigned long long nVol = 0, nIso = 0, nIsoHits = 0, nVolRays = 0, nIsoRays = 0;
@@ -1215,14 +1288,37 @@ namespace isaac_hip
                     if(isoHit)
                     {
                         const F3 pu = divv(add(ray.start, muls(ray.dir, depth)), sc);
-                        const PostSample h = post_classify<DT, LINEAR>(Q, lut, F.X, F.Y, F.Z, pu.x, pu.y, pu.z);
-                        auto ia = [&](float x, float y, float z) {
-                            return gmin(post_classify<DT, LINEAR>(Q, lut, F.X, F.Y, F.Z, x, y, z).ia, 1.0f);
-                        };
-                        F3 g;
-                        g.x = ia(pu.x + 1.0f, pu.y, pu.z) - ia(pu.x - 1.0f, pu.y, pu.z);
-                        g.y = ia(pu.x, pu.y + 1.0f, pu.z) - ia(pu.x, pu.y - 1.0f, pu.z);
-                        g.z = ia(pu.x, pu.y, pu.z + 1.0f) - ia(pu.x, pu.y, pu.z - 1.0f);
+                        // hit colour, then the six central-difference samples of the iso value
+                        // (one loop, so the per-sample classification is inlined once)
+                        float hr = 0.f, hg = 0.f, hb = 0.f;
+                        F3 g = f3(0.f, 0.f, 0.f);
+#pragma unroll 1
+                        for(int d = -1; d < 6; d++)
+                        {
+                            const float sx = d == 0 ? 1.0f : (d == 1 ? -1.0f : 0.0f);
+                            const float sy = d == 2 ? 1.0f : (d == 3 ? -1.0f : 0.0f);
+                            const float sz = d == 4 ? 1.0f : (d == 5 ? -1.0f : 0.0f);
+                            const PostSample h = post_classify<DT, LINEAR>(Q, lut, F.X, F.Y, F.Z, pu.x + sx, pu.y + sy, pu.z + sz);
+                            const float v = gmin(h.ia, 1.0f);
+                            if(d < 0)
+                            {
+                                hr = h.ir;
+                                hg = h.ig;
+                                hb = h.ib;
+                            }
+                            else if(d == 0)
+                                g.x = v;
+                            else if(d == 1)
+                                g.x = g.x - v;
+                            else if(d == 2)
+                                g.y = v;
+                            else if(d == 3)
+                                g.y = g.y - v;
+                            else if(d == 4)
+                                g.z = v;
+                            else
+                                g.z = g.z - v;
+                        }
                         float gl = len3(g);
                         if(isoFirst || gl == 0.0f)
                         {
@@ -1230,7 +1326,7 @@ namespace isaac_hip
                             gl = 1.0f;
                         }
                         normal = f3(-g.x / gl, -g.y / gl, -g.z / gl);
-                        color = to_byte4(h.ir, h.ig, h.ib, 1.0f);
+                        color = to_byte4(hr, hg, hb, 1.0f);
                     }
                     if(volBlend)
                     {
@@ -1393,24 +1489,13 @@ namespace isaac_hip
 
     static_assert(sizeof(FrameParams) + sizeof(PostParams) + sizeof(ParticleParams) <= 4096, "kernel argument block");
 
-    hipError_t launch_brick_frame(
-        const FrameParams& F,
-        const PostParams& Q,
-        const ParticleParams& PP,
-        bool counting,
-        hipStream_t s,
-        hipEvent_t afterMarch,
-        hipEvent_t afterSsao)
+    hipError_t launch_brick_march(const FrameParams& F, const PostParams& Q, const ParticleParams& PP, bool counting, hipStream_t s)
     {
         const unsigned nb = unsigned(F.tilesX) * unsigned(F.tilesY);
         if(nb == 0)
-        {
-            (void) hipEventRecord(afterMarch, s);
-            (void) hipEventRecord(afterSsao, s);
             return hipSuccess;
-        }
         const dim3 grid(nb), block(256);
-        const size_t lds = size_t(Q.nLut) * Q.T * sizeof(float4);
+        const size_t lds = size_t(Q.nsrc) * Q.T * sizeof(float4);
         const int v = (Q.dtype * 4) | (F.linear ? 2 : 0) | (counting ? 1 : 0);
         switch(v)
         {
@@ -1427,11 +1512,6 @@ namespace isaac_hip
         case 10: hipLaunchKernelGGL((brick_march_kernel<2, true, false>), grid, block, lds, s, F, Q, PP); break;
         default: hipLaunchKernelGGL((brick_march_kernel<2, true, true>), grid, block, lds, s, F, Q, PP); break;
         }
-        (void) hipEventRecord(afterMarch, s);
-        if(F.aoEnabled)
-            hipLaunchKernelGGL(ssao_kernel, grid, block, 0, s, F);
-        (void) hipEventRecord(afterSsao, s);
-        hipLaunchKernelGGL(shade_kernel, grid, block, 0, s, F);
         return hipGetLastError();
     }
 
@@ -1472,22 +1552,30 @@ namespace isaac_hip
         return hipGetLastError();
     }
 
-    hipError_t launch_frame(
-        const FrameParams& F,
-        const ParticleParams& PP,
-        bool counting,
-        hipStream_t s,
-        hipEvent_t afterMarch,
-        hipEvent_t afterSsao)
+    hipError_t launch_march(const FrameParams& F, const ParticleParams& PP, bool counting, hipStream_t s)
     {
         const unsigned nb = unsigned(F.tilesX) * unsigned(F.tilesY);
         if(nb == 0)
-        {
-            (void) hipEventRecord(afterMarch, s);
-            (void) hipEventRecord(afterSsao, s);
             return hipSuccess;
-        }
         const dim3 grid(nb), block(256);
+        if(!counting && !F.prefetch && (F.waves == 6 || F.waves == 8))
+        {
+            if(F.waves == 6)
+            {
+                if(F.linear)
+                    hipLaunchKernelGGL((march_kernel_w6<true>), grid, block, 0, s, F, PP);
+                else
+                    hipLaunchKernelGGL((march_kernel_w6<false>), grid, block, 0, s, F, PP);
+            }
+            else
+            {
+                if(F.linear)
+                    hipLaunchKernelGGL((march_kernel_w8<true>), grid, block, 0, s, F, PP);
+                else
+                    hipLaunchKernelGGL((march_kernel_w8<false>), grid, block, 0, s, F, PP);
+            }
+            return hipGetLastError();
+        }
         const int v = (F.linear ? 4 : 0) | (counting ? 2 : 0) | (F.prefetch ? 1 : 0);
         switch(v)
         {
@@ -1500,11 +1588,24 @@ namespace isaac_hip
         case 6: hipLaunchKernelGGL((march_kernel<true, true, false>), grid, block, 0, s, F, PP); break;
         default: hipLaunchKernelGGL((march_kernel<true, true, true>), grid, block, 0, s, F, PP); break;
         }
-        (void) hipEventRecord(afterMarch, s);
-        if(F.aoEnabled)
-            hipLaunchKernelGGL(ssao_kernel, grid, block, 0, s, F);
-        (void) hipEventRecord(afterSsao, s);
-        hipLaunchKernelGGL(shade_kernel, grid, block, 0, s, F);
+        return hipGetLastError();
+    }
+
+    hipError_t launch_ssao(const FrameParams& F, hipStream_t s)
+    {
+        const unsigned nb = unsigned(F.tilesX) * unsigned(F.tilesY);
+        if(nb == 0)
+            return hipSuccess;
+        hipLaunchKernelGGL(ssao_kernel, dim3(nb), dim3(256), 0, s, F);
+        return hipGetLastError();
+    }
+
+    hipError_t launch_shade(const FrameParams& F, hipStream_t s)
+    {
+        const unsigned nb = unsigned(F.tilesX) * unsigned(F.tilesY);
+        if(nb == 0)
+            return hipSuccess;
+        hipLaunchKernelGGL(shade_kernel, dim3(nb), dim3(256), 0, s, F);
         return hipGetLastError();
     }
 } // namespace isaac_hip

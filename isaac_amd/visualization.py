@@ -386,6 +386,7 @@ class IsaacVisualization:
         return {name: float(getattr(t, name)) for name, _ in _lib.Timings._fields_}
 
     TUNE_MARCH_PREFETCH = 0
+    TUNE_MARCH_WAVES = 1
 
     def set_tuning(self, key: int, value: int) -> None:
         check(lib().isaac_hip_set_tuning(self._ctx, int(key), int(value)), self._ctx, "set_tuning")
@@ -420,6 +421,56 @@ class IsaacVisualization:
         self._d2h(pairs, pp.value)
         self._d2h(scales, ps.value)
         return pairs, scales
+
+    # learned occlusion (no counterpart in the reference; include/isaac_hip.h)
+    AO_PLAIN = 0
+    AO_LEARNED = 1
+    ao_mode = 0
+
+    def set_ao_mode(self, mode: int) -> None:
+        check(lib().isaac_hip_set_ao_mode(self._ctx, int(mode)), self._ctx, "set_ao_mode")
+        self.ao_mode = int(mode)
+
+    def aonet_set_params(self, params: np.ndarray) -> None:
+        p = np.ascontiguousarray(params, np.float32)
+        check(lib().isaac_hip_aonet_set_params(self._ctx, p.ctypes.data, p.size), self._ctx, "aonet_set_params")
+
+    def aonet_get_params(self) -> np.ndarray:
+        n = int(lib().isaac_hip_aonet_param_count())
+        p = np.zeros(n, np.float32)
+        check(lib().isaac_hip_aonet_get_params(self._ctx, p.ctypes.data, n), self._ctx, "aonet_get_params")
+        return p
+
+    def aonet_clear_batch(self) -> None:
+        check(lib().isaac_hip_aonet_clear_batch(self._ctx), self._ctx, "aonet_clear_batch")
+
+    def aonet_add_crops(self, xy: np.ndarray, side: int) -> None:
+        a = np.ascontiguousarray(xy, np.int32).reshape(-1, 2)
+        check(lib().isaac_hip_aonet_add_crops(self._ctx, a.shape[0], a.ctypes.data, int(side)), self._ctx, "aonet_add_crops")
+
+    def aonet_train_step(self, lr: float) -> float:
+        loss = ctypes.c_float()
+        check(lib().isaac_hip_aonet_train_step(self._ctx, float(lr), ctypes.byref(loss)), self._ctx, "aonet_train_step")
+        return float(loss.value)
+
+    def aonet_loss_grad(self, features: np.ndarray, target: np.ndarray):
+        """(loss, gradient) of the network on host features (n, h, w, 8) and targets (n, h, w)."""
+        f = np.ascontiguousarray(features, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        n, h, w = t.shape
+        g = np.zeros(int(lib().isaac_hip_aonet_param_count()), np.float32)
+        loss = ctypes.c_float()
+        check(lib().isaac_hip_aonet_loss_grad(self._ctx, n, h, w, f.ctypes.data, t.ctypes.data, ctypes.byref(loss), g.ctypes.data),
+              self._ctx, "aonet_loss_grad")
+        return float(loss.value), g
+
+    def aonet_forward(self, features: np.ndarray) -> np.ndarray:
+        """Raw network output (n, h, w) on host features (n, h, w, 8)."""
+        f = np.ascontiguousarray(features, np.float32)
+        n, h, w = f.shape[:3]
+        out = np.zeros((n, h, w), np.float32)
+        check(lib().isaac_hip_aonet_forward(self._ctx, n, h, w, f.ctypes.data, out.ctypes.data), self._ctx, "aonet_forward")
+        return out
 
     def set_float_output(self, on: bool) -> None:
         check(lib().isaac_hip_set_float_output(self._ctx, int(on)), self._ctx, "set_float_output")

@@ -267,15 +267,15 @@ namespace isaac_oracle
             }
             if(st.iso[s] > 0.0f)
             {
-                const R a = col.w / R(st.iso[s]) * R(0.5);
+                // iso value = alpha * (0.5 / threshold), the scale taken once per source
+                const R a = col.w * (R(0.5) / R(st.iso[s]));
                 if(o.iso.w < a)
                     o.iso = V4<R>{col.x, col.y, col.z, a};
             }
         }
-        if(twf != 0.0f)
-            o.vol = o.vol / R(twf);
-        else
-            o.vol = V4<R>{0, 0, 0, 0};
+        // normalised by the volume-source weight sum through its reciprocal
+        const R inv = twf != 0.0f ? R(1) / R(twf) : R(0);
+        o.vol = o.vol * inv;
         return o;
     }
 
