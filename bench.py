@@ -143,6 +143,56 @@ def measure(vis, poses, out, steps, warmup):
     return wall, total_samples, avg_us, avg_counts, counts
 
 
+def learned_ao(vis, poses, out, args):
+    """Train the occlusion network on the GPU (fresh frames of this scene at random
+    poses), gate it on held-out poses against the plain SSAO pass, and time the
+    frame with each occlusion pass in the combined and FP16-brick forms."""
+    import numpy as np
+    import torch
+
+    from isaac_amd import aonet, scenes
+
+    vis.set_render_form(*FORMS["combined"])
+    vis.set_ao_mode(vis.AO_PLAIN)
+    t0 = time.perf_counter()
+    vis.aonet_set_params(aonet.init_params(0))
+    losses = aonet.train(vis, steps=args.ao_train_steps, seed=1)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    held_out = [scenes.DEFAULT_POSE, scenes.FILL_POSE, scenes.random_pose(np.random.default_rng(99))]
+    gate = aonet.gate(vis, held_out, floor_db=40.0)
+    timing = {}
+    for name in ("combined", "bricks-f16"):
+        vis.set_render_form(*FORMS[name])
+        for mode, label in ((vis.AO_PLAIN, "plain"), (vis.AO_LEARNED, "learned")):
+            if mode == vis.AO_LEARNED and not gate["passed"]:
+                continue
+            vis.set_ao_mode(mode)
+            w2, s2, us2, c2, _ = measure(vis, poses, out, args.form_steps, 2)
+            timing[f"{name}/{label}"] = {
+                "ms_per_step": w2 / args.form_steps * 1e3,
+                "occlusion_ms": us2["ssao"] / 1e3,
+                "march_ms": us2["march"] / 1e3,
+            }
+    vis.set_ao_mode(vis.AO_PLAIN)
+    vis.set_render_form(*FORMS["combined"])
+    res = {
+        "train_steps": args.ao_train_steps,
+        "train_s": train_s,
+        "train_mse_first20": float(np.mean(losses[:20])),
+        "train_mse_last20": float(np.mean(losses[-20:])),
+        "gate_floor_db": 40.0,
+        "gate_passed": bool(gate["passed"]),
+        "image_psnr_db": gate["image_psnr"],
+        "occlusion_map_psnr_db": gate["ao_psnr"],
+        "image_psnr_without_occlusion_db": gate["image_psnr_no_ao"],
+        "frames": timing,
+    }
+    if "bricks-f16/learned" in timing:
+        res["speedup_learned_f16_vs_plain_f32"] = timing["combined/plain"]["ms_per_step"] / timing["bricks-f16/learned"]["ms_per_step"]
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,6 +207,8 @@ def main() -> int:
     ap.add_argument("--prefetch", type=int, default=0, help="march kernel: fetch step i+1 before compositing step i")
     ap.add_argument("--march-waves", type=int, default=0, help="march occupancy target (0, 6, 8)")
     ap.add_argument("--no-tune", action="store_true", help="keep --prefetch / --march-waves instead of autotuning")
+    ap.add_argument("--no-learned-ao", action="store_true", help="skip training / timing the learned occlusion pass")
+    ap.add_argument("--ao-train-steps", type=int, default=600)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain device copy of this size first (scales profiler byte counters)")
     args = ap.parse_args()
@@ -272,6 +324,8 @@ def main() -> int:
                 "march_steps_per_frame": c2["march_steps"],
             }
         line["forms"] = forms
+    if not args.no_learned_ao:
+        line["learned_ao"] = learned_ao(vis, poses, out, args)
     vis.close()
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))

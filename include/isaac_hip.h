@@ -226,11 +226,12 @@ extern "C"
         ISAAC_HIP_FORM_BRICKS = 1
     };
     int isaac_hip_set_render_form(isaac_hip_ctx* ctx, int32_t form, int32_t brick_dtype);
-    /* device pointers of volume/field source i's bricks: *pairs = nbricks * feature_dim
-     * blocks of 648 pairs (brick index = (bz*nby + by)*nbx + bx, x fastest; pair
-     * (lz*9 + ly)*8 + lx holds (v[x], v[x+1]) at voxel (8bx+lx, 8by+ly, 8bz+lz),
-     * coordinates clamped into the local box), each pair *pair_bytes (8 or 4) bytes;
-     * *scales = nbricks * feature_dim floats.  Valid after update_buffers in BRICKS form. */
+    /* device pointers of volume/field source i's bricks: *pairs = nbricks * 648 * feature_dim
+     * pairs, ordered [brick][pair row][component] (brick index = (bz*nby + by)*nbx + bx,
+     * x fastest; pair row (lz*9 + ly)*8 + lx holds (v[x], v[x+1]) at voxel
+     * (8bx+lx, 8by+ly, 8bz+lz), coordinates clamped into the local box), each pair
+     * *pair_bytes (8 or 4) bytes; *scales = nbricks * feature_dim floats.  Valid after
+     * update_buffers in BRICKS form. */
     int isaac_hip_get_bricks(
         const isaac_hip_ctx* ctx,
         int source,

@@ -22,107 +22,159 @@ namespace isaac_hip
     }
 
     // ------------------------------------------------------------------ convolution
-    // Workgroup = 64 consecutive pixels of one image row, four waves of 16 pixels,
-    // all output channels.  The 3 x 66 x Cin input patch is staged in LDS once.
-    template<int COUT, bool FWD>
-    __global__ __launch_bounds__(256) void conv3x3_kernel(const ConvArgs a)
+    // stage rows y0-1 .. y0+rows of the 66-pixel-wide window at x0-1 (zero outside
+    // the image) into LDS as [row][66][cin]; channels < ca come upsampled from `up`
+    __device__ inline void stage_patch(
+        _Float16* patch,
+        const _Float16* __restrict__ x,
+        const _Float16* __restrict__ up,
+        int ca,
+        int cin,
+        int n,
+        int h,
+        int w,
+        int y0,
+        int x0,
+        int rows)
     {
-        extern __shared__ _Float16 patch[];
-        const int x0 = blockIdx.x * 64, y = blockIdx.y, n = blockIdx.z;
-        const int cin = a.cin;
-        const int units = 3 * 66 * cin / 4;
-        for(int u = threadIdx.x; u < units; u += 256)
+        const int cs = cin - ca;
+        const int units = rows * 66 * cin / 4;
+        for(int u = threadIdx.x; u < units; u += blockDim.x)
         {
             const int e = u * 4;
             const int pix = e / cin, c = e - pix * cin;
             const int r = pix / 66, col = pix - r * 66;
-            const int yy = y + r - 1, xx = x0 + col - 1;
+            const int yy = y0 + r - 1, xx = x0 + col - 1;
             h4 v = {0, 0, 0, 0};
-            if(yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_)
-                v = *reinterpret_cast<const h4*>(a.x + ((size_t(n) * a.h + yy) * a.w_ + xx) * cin + c);
+            if(yy >= 0 && yy < h && xx >= 0 && xx < w)
+            {
+                if(c < ca)
+                    v = *reinterpret_cast<const h4*>(up + ((size_t(n) * (h / 2) + yy / 2) * (w / 2) + xx / 2) * ca + c);
+                else
+                    v = *reinterpret_cast<const h4*>(x + ((size_t(n) * h + yy) * w + xx) * cs + (c - ca));
+            }
             *reinterpret_cast<h4*>(patch + e) = v;
         }
+    }
+
+    // Workgroup = ROWS image rows x 64 pixels; four waves split the 64-pixel row
+    // segment into 16-pixel columns, and each wave computes its column in all ROWS
+    // rows for all output channels, so every weight fragment it loads feeds ROWS
+    // MFMAs.  The (ROWS + 2) x 66 x cin input patch is staged in LDS once.
+    template<int COUT, bool FWD, int ROWS>
+    __global__ __launch_bounds__(256) void conv3x3_kernel(const ConvArgs a)
+    {
+        extern __shared__ _Float16 patch[];
+        const int x0 = blockIdx.x * 64, y0 = blockIdx.y * ROWS, n = blockIdx.z;
+        const int cin = a.cin;
+        stage_patch(patch, a.x, a.up, a.ca, cin, n, a.h, a.w_, y0, x0, ROWS + 2);
         __syncthreads();
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
         const int pl = wave * 16 + (lane & 15);
         const int kq = 4 * (lane >> 4);
-        f4 acc[COUT / 16];
+        f4 acc[ROWS][COUT / 16];
 #pragma unroll
-        for(int cb = 0; cb < COUT / 16; cb++)
-            acc[cb] = f4{0.f, 0.f, 0.f, 0.f};
+        for(int r = 0; r < ROWS; r++)
+#pragma unroll
+            for(int cb = 0; cb < COUT / 16; cb++)
+                acc[r][cb] = f4{0.f, 0.f, 0.f, 0.f};
         const _Float16* wrow = a.w + size_t(lane & 15) * a.kp;
         for(int kc = 0; kc < a.kp; kc += 16)
         {
             const int kk = kc + kq;
             const int tap = kk / cin, ci = kk - tap * cin;
-            h4 b = {0, 0, 0, 0};
-            if(tap < 9)
-            {
-                const int ky = tap / 3, kx = tap - ky * 3;
-                b = *reinterpret_cast<const h4*>(patch + (ky * 66 + pl + kx) * cin + ci);
-            }
+            h4 w[COUT / 16];
 #pragma unroll
             for(int cb = 0; cb < COUT / 16; cb++)
+                w[cb] = *reinterpret_cast<const h4*>(wrow + size_t(cb * 16) * a.kp + kk);
+            const bool valid = tap < 9;
+            const int ky = valid ? tap / 3 : 0, kx = valid ? tap - ky * 3 : 0;
+#pragma unroll
+            for(int r = 0; r < ROWS; r++)
             {
-                const h4 w = *reinterpret_cast<const h4*>(wrow + size_t(cb * 16) * a.kp + kk);
-                acc[cb] = mfma16(w, b, acc[cb]);
+                h4 b = {0, 0, 0, 0};
+                if(valid)
+                    b = *reinterpret_cast<const h4*>(patch + ((r + ky) * 66 + pl + kx) * cin + ci);
+#pragma unroll
+                for(int cb = 0; cb < COUT / 16; cb++)
+                    acc[r][cb] = mfma16(w[cb], b, acc[r][cb]);
             }
         }
         const int x = x0 + pl;
         if(x >= a.w_)
             return;
-        const size_t p = (size_t(n) * a.h + y) * a.w_ + x;
 #pragma unroll
-        for(int cb = 0; cb < COUT / 16; cb++)
+        for(int r = 0; r < ROWS; r++)
         {
-            const int co = cb * 16 + 4 * (lane >> 4);
-            if(FWD)
-            {
-                h4 o;
+            const int y = y0 + r;
+            if(y >= a.h)
+                break;
+            const size_t p = (size_t(n) * a.h + y) * a.w_ + x;
 #pragma unroll
-                for(int r = 0; r < 4; r++)
-                {
-                    const float v = acc[cb][r] + a.bias[co + r];
-                    o[r] = _Float16(v > 0.0f ? v : 0.0f);
-                }
-                *reinterpret_cast<h4*>(reinterpret_cast<_Float16*>(a.y) + p * COUT + co) = o;
-            }
-            else
+            for(int cb = 0; cb < COUT / 16; cb++)
             {
-                float4* dst = reinterpret_cast<float4*>(reinterpret_cast<float*>(a.y) + p * COUT + co);
-                float4 d = *dst;
-                d.x += acc[cb][0];
-                d.y += acc[cb][1];
-                d.z += acc[cb][2];
-                d.w += acc[cb][3];
-                *dst = d;
+                const int co = cb * 16 + 4 * (lane >> 4);
+                if(FWD)
+                {
+                    h4 o;
+#pragma unroll
+                    for(int q = 0; q < 4; q++)
+                    {
+                        const float v = acc[r][cb][q] + a.bias[co + q];
+                        o[q] = _Float16(v > 0.0f ? v : 0.0f);
+                    }
+                    *reinterpret_cast<h4*>(reinterpret_cast<_Float16*>(a.y) + p * COUT + co) = o;
+                }
+                else
+                {
+                    float4* dst = reinterpret_cast<float4*>(reinterpret_cast<float*>(a.y) + p * COUT + co);
+                    float4 d = *dst;
+                    d.x += acc[r][cb][0];
+                    d.y += acc[r][cb][1];
+                    d.z += acc[r][cb][2];
+                    d.w += acc[r][cb][3];
+                    *dst = d;
+                }
             }
         }
     }
 
-    template<int COUT, bool FWD>
+    template<int COUT, bool FWD, int ROWS>
     hipError_t conv_launch(const ConvArgs& a, hipStream_t s)
     {
-        const dim3 grid(unsigned((a.w_ + 63) / 64), unsigned(a.h), unsigned(a.n));
-        const size_t lds = size_t(3 * 66) * a.cin * sizeof(_Float16);
-        hipLaunchKernelGGL((conv3x3_kernel<COUT, FWD>), grid, dim3(256), lds, s, a);
+        const dim3 grid(unsigned((a.w_ + 63) / 64), unsigned((a.h + ROWS - 1) / ROWS), unsigned(a.n));
+        const size_t lds = size_t((ROWS + 2) * 66) * a.cin * sizeof(_Float16);
+        hipLaunchKernelGGL((conv3x3_kernel<COUT, FWD, ROWS>), grid, dim3(256), lds, s, a);
         return hipGetLastError();
     }
 
-    hipError_t launch_conv3x3(const ConvArgs& a, int cout, bool fwd, hipStream_t s)
+    template<int COUT, bool FWD>
+    hipError_t conv_rows(const ConvArgs& a, size_t max_lds, hipStream_t s)
     {
-        if(a.cin % 4 != 0 || a.kp % 16 != 0 || a.kp < 9 * a.cin || 3 * 66 * a.cin * 2 > 64 * 1024)
+        // four rows per workgroup when the patch fits and the grid still covers the
+        // chip several times over; else one
+        const size_t lds4 = size_t(6 * 66) * a.cin * sizeof(_Float16);
+        const int64_t groups4 = int64_t((a.w_ + 63) / 64) * ((a.h + 3) / 4) * a.n;
+        if(lds4 <= max_lds && groups4 >= 1024)
+            return conv_launch<COUT, FWD, 4>(a, s);
+        return conv_launch<COUT, FWD, 1>(a, s);
+    }
+
+    hipError_t launch_conv3x3(const ConvArgs& a, int cout, bool fwd, size_t max_lds, hipStream_t s)
+    {
+        if(a.cin % 4 != 0 || a.ca % 4 != 0 || a.ca > a.cin || (a.ca > 0 && !a.up) || a.kp % 16 != 0 || a.kp < 9 * a.cin
+           || size_t(3 * 66) * a.cin * 2 > max_lds || (a.up && (a.h % 2 || a.w_ % 2)))
             return hipErrorInvalidValue;
         if(a.n == 0 || a.h == 0 || a.w_ == 0)
             return hipSuccess;
         switch(cout)
         {
-        case 16: return fwd ? conv_launch<16, true>(a, s) : conv_launch<16, false>(a, s);
-        case 32: return fwd ? conv_launch<32, true>(a, s) : conv_launch<32, false>(a, s);
-        case 48: return fwd ? conv_launch<48, true>(a, s) : conv_launch<48, false>(a, s);
-        case 64: return fwd ? conv_launch<64, true>(a, s) : conv_launch<64, false>(a, s);
-        case 96: return fwd ? conv_launch<96, true>(a, s) : conv_launch<96, false>(a, s);
-        case 128: return fwd ? conv_launch<128, true>(a, s) : conv_launch<128, false>(a, s);
+        case 16: return fwd ? conv_rows<16, true>(a, max_lds, s) : conv_rows<16, false>(a, max_lds, s);
+        case 32: return fwd ? conv_rows<32, true>(a, max_lds, s) : conv_rows<32, false>(a, max_lds, s);
+        case 48: return fwd ? hipErrorInvalidValue : conv_rows<48, false>(a, max_lds, s);
+        case 64: return fwd ? conv_rows<64, true>(a, max_lds, s) : conv_rows<64, false>(a, max_lds, s);
+        case 96: return fwd ? hipErrorInvalidValue : conv_rows<96, false>(a, max_lds, s);
+        case 128: return fwd ? hipErrorInvalidValue : conv_rows<128, false>(a, max_lds, s);
         default: return hipErrorInvalidValue;
         }
     }
@@ -134,6 +186,8 @@ namespace isaac_hip
     // segments [q*segs, (q+1)*segs); one FP32 atomic per tile element at the end.
     __global__ __launch_bounds__(256) void conv_dw_kernel(
         const _Float16* __restrict__ x,
+        const _Float16* __restrict__ up,
+        int ca,
         const _Float16* __restrict__ dz,
         float* __restrict__ dw,
         int n,
@@ -171,18 +225,7 @@ namespace isaac_hip
             const int64_t row = sg / segW;
             const int y = int(row % h), img = int(row / h);
             __syncthreads();
-            const int units = 3 * 66 * cin / 4;
-            for(int u = threadIdx.x; u < units; u += 256)
-            {
-                const int e = u * 4;
-                const int pix = e / cin, c = e - pix * cin;
-                const int r = pix / 66, col = pix - r * 66;
-                const int yy = y + r - 1, xx = xs + col - 1;
-                h4 v = {0, 0, 0, 0};
-                if(yy >= 0 && yy < h && xx >= 0 && xx < w)
-                    v = *reinterpret_cast<const h4*>(x + ((size_t(img) * h + yy) * w + xx) * cin + c);
-                *reinterpret_cast<h4*>(patch + e) = v;
-            }
+            stage_patch(patch, x, up, ca, cin, img, h, w, y, xs, 3);
             const int dunits = 64 * cout / 4;
             for(int u = threadIdx.x; u < dunits; u += 256)
             {
@@ -236,6 +279,8 @@ namespace isaac_hip
 
     hipError_t launch_conv3x3_dw(
         const _Float16* x,
+        const _Float16* up,
+        int ca,
         const _Float16* dz,
         float* dw,
         int n,
@@ -245,7 +290,7 @@ namespace isaac_hip
         int cout,
         hipStream_t s)
     {
-        if(cin % 4 != 0 || cout % 16 != 0)
+        if(cin % 4 != 0 || cout % 16 != 0 || ca % 4 != 0 || ca > cin || (ca > 0 && !up) || (up && (h % 2 || w % 2)))
             return hipErrorInvalidValue;
         const int kdim = 9 * cin;
         const int64_t nseg = int64_t(n) * h * ((w + 63) / 64);
@@ -261,7 +306,7 @@ namespace isaac_hip
         const size_t lds = (size_t(3 * 66) * cin + size_t(64) * cout) * sizeof(_Float16);
         if(lds > 64 * 1024)
             return hipErrorInvalidValue;
-        hipLaunchKernelGGL(conv_dw_kernel, grid, dim3(256), lds, s, x, dz, dw, n, h, w, cin, cout, kdim, int(segs));
+        hipLaunchKernelGGL(conv_dw_kernel, grid, dim3(256), lds, s, x, up, ca, dz, dw, n, h, w, cin, cout, kdim, int(segs));
         return hipGetLastError();
     }
 

@@ -15,22 +15,31 @@
 
 namespace isaac_hip
 {
+    // Input of a convolution: channels [0, ca) of pixel (y, x) come from `up` at
+    // (y/2, x/2) (nearest-neighbour upsampling of an (h/2, w/2, ca) tensor) and
+    // channels [ca, cin) from `x` (h, w, cin - ca); with up == nullptr all cin
+    // channels come from x.  This fuses the decoder's upsample + concat.
     struct ConvArgs
     {
-        const _Float16* x; // [n][h][w][cin], cin a multiple of 4
+        const _Float16* x; // [n][h][w][cin - ca]
+        const _Float16* up; // [n][h/2][w/2][ca] or nullptr
+        int32_t ca; // channels from `up` (a multiple of 4; 0 without `up`)
         const _Float16* w; // [cout][kp]
         const float* bias; // [cout] (forward only)
         void* y; // forward: _Float16 [n][h][w][cout] = relu(conv + bias); backward data: float, +=
-        int32_t n, h, w_, cin, kp;
+        int32_t n, h, w_, cin, kp; // cin a multiple of 4
     };
 
-    // cout in {16, 32, 48, 64, 96, 128}; relu_fp16_out: forward form, else FP32 accumulate
-    hipError_t launch_conv3x3(const ConvArgs& a, int cout, bool relu_fp16_out, hipStream_t s);
+    // cout in {16, 32, 48, 64, 96, 128}; relu_fp16_out: forward form, else FP32 accumulate.
+    // max_lds: the device's LDS bytes per workgroup (picks the rows per workgroup).
+    hipError_t launch_conv3x3(const ConvArgs& a, int cout, bool relu_fp16_out, size_t max_lds, hipStream_t s);
 
     // dW[co][k] += sum_p dz[p][co] * col(x)[p][k] for k < 9*cin (FP32 atomics into dw,
     // rows of 9*cin floats: the master weight layout [cout][9][cin])
     hipError_t launch_conv3x3_dw(
         const _Float16* x,
+        const _Float16* up,
+        int ca,
         const _Float16* dz,
         float* dw,
         int n,

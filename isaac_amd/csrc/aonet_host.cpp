@@ -43,7 +43,7 @@ namespace isaac_hip
         std::vector<void*> allocs;
         _Float16* x0 = nullptr;
         _Float16* act[kNumLayers] = {};
-        _Float16 *p0 = nullptr, *p1 = nullptr, *p2 = nullptr, *c2 = nullptr, *c1 = nullptr, *c0 = nullptr;
+        _Float16 *p0 = nullptr, *p1 = nullptr, *p2 = nullptr;
         float* target = nullptr;
         float* out = nullptr;
         float* loss = nullptr;
@@ -90,9 +90,6 @@ namespace isaac_hip
             chk(get(p0, half * 16));
             chk(get(p1, quarter * 32));
             chk(get(p2, eighth * 64));
-            chk(get(c2, quarter * 128));
-            chk(get(c1, half * 96));
-            chk(get(c0, full * 48));
             chk(get(target, full));
             chk(get(out, full));
             chk(get(loss, 1));
@@ -132,6 +129,7 @@ namespace isaac_hip
         _Float16* wb[kNumLayers] = {};
         int64_t step = 0;
         bool ready = false;
+        size_t maxLds = 64 * 1024; // LDS bytes per workgroup on this device
         AoWorkspace infer, train;
         int batchCount = 0, batchSide = 0;
 
@@ -178,6 +176,10 @@ namespace isaac_hip
             }
             net->headOff = off;
             net->nparams = off + kHeadIn + 1;
+            int lds = 0;
+            // dynamic LDS beyond 64 KiB per workgroup is not requested
+            if(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) == hipSuccess && lds > 0)
+                net->maxLds = std::min<size_t>(size_t(lds), 64 * 1024);
             hipError_t e = hipSuccess;
             auto chk = [&](hipError_t r) {
                 if(e == hipSuccess)
@@ -214,10 +216,23 @@ namespace isaac_hip
             return hipSuccess;
         }
 
-        hipError_t conv_fwd(AoNet* net, int i, const _Float16* in, _Float16* out, int n, int h, int w, hipStream_t s)
+        // input of a decoder layer: upsampled `up` (ca channels) followed by the skip tensor
+        hipError_t conv_fwd(
+            AoNet* net,
+            int i,
+            const _Float16* in,
+            _Float16* out,
+            int n,
+            int h,
+            int w,
+            hipStream_t s,
+            const _Float16* up = nullptr,
+            int ca = 0)
         {
             ConvArgs a{};
             a.x = in;
+            a.up = up;
+            a.ca = ca;
             a.w = net->wf[i];
             a.bias = net->master + net->boff[i];
             a.y = out;
@@ -226,7 +241,7 @@ namespace isaac_hip
             a.w_ = w;
             a.cin = kLayers[i].cin;
             a.kp = net->kp[i];
-            return launch_conv3x3(a, kLayers[i].cout, true, s);
+            return launch_conv3x3(a, kLayers[i].cout, true, net->maxLds, s);
         }
 
         // U-Net forward over ws.x0 (n images of h x w, multiples of 8)
@@ -248,29 +263,39 @@ namespace isaac_hip
             AOCHK(launch_pool2(ws.act[5], ws.p2, n, h2, w2, 64, s));
             AOCHK(conv_fwd(net, 6, ws.p2, ws.act[6], n, h3, w3, s));
             AOCHK(conv_fwd(net, 7, ws.act[6], ws.act[7], n, h3, w3, s));
-            AOCHK(launch_upcat(ws.act[7], ws.act[5], ws.c2, n, h2, w2, 64, 64, s));
-            AOCHK(conv_fwd(net, 8, ws.c2, ws.act[8], n, h2, w2, s));
-            AOCHK(launch_upcat(ws.act[8], ws.act[3], ws.c1, n, h1, w1, 64, 32, s));
-            AOCHK(conv_fwd(net, 9, ws.c1, ws.act[9], n, h1, w1, s));
-            AOCHK(launch_upcat(ws.act[9], ws.act[1], ws.c0, n, h, w, 32, 16, s));
-            AOCHK(conv_fwd(net, 10, ws.c0, ws.act[10], n, h, w, s));
+            AOCHK(conv_fwd(net, 8, ws.act[5], ws.act[8], n, h2, w2, s, ws.act[7], 64));
+            AOCHK(conv_fwd(net, 9, ws.act[3], ws.act[9], n, h1, w1, s, ws.act[8], 64));
+            AOCHK(conv_fwd(net, 10, ws.act[1], ws.act[10], n, h, w, s, ws.act[9], 32));
             return hipSuccess;
         }
 
         // backward of layer i given ws.g[i] (gradient wrt its ReLU output): weight and
         // bias gradients, and (when gin != nullptr) += gradient wrt its input
-        hipError_t layer_bwd(AoNet* net, AoWorkspace& ws, int i, const _Float16* in, float* gin, int n, int h, int w, hipStream_t s)
+        hipError_t layer_bwd(
+            AoNet* net,
+            AoWorkspace& ws,
+            int i,
+            const _Float16* in,
+            float* gin,
+            int n,
+            int h,
+            int w,
+            hipStream_t s,
+            const _Float16* up = nullptr,
+            int ca = 0)
         {
             const int cin = kLayers[i].cin, cout = kLayers[i].cout;
             const int64_t npx = int64_t(n) * h * w;
             hipError_t e;
             AOCHK(launch_relu_bwd(ws.g[i], ws.act[i], ws.dz, npx * cout, s));
-            AOCHK(launch_conv3x3_dw(in, ws.dz, net->grad + net->woff[i], n, h, w, cin, cout, s));
+            AOCHK(launch_conv3x3_dw(in, up, ca, ws.dz, net->grad + net->woff[i], n, h, w, cin, cout, s));
             AOCHK(launch_bias_grad(ws.dz, net->grad + net->boff[i], npx, cout, s));
             if(gin)
             {
                 ConvArgs a{};
                 a.x = ws.dz;
+                a.up = nullptr;
+                a.ca = 0;
                 a.w = net->wb[i];
                 a.bias = nullptr;
                 a.y = gin;
@@ -279,7 +304,7 @@ namespace isaac_hip
                 a.w_ = w;
                 a.cin = cout;
                 a.kp = 9 * cout;
-                AOCHK(launch_conv3x3(a, cin, false, s));
+                AOCHK(launch_conv3x3(a, cin, false, net->maxLds, s));
             }
             return hipSuccess;
         }
@@ -308,11 +333,11 @@ namespace isaac_hip
             AOCHK(zero(ws.gc0, full * 48, s));
             AOCHK(launch_head_train(ws.act[10], net->master + net->headOff, ws.target, ws.loss, net->grad + net->headOff,
                                     ws.g[10], ws.out, int64_t(full), s));
-            AOCHK(layer_bwd(net, ws, 10, ws.c0, ws.gc0, n, h, w, s));
+            AOCHK(layer_bwd(net, ws, 10, ws.act[1], ws.gc0, n, h, w, s, ws.act[9], 32));
             AOCHK(launch_upcat_bwd(ws.gc0, ws.g[9], ws.g[1], n, h, w, 32, 16, s));
-            AOCHK(layer_bwd(net, ws, 9, ws.c1, ws.gc1, n, h1, w1, s));
+            AOCHK(layer_bwd(net, ws, 9, ws.act[3], ws.gc1, n, h1, w1, s, ws.act[8], 64));
             AOCHK(launch_upcat_bwd(ws.gc1, ws.g[8], ws.g[3], n, h1, w1, 64, 32, s));
-            AOCHK(layer_bwd(net, ws, 8, ws.c2, ws.gc2, n, h2, w2, s));
+            AOCHK(layer_bwd(net, ws, 8, ws.act[5], ws.gc2, n, h2, w2, s, ws.act[7], 64));
             AOCHK(launch_upcat_bwd(ws.gc2, ws.g[7], ws.g[5], n, h2, w2, 64, 64, s));
             AOCHK(layer_bwd(net, ws, 7, ws.act[6], ws.g[6], n, h3, w3, s));
             AOCHK(layer_bwd(net, ws, 6, ws.p2, ws.gp2, n, h3, w3, s));

@@ -111,10 +111,11 @@ namespace isaac_hip
 
     // ------------------------------------------------------------------ precision form (field bricks)
     // A source re-laid into 8^3 bricks with a one-voxel apron on the + side, stored
-    // as x-pairs (v[x], v[x+1]) for x in 0..7, y, z in 0..8 (648 pairs per brick and
-    // component), so the 8 trilinear taps of any sample come from one brick with 4
-    // aligned pair loads.  Pairs are float2 (F32) or two 16-bit values (F16, BF16)
-    // holding v / scale, with one FP32 scale per brick and component.
+    // as x-pairs (v[x], v[x+1]) for x in 0..7, y, z in 0..8 (648 pair rows per brick,
+    // the source's components side by side in each row), so the 8 trilinear taps of
+    // any sample come from one brick with 4 aligned pair loads per component.  Pairs
+    // are float2 (F32) or two 16-bit values (F16, BF16) holding v / scale, with one
+    // FP32 scale per brick and component.
     constexpr int kBrick = 8;
     constexpr int kBrickPairs = 8 * 9 * 9;
     constexpr int kMaxBrickSources = 4;
@@ -122,7 +123,7 @@ namespace isaac_hip
 
     struct BrickSource
     {
-        const void* data; // [brick][component][648 pairs]
+        const void* data; // [brick][648 pair rows][component]
         const float* scale; // [brick][component]
         int32_t dim;
         int32_t active; // weight + iso > 0

@@ -1077,31 +1077,33 @@ namespace isaac_hip
         return t;
     }
 
-    // raw sample of every component of one source (v[c] for c < S.dim)
+    // raw sample of every component of one source (v[c] for c < S.dim).  Pair rows
+    // hold the components of a source side by side, so the row loads of all
+    // components are adjacent and the compiler merges them into wide loads.
     template<int DT, bool LINEAR>
     __device__ inline void brick_sample(const BrickSource& S, const BrickTaps& t, float v[4])
     {
-        const uint32_t bc0 = t.brick * uint32_t(S.dim);
+        const uint32_t dim = uint32_t(S.dim);
+        const uint32_t row0 = t.brick * uint32_t(kBrickPairs);
         const float oz = 1.0f - t.fz, oy = 1.0f - t.fy, ox = 1.0f - t.fx;
+        const float* sc = S.scale + t.brick * dim;
 #pragma unroll
         for(int c = 0; c < 4; c++)
         {
             if(c >= S.dim)
                 break;
-            const uint32_t base = (bc0 + uint32_t(c)) * kBrickPairs;
-            const float sc = S.scale[bc0 + c];
             if(!LINEAR)
             {
                 float a, b;
-                load_pair<DT>(S.data, base + t.i00, a, b);
-                v[c] = a * sc;
+                load_pair<DT>(S.data, (row0 + t.i00) * dim + c, a, b);
+                v[c] = a * sc[c];
                 continue;
             }
             float d000, d100, d010, d110, d001, d101, d011, d111;
-            load_pair<DT>(S.data, base + t.i00, d000, d100);
-            load_pair<DT>(S.data, base + t.i10, d010, d110);
-            load_pair<DT>(S.data, base + t.i01, d001, d101);
-            load_pair<DT>(S.data, base + t.i11, d011, d111);
+            load_pair<DT>(S.data, (row0 + t.i00) * dim + c, d000, d100);
+            load_pair<DT>(S.data, (row0 + t.i10) * dim + c, d010, d110);
+            load_pair<DT>(S.data, (row0 + t.i01) * dim + c, d001, d101);
+            load_pair<DT>(S.data, (row0 + t.i11) * dim + c, d011, d111);
             if(t.dup)
             {
                 d100 = d000;
@@ -1113,7 +1115,7 @@ namespace isaac_hip
             const float A = a0 * oy + a1 * t.fy;
             const float b0 = d100 * oz + d101 * t.fz, b1 = d110 * oz + d111 * t.fz;
             const float B = b0 * oy + b1 * t.fy;
-            v[c] = (A * ox + B * t.fx) * sc;
+            v[c] = (A * ox + B * t.fx) * sc[c];
         }
     }
 
@@ -1419,20 +1421,22 @@ namespace isaac_hip
         const float* scale,
         void* out)
     {
+        // element i = (brick*648 + pair)*dim + component
         const int64_t total = nbricks * dim * kBrickPairs;
         for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += int64_t(gridDim.x) * blockDim.x)
         {
-            const int64_t bc = i / kBrickPairs;
-            const int pi = int(i - bc * kBrickPairs);
-            const int64_t brick = bc / dim;
-            const int c = int(bc - brick * dim);
+            const int64_t row = i / dim;
+            const int c = int(i - row * dim);
+            const int64_t brick = row / kBrickPairs;
+            const int pi = int(row - brick * kBrickPairs);
+            const int64_t bc = brick * dim + c;
             const int lx = pi % 8, ly = (pi / 8) % 9, lz = pi / 72;
             const int Bx = int(brick % nbx), By = int((brick / nbx) % nby), Bz = int(brick / (int64_t(nbx) * nby));
             const int x0 = min(Bx * 8 + lx, X - 1), x1 = min(Bx * 8 + lx + 1, X - 1);
             const int y = min(By * 8 + ly, Y - 1), z = min(Bz * 8 + lz, Z - 1);
-            const int64_t row = int64_t(y) * sy + int64_t(z) * sz;
-            const float v0 = src[(int64_t(x0) * sx + row) * dim + c];
-            const float v1 = src[(int64_t(x1) * sx + row) * dim + c];
+            const int64_t yz = int64_t(y) * sy + int64_t(z) * sz;
+            const float v0 = src[(int64_t(x0) * sx + yz) * dim + c];
+            const float v1 = src[(int64_t(x1) * sx + yz) * dim + c];
             if(DT == kDtF32)
                 reinterpret_cast<float2*>(out)[i] = make_float2(v0, v1);
             else

@@ -404,7 +404,7 @@ class IsaacVisualization:
 
     def bricks(self, source: int):
         """Host copies of a source's bricks: (pairs, scales) with pairs shaped
-        (nbricks, dim, 9, 9, 8, 2) as float32 (F32) or uint16 bit patterns (F16/BF16)
+        (nbricks, 9, 9, 8, dim, 2) as float32 (F32) or uint16 bit patterns (F16/BF16)
         and scales (nbricks, dim)."""
         pp, ps = ctypes.c_void_p(), ctypes.c_void_p()
         nb, pb = ctypes.c_int64(), ctypes.c_int32()
@@ -416,7 +416,7 @@ class IsaacVisualization:
         dim = int(src.feature_dim) if src.persistent else 1  # non-persistent: the functor-applied scalar
         self.synchronize()
         dt = np.float32 if pb.value == 8 else np.uint16
-        pairs = np.zeros((nb.value, dim, 9, 9, 8, 2), dt)
+        pairs = np.zeros((nb.value, 9, 9, 8, dim, 2), dt)
         scales = np.zeros((nb.value, dim), np.float32)
         self._d2h(pairs, pp.value)
         self._d2h(scales, ps.value)

@@ -38,7 +38,7 @@ def brick_frame(spec, gpu, dtype, counting=True):
 
 
 def expected_bricks(data, dtype):
-    """Pairs (nbricks, dim, 9, 9, 8, 2) and scales (nbricks, dim) of a (Z, Y, X[, C]) source."""
+    """Pairs (nbricks, 9, 9, 8, dim, 2) and scales (nbricks, dim) of a (Z, Y, X[, C]) source."""
     import torch
 
     if data.ndim == 3:
@@ -60,10 +60,9 @@ def expected_bricks(data, dtype):
     v0 = region[:, :, :, 0:8, :]
     v1 = region[:, :, :, 1:9, :]
     pairs = np.stack([v0, v1], axis=-1)  # (nb, 9z, 9y, 8x, C, 2)
-    pairs = np.moveaxis(pairs, 4, 1)  # (nb, C, 9, 9, 8, 2)
     if dtype == F32:
         return pairs.astype(np.float32), scales
-    q = (pairs / scales[:, :, None, None, None, None]).astype(np.float32)
+    q = (pairs / scales[:, None, None, None, :, None]).astype(np.float32)
     if dtype == F16:
         return q.astype(np.float16).view(np.uint16), scales
     t = torch.from_numpy(np.ascontiguousarray(q)).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
