@@ -245,6 +245,45 @@ extern "C"
     int isaac_hip_set_float_output(isaac_hip_ctx* ctx, int enabled);
     int isaac_hip_get_float_image(const isaac_hip_ctx* ctx, const void** rgba_f32);
 
+    /* ---------------------------------------------------------------- batched form
+     * No counterpart in the reference (one frame of one sub-volume per drawCallBack,
+     * lib/isaac.hpp:2140-2543).  A batch renders frames = contexts x views, all at
+     * width x height: one persistent march launch over every 16x16 tile of every
+     * frame (workgroups pop chunks of tiles from a global counter), one occlusion
+     * pass over all frames (plain SSAO, or the learned pass of context 0, evaluated
+     * over several frames at a time) and one shading launch.  Each context is one
+     * sub-volume in the brick form with update_buffers done, and all contexts share
+     * the steering state of context 0 (transfer functions, chains, weights, iso,
+     * interpolation, brick dtype, occlusion).  Contexts must outlive the batch, and
+     * the batch uses context 0's stream. */
+    typedef struct isaac_hip_batch isaac_hip_batch;
+    enum isaac_hip_batch_mode
+    {
+        ISAAC_HIP_BATCH_PERSISTENT = 0, /* the launch sequence, enqueued directly */
+        ISAAC_HIP_BATCH_GRAPH = 1, /* the same sequence captured once in a hipGraph, then replayed */
+        ISAAC_HIP_BATCH_PER_FRAME = 2 /* one march / occlusion / shading sequence per frame (unbatched reference) */
+    };
+    enum isaac_hip_batch_tuning_key
+    {
+        ISAAC_HIP_BATCH_TUNE_CHUNK = 0, /* tiles per pop of the work counter (default 4) */
+        ISAAC_HIP_BATCH_TUNE_GROUPS_PER_CU = 1, /* persistent workgroups per compute unit (default 4) */
+        ISAAC_HIP_BATCH_TUNE_AO_CHUNK = 2 /* frames per evaluation of the learned occlusion network (default 16) */
+    };
+    int isaac_hip_batch_create(isaac_hip_ctx* const* ctxs, int32_t n_ctx, int32_t width, int32_t height, isaac_hip_batch** out);
+    void isaac_hip_batch_destroy(isaac_hip_batch* batch);
+    const char* isaac_hip_batch_last_error(const isaac_hip_batch* batch);
+    int isaac_hip_batch_set_tuning(isaac_hip_batch* batch, int32_t key, int32_t value);
+    /* frame f = (context f / n_views, view f % n_views); proj and mv hold 16 column-major
+     * doubles per frame (n_ctx * n_views each); bg as for isaac_hip_render */
+    int isaac_hip_batch_set_views(isaac_hip_batch* batch, int32_t n_views, const double* proj, const double* mv, const float bg[4]);
+    /* enqueue all frames on the stream (mode: ISAAC_HIP_BATCH_*); returns after enqueueing */
+    int isaac_hip_batch_render(isaac_hip_batch* batch, int32_t mode);
+    int isaac_hip_batch_synchronize(isaac_hip_batch* batch);
+    /* GPU time of the last isaac_hip_batch_render (waits for it) */
+    int isaac_hip_batch_last_time_us(isaac_hip_batch* batch, double* us);
+    /* device pointer of the frames' RGBA8 images, frame-major (nframes * width * height * 4 bytes) */
+    int isaac_hip_batch_get_images(const isaac_hip_batch* batch, const void** rgba, int32_t* nframes);
+
     /* ---------------------------------------------------------------- learned occlusion
      * No counterpart in the reference.  A U-Net over the G-buffer (input: depth /
      * maxGlobalSizeScaled, background flag, normal; 3x3 convolutions with 16/32/64/64

@@ -118,6 +118,15 @@ _SIGS = {
     "isaac_hip_set_float_output": (ctypes.c_int, [P, ctypes.c_int]),
     "isaac_hip_get_float_image": (ctypes.c_int, [P, P]),
     "isaac_hip_set_ao_mode": (ctypes.c_int, [P, I32]),
+    "isaac_hip_batch_create": (ctypes.c_int, [P, I32, I32, I32, P]),
+    "isaac_hip_batch_destroy": (None, [P]),
+    "isaac_hip_batch_last_error": (ctypes.c_char_p, [P]),
+    "isaac_hip_batch_set_tuning": (ctypes.c_int, [P, I32, I32]),
+    "isaac_hip_batch_set_views": (ctypes.c_int, [P, I32, P, P, P]),
+    "isaac_hip_batch_render": (ctypes.c_int, [P, I32]),
+    "isaac_hip_batch_synchronize": (ctypes.c_int, [P]),
+    "isaac_hip_batch_last_time_us": (ctypes.c_int, [P, P]),
+    "isaac_hip_batch_get_images": (ctypes.c_int, [P, P, P]),
     "isaac_hip_aonet_param_count": (ctypes.c_int64, []),
     "isaac_hip_aonet_set_params": (ctypes.c_int, [P, P, ctypes.c_int64]),
     "isaac_hip_aonet_get_params": (ctypes.c_int, [P, P, ctypes.c_int64]),

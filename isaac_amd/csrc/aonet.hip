@@ -496,9 +496,10 @@ namespace isaac_hip
             const int item = int(i / (int64_t(a.w) * a.h));
             const int gx = xx + (a.origin ? a.origin[2 * item] : 0), gy = yy + (a.origin ? a.origin[2 * item + 1] : 0);
             float d = FLT_MAX, nx = 0.f, ny = 0.f, nz = 0.f, t = 0.f;
+            const float invScale = a.invScaleItem ? a.invScaleItem[item] : a.invScale;
             if(gx >= 0 && gx < a.imgW && gy >= 0 && gy < a.imgH)
             {
-                const size_t p = size_t(gy) * a.imgW + gx;
+                const size_t p = size_t(item) * size_t(a.imgStride) + size_t(gy) * a.imgW + gx;
                 d = a.depth[p];
                 nx = a.normal[3 * p + 0];
                 ny = a.normal[3 * p + 1];
@@ -508,7 +509,7 @@ namespace isaac_hip
             }
             const bool bg = !(d < 3.0e38f);
             h4 lo, hi;
-            lo[0] = _Float16(bg ? 0.0f : d * a.invScale);
+            lo[0] = _Float16(bg ? 0.0f : d * invScale);
             lo[1] = _Float16(bg ? 1.0f : 0.0f);
             lo[2] = _Float16(nx);
             lo[3] = _Float16(ny);
@@ -545,25 +546,26 @@ namespace isaac_hip
         return s;
     }
 
-    __global__ __launch_bounds__(256) void head_infer_kernel(const _Float16* d0, const float* wh, float* ao, int h, int w, int imgW, int imgH)
+    __global__ __launch_bounds__(256) void head_infer_kernel(const _Float16* d0, const float* wh, float* ao, int n, int h, int w, int imgW, int imgH)
     {
-        const int64_t count = int64_t(imgW) * imgH;
+        const int64_t count = int64_t(n) * imgW * imgH;
         for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x)
         {
-            const int xx = int(i % imgW), yy = int(i / imgW);
-            const float v = head_dot(d0, wh, int64_t(yy) * w + xx);
+            const int xx = int(i % imgW), yy = int((i / imgW) % imgH);
+            const int64_t item = i / (int64_t(imgW) * imgH);
+            const float v = head_dot(d0, wh, (item * h + yy) * w + xx);
             ao[i] = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
         }
     }
 
-    hipError_t launch_head_infer(const _Float16* d0, const float* wh, float* ao, int h, int w, int imgW, int imgH, hipStream_t s)
+    hipError_t launch_head_infer(const _Float16* d0, const float* wh, float* ao, int n, int h, int w, int imgW, int imgH, hipStream_t s)
     {
         if(imgW > w || imgH > h)
             return hipErrorInvalidValue;
-        const int64_t count = int64_t(imgW) * imgH;
+        const int64_t count = int64_t(n) * imgW * imgH;
         if(count == 0)
             return hipSuccess;
-        hipLaunchKernelGGL(head_infer_kernel, dim3(grid_for(count)), dim3(256), 0, s, d0, wh, ao, h, w, imgW, imgH);
+        hipLaunchKernelGGL(head_infer_kernel, dim3(grid_for(count)), dim3(256), 0, s, d0, wh, ao, n, h, w, imgW, imgH);
         return hipGetLastError();
     }
 

@@ -73,11 +73,23 @@ namespace isaac_hip
         _Float16* x; // [n][h][w][8]
         float* target; // [n][h][w] or nullptr
         int32_t n, h, w;
+        int64_t imgStride; // elements between the images of consecutive items (0: one image)
+        const float* invScaleItem; // per item 1 / scale (device), or nullptr = invScale
     };
     hipError_t launch_features(const FeatureArgs& a, hipStream_t s);
 
-    // 1x1 head 16 -> 1.  Inference: ao[y][x] = clamp(out, 0, 1) for x < imgW, y < imgH.
-    hipError_t launch_head_infer(const _Float16* d0, const float* wh, float* ao, int h, int w, int imgW, int imgH, hipStream_t s);
+    // 1x1 head 16 -> 1.  Inference over n items: ao[item][y][x] = clamp(out, 0, 1) for
+    // x < imgW, y < imgH (item images of imgW * imgH floats, back to back).
+    hipError_t launch_head_infer(
+        const _Float16* d0,
+        const float* wh,
+        float* ao,
+        int n,
+        int h,
+        int w,
+        int imgW,
+        int imgH,
+        hipStream_t s);
     // Training: out = d0 . wh + bh (wh[16] = bh); loss += sum (out - t)^2;
     // gwh[0..15] += sum dout * d0, gwh[16] += sum dout; dd0 = dout * wh with
     // dout = 2 (out - t); `out` (nullable) receives the raw outputs

@@ -130,13 +130,14 @@ namespace isaac_hip
         int64_t step = 0;
         bool ready = false;
         size_t maxLds = 64 * 1024; // LDS bytes per workgroup on this device
-        AoWorkspace infer, train;
+        AoWorkspace infer, train, batch;
         int batchCount = 0, batchSide = 0;
 
         ~AoNet()
         {
             infer.release();
             train.release();
+            batch.release();
             for(void* p : {(void*) master, (void*) grad, (void*) m, (void*) v})
                 if(p)
                     (void) hipFree(p);
@@ -398,7 +399,60 @@ namespace isaac_hip
         f.w = Wp;
         HIPCHK(c, launch_features(f, c->stream));
         HIPCHK(c, forward(net, net->infer, 1, Hp, Wp, c->stream));
-        HIPCHK(c, launch_head_infer(net->infer.act[10], net->master + net->headOff, c->ao, Hp, Wp, W, H, c->stream));
+        HIPCHK(c, launch_head_infer(net->infer.act[10], net->master + net->headOff, c->ao, 1, Hp, Wp, W, H, c->stream));
+        return 0;
+    }
+
+    int aonet_reserve_frames(isaac_hip_ctx* c, int chunk, int W, int H)
+    {
+        AoNet* net = c->aonet;
+        if(!net || !net->ready)
+            return fail(c, ISAAC_HIP_ERR_STATE, "learned occlusion selected without network parameters");
+        const int Wp = (W + 7) / 8 * 8, Hp = (H + 7) / 8 * 8;
+        HIPCHK(c, net->batch.ensure(chunk, Hp, Wp, false));
+        return 0;
+    }
+
+    // learned occlusion of nframes frame-major G-buffer images (W x H each), `chunk` frames
+    // per network evaluation; allocates nothing once aonet_reserve_frames has run
+    int aonet_occlusion_frames(
+        isaac_hip_ctx* c,
+        const float* depth,
+        const float* normal,
+        float* ao,
+        int nframes,
+        int W,
+        int H,
+        const float* invScaleDev,
+        int chunk,
+        hipStream_t s)
+    {
+        AoNet* net = c->aonet;
+        if(!net || !net->ready)
+            return fail(c, ISAAC_HIP_ERR_STATE, "learned occlusion selected without network parameters");
+        const int Wp = (W + 7) / 8 * 8, Hp = (H + 7) / 8 * 8;
+        if(net->batch.n < chunk || net->batch.h != Hp || net->batch.w != Wp)
+            return fail(c, ISAAC_HIP_ERR_STATE, "batch occlusion workspace not reserved");
+        const size_t img = size_t(W) * H;
+        for(int f0 = 0; f0 < nframes; f0 += chunk)
+        {
+            const int n = std::min(chunk, nframes - f0);
+            FeatureArgs fa{};
+            fa.depth = depth + img * f0;
+            fa.normal = normal + img * 3 * f0;
+            fa.imgW = W;
+            fa.imgH = H;
+            fa.invScale = 1.0f;
+            fa.invScaleItem = invScaleDev + f0;
+            fa.imgStride = int64_t(img);
+            fa.x = net->batch.x0;
+            fa.n = n;
+            fa.h = Hp;
+            fa.w = Wp;
+            HIPCHK(c, launch_features(fa, s));
+            HIPCHK(c, forward(net, net->batch, n, Hp, Wp, s));
+            HIPCHK(c, launch_head_infer(net->batch.act[10], net->master + net->headOff, ao + img * f0, n, Hp, Wp, W, H, s));
+        }
         return 0;
     }
 

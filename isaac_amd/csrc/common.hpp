@@ -148,6 +148,22 @@ namespace isaac_hip
         BrickSource src[kMaxBrickSources];
     };
 
+    // A batch of frames (sub-volume x view) rendered by one persistent launch; the
+    // per-frame parameter blocks live in device memory.
+    struct FrameParams;
+    struct ParticleParams;
+    struct BatchParams
+    {
+        const struct FrameParams* frames;
+        const PostParams* posts;
+        const struct ParticleParams* parts;
+        const uint32_t* tileStart; // nframes + 1 prefix sums of 16x16 tiles per frame
+        int32_t nframes;
+        uint32_t ntiles;
+        int32_t chunk; // tiles per pop of the work counter
+        uint32_t* counter; // zeroed before the launch
+    };
+
     struct ParticleSourceDev
     {
         const int32_t* voff; // ncells + 1 exclusive offsets into recs (visible particles)

@@ -107,8 +107,44 @@ namespace isaac_hip
         return code;
     }
 
+    // G-buffer targets of one frame
+    struct FrameTargets
+    {
+        int W, H;
+        uint32_t* color;
+        float* depth;
+        float* normal;
+        float* ao;
+        float4* volColor;
+        float4* outFloat;
+        unsigned long long* counters;
+    };
+    // kernel parameter blocks of one frame (host.cpp)
+    int build_frame(
+        isaac_hip_ctx* c,
+        const double proj[16],
+        const double mv[16],
+        const float bg[4],
+        const int32_t viewport[4],
+        const FrameTargets& t,
+        FrameParams& F,
+        PostParams& Q,
+        ParticleParams& PP);
+
     // learned occlusion pass (aonet_host.cpp)
     int aonet_occlusion(isaac_hip_ctx* c);
+    int aonet_reserve_frames(isaac_hip_ctx* c, int chunk, int W, int H);
+    int aonet_occlusion_frames(
+        isaac_hip_ctx* c,
+        const float* depth,
+        const float* normal,
+        float* ao,
+        int nframes,
+        int W,
+        int H,
+        const float* invScaleDev,
+        int chunk,
+        hipStream_t s);
     void aonet_destroy(isaac_hip_ctx* c);
     int64_t aonet_param_count();
 

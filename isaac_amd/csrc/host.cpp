@@ -433,6 +433,193 @@ namespace
     }
 } // namespace
 
+namespace isaac_hip
+{
+    // Kernel parameter blocks of one frame of context c (drawCallBack's setup,
+    // lib/isaac.hpp:2140-2283) rendered into the G-buffer targets t.
+    int build_frame(
+        isaac_hip_ctx* c,
+        const double proj[16],
+        const double mv[16],
+        const float bg[4],
+        const int32_t viewport[4],
+        const FrameTargets& t,
+        FrameParams& F,
+        PostParams& Q,
+        ParticleParams& PP)
+    {
+        const int W = t.W, H = t.H;
+        F = FrameParams{};
+        Q = PostParams{};
+        PP = ParticleParams{};
+        // inverse MVP in binary32 (lib/isaac.hpp:2174-2178)
+        M4 P, MV;
+        for(int col = 0; col < 4; col++)
+            for(int r = 0; r < 4; r++)
+            {
+                P.m[col][r] = float(proj[col * 4 + r]);
+                MV.m[col][r] = float(mv[col * 4 + r]);
+            }
+        M4 inv = mat_inverse(mat_mul(P, MV));
+        for(int col = 0; col < 4; col++)
+            for(int r = 0; r < 4; r++)
+                F.inv[col * 4 + r] = inv.m[col][r];
+        F.W = W;
+        F.H = H;
+        int vp[4] = {0, 0, W, H};
+        if(viewport)
+            for(int i = 0; i < 4; i++)
+                vp[i] = viewport[i];
+        F.vx0 = std::max(0, vp[0]);
+        F.vy0 = std::max(0, vp[1]);
+        F.vx1 = std::min(W, vp[0] + vp[2]);
+        F.vy1 = std::min(H, vp[1] + vp[3]);
+        if(F.vx1 < F.vx0)
+            F.vx1 = F.vx0;
+        if(F.vy1 < F.vy0)
+            F.vy1 = F.vy0;
+        F.tilesX = (F.vx1 - F.vx0 + 15) / 16;
+        F.tilesY = (F.vy1 - F.vy0 + 15) / 16;
+        F.fbW = float(W);
+        F.fbH = float(H);
+        F.maxSize = float(c->maxGsScaled) * 0.5f;
+        for(int i = 0; i < 3; i++)
+        {
+            F.off[i] = float(int32_t(c->gsScaled[i]) / 2 - c->posScaled[i]);
+            F.bbMin[i] = float(std::max(int32_t(0), -c->posScaled[i]));
+            F.bbMax[i] = float(std::min(int32_t(c->lsScaled[i]), int32_t(c->gsScaled[i]) - c->posScaled[i]));
+            F.outerLow[i] = c->position[i] == 0;
+            F.outerHigh[i] = (int64_t(c->position[i]) + c->local[i] == c->global[i]);
+            F.scale[i] = c->scale[i];
+        }
+        F.nclip = c->nclip;
+        for(int i = 0; i < c->nclip; i++)
+            for(int k = 0; k < 3; k++)
+            {
+                F.cpos[i][k] = c->clipPos[i][k] * float(c->gsScaled[k]) * 0.5f + F.off[k];
+                F.cnorm[i][k] = c->clipNormal[i][k];
+            }
+        F.step = c->step;
+        F.linear = c->interpolation ? 1 : 0;
+        F.prefetch = c->tunePrefetch;
+        F.waves = c->tuneWaves;
+        F.X = int32_t(c->local[0]);
+        F.Y = int32_t(c->local[1]);
+        F.Z = int32_t(c->local[2]);
+        F.tex = c->tex;
+        const int nvf = c->nv + c->nf;
+        F.isoEnabled = 0;
+        for(int s = 0; s < nvf; s++)
+            if(c->iso[s] > 0.0f)
+                F.isoEnabled = 1;
+        float twAll = 0.0f;
+        for(int s = 0; s < c->nsrc; s++)
+            twAll += c->weight[s];
+        F.totalWeightAll = twAll;
+        F.volumeEnabled = twAll > 0.0f;
+        F.minSize = float(std::min(int(c->global[0]), std::min(int(c->global[1]), int(c->global[2]))));
+        F.renderMode = c->renderMode;
+        F.aoEnabled = (c->aoEnabled && c->aoWeight > 0.0f) ? 1 : 0;
+        F.aoWeight = c->aoWeight;
+        float bgv[4] = {0, 0, 0, 0};
+        if(bg)
+            for(int i = 0; i < 4; i++)
+                bgv[i] = bg[i];
+        // bgColor = {bg[3], bg[2], bg[1], bg[0]} (lib/isaac.hpp:2245-2249)
+        F.bg[0] = bgv[3];
+        F.bg[1] = bgv[2];
+        F.bg[2] = bgv[1];
+        F.bg[3] = bgv[0];
+        F.bgClear = to_byte4(F.bg[0], F.bg[1], F.bg[2], 0.0f);
+        F.maxGlobalSizeScaled = float(c->maxGsScaled);
+        hsva(halton(0, 3) * float(2 * M_PI), 1.0f, 1.0f, 1.0f, F.rankHue);
+        F.color = t.color;
+        F.depth = t.depth;
+        F.normal = t.normal;
+        F.ao = t.ao;
+        F.volColor = t.volColor;
+        F.counters = t.counters;
+        // particles
+        PP.nsrc = 0;
+        for(int i = 0; i < 3; i++)
+        {
+            PP.grid[i] = int32_t(c->pgrid[i]);
+            PP.pscale[i] = float(c->lsScaled[i]) / float(c->pgrid[i]);
+        }
+        const int T = c->cfg.transfer_size;
+        for(int s = 0; s < c->np; s++)
+        {
+            const PartSlot& p = c->parts[s];
+            const int gi = nvf + s;
+            if(c->weight[gi] == 0.0f || !p.set)
+                continue;
+            if(p.capCells != c->pgrid[0] * c->pgrid[1] * c->pgrid[2])
+                return fail(c, ISAAC_HIP_ERR_STATE, "particle cell lists are stale; call isaac_hip_update_buffers");
+            if(PP.nsrc >= 4)
+                return fail(c, ISAAC_HIP_ERR_UNSUPPORTED, "at most 4 active particle sources per frame");
+            ParticleSourceDev& S = PP.src[PP.nsrc++];
+            S.voff = p.B.voffset;
+            S.recs = p.B.recs;
+            S.radius = p.B.recRadius;
+            S.constRadius = p.desc.constant_radius;
+            S.weight = c->weight[gi];
+            S.lut = c->lutDev + size_t(gi) * T;
+        }
+        F.outFloat = t.outFloat;
+        if(c->form == ISAAC_HIP_FORM_BRICKS)
+        {
+            // binary64 camera inverse for the ray setup
+            M4T<double> Pd, MVd;
+            for(int col = 0; col < 4; col++)
+                for(int r = 0; r < 4; r++)
+                {
+                    Pd.m[col][r] = proj[col * 4 + r];
+                    MVd.m[col][r] = mv[col * 4 + r];
+                }
+            const M4T<double> invd = mat_inverse(mat_mul(Pd, MVd));
+            for(int col = 0; col < 4; col++)
+                for(int r = 0; r < 4; r++)
+                    Q.invd[col * 4 + r] = invd.m[col][r];
+            Q.maxSized = double(c->maxGsScaled) * 0.5;
+            for(int i = 0; i < c->nclip; i++)
+                for(int k = 0; k < 3; k++)
+                    Q.cposd[i][k] = double(c->clipPos[i][k]) * double(c->gsScaled[k]) * 0.5 + double(F.off[k]);
+            Q.dtype = c->brickDtype;
+            Q.nbx = int32_t((c->local[0] + 7) / 8);
+            Q.nby = int32_t((c->local[1] + 7) / 8);
+            Q.nbz = int32_t((c->local[2] + 7) / 8);
+            Q.T = T;
+            Q.luts = c->lutDev;
+            float tw = 0.0f;
+            for(int s = 0; s < nvf; s++)
+                tw += c->weight[s];
+            Q.invTotalWeight = tw != 0.0f ? 1.0f / tw : 0.0f;
+            for(int s = 0; s < nvf; s++)
+            {
+                const VolSlot& v = c->vols[s];
+                if(!v.set || c->weight[s] + c->iso[s] <= 0.0f)
+                    continue;
+                if(!v.bricked)
+                    return fail(c, ISAAC_HIP_ERR_STATE, "source " + std::to_string(s) + " enabled after update_buffers; call it again");
+                BrickSource& B = Q.src[Q.nsrc++];
+                B.data = v.brick;
+                B.scale = v.brickScale;
+                B.dim = v.brickDim;
+                B.active = 1;
+                B.weight = c->weight[s];
+                B.iso = c->iso[s];
+                B.isoScale = B.iso > 0.0f ? 0.5f / B.iso : 0.0f;
+                B.chain = v.desc.persistent ? c->chain[s] : ChainDesc{};
+                B.lutSlot = s;
+            }
+            // the enabled sources' transfer functions are staged in LDS (64 KiB per workgroup)
+            if(size_t(Q.nsrc) * size_t(T) * sizeof(float4) > 64 * 1024)
+                return fail(c, ISAAC_HIP_ERR_UNSUPPORTED, "brick form: enabled sources x transfer size above 4096 entries");
+        }
+        return 0;
+    }
+} // namespace isaac_hip
+
 extern "C"
 {
     const char* isaac_hip_version(void)
@@ -1086,172 +1273,14 @@ extern "C"
         if(c->builtForm != c->form)
             return fail(c, ISAAC_HIP_ERR_STATE, "render form changed; call isaac_hip_update_buffers");
         const int W = c->cfg.framebuffer_size[0], H = c->cfg.framebuffer_size[1];
-        FrameParams F{};
-        // inverse MVP in binary32 (lib/isaac.hpp:2174-2178)
-        M4 P, MV;
-        for(int col = 0; col < 4; col++)
-            for(int r = 0; r < 4; r++)
-            {
-                P.m[col][r] = float(proj[col * 4 + r]);
-                MV.m[col][r] = float(mv[col * 4 + r]);
-            }
-        M4 inv = mat_inverse(mat_mul(P, MV));
-        for(int col = 0; col < 4; col++)
-            for(int r = 0; r < 4; r++)
-                F.inv[col * 4 + r] = inv.m[col][r];
-        F.W = W;
-        F.H = H;
-        int vp[4] = {0, 0, W, H};
-        if(viewport)
-            for(int i = 0; i < 4; i++)
-                vp[i] = viewport[i];
-        F.vx0 = std::max(0, vp[0]);
-        F.vy0 = std::max(0, vp[1]);
-        F.vx1 = std::min(W, vp[0] + vp[2]);
-        F.vy1 = std::min(H, vp[1] + vp[3]);
-        if(F.vx1 < F.vx0)
-            F.vx1 = F.vx0;
-        if(F.vy1 < F.vy0)
-            F.vy1 = F.vy0;
-        F.tilesX = (F.vx1 - F.vx0 + 15) / 16;
-        F.tilesY = (F.vy1 - F.vy0 + 15) / 16;
-        F.fbW = float(W);
-        F.fbH = float(H);
-        F.maxSize = float(c->maxGsScaled) * 0.5f;
-        for(int i = 0; i < 3; i++)
+        FrameTargets tg{W, H, c->color, c->depth, c->normal, c->ao, c->volColor, c->floatOut ? c->outFloat : nullptr, c->counters};
+        FrameParams F;
+        PostParams Q;
+        ParticleParams PP;
         {
-            F.off[i] = float(int32_t(c->gsScaled[i]) / 2 - c->posScaled[i]);
-            F.bbMin[i] = float(std::max(int32_t(0), -c->posScaled[i]));
-            F.bbMax[i] = float(std::min(int32_t(c->lsScaled[i]), int32_t(c->gsScaled[i]) - c->posScaled[i]));
-            F.outerLow[i] = c->position[i] == 0;
-            F.outerHigh[i] = (int64_t(c->position[i]) + c->local[i] == c->global[i]);
-            F.scale[i] = c->scale[i];
-        }
-        F.nclip = c->nclip;
-        for(int i = 0; i < c->nclip; i++)
-            for(int k = 0; k < 3; k++)
-            {
-                F.cpos[i][k] = c->clipPos[i][k] * float(c->gsScaled[k]) * 0.5f + F.off[k];
-                F.cnorm[i][k] = c->clipNormal[i][k];
-            }
-        F.step = c->step;
-        F.linear = c->interpolation ? 1 : 0;
-        F.prefetch = c->tunePrefetch;
-        F.waves = c->tuneWaves;
-        F.X = int32_t(c->local[0]);
-        F.Y = int32_t(c->local[1]);
-        F.Z = int32_t(c->local[2]);
-        F.tex = c->tex;
-        const int nvf = c->nv + c->nf;
-        F.isoEnabled = 0;
-        for(int s = 0; s < nvf; s++)
-            if(c->iso[s] > 0.0f)
-                F.isoEnabled = 1;
-        float twAll = 0.0f;
-        for(int s = 0; s < c->nsrc; s++)
-            twAll += c->weight[s];
-        F.totalWeightAll = twAll;
-        F.volumeEnabled = twAll > 0.0f;
-        F.minSize = float(std::min(int(c->global[0]), std::min(int(c->global[1]), int(c->global[2]))));
-        F.renderMode = c->renderMode;
-        F.aoEnabled = (c->aoEnabled && c->aoWeight > 0.0f) ? 1 : 0;
-        F.aoWeight = c->aoWeight;
-        float bgv[4] = {0, 0, 0, 0};
-        if(bg)
-            for(int i = 0; i < 4; i++)
-                bgv[i] = bg[i];
-        // bgColor = {bg[3], bg[2], bg[1], bg[0]} (lib/isaac.hpp:2245-2249)
-        F.bg[0] = bgv[3];
-        F.bg[1] = bgv[2];
-        F.bg[2] = bgv[1];
-        F.bg[3] = bgv[0];
-        F.bgClear = to_byte4(F.bg[0], F.bg[1], F.bg[2], 0.0f);
-        F.maxGlobalSizeScaled = float(c->maxGsScaled);
-        hsva(halton(0, 3) * float(2 * M_PI), 1.0f, 1.0f, 1.0f, F.rankHue);
-        F.color = c->color;
-        F.depth = c->depth;
-        F.normal = c->normal;
-        F.ao = c->ao;
-        F.volColor = c->volColor;
-        F.counters = c->counters;
-        // particles
-        ParticleParams PP{};
-        PP.nsrc = 0;
-        for(int i = 0; i < 3; i++)
-        {
-            PP.grid[i] = int32_t(c->pgrid[i]);
-            PP.pscale[i] = float(c->lsScaled[i]) / float(c->pgrid[i]);
-        }
-        const int T = c->cfg.transfer_size;
-        for(int s = 0; s < c->np; s++)
-        {
-            const PartSlot& p = c->parts[s];
-            const int gi = nvf + s;
-            if(c->weight[gi] == 0.0f || !p.set)
-                continue;
-            if(p.capCells != c->pgrid[0] * c->pgrid[1] * c->pgrid[2])
-                return fail(c, ISAAC_HIP_ERR_STATE, "particle cell lists are stale; call isaac_hip_update_buffers");
-            if(PP.nsrc >= 4)
-                return fail(c, ISAAC_HIP_ERR_UNSUPPORTED, "at most 4 active particle sources per frame");
-            ParticleSourceDev& S = PP.src[PP.nsrc++];
-            S.voff = p.B.voffset;
-            S.recs = p.B.recs;
-            S.radius = p.B.recRadius;
-            S.constRadius = p.desc.constant_radius;
-            S.weight = c->weight[gi];
-            S.lut = c->lutDev + size_t(gi) * T;
-        }
-        F.outFloat = c->floatOut ? c->outFloat : nullptr;
-        PostParams Q{};
-        if(c->form == ISAAC_HIP_FORM_BRICKS)
-        {
-            // binary64 camera inverse for the ray setup
-            M4T<double> Pd, MVd;
-            for(int col = 0; col < 4; col++)
-                for(int r = 0; r < 4; r++)
-                {
-                    Pd.m[col][r] = proj[col * 4 + r];
-                    MVd.m[col][r] = mv[col * 4 + r];
-                }
-            const M4T<double> invd = mat_inverse(mat_mul(Pd, MVd));
-            for(int col = 0; col < 4; col++)
-                for(int r = 0; r < 4; r++)
-                    Q.invd[col * 4 + r] = invd.m[col][r];
-            Q.maxSized = double(c->maxGsScaled) * 0.5;
-            for(int i = 0; i < c->nclip; i++)
-                for(int k = 0; k < 3; k++)
-                    Q.cposd[i][k] = double(c->clipPos[i][k]) * double(c->gsScaled[k]) * 0.5 + double(F.off[k]);
-            Q.dtype = c->brickDtype;
-            Q.nbx = int32_t((c->local[0] + 7) / 8);
-            Q.nby = int32_t((c->local[1] + 7) / 8);
-            Q.nbz = int32_t((c->local[2] + 7) / 8);
-            Q.T = T;
-            Q.luts = c->lutDev;
-            float tw = 0.0f;
-            for(int s = 0; s < nvf; s++)
-                tw += c->weight[s];
-            Q.invTotalWeight = tw != 0.0f ? 1.0f / tw : 0.0f;
-            for(int s = 0; s < nvf; s++)
-            {
-                const VolSlot& v = c->vols[s];
-                if(!v.set || c->weight[s] + c->iso[s] <= 0.0f)
-                    continue;
-                if(!v.bricked)
-                    return fail(c, ISAAC_HIP_ERR_STATE, "source " + std::to_string(s) + " enabled after update_buffers; call it again");
-                BrickSource& B = Q.src[Q.nsrc++];
-                B.data = v.brick;
-                B.scale = v.brickScale;
-                B.dim = v.brickDim;
-                B.active = 1;
-                B.weight = c->weight[s];
-                B.iso = c->iso[s];
-                B.isoScale = B.iso > 0.0f ? 0.5f / B.iso : 0.0f;
-                B.chain = v.desc.persistent ? c->chain[s] : ChainDesc{};
-                B.lutSlot = s;
-            }
-            // the enabled sources' transfer functions are staged in LDS (64 KiB per workgroup)
-            if(size_t(Q.nsrc) * size_t(T) * sizeof(float4) > 64 * 1024)
-                return fail(c, ISAAC_HIP_ERR_UNSUPPORTED, "brick form: enabled sources x transfer size above 4096 entries");
+            int rc = build_frame(c, proj, mv, bg, viewport, tg, F, Q, PP);
+            if(rc)
+                return rc;
         }
         if(c->counting)
             HIPCHK(c, launch_fill_u32(reinterpret_cast<uint32_t*>(c->counters), 0u, 2 * ISAAC_HIP_CNT_COUNT, c->stream));

@@ -423,18 +423,29 @@ namespace isaac_hip
 
     // ------------------------------------------------------------------ pixel tiling
     // workgroup -> 16x16 tile, XCD-banded; returns false for threads outside the viewport
-    __device__ inline bool tile_pixel(const FrameParams& F, int& px, int& py)
+    // pixel of this thread in 16x16 tile `b` of the viewport (row-major tiles)
+    __device__ inline bool tile_pixel_at(const FrameParams& F, uint32_t b, int& px, int& py)
+    {
+        const uint32_t tx = b % uint32_t(F.tilesX), ty = b / uint32_t(F.tilesX);
+        const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+        px = F.vx0 + int(tx * 16u + (wave & 1u) * 8u + (lane & 7u));
+        py = F.vy0 + int(ty * 16u + (wave >> 1) * 8u + (lane >> 3));
+        return b < uint32_t(F.tilesX) * uint32_t(F.tilesY) && px < F.vx1 && py < F.vy1;
+    }
+
+    __device__ inline uint32_t xcd_tile(const FrameParams& F)
     {
         const uint32_t nb = uint32_t(F.tilesX) * uint32_t(F.tilesY);
         uint32_t b = blockIdx.x;
         const uint32_t full = nb & ~7u;
         if(b < full)
             b = (b & 7u) * (full >> 3) + (b >> 3);
-        const uint32_t tx = b % uint32_t(F.tilesX), ty = b / uint32_t(F.tilesX);
-        const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-        px = F.vx0 + int(tx * 16u + (wave & 1u) * 8u + (lane & 7u));
-        py = F.vy0 + int(ty * 16u + (wave >> 1) * 8u + (lane >> 3));
-        return px < F.vx1 && py < F.vy1;
+        return b;
+    }
+
+    __device__ inline bool tile_pixel(const FrameParams& F, int& px, int& py)
+    {
+        return tile_pixel_at(F, xcd_tile(F), px, py);
     }
 
     __device__ inline void block_count_add(unsigned long long* dst, unsigned long long v)
@@ -781,10 +792,10 @@ namespace isaac_hip
     }
 
     // ------------------------------------------------------------------ SSAO
-    __global__ __launch_bounds__(256) void ssao_kernel(const FrameParams F)
+    __device__ __attribute__((always_inline)) inline void ssao_body(const FrameParams& F, uint32_t tile)
     {
         int px, py;
-        if(!tile_pixel(F, px, py))
+        if(!tile_pixel_at(F, tile, px, py))
             return;
         const float ref = F.depth[size_t(py) * F.W + px];
         float occ = 0.0f;
@@ -803,14 +814,19 @@ namespace isaac_hip
         F.ao[size_t(py) * F.W + px] = gclamp(occ / 42.0f, 0.0f, 1.0f);
     }
 
+    __global__ __launch_bounds__(256) void ssao_kernel(const FrameParams F)
+    {
+        ssao_body(F, xcd_tile(F));
+    }
+
     // ------------------------------------------------------------------ shading + blend
     // ShadingKernel (lib/isaac/isaac_common_kernel.hpp:220-320), then the volume
     // pass's final blend color + (1 - color.a) * solid
     // (lib/isaac/isaac_volume_kernel.hpp:83-86) for pixels whose volume pass ran.
-    __global__ __launch_bounds__(256) void shade_kernel(const FrameParams F)
+    __device__ __attribute__((always_inline)) inline void shade_body(const FrameParams& F, uint32_t tile)
     {
         int px, py;
-        if(!tile_pixel(F, px, py))
+        if(!tile_pixel_at(F, tile, px, py))
             return;
         const size_t p = size_t(py) * size_t(F.W) + size_t(px);
         const uint32_t cb = F.color[p];
@@ -894,6 +910,11 @@ namespace isaac_hip
         F.color[p] = out;
         if(depthChanged)
             F.depth[p] = depth;
+    }
+
+    __global__ __launch_bounds__(256) void shade_kernel(const FrameParams F)
+    {
+        shade_body(F, xcd_tile(F));
     }
 
     // ------------------------------------------------------------------ precision form
@@ -1173,11 +1194,9 @@ namespace isaac_hip
     // march_kernel (clear, particle DDA, one traversal serving iso and volume,
     // G-buffer + premultiplied volume colour out); sample positions
     // startU + stepVec*i with startSteps / end steps taken from the binary64 ray.
-    template<int DT, bool LINEAR, bool COUNT>
-    __global__ __launch_bounds__(256) void brick_march_kernel(const FrameParams F, const PostParams Q, const ParticleParams PP)
+    // the enabled sources' transfer functions, staged in LDS in source order
+    __device__ inline void stage_luts(float4* lds_lut, const PostParams& Q)
     {
-        // the enabled sources' transfer functions, staged in LDS in source order
-        extern __shared__ float4 lds_lut[];
 #pragma unroll
         for(int j = 0; j < kMaxBrickSources; j++)
         {
@@ -1187,10 +1206,18 @@ namespace isaac_hip
             for(int i = threadIdx.x; i < Q.T; i += blockDim.x)
                 lds_lut[j * Q.T + i] = src[i];
         }
-        __syncthreads();
-        const float4* lut = lds_lut;
+    }
+
+    template<int DT, bool LINEAR, bool COUNT>
+    __device__ __attribute__((always_inline)) inline void brick_march_body(
+        const FrameParams& F,
+        const PostParams& Q,
+        const ParticleParams& PP,
+        const float4* lut,
+        uint32_t tile)
+    {
         int px, py;
-        const bool valid = tile_pixel(F, px, py);
+        const bool valid = tile_pixel_at(F, tile, px, py);
         unsigned long long nVol = 0, nIso = 0, nIsoHits = 0, nVolRays = 0, nIsoRays = 0;
         unsigned long long nCells = 0, nTests = 0, nPartRays = 0, nSteps = 0;
         if(valid)
@@ -1360,6 +1387,73 @@ namespace isaac_hip
         }
     }
 
+    template<int DT, bool LINEAR, bool COUNT>
+    __global__ __launch_bounds__(256) void brick_march_kernel(const FrameParams F, const PostParams Q, const ParticleParams PP)
+    {
+        extern __shared__ float4 lds_lut[];
+        stage_luts(lds_lut, Q);
+        __syncthreads();
+        brick_march_body<DT, LINEAR, COUNT>(F, Q, PP, lds_lut, xcd_tile(F));
+    }
+
+    // ------------------------------------------------------------------ batched form
+    // One persistent launch over many frames (sub-volume x view): workgroups pop
+    // chunks of 16x16 tiles from a global counter until the work is exhausted
+    // (every workgroup reaches the exit when the counter passes the last chunk).
+    // Frames share the transfer functions (staged once per workgroup).
+    template<int DT, bool LINEAR>
+    __global__ __launch_bounds__(256) void batch_march_kernel(const BatchParams B)
+    {
+        extern __shared__ float4 lds_lut[];
+        __shared__ uint32_t s_work;
+        stage_luts(lds_lut, B.posts[0]);
+        for(;;)
+        {
+            __syncthreads();
+            if(threadIdx.x == 0)
+                s_work = atomicAdd(B.counter, 1u);
+            __syncthreads();
+            // uniform across the workgroup: keep the work index in a scalar register so
+            // the frame's parameter blocks are read with scalar loads
+            const uint32_t t0 = uint32_t(__builtin_amdgcn_readfirstlane(int(s_work))) * uint32_t(B.chunk);
+            if(t0 >= B.ntiles)
+                break;
+            const uint32_t t1 = min(t0 + uint32_t(B.chunk), B.ntiles);
+            // frame of tile t0: largest i with tileStart[i] <= t0
+            int lo = 0, hi = B.nframes - 1;
+            while(lo < hi)
+            {
+                const int mid = (lo + hi + 1) >> 1;
+                if(B.tileStart[mid] <= t0)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            int item = __builtin_amdgcn_readfirstlane(lo);
+            for(uint32_t t = t0; t < t1; t++)
+            {
+                while(t >= B.tileStart[item + 1])
+                    item++;
+                item = __builtin_amdgcn_readfirstlane(item);
+                brick_march_body<DT, LINEAR, false>(B.frames[item], B.posts[item], B.parts[item], lds_lut, t - B.tileStart[item]);
+            }
+        }
+    }
+
+    // per-frame passes of the batch: grid (tiles of the largest frame, frames)
+    __global__ __launch_bounds__(256) void batch_ssao_kernel(const FrameParams* frames)
+    {
+        const FrameParams& F = frames[blockIdx.y];
+        if(F.aoEnabled)
+            ssao_body(F, blockIdx.x);
+    }
+
+    __global__ __launch_bounds__(256) void batch_shade_kernel(const FrameParams* frames)
+    {
+        shade_body(frames[blockIdx.y], blockIdx.x);
+    }
+
+
     // per brick and component: scale = max |v| over the 9^3 region (1 if all zero)
     __global__ __launch_bounds__(256) void brick_scale_kernel(
         const float* src,
@@ -1516,6 +1610,39 @@ namespace isaac_hip
         case 10: hipLaunchKernelGGL((brick_march_kernel<2, true, false>), grid, block, lds, s, F, Q, PP); break;
         default: hipLaunchKernelGGL((brick_march_kernel<2, true, true>), grid, block, lds, s, F, Q, PP); break;
         }
+        return hipGetLastError();
+    }
+
+    hipError_t launch_batch_march(const BatchParams& B, int dtype, bool linear, size_t lds, int grid, hipStream_t s)
+    {
+        if(B.nframes <= 0 || grid <= 0)
+            return hipSuccess;
+        const int v = dtype * 2 + (linear ? 1 : 0);
+        switch(v)
+        {
+        case 0: hipLaunchKernelGGL((batch_march_kernel<0, false>), dim3(grid), dim3(256), lds, s, B); break;
+        case 1: hipLaunchKernelGGL((batch_march_kernel<0, true>), dim3(grid), dim3(256), lds, s, B); break;
+        case 2: hipLaunchKernelGGL((batch_march_kernel<1, false>), dim3(grid), dim3(256), lds, s, B); break;
+        case 3: hipLaunchKernelGGL((batch_march_kernel<1, true>), dim3(grid), dim3(256), lds, s, B); break;
+        case 4: hipLaunchKernelGGL((batch_march_kernel<2, false>), dim3(grid), dim3(256), lds, s, B); break;
+        default: hipLaunchKernelGGL((batch_march_kernel<2, true>), dim3(grid), dim3(256), lds, s, B); break;
+        }
+        return hipGetLastError();
+    }
+
+    hipError_t launch_batch_ssao(const FrameParams* frames, int nframes, int maxTiles, hipStream_t s)
+    {
+        if(nframes <= 0 || maxTiles <= 0)
+            return hipSuccess;
+        hipLaunchKernelGGL(batch_ssao_kernel, dim3(unsigned(maxTiles), unsigned(nframes)), dim3(256), 0, s, frames);
+        return hipGetLastError();
+    }
+
+    hipError_t launch_batch_shade(const FrameParams* frames, int nframes, int maxTiles, hipStream_t s)
+    {
+        if(nframes <= 0 || maxTiles <= 0)
+            return hipSuccess;
+        hipLaunchKernelGGL(batch_shade_kernel, dim3(unsigned(maxTiles), unsigned(nframes)), dim3(256), 0, s, frames);
         return hipGetLastError();
     }
 
