@@ -193,6 +193,69 @@ def learned_ao(vis, poses, out, args):
     return res
 
 
+def batched_workload(args):
+    """48 sub-volumes of side U{32..256} (FP16 bricks: Gaussian-mixture density with an
+    iso-surface + float3 noise via length) x 8 seeded views at 960x540 with learned
+    occlusion: the whole set as one persistent launch sequence replayed from a hipGraph,
+    against the same frames rendered one launch sequence per frame."""
+    import numpy as np
+    import torch
+
+    from isaac_amd import aonet
+    from isaac_amd import batch as B
+
+    t0 = time.perf_counter()
+    vises, sides = B.make_subvolumes(0, count=args.batch_count, seed=5, fb=(960, 540), dtype=1)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    projs, mvs = B.views_for(vises, args.batch_views, seed=5)
+    nframes = len(vises) * args.batch_views
+    # occlusion network: trained on the largest sub-volume at random poses, shared by all frames
+    big = int(np.argmax(sides))
+    t1 = time.perf_counter()
+    vises[big].aonet_set_params(aonet.init_params(0))
+    losses = aonet.train(vises[big], steps=args.ao_train_steps, seed=2)
+    params = vises[big].aonet_get_params()
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+    vises[0].aonet_set_params(params)
+    batch = B.Batch(vises, 960, 540)
+    bg = vises[0].background_color
+    res = {"sub_volumes": len(vises), "views_each": args.batch_views, "frames": nframes, "framebuffer": [960, 540],
+           "voxels": int(sum(s ** 3 for s in sides)), "rays": nframes * 960 * 540, "dtype": "f16 bricks",
+           "setup_s": t_build, "train_s": t_train, "train_mse_last20": float(np.mean(losses[-20:]))}
+    # plain occlusion images, then learned: PSNR of the whole set
+    vises[0].set_ao_mode(vises[0].AO_PLAIN)
+    batch.set_views(projs, mvs, bg=bg)
+    batch.render(B.GRAPH)
+    plain = batch.images()
+    vises[0].set_ao_mode(vises[0].AO_LEARNED)
+    batch.set_views(projs, mvs, bg=bg)
+    res["tuning"] = B.autotune(batch)
+    batch.render(B.GRAPH)
+    learned = batch.images()
+    res["learned_vs_plain_psnr_db"] = aonet.psnr(learned, plain, 255.0)
+    del plain, learned
+    times = {}
+    for mode, name in ((B.GRAPH, "graph"), (B.PERSISTENT, "persistent"), (B.PER_FRAME, "per_frame")):
+        batch.render(mode)
+        batch.synchronize()
+        ts = []
+        for _ in range(args.batch_repeats):
+            batch.render(mode)
+            ts.append(batch.last_time_us())
+        times[name] = float(np.median(ts)) / 1e3
+    res["ms"] = times
+    res["frames_per_s"] = nframes / (times["graph"] * 1e-3)
+    res["rays_per_s"] = res["rays"] / (times["graph"] * 1e-3)
+    res["speedup_vs_per_frame_launches"] = times["per_frame"] / times["graph"]
+    batch.close()
+    for v in vises:
+        v.close()
+    torch.cuda.synchronize()
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,6 +272,10 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true", help="keep --prefetch / --march-waves instead of autotuning")
     ap.add_argument("--no-learned-ao", action="store_true", help="skip training / timing the learned occlusion pass")
     ap.add_argument("--ao-train-steps", type=int, default=600)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched sub-volumes x views workload")
+    ap.add_argument("--batch-count", type=int, default=48)
+    ap.add_argument("--batch-views", type=int, default=8)
+    ap.add_argument("--batch-repeats", type=int, default=5)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain device copy of this size first (scales profiler byte counters)")
     args = ap.parse_args()
@@ -327,6 +394,8 @@ def main() -> int:
     if not args.no_learned_ao:
         line["learned_ao"] = learned_ao(vis, poses, out, args)
     vis.close()
+    if not args.no_batch:
+        line["batched"] = batched_workload(args)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
         spec0 = spec.with_camera(poses[0])
