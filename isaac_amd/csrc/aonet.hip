@@ -1,7 +1,9 @@
 // Learned occlusion pass: U-Net kernels on gfx950 (forward, backward, Adam).
 //
-// Convolutions are implicit GEMMs on v_mfma_f32_16x16x16_f16.  For one
-// 16x16x16 step, lane l of the wave supplies
+// Convolutions are implicit GEMMs on the matrix cores: the forward layers on
+// v_mfma_f32_16x16x32_f16 (below, "forward convolution"), backward data and the
+// weight gradient on v_mfma_f32_16x16x16_f16.  For one 16x16x16 step, lane l of
+// the wave supplies
 //   A (16 x 16, rows = output channels):  row l % 16, k = 4*(l / 16) .. +3
 //   B (16 x 16, cols = pixels):           col l % 16, k = 4*(l / 16) .. +3
 // and receives D rows 4*(l / 16) .. +3 of column l % 16.  With k = tap*Cin + ci
@@ -57,12 +59,13 @@ namespace isaac_hip
         }
     }
 
+    // Backward data: dx (FP32, +=) = conv(dz, flipped transposed weights).
     // Workgroup = ROWS image rows x 64 pixels; four waves split the 64-pixel row
     // segment into 16-pixel columns, and each wave computes its column in all ROWS
     // rows for all output channels, so every weight fragment it loads feeds ROWS
     // MFMAs.  The (ROWS + 2) x 66 x cin input patch is staged in LDS once.
-    template<int COUT, bool FWD, int ROWS>
-    __global__ __launch_bounds__(256) void conv3x3_kernel(const ConvArgs a)
+    template<int COUT, int ROWS>
+    __global__ __launch_bounds__(256) void conv3x3_bwd_kernel(const ConvArgs a)
     {
         extern __shared__ _Float16 patch[];
         const int x0 = blockIdx.x * 64, y0 = blockIdx.y * ROWS, n = blockIdx.z;
@@ -114,53 +117,39 @@ namespace isaac_hip
             for(int cb = 0; cb < COUT / 16; cb++)
             {
                 const int co = cb * 16 + 4 * (lane >> 4);
-                if(FWD)
-                {
-                    h4 o;
-#pragma unroll
-                    for(int q = 0; q < 4; q++)
-                    {
-                        const float v = acc[r][cb][q] + a.bias[co + q];
-                        o[q] = _Float16(v > 0.0f ? v : 0.0f);
-                    }
-                    *reinterpret_cast<h4*>(reinterpret_cast<_Float16*>(a.y) + p * COUT + co) = o;
-                }
-                else
-                {
-                    float4* dst = reinterpret_cast<float4*>(reinterpret_cast<float*>(a.y) + p * COUT + co);
-                    float4 d = *dst;
-                    d.x += acc[r][cb][0];
-                    d.y += acc[r][cb][1];
-                    d.z += acc[r][cb][2];
-                    d.w += acc[r][cb][3];
-                    *dst = d;
-                }
+                float4* dst = reinterpret_cast<float4*>(reinterpret_cast<float*>(a.y) + p * COUT + co);
+                float4 d = *dst;
+                d.x += acc[r][cb][0];
+                d.y += acc[r][cb][1];
+                d.z += acc[r][cb][2];
+                d.w += acc[r][cb][3];
+                *dst = d;
             }
         }
     }
 
-    template<int COUT, bool FWD, int ROWS>
-    hipError_t conv_launch(const ConvArgs& a, hipStream_t s)
+    template<int COUT, int ROWS>
+    hipError_t conv_bwd_launch(const ConvArgs& a, hipStream_t s)
     {
         const dim3 grid(unsigned((a.w_ + 63) / 64), unsigned((a.h + ROWS - 1) / ROWS), unsigned(a.n));
         const size_t lds = size_t((ROWS + 2) * 66) * a.cin * sizeof(_Float16);
-        hipLaunchKernelGGL((conv3x3_kernel<COUT, FWD, ROWS>), grid, dim3(256), lds, s, a);
+        hipLaunchKernelGGL((conv3x3_bwd_kernel<COUT, ROWS>), grid, dim3(256), lds, s, a);
         return hipGetLastError();
     }
 
-    template<int COUT, bool FWD>
-    hipError_t conv_rows(const ConvArgs& a, size_t max_lds, hipStream_t s)
+    template<int COUT>
+    hipError_t conv_bwd_rows(const ConvArgs& a, size_t max_lds, hipStream_t s)
     {
         // four rows per workgroup when the patch fits and the grid still covers the
         // chip several times over; else one
         const size_t lds4 = size_t(6 * 66) * a.cin * sizeof(_Float16);
         const int64_t groups4 = int64_t((a.w_ + 63) / 64) * ((a.h + 3) / 4) * a.n;
         if(lds4 <= max_lds && groups4 >= 1024)
-            return conv_launch<COUT, FWD, 4>(a, s);
-        return conv_launch<COUT, FWD, 1>(a, s);
+            return conv_bwd_launch<COUT, 4>(a, s);
+        return conv_bwd_launch<COUT, 1>(a, s);
     }
 
-    hipError_t launch_conv3x3(const ConvArgs& a, int cout, bool fwd, size_t max_lds, hipStream_t s)
+    hipError_t launch_conv3x3_bwd(const ConvArgs& a, int cout, size_t max_lds, hipStream_t s)
     {
         if(a.cin % 4 != 0 || a.ca % 4 != 0 || a.ca > a.cin || (a.ca > 0 && !a.up) || a.kp % 16 != 0 || a.kp < 9 * a.cin
            || size_t(3 * 66) * a.cin * 2 > max_lds || (a.up && (a.h % 2 || a.w_ % 2)))
@@ -169,14 +158,294 @@ namespace isaac_hip
             return hipSuccess;
         switch(cout)
         {
-        case 16: return fwd ? conv_rows<16, true>(a, max_lds, s) : conv_rows<16, false>(a, max_lds, s);
-        case 32: return fwd ? conv_rows<32, true>(a, max_lds, s) : conv_rows<32, false>(a, max_lds, s);
-        case 48: return fwd ? hipErrorInvalidValue : conv_rows<48, false>(a, max_lds, s);
-        case 64: return fwd ? conv_rows<64, true>(a, max_lds, s) : conv_rows<64, false>(a, max_lds, s);
-        case 96: return fwd ? hipErrorInvalidValue : conv_rows<96, false>(a, max_lds, s);
-        case 128: return fwd ? hipErrorInvalidValue : conv_rows<128, false>(a, max_lds, s);
+        case 16: return conv_bwd_rows<16>(a, max_lds, s);
+        case 32: return conv_bwd_rows<32>(a, max_lds, s);
+        case 48: return conv_bwd_rows<48>(a, max_lds, s);
+        case 64: return conv_bwd_rows<64>(a, max_lds, s);
+        case 96: return conv_bwd_rows<96>(a, max_lds, s);
+        case 128: return conv_bwd_rows<128>(a, max_lds, s);
         default: return hipErrorInvalidValue;
         }
+    }
+
+    // ------------------------------------------------------------------ forward convolution
+    // Forward layers run on v_mfma_f32_16x16x32_f16 (gfx950): lane l supplies
+    //   A row l % 16 (output channel), k = 8*(l / 16) .. +7
+    //   B col l % 16 (pixel),          k = 8*(l / 16) .. +7
+    // and receives D rows 4*(l / 16) .. +3 of column l % 16.  Cin is a multiple of 8,
+    // so a lane's eight k are eight consecutive channels of one tap: one 16-byte load
+    // each.  Layer shapes are template parameters, so the tap / channel of every
+    // K-chunk is a compile-time expression of the lane's group.
+    typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+
+    __device__ inline f4 mfma32(h8 a, h8 b, f4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+
+    // LDS pixel stride (halves) of a patch with CIN channels: an odd number of
+    // 16-byte units, so 16 consecutive pixels of one 16-byte read hit 16 different
+    // bank groups
+    template<int CIN>
+    constexpr int patch_stride()
+    {
+        return ((CIN / 8) % 2) ? CIN : CIN + 8;
+    }
+
+    // All of a thread's 16-byte loads are issued before the first LDS write, so the
+    // workgroup waits for one global-load latency, not one per loop trip.
+    template<int CIN, int CA, int PROWS>
+    __device__ inline void stage_patch8(
+        _Float16* patch,
+        const _Float16* __restrict__ x,
+        const _Float16* __restrict__ up,
+        int n,
+        int h,
+        int w,
+        int y0,
+        int x0)
+    {
+        constexpr int CS = CIN - CA, U = CIN / 8, PS = patch_stride<CIN>();
+        constexpr int TOTAL = PROWS * 66 * U, ITER = (TOTAL + 255) / 256;
+        h8 v[ITER];
+#pragma unroll
+        for(int it = 0; it < ITER; it++)
+        {
+            const int u = int(threadIdx.x) + it * 256;
+            v[it] = h8{0, 0, 0, 0, 0, 0, 0, 0};
+            if(TOTAL % 256 == 0 || u < TOTAL)
+            {
+                const int pix = u / U, c = (u - pix * U) * 8;
+                const int r = pix / 66, col = pix - r * 66;
+                const int yy = y0 + r - 1, xx = x0 + col - 1;
+                if(yy >= 0 && yy < h && xx >= 0 && xx < w)
+                {
+                    if(CA > 0 && c < CA)
+                        v[it] = *reinterpret_cast<const h8*>(up + ((size_t(n) * (h / 2) + yy / 2) * (w / 2) + xx / 2) * CA + c);
+                    else
+                        v[it] = *reinterpret_cast<const h8*>(x + ((size_t(n) * h + yy) * w + xx) * CS + (c - CA));
+                }
+            }
+        }
+#pragma unroll
+        for(int it = 0; it < ITER; it++)
+        {
+            const int u = int(threadIdx.x) + it * 256;
+            if(TOTAL % 256 == 0 || u < TOTAL)
+            {
+                const int pix = u / U, c = (u - pix * U) * 8;
+                *reinterpret_cast<h8*>(patch + pix * PS + c) = v[it];
+            }
+        }
+    }
+
+    // Workgroup = ROWS rows x 64 pixels, wave = a 16-pixel column in all ROWS rows
+    // and all COUT channels: each A fragment feeds ROWS MFMAs, each B fragment
+    // COUT / 16.
+    template<int CIN, int COUT, int CA, int ROWS>
+    __global__ __launch_bounds__(256) void conv_fwd_kernel(const ConvArgs a)
+    {
+        constexpr int K = 9 * CIN, KP = (K + 31) / 32 * 32, NCH = KP / 32, NCB = COUT / 16;
+        constexpr int PS = patch_stride<CIN>();
+        extern __shared__ _Float16 patch[];
+        const int x0 = blockIdx.x * 64, y0 = blockIdx.y * ROWS, n = blockIdx.z;
+        stage_patch8<CIN, CA, ROWS + 2>(patch, a.x, a.up, n, a.h, a.w_, y0, x0);
+        __syncthreads();
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int pl = wave * 16 + (lane & 15);
+        const int g = lane >> 4;
+        f4 acc[ROWS][NCB];
+#pragma unroll
+        for(int r = 0; r < ROWS; r++)
+#pragma unroll
+            for(int cb = 0; cb < NCB; cb++)
+                acc[r][cb] = f4{0.f, 0.f, 0.f, 0.f};
+        // weight fragments run PD chunks ahead of the MFMAs (a ring of PD x NCB)
+        constexpr int PD = NCH < 16 / NCB ? NCH : 16 / NCB;
+        const _Float16* wrow = a.w + size_t(lane & 15) * KP + 8 * g;
+        h8 wq[PD][NCB];
+#pragma unroll
+        for(int p = 0; p < PD; p++)
+#pragma unroll
+            for(int cb = 0; cb < NCB; cb++)
+                wq[p][cb] = *reinterpret_cast<const h8*>(wrow + size_t(cb * 16) * KP + 32 * p);
+#pragma unroll
+        for(int j = 0; j < NCH; j++)
+        {
+            h8 wa[NCB];
+#pragma unroll
+            for(int cb = 0; cb < NCB; cb++)
+                wa[cb] = wq[j % PD][cb];
+            if(j + PD < NCH)
+            {
+#pragma unroll
+                for(int cb = 0; cb < NCB; cb++)
+                    wq[j % PD][cb] = *reinterpret_cast<const h8*>(wrow + size_t(cb * 16) * KP + 32 * (j + PD));
+            }
+            const int k0 = 32 * j + 8 * g;
+            const int tap = k0 / CIN, ci = k0 - tap * CIN;
+            const bool valid = k0 < K;
+            const int ky = valid ? tap / 3 : 0, kx = valid ? tap - 3 * (tap / 3) : 0;
+            const _Float16* pb = patch + (ky * 66 + pl + kx) * PS + ci;
+#pragma unroll
+            for(int r = 0; r < ROWS; r++)
+            {
+                h8 b = {0, 0, 0, 0, 0, 0, 0, 0};
+                if(valid)
+                    b = *reinterpret_cast<const h8*>(pb + r * 66 * PS);
+#pragma unroll
+                for(int cb = 0; cb < NCB; cb++)
+                    acc[r][cb] = mfma32(wa[cb], b, acc[r][cb]);
+            }
+        }
+        // epilogue: bias + ReLU rounded to FP16 (the activation), then optionally its
+        // 2x2 average pool (rows r-1, r of this wave; x and x+1 in lanes l, l^1) and
+        // the 1x1 head (channels of lane groups 1..3 gathered into group 0)
+        const int x = x0 + pl;
+        const bool xin = x < a.w_;
+        _Float16* y = reinterpret_cast<_Float16*>(a.y);
+        h4 prev[NCB];
+#pragma unroll
+        for(int r = 0; r < ROWS; r++)
+        {
+            const int yy = y0 + r;
+            const bool in = xin && yy < a.h;
+            const size_t p = (size_t(n) * a.h + yy) * a.w_ + x;
+            h4 o[NCB];
+#pragma unroll
+            for(int cb = 0; cb < NCB; cb++)
+            {
+                const int co = cb * 16 + 4 * g;
+#pragma unroll
+                for(int q = 0; q < 4; q++)
+                {
+                    const float v = acc[r][cb][q] + a.bias[co + q];
+                    o[cb][q] = _Float16(v > 0.0f ? v : 0.0f);
+                }
+                if(in && y)
+                    *reinterpret_cast<h4*>(y + p * COUT + co) = o[cb];
+            }
+            if((ROWS % 2) == 0 && a.pool && (r & 1))
+            {
+                const size_t p2 = (size_t(n) * (a.h / 2) + yy / 2) * (a.w_ / 2) + x / 2;
+#pragma unroll
+                for(int cb = 0; cb < NCB; cb++)
+                {
+                    h4 m;
+#pragma unroll
+                    for(int q = 0; q < 4; q++)
+                    {
+                        const float v00 = float(prev[cb][q]), v10 = float(o[cb][q]);
+                        const float v01 = __shfl_xor(v00, 1), v11 = __shfl_xor(v10, 1);
+                        m[q] = _Float16(((v00 + v01) + (v10 + v11)) * 0.25f);
+                    }
+                    if(in && (lane & 1) == 0)
+                        *reinterpret_cast<h4*>(a.pool + p2 * COUT + cb * 16 + 4 * g) = m;
+                }
+            }
+            if constexpr(COUT == 16)
+            {
+                if(a.head)
+                {
+                    // channels 4g'..4g'+3 of this pixel from lane l + 16 g'
+                    typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+                    const u2 bits = __builtin_bit_cast(u2, o[0]);
+                    h4 ch[4];
+                    ch[0] = o[0];
+#pragma unroll
+                    for(int gg = 1; gg < 4; gg++)
+                    {
+                        u2 t;
+                        t.x = __shfl_down(bits.x, 16 * gg);
+                        t.y = __shfl_down(bits.y, 16 * gg);
+                        ch[gg] = __builtin_bit_cast(h4, t);
+                    }
+                    if(g == 0 && x < a.imgW && yy < a.imgH)
+                    {
+                        float sum = a.headW[16];
+#pragma unroll
+                        for(int q = 0; q < 4; q++)
+#pragma unroll
+                            for(int e = 0; e < 4; e++)
+                                sum += float(ch[q][e]) * a.headW[q * 4 + e];
+                        a.head[(size_t(n) * a.imgH + yy) * a.imgW + x] = sum < 0.0f ? 0.0f : (sum > 1.0f ? 1.0f : sum);
+                    }
+                }
+            }
+#pragma unroll
+            for(int cb = 0; cb < NCB; cb++)
+                prev[cb] = o[cb];
+        }
+    }
+
+    template<int CIN>
+    constexpr size_t patch_bytes(int rows)
+    {
+        return size_t((rows + 2) * 66) * patch_stride<CIN>() * sizeof(_Float16);
+    }
+
+    template<int CIN, int COUT, int CA, int ROWS>
+    hipError_t conv_fwd_launch(const ConvArgs& a, hipStream_t s)
+    {
+        const dim3 grid(unsigned((a.w_ + 63) / 64), unsigned((a.h + ROWS - 1) / ROWS), unsigned(a.n));
+        hipLaunchKernelGGL((conv_fwd_kernel<CIN, COUT, CA, ROWS>), grid, dim3(256), patch_bytes<CIN>(ROWS), s, a);
+        return hipGetLastError();
+    }
+
+    // rows per workgroup: the most (of 8, 4, 2, 1) whose patch fits max_lds and whose
+    // grid still has >= 1024 workgroups; else the fewest that fits (variants whose
+    // patch exceeds 64 KiB are not built)
+    template<int CIN, int COUT, int CA>
+    hipError_t conv_fwd_rows(const ConvArgs& a, size_t max_lds, hipStream_t s)
+    {
+        constexpr int KP = (9 * CIN + 31) / 32 * 32;
+        constexpr size_t kLdsCap = 64 * 1024;
+        if(a.cin != CIN || a.ca != CA || a.kp != KP || (a.head && COUT != 16))
+            return hipErrorInvalidValue;
+        // a fused pool needs both rows of each pair in one workgroup
+        const bool pool = a.pool != nullptr;
+        const auto groups = [&](int rows) { return int64_t((a.w_ + 63) / 64) * ((a.h + rows - 1) / rows) * a.n; };
+        if constexpr(patch_bytes<CIN>(8) <= kLdsCap)
+            if(patch_bytes<CIN>(8) <= max_lds && groups(8) >= 1024)
+                return conv_fwd_launch<CIN, COUT, CA, 8>(a, s);
+        if constexpr(patch_bytes<CIN>(4) <= kLdsCap)
+            if(patch_bytes<CIN>(4) <= max_lds && groups(4) >= 1024)
+                return conv_fwd_launch<CIN, COUT, CA, 4>(a, s);
+        if constexpr(patch_bytes<CIN>(2) <= kLdsCap)
+            if(patch_bytes<CIN>(2) <= max_lds && (pool || groups(2) >= 1024))
+                return conv_fwd_launch<CIN, COUT, CA, 2>(a, s);
+        if(!pool && patch_bytes<CIN>(1) <= max_lds)
+            return conv_fwd_launch<CIN, COUT, CA, 1>(a, s);
+        return hipErrorInvalidValue;
+    }
+
+    hipError_t launch_conv_fwd(const ConvArgs& a, int cout, size_t max_lds, hipStream_t s)
+    {
+        if((a.ca > 0 && !a.up) || ((a.up || a.pool) && (a.h % 2 || a.w_ % 2))
+           || (a.head && (!a.headW || a.imgW > a.w_ || a.imgH > a.h)))
+            return hipErrorInvalidValue;
+        if(a.n == 0 || a.h == 0 || a.w_ == 0)
+            return hipSuccess;
+        const int cin = a.cin, ca = a.ca;
+        if(cin == 8 && cout == 16 && ca == 0)
+            return conv_fwd_rows<8, 16, 0>(a, max_lds, s);
+        if(cin == 16 && cout == 16 && ca == 0)
+            return conv_fwd_rows<16, 16, 0>(a, max_lds, s);
+        if(cin == 16 && cout == 32 && ca == 0)
+            return conv_fwd_rows<16, 32, 0>(a, max_lds, s);
+        if(cin == 32 && cout == 32 && ca == 0)
+            return conv_fwd_rows<32, 32, 0>(a, max_lds, s);
+        if(cin == 32 && cout == 64 && ca == 0)
+            return conv_fwd_rows<32, 64, 0>(a, max_lds, s);
+        if(cin == 64 && cout == 64 && ca == 0)
+            return conv_fwd_rows<64, 64, 0>(a, max_lds, s);
+        if(cin == 128 && cout == 64 && ca == 64)
+            return conv_fwd_rows<128, 64, 64>(a, max_lds, s);
+        if(cin == 96 && cout == 32 && ca == 64)
+            return conv_fwd_rows<96, 32, 64>(a, max_lds, s);
+        if(cin == 48 && cout == 16 && ca == 32)
+            return conv_fwd_rows<48, 16, 32>(a, max_lds, s);
+        return hipErrorInvalidValue;
     }
 
     // ------------------------------------------------------------------ weight gradient
@@ -366,32 +635,6 @@ namespace isaac_hip
         return hipGetLastError();
     }
 
-    // output element i = ((n*ho + yo)*wo + xo)*c + ch
-    __global__ __launch_bounds__(256) void pool2_kernel(const _Float16* x, _Float16* y, int n, int h, int w, int c)
-    {
-        const int ho = h / 2, wo = w / 2;
-        const int64_t count = int64_t(n) * ho * wo * c;
-        for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x)
-        {
-            const int ch = int(i % c);
-            const int64_t q = i / c;
-            const int xo = int(q % wo), yo = int((q / wo) % ho);
-            const int64_t img = q / (int64_t(wo) * ho);
-            const _Float16* b = x + ((img * h + 2 * yo) * w + 2 * xo) * c + ch;
-            const float s = (float(b[0]) + float(b[c])) + (float(b[size_t(w) * c]) + float(b[size_t(w) * c + c]));
-            y[i] = _Float16(s * 0.25f);
-        }
-    }
-
-    hipError_t launch_pool2(const _Float16* x, _Float16* y, int n, int h, int w, int c, hipStream_t s)
-    {
-        const int64_t count = int64_t(n) * (h / 2) * (w / 2) * c;
-        if(count == 0)
-            return hipSuccess;
-        hipLaunchKernelGGL(pool2_kernel, dim3(grid_for(count)), dim3(256), 0, s, x, y, n, h, w, c);
-        return hipGetLastError();
-    }
-
     // dx (n, h, w, c) += dy(n, h/2, w/2, c) / 4; one thread per dx element
     __global__ __launch_bounds__(256) void pool2_bwd_kernel(const float* dy, float* dx, int n, int h, int w, int c)
     {
@@ -412,38 +655,6 @@ namespace isaac_hip
         if(count == 0)
             return hipSuccess;
         hipLaunchKernelGGL(pool2_bwd_kernel, dim3(grid_for(count)), dim3(256), 0, s, dy, dx, n, h, w, c);
-        return hipGetLastError();
-    }
-
-    // y (n, h, w, ca+cs): channels [0, ca) from a (n, h/2, w/2, ca), [ca, ca+cs) from skip
-    __global__ __launch_bounds__(256) void upcat_kernel(const _Float16* a, const _Float16* skip, _Float16* y, int n, int h, int w, int ca, int cs)
-    {
-        const int ct = ca + cs;
-        const int64_t count = int64_t(n) * h * w * ct / 4;
-        for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x)
-        {
-            const int64_t e = i * 4;
-            const int ch = int(e % ct);
-            const int64_t q = e / ct;
-            const int xx = int(q % w), yy = int((q / w) % h);
-            const int64_t img = q / (int64_t(w) * h);
-            h4 v;
-            if(ch < ca)
-                v = *reinterpret_cast<const h4*>(a + ((img * (h / 2) + yy / 2) * (w / 2) + xx / 2) * ca + ch);
-            else
-                v = *reinterpret_cast<const h4*>(skip + q * cs + (ch - ca));
-            *reinterpret_cast<h4*>(y + e) = v;
-        }
-    }
-
-    hipError_t launch_upcat(const _Float16* a, const _Float16* skip, _Float16* y, int n, int h, int w, int ca, int cs, hipStream_t s)
-    {
-        if(ca % 4 || cs % 4 || h % 2 || w % 2)
-            return hipErrorInvalidValue;
-        const int64_t count = int64_t(n) * h * w * (ca + cs) / 4;
-        if(count == 0)
-            return hipSuccess;
-        hipLaunchKernelGGL(upcat_kernel, dim3(grid_for(count)), dim3(256), 0, s, a, skip, y, n, h, w, ca, cs);
         return hipGetLastError();
     }
 
@@ -544,29 +755,6 @@ namespace isaac_hip
                 s += float(u[r]) * wh[q * 4 + r];
         }
         return s;
-    }
-
-    __global__ __launch_bounds__(256) void head_infer_kernel(const _Float16* d0, const float* wh, float* ao, int n, int h, int w, int imgW, int imgH)
-    {
-        const int64_t count = int64_t(n) * imgW * imgH;
-        for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x)
-        {
-            const int xx = int(i % imgW), yy = int((i / imgW) % imgH);
-            const int64_t item = i / (int64_t(imgW) * imgH);
-            const float v = head_dot(d0, wh, (item * h + yy) * w + xx);
-            ao[i] = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
-        }
-    }
-
-    hipError_t launch_head_infer(const _Float16* d0, const float* wh, float* ao, int n, int h, int w, int imgW, int imgH, hipStream_t s)
-    {
-        if(imgW > w || imgH > h)
-            return hipErrorInvalidValue;
-        const int64_t count = int64_t(n) * imgW * imgH;
-        if(count == 0)
-            return hipSuccess;
-        hipLaunchKernelGGL(head_infer_kernel, dim3(grid_for(count)), dim3(256), 0, s, d0, wh, ao, n, h, w, imgW, imgH);
-        return hipGetLastError();
     }
 
     __global__ __launch_bounds__(256) void head_train_kernel(

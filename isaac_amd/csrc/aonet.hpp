@@ -1,11 +1,12 @@
 // Kernels of the learned occlusion pass: a U-Net over the G-buffer whose 3x3
 // convolutions run as implicit GEMMs on the gfx950 matrix cores
-// (v_mfma_f32_16x16x16_f16: FP16 operands, FP32 accumulation).
+// (v_mfma_f32_16x16x32_f16 forward, v_mfma_f32_16x16x16_f16 backward: FP16
+// operands, FP32 accumulation).
 //
 // Tensors are channels-last: element (n, y, x, c) at ((n*H + y)*W + x)*C + c.
-// Convolution weights are [Cout][Kp] FP16 with k = tap*Cin + ci (tap = ky*3 + kx
-// reads pixel (y + ky - 1, x + kx - 1), zero padding), Kp = 9*Cin rounded up to
-// a multiple of 16 with zero columns.
+// Forward convolution weights are [Cout][Kp] FP16 with k = tap*Cin + ci (tap =
+// ky*3 + kx reads pixel (y + ky - 1, x + kx - 1), zero padding), Kp = 9*Cin
+// rounded up to a multiple of 32 with zero columns.
 #pragma once
 
 #include <hip/hip_fp16.h>
@@ -26,13 +27,27 @@ namespace isaac_hip
         int32_t ca; // channels from `up` (a multiple of 4; 0 without `up`)
         const _Float16* w; // [cout][kp]
         const float* bias; // [cout] (forward only)
-        void* y; // forward: _Float16 [n][h][w][cout] = relu(conv + bias); backward data: float, +=
+        void* y; // forward: _Float16 [n][h][w][cout] = relu(conv + bias) (nullable); backward data: float, +=
         int32_t n, h, w_, cin, kp; // cin a multiple of 4
+        // forward only (launch_conv_fwd), all optional:
+        _Float16* pool; // [n][h/2][w/2][cout]: 2x2 average of the FP16 activation (h, w even)
+        float* head; // cout 16: [n][imgH][imgW] = clamp(act . headW[0..15] + headW[16], 0, 1)
+        const float* headW;
+        int32_t imgW, imgH;
     };
 
-    // cout in {16, 32, 48, 64, 96, 128}; relu_fp16_out: forward form, else FP32 accumulate.
-    // max_lds: the device's LDS bytes per workgroup (picks the rows per workgroup).
-    hipError_t launch_conv3x3(const ConvArgs& a, int cout, bool relu_fp16_out, size_t max_lds, hipStream_t s);
+    // Backward data of a layer: y (FP32, +=) = 3x3 convolution of x with the layer's
+    // flipped, transposed weights ([cin][kp], kp = 9*cout); `cout` here is the
+    // layer's cin, in {16, 32, 48, 64, 96, 128}.  max_lds: the device's LDS bytes per
+    // workgroup (picks the rows per workgroup).
+    hipError_t launch_conv3x3_bwd(const ConvArgs& a, int cout, size_t max_lds, hipStream_t s);
+
+    // Forward layer y = relu(conv(x) + bias) on v_mfma_f32_16x16x32_f16 for the
+    // network's (cin, cout, ca) shapes: (8,16,0) (16,16,0) (16,32,0) (32,32,0)
+    // (32,64,0) (64,64,0) (128,64,64) (96,32,64) (48,16,32); weights [cout][kp] with
+    // kp = 9*cin rounded up to a multiple of 32 (zero columns).  The fused pool and
+    // head outputs equal launch_pool2 / launch_head_infer applied to the activation.
+    hipError_t launch_conv_fwd(const ConvArgs& a, int cout, size_t max_lds, hipStream_t s);
 
     // dW[co][k] += sum_p dz[p][co] * col(x)[p][k] for k < 9*cin (FP32 atomics into dw,
     // rows of 9*cin floats: the master weight layout [cout][9][cin])
@@ -52,11 +67,10 @@ namespace isaac_hip
     hipError_t launch_bias_grad(const _Float16* dz, float* db, int64_t npx, int c, hipStream_t s);
     // dz = half(dy * (y > 0))
     hipError_t launch_relu_bwd(const float* dy, const _Float16* y, _Float16* dz, int64_t count, hipStream_t s);
-    // 2x2 average pool (h, w even) and its backward (dx += dy / 4)
-    hipError_t launch_pool2(const _Float16* x, _Float16* y, int n, int h, int w, int c, hipStream_t s);
+    // backward of the 2x2 average pool (h, w even): dx += dy / 4
     hipError_t launch_pool2_bwd(const float* dy, float* dx, int n, int h, int w, int c, hipStream_t s);
-    // y = concat(nearest-upsample(a), skip) at the skip's size; backward splits dy
-    hipError_t launch_upcat(const _Float16* a, const _Float16* skip, _Float16* y, int n, int h, int w, int ca, int cs, hipStream_t s);
+    // backward of y = concat(nearest-upsample(a), skip): da += 2x2 sums of dy[.., :ca],
+    // dskip += dy[.., ca:]
     hipError_t launch_upcat_bwd(const float* dy, float* da, float* dskip, int n, int h, int w, int ca, int cs, hipStream_t s);
 
     // network input from a G-buffer window: 8 channels (depth / scale or 0, background
@@ -78,18 +92,6 @@ namespace isaac_hip
     };
     hipError_t launch_features(const FeatureArgs& a, hipStream_t s);
 
-    // 1x1 head 16 -> 1.  Inference over n items: ao[item][y][x] = clamp(out, 0, 1) for
-    // x < imgW, y < imgH (item images of imgW * imgH floats, back to back).
-    hipError_t launch_head_infer(
-        const _Float16* d0,
-        const float* wh,
-        float* ao,
-        int n,
-        int h,
-        int w,
-        int imgW,
-        int imgH,
-        hipStream_t s);
     // Training: out = d0 . wh + bh (wh[16] = bh); loss += sum (out - t)^2;
     // gwh[0..15] += sum dout * d0, gwh[16] += sum dout; dd0 = dout * wh with
     // dout = 2 (out - t); `out` (nullable) receives the raw outputs

@@ -29,9 +29,10 @@ namespace isaac_hip
         constexpr int kHeadIn = 16;
         constexpr int kMaxBatch = 64;
 
+        // forward weight row length: 9*cin rounded up to the forward MFMA's K of 32
         int kpad(int cin)
         {
-            return (9 * cin + 15) / 16 * 16;
+            return (9 * cin + 31) / 32 * 32;
         }
     } // namespace
 
@@ -228,9 +229,11 @@ namespace isaac_hip
             int w,
             hipStream_t s,
             const _Float16* up = nullptr,
-            int ca = 0)
+            int ca = 0,
+            _Float16* pool = nullptr)
         {
             ConvArgs a{};
+            a.pool = pool;
             a.x = in;
             a.up = up;
             a.ca = ca;
@@ -242,11 +245,23 @@ namespace isaac_hip
             a.w_ = w;
             a.cin = kLayers[i].cin;
             a.kp = net->kp[i];
-            return launch_conv3x3(a, kLayers[i].cout, true, net->maxLds, s);
+            return launch_conv_fwd(a, kLayers[i].cout, net->maxLds, s);
         }
 
-        // U-Net forward over ws.x0 (n images of h x w, multiples of 8)
-        hipError_t forward(AoNet* net, AoWorkspace& ws, int n, int h, int w, hipStream_t s)
+        // U-Net forward over ws.x0 (n images of h x w, multiples of 8); the pools are
+        // fused into the layers before them.  With `ao`, the head is fused into the
+        // last layer (clamped occlusion of the imgW x imgH corner of each image into
+        // ao) and that layer's activation is not stored.
+        hipError_t forward(
+            AoNet* net,
+            AoWorkspace& ws,
+            int n,
+            int h,
+            int w,
+            hipStream_t s,
+            float* ao = nullptr,
+            int imgW = 0,
+            int imgH = 0)
         {
             const int h1 = h / 2, w1 = w / 2, h2 = h / 4, w2 = w / 4, h3 = h / 8, w3 = w / 8;
             hipError_t e;
@@ -254,20 +269,34 @@ namespace isaac_hip
     if((e = (x)) != hipSuccess)                                                                                       \
     return e
             AOCHK(conv_fwd(net, 0, ws.x0, ws.act[0], n, h, w, s));
-            AOCHK(conv_fwd(net, 1, ws.act[0], ws.act[1], n, h, w, s));
-            AOCHK(launch_pool2(ws.act[1], ws.p0, n, h, w, 16, s));
+            AOCHK(conv_fwd(net, 1, ws.act[0], ws.act[1], n, h, w, s, nullptr, 0, ws.p0));
             AOCHK(conv_fwd(net, 2, ws.p0, ws.act[2], n, h1, w1, s));
-            AOCHK(conv_fwd(net, 3, ws.act[2], ws.act[3], n, h1, w1, s));
-            AOCHK(launch_pool2(ws.act[3], ws.p1, n, h1, w1, 32, s));
+            AOCHK(conv_fwd(net, 3, ws.act[2], ws.act[3], n, h1, w1, s, nullptr, 0, ws.p1));
             AOCHK(conv_fwd(net, 4, ws.p1, ws.act[4], n, h2, w2, s));
-            AOCHK(conv_fwd(net, 5, ws.act[4], ws.act[5], n, h2, w2, s));
-            AOCHK(launch_pool2(ws.act[5], ws.p2, n, h2, w2, 64, s));
+            AOCHK(conv_fwd(net, 5, ws.act[4], ws.act[5], n, h2, w2, s, nullptr, 0, ws.p2));
             AOCHK(conv_fwd(net, 6, ws.p2, ws.act[6], n, h3, w3, s));
             AOCHK(conv_fwd(net, 7, ws.act[6], ws.act[7], n, h3, w3, s));
             AOCHK(conv_fwd(net, 8, ws.act[5], ws.act[8], n, h2, w2, s, ws.act[7], 64));
             AOCHK(conv_fwd(net, 9, ws.act[3], ws.act[9], n, h1, w1, s, ws.act[8], 64));
-            AOCHK(conv_fwd(net, 10, ws.act[1], ws.act[10], n, h, w, s, ws.act[9], 32));
-            return hipSuccess;
+            if(!ao)
+                return conv_fwd(net, 10, ws.act[1], ws.act[10], n, h, w, s, ws.act[9], 32);
+            ConvArgs a{};
+            a.x = ws.act[1];
+            a.up = ws.act[9];
+            a.ca = 32;
+            a.w = net->wf[10];
+            a.bias = net->master + net->boff[10];
+            a.y = nullptr;
+            a.n = n;
+            a.h = h;
+            a.w_ = w;
+            a.cin = kLayers[10].cin;
+            a.kp = net->kp[10];
+            a.head = ao;
+            a.headW = net->master + net->headOff;
+            a.imgW = imgW;
+            a.imgH = imgH;
+            return launch_conv_fwd(a, kLayers[10].cout, net->maxLds, s);
         }
 
         // backward of layer i given ws.g[i] (gradient wrt its ReLU output): weight and
@@ -305,7 +334,7 @@ namespace isaac_hip
                 a.w_ = w;
                 a.cin = cout;
                 a.kp = 9 * cout;
-                AOCHK(launch_conv3x3(a, cin, false, net->maxLds, s));
+                AOCHK(launch_conv3x3_bwd(a, cin, net->maxLds, s));
             }
             return hipSuccess;
         }
@@ -398,8 +427,7 @@ namespace isaac_hip
         f.h = Hp;
         f.w = Wp;
         HIPCHK(c, launch_features(f, c->stream));
-        HIPCHK(c, forward(net, net->infer, 1, Hp, Wp, c->stream));
-        HIPCHK(c, launch_head_infer(net->infer.act[10], net->master + net->headOff, c->ao, 1, Hp, Wp, W, H, c->stream));
+        HIPCHK(c, forward(net, net->infer, 1, Hp, Wp, c->stream, c->ao, W, H));
         return 0;
     }
 
@@ -450,8 +478,7 @@ namespace isaac_hip
             fa.h = Hp;
             fa.w = Wp;
             HIPCHK(c, launch_features(fa, s));
-            HIPCHK(c, forward(net, net->batch, n, Hp, Wp, s));
-            HIPCHK(c, launch_head_infer(net->batch.act[10], net->master + net->headOff, ao + img * f0, n, Hp, Wp, W, H, s));
+            HIPCHK(c, forward(net, net->batch, n, Hp, Wp, s, ao + img * f0, W, H));
         }
         return 0;
     }

@@ -88,6 +88,36 @@ def test_gradients_match_fp64_oracle(gpu):
     vis.close()
 
 
+def test_render_path_occlusion_matches_network(gpu):
+    """The render path's learned occlusion map (pools and head fused into the
+    convolutions, frame padded to multiples of 8 with background) equals
+    clamp(network(features of its own G-buffer)) evaluated in float64."""
+    from isaac_amd import aonet
+    from oracle import aonet_oracle as A
+
+    spec = scenes.prod_scene(n=32, fb=(100, 60), n_particles=2000)
+    vis = _net_vis(gpu, spec)
+    p = aonet.init_params(5)
+    vis.aonet_set_params(p)
+    vis.set_ao_mode(vis.AO_LEARNED)
+    vis.draw()
+    _, depth, normal, ao = vis.gbuffer()
+    scale = max(int(np.float32(g) * np.float32(s)) for g, s in zip(spec.global_size, spec.scale))
+    H, W = depth.shape
+    Hp, Wp = (H + 7) // 8 * 8, (W + 7) // 8 * 8
+    d = np.full((1, Hp, Wp), np.finfo(np.float32).max, np.float64)
+    nrm = np.zeros((1, Hp, Wp, 3))
+    d[0, :H, :W] = depth
+    nrm[0, :H, :W] = normal
+    assert np.count_nonzero(depth < 3e38) > 0.2 * depth.size  # the frame has geometry
+    x16 = A.features(d, nrm, float(scale)).astype(np.float16).astype(np.float64)
+    want = np.clip(A.forward(p.astype(np.float64), x16)[0][0, :H, :W], 0.0, 1.0)
+    err = np.abs(ao.astype(np.float64) - want)
+    print(f"render-path occlusion vs float64 network: max |err| {err.max():.4g}, mean {err.mean():.3g}", flush=True)
+    assert err.max() <= 2e-2
+    vis.close()
+
+
 def _training_scene():
     # particles and an iso-surface give the depth structure the occlusion term depends on
     return scenes.prod_scene(n=128, fb=(1920, 1080), n_particles=250_000)
