@@ -195,6 +195,21 @@ extern "C"
         uint8_t* out_rgba,
         int out_is_device);
     int isaac_hip_synchronize(isaac_hip_ctx* ctx);
+    /* The min/max request (sendMinMax, lib/isaac.hpp:1951-2006, one rank): per
+     * source, in the combined order (volume, field, particle), the min and max of
+     * its functor-chain values over the local volume (MinMaxKernel,
+     * lib/isaac/isaac_min_max_kernel.hpp:22-57; CalcMinMaxIterator,
+     * lib/isaac.hpp:372-432) or over its particles (MinMaxParticleKernel,
+     * lib/isaac/isaac_min_max_kernel.hpp:60-97; lib/isaac.hpp:434-497).  Persistent
+     * volume sources and particle sources are read through their descriptors (the
+     * pointers must be valid), non-persistent sources from the texture the last
+     * isaac_hip_update_buffers filled.  Comparisons follow glm's min/max: NaN is
+     * skipped, the start values are FLT_MAX / -FLT_MAX (so +inf never becomes the
+     * min nor -inf the max), and ties keep the first element in the reference's
+     * visiting order.  A source without values (unset, no particles, a
+     * non-persistent source never updated) reports FLT_MAX / -FLT_MAX.
+     * min_out / max_out: host, count >= number of sources.  Blocks until done. */
+    int isaac_hip_min_max(isaac_hip_ctx* ctx, float* min_out, float* max_out, int32_t count);
 
     /* ---------------------------------------------------------------- tuning
      * Launch-schedule knobs with no counterpart in the reference (results do not

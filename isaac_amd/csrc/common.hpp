@@ -276,4 +276,25 @@ namespace isaac_hip
         uint64_t hi = __umulhi(n, mul);
         return (uint32_t) ((hi + n) >> shift);
     }
+
+    // min/max with the visiting-order key of each value: glm's min(x, y) = (y < x) ? y : x
+    // keeps x on ties, so among equal values the earliest element wins; a NaN y fails
+    // both comparisons and is never taken
+    __device__ inline void mm_take_min(float& v, uint64_t& k, float v2, uint64_t k2)
+    {
+        if(v2 < v || (v2 == v && k2 < k))
+        {
+            v = v2;
+            k = k2;
+        }
+    }
+
+    __device__ inline void mm_take_max(float& v, uint64_t& k, float v2, uint64_t k2)
+    {
+        if(v < v2 || (v2 == v && k2 < k))
+        {
+            v = v2;
+            k = k2;
+        }
+    }
 } // namespace isaac_hip

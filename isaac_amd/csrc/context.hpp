@@ -91,6 +91,9 @@ struct isaac_hip_ctx
     float4* outFloat = nullptr;
     int aoMode = ISAAC_HIP_AO_PLAIN;
     isaac_hip::AoNet* aonet = nullptr;
+    // min/max request: per-workgroup partials and one (min, max) per source
+    isaac_hip::MinMaxPartial* mmPartials = nullptr;
+    float2* mmResult = nullptr;
     // events: 0 update start, 1 after source copies, 2 after binning, 3 after
     // classification (or brick build); 4 draw start, 5 after the march, 6 after the
     // occlusion pass, 7 after shading, 8 after the framebuffer copy

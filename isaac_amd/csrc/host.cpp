@@ -819,7 +819,8 @@ extern "C"
         for(auto& p : c->parts)
             free_part(p);
         aonet_destroy(c);
-        void* bufs[] = {c->lutDev, c->tex, c->color, c->depth, c->normal, c->ao, c->volColor, c->counters, c->outFloat};
+        void* bufs[] = {c->lutDev, c->tex,      c->color,      c->depth,   c->normal,  c->ao,
+                        c->volColor, c->counters, c->outFloat, c->mmPartials, c->mmResult};
         for(void* b : bufs)
             if(b)
                 (void) hipFree(b);
@@ -1476,6 +1477,90 @@ extern "C"
             *texels = c->tex;
         if(texel_bytes)
             *texel_bytes = 8;
+        return 0;
+    }
+
+    int isaac_hip_min_max(isaac_hip_ctx* c, float* min_out, float* max_out, int32_t count)
+    {
+        if(!c || !min_out || !max_out)
+            return ISAAC_HIP_ERR_ARG;
+        const int nvf = c->nv + c->nf, total = c->nsrc;
+        if(count < total)
+            return fail(c, ISAAC_HIP_ERR_ARG, "min_max: count below the number of sources");
+        HIPCHK(c, hipSetDevice(c->device));
+        if(!c->mmPartials)
+            HIPCHK(c, hipMalloc(&c->mmPartials, sizeof(MinMaxPartial) * kMinMaxGroups));
+        if(!c->mmResult)
+            HIPCHK(c, hipMalloc(&c->mmResult, sizeof(float2) * std::max(total, 1)));
+        std::vector<char> computed(static_cast<size_t>(total), 0);
+        for(int s = 0; s < nvf; s++)
+        {
+            const VolSlot& v = c->vols[s];
+            MinMaxVolumeParams P{};
+            P.X = int32_t(c->local[0]);
+            P.Y = int32_t(c->local[1]);
+            P.Z = int32_t(c->local[2]);
+            P.nvox = int64_t(P.X) * P.Y * P.Z;
+            if(!v.set || P.nvox == 0)
+                continue;
+            if(P.nvox >= (int64_t(1) << 32))
+                return fail(c, ISAAC_HIP_ERR_UNSUPPORTED, "min_max: local volume above 2^32 voxels");
+            make_fastdiv(uint32_t(P.X), P.divX_mul, P.divX_shift);
+            make_fastdiv(uint32_t(P.X * P.Y), P.divXY_mul, P.divXY_shift);
+            if(v.desc.persistent)
+            {
+                P.data = (const float*) v.desc.data;
+                P.dim = v.desc.feature_dim;
+                for(int k = 0; k < 3; k++)
+                    P.stride[k] = v.desc.stride[k];
+                P.applyChain = 1;
+                P.chain = c->chain[s];
+            }
+            else
+            {
+                // the texture the last buffer update filled with the chain values
+                if(!v.npTex)
+                    continue;
+                const int64_t g = v.desc.guard_size;
+                const int64_t gX = c->local[0] + 2 * g, gY = c->local[1] + 2 * g;
+                P.data = v.npTex + (g + g * gX + g * gX * gY);
+                P.dim = 1;
+                P.stride[0] = 1;
+                P.stride[1] = gX;
+                P.stride[2] = gX * gY;
+                P.applyChain = 0;
+            }
+            HIPCHK(c, launch_min_max_volume(P, c->mmPartials, c->mmResult + s, c->stream));
+            computed[size_t(s)] = 1;
+        }
+        for(int s = 0; s < c->np; s++)
+        {
+            const PartSlot& p = c->parts[s];
+            if(!p.set)
+                continue;
+            MinMaxParticleParams P{};
+            P.pos = p.desc.position;
+            P.attr = p.desc.attribute;
+            P.dim = p.desc.feature_dim;
+            P.n = p.desc.count;
+            for(int i = 0; i < 3; i++)
+            {
+                P.grid[i] = int32_t(c->pgrid[i]);
+                P.cw[i] = float(c->local[i]) / float(c->pgrid[i]);
+            }
+            P.chain = c->chain[nvf + s];
+            HIPCHK(c, launch_min_max_particles(P, c->mmPartials, c->mmResult + nvf + s, c->stream));
+            computed[size_t(nvf + s)] = 1;
+        }
+        std::vector<float2> res(static_cast<size_t>(total));
+        if(total)
+            HIPCHK(c, hipMemcpyAsync(res.data(), c->mmResult, sizeof(float2) * total, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for(int s = 0; s < total; s++)
+        {
+            min_out[s] = computed[size_t(s)] ? res[size_t(s)].x : FLT_MAX;
+            max_out[s] = computed[size_t(s)] ? res[size_t(s)].y : -FLT_MAX;
+        }
         return 0;
     }
 

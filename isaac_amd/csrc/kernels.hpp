@@ -75,6 +75,72 @@ namespace isaac_hip
         unsigned long long* scanTmp; // scan workspace (bin_scan_workspace_elems entries)
     };
 
+    // ---- min/max request (MinMaxKernel / MinMaxParticleKernel,
+    // lib/isaac/isaac_min_max_kernel.hpp:22-97).  Every value carries the position
+    // of its element in the reference's visiting order (column x + y*X, then z, then
+    // the particle's place in its cell), and ties keep the earlier element, so the
+    // parallel reduction returns the bits the sequential loops return (+0 vs -0).
+    // NaN never replaces a value; the start values are FLT_MAX / -FLT_MAX.
+    struct MinMaxPartial
+    {
+        float vmin, vmax;
+        uint64_t kmin, kmax; // 0 = the start value
+    };
+
+    constexpr int kMinMaxGroups = 1024; // partial results per source (workgroups)
+
+    struct MinMaxVolumeParams
+    {
+        const float* data; // local voxel (0,0,0)
+        int64_t stride[3]; // element strides (elements of dim floats)
+        int32_t dim;
+        int32_t applyChain; // 0: data already holds the chain values (texture of a non-persistent source)
+        int32_t X, Y, Z;
+        uint32_t divX_mul, divX_shift, divXY_mul, divXY_shift;
+        int64_t nvox; // < 2^32
+        ChainDesc chain;
+    };
+
+    struct MinMaxParticleParams
+    {
+        const float* pos; // n x 3
+        const float* attr; // n x dim
+        int32_t dim;
+        int64_t n;
+        int32_t grid[3];
+        float cw[3]; // cell width = local / grid (float), as in the cell lists
+        ChainDesc chain;
+    };
+
+    // the workgroup's (256 threads) combined partial, valid in thread 0
+    __device__ inline MinMaxPartial mm_block_reduce(MinMaxPartial a)
+    {
+        for(int off = 32; off > 0; off >>= 1)
+        {
+            const float v0 = __shfl_xor(a.vmin, off), v1 = __shfl_xor(a.vmax, off);
+            const uint64_t k0 = __shfl_xor(a.kmin, off), k1 = __shfl_xor(a.kmax, off);
+            mm_take_min(a.vmin, a.kmin, v0, k0);
+            mm_take_max(a.vmax, a.kmax, v1, k1);
+        }
+        __shared__ MinMaxPartial waves[4];
+        if((threadIdx.x & 63) == 0)
+            waves[threadIdx.x >> 6] = a;
+        __syncthreads();
+        if(threadIdx.x == 0)
+            for(int w = 1; w < 4; w++)
+            {
+                mm_take_min(a.vmin, a.kmin, waves[w].vmin, waves[w].kmin);
+                mm_take_max(a.vmax, a.kmax, waves[w].vmax, waves[w].kmax);
+            }
+        return a;
+    }
+
+    // partials: kMinMaxGroups entries of workspace; out: (min, max) of the source
+    hipError_t launch_min_max_volume(const MinMaxVolumeParams& P, MinMaxPartial* partials, float2* out, hipStream_t s);
+    hipError_t launch_min_max_particles(const MinMaxParticleParams& P, MinMaxPartial* partials, float2* out, hipStream_t s);
+    // reduces `count` partials into out (one workgroup)
+    hipError_t launch_min_max_final(const MinMaxPartial* partials, int count, float2* out, hipStream_t s);
+
     size_t bin_scan_workspace_elems(int64_t ncells);
     hipError_t launch_bin(const BinParams& P, const BinBuffers& B, hipStream_t s);
     hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s);

@@ -101,6 +101,43 @@ namespace isaac_hip
         }
     }
 
+    // ---- min/max request over particles (MinMaxParticleKernel,
+    // lib/isaac/isaac_min_max_kernel.hpp:60-97): the reference visits cell columns
+    // (x + y*gx) in the host loop's order, z inside a column, and a cell's particles
+    // in list order, which for these cell lists is ascending particle index: the key
+    // is ((x + y*gx) * gz + z) * 2^32 + index + 1.
+    __global__ __launch_bounds__(256) void min_max_particle_kernel(const MinMaxParticleParams P, MinMaxPartial* partials)
+    {
+        MinMaxPartial a{FLT_MAX, -FLT_MAX, 0, 0};
+        for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < P.n; i += int64_t(gridDim.x) * blockDim.x)
+        {
+            float q;
+            const int cx = cell_coord(P.pos[3 * i + 0], P.cw[0], P.grid[0], q);
+            const int cy = cell_coord(P.pos[3 * i + 1], P.cw[1], P.grid[1], q);
+            const int cz = cell_coord(P.pos[3 * i + 2], P.cw[2], P.grid[2], q);
+            const float v = apply_chain_p(P.chain, P.attr + i * P.dim, P.dim);
+            const uint64_t cell = (uint64_t(cx) + uint64_t(cy) * uint64_t(P.grid[0])) * uint64_t(P.grid[2]) + uint64_t(cz);
+            const uint64_t key = (cell << 32) + uint64_t(i) + 1;
+            mm_take_min(a.vmin, a.kmin, v, key);
+            mm_take_max(a.vmax, a.kmax, v, key);
+        }
+        a = mm_block_reduce(a);
+        if(threadIdx.x == 0)
+            partials[blockIdx.x] = a;
+    }
+
+    hipError_t launch_min_max_particles(const MinMaxParticleParams& P, MinMaxPartial* partials, float2* out, hipStream_t s)
+    {
+        if(P.n < 0 || P.n >= (int64_t(1) << 32) - 1)
+            return hipErrorInvalidValue;
+        if(P.n == 0)
+            return launch_min_max_final(partials, 0, out, s);
+        const int groups = int(std::min<int64_t>((P.n + 255) / 256, kMinMaxGroups));
+        hipLaunchKernelGGL(min_max_particle_kernel, dim3(groups), dim3(256), 0, s, P, partials);
+        hipError_t e = hipGetLastError();
+        return e != hipSuccess ? e : launch_min_max_final(partials, groups, out, s);
+    }
+
     // ---- exclusive scan of packed 64-bit counts (3-phase, 1024-wide blocks)
     constexpr int kScanBlock = 1024;
 

@@ -1659,6 +1659,63 @@ namespace isaac_hip
         return hipGetLastError();
     }
 
+    // ------------------------------------------------------------------ min/max request
+    // MinMaxKernel (lib/isaac/isaac_min_max_kernel.hpp:22-57) as a grid-stride
+    // reduction over the voxels; the visiting-order key of voxel (x, y, z) is
+    // (x + y*X) * Z + z + 1 (columns in the host loop's order, z inside a column).
+    __global__ __launch_bounds__(256) void min_max_volume_kernel(const MinMaxVolumeParams P, MinMaxPartial* partials)
+    {
+        MinMaxPartial a{FLT_MAX, -FLT_MAX, 0, 0};
+        const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+        for(int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < P.nvox; idx += stride)
+        {
+            const uint32_t u = (uint32_t) idx;
+            const uint32_t zq = fastdiv(u, P.divXY_mul, P.divXY_shift);
+            const uint32_t rem = u - zq * uint32_t(P.X * P.Y);
+            const uint32_t yq = fastdiv(rem, P.divX_mul, P.divX_shift);
+            const int x = int(rem - yq * uint32_t(P.X)), y = int(yq), z = int(zq);
+            const float* e = P.data + (int64_t(x) * P.stride[0] + int64_t(y) * P.stride[1] + int64_t(z) * P.stride[2]) * P.dim;
+            float v = e[0];
+            if(P.applyChain)
+                v = apply_chain(P.chain, e[0], P.dim > 1 ? e[1] : 0.f, P.dim > 2 ? e[2] : 0.f, P.dim > 3 ? e[3] : 0.f, P.dim);
+            const uint64_t key = (uint64_t(x) + uint64_t(y) * uint64_t(P.X)) * uint64_t(P.Z) + uint64_t(z) + 1;
+            mm_take_min(a.vmin, a.kmin, v, key);
+            mm_take_max(a.vmax, a.kmax, v, key);
+        }
+        a = mm_block_reduce(a);
+        if(threadIdx.x == 0)
+            partials[blockIdx.x] = a;
+    }
+
+    __global__ __launch_bounds__(256) void min_max_final_kernel(const MinMaxPartial* partials, int count, float2* out)
+    {
+        MinMaxPartial a{FLT_MAX, -FLT_MAX, 0, 0};
+        for(int i = threadIdx.x; i < count; i += blockDim.x)
+        {
+            mm_take_min(a.vmin, a.kmin, partials[i].vmin, partials[i].kmin);
+            mm_take_max(a.vmax, a.kmax, partials[i].vmax, partials[i].kmax);
+        }
+        a = mm_block_reduce(a);
+        if(threadIdx.x == 0)
+            *out = make_float2(a.vmin, a.vmax);
+    }
+
+    hipError_t launch_min_max_final(const MinMaxPartial* partials, int count, float2* out, hipStream_t s)
+    {
+        hipLaunchKernelGGL(min_max_final_kernel, dim3(1), dim3(256), 0, s, partials, count, out);
+        return hipGetLastError();
+    }
+
+    hipError_t launch_min_max_volume(const MinMaxVolumeParams& P, MinMaxPartial* partials, float2* out, hipStream_t s)
+    {
+        if(P.nvox <= 0 || P.nvox >= (int64_t(1) << 32))
+            return hipErrorInvalidValue;
+        const int groups = int(std::min<int64_t>((P.nvox + 255) / 256, kMinMaxGroups));
+        hipLaunchKernelGGL(min_max_volume_kernel, dim3(groups), dim3(256), 0, s, P, partials);
+        hipError_t e = hipGetLastError();
+        return e != hipSuccess ? e : launch_min_max_final(partials, groups, out, s);
+    }
+
     hipError_t launch_persistent_copy(
         const float* src,
         const int64_t stride[3],

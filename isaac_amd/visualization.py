@@ -378,6 +378,16 @@ class IsaacVisualization:
     def synchronize(self) -> None:
         check(lib().isaac_hip_synchronize(self._ctx), self._ctx, "synchronize")
 
+    def min_max(self) -> Tuple[np.ndarray, np.ndarray]:
+        """The min/max request (sendMinMax, lib/isaac.hpp:1951-2006): per source in the
+        combined order, min and max of its functor-chain values (float32 arrays).
+        Non-persistent sources report the values of the last update_buffers."""
+        n = len(self.volume_sources) + len(self.field_sources) + len(self.particle_sources)
+        mins = np.zeros(max(n, 1), np.float32)
+        maxs = np.zeros(max(n, 1), np.float32)
+        check(lib().isaac_hip_min_max(self._ctx, mins.ctypes.data, maxs.ctypes.data, max(n, 1)), self._ctx, "min_max")
+        return mins[:n], maxs[:n]
+
     def timings(self) -> Dict[str, float]:
         t = _lib.Timings()
         check(lib().isaac_hip_get_timings(self._ctx, ctypes.byref(t)), self._ctx, "get_timings")

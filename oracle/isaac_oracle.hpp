@@ -753,6 +753,92 @@ namespace isaac_oracle
         return cl;
     }
 
+    // --------------------------------------------------------------- min/max request
+    // glm 0.9.9 scalar min/max: min(x, y) = (y < x) ? y : x, max(x, y) = (x < y) ? y : x
+    // (a NaN y never replaces x; on ties the running value stays)
+    inline float glm_min(float x, float y)
+    {
+        return (y < x) ? y : x;
+    }
+
+    inline float glm_max(float x, float y)
+    {
+        return (x < y) ? y : x;
+    }
+
+    // MinMaxKernel (lib/isaac/isaac_min_max_kernel.hpp:22-57): one thread per (x, y)
+    // column walks z over the functor-chain values; the host then reduces the columns
+    // in order x + y*X with strict comparisons (CalcMinMaxIterator,
+    // lib/isaac.hpp:372-432).  Non-persistent sources read their texture, which holds
+    // the same chain values.
+    inline void min_max_volume(const VolumeSource& vs, const Chain& chain, const Sizes& sz, float& outMin, float& outMax)
+    {
+        const int64_t X = sz.localSize[0], Y = sz.localSize[1], Z = sz.localSize[2];
+        std::vector<float> colMin(size_t(X * Y)), colMax(size_t(X * Y));
+        for(int64_t y = 0; y < Y; y++)
+            for(int64_t x = 0; x < X; x++)
+            {
+                float mn = std::numeric_limits<float>::max();
+                float mx = -std::numeric_limits<float>::max();
+                for(int64_t z = 0; z < Z; z++)
+                {
+                    const float* e = vs.data + (x * vs.stride[0] + y * vs.stride[1] + z * vs.stride[2]) * vs.dim;
+                    const float v = apply_chain<float>(chain, e, vs.dim);
+                    mn = glm_min(mn, v);
+                    mx = glm_max(mx, v);
+                }
+                colMin[size_t(x + y * X)] = mn;
+                colMax[size_t(x + y * X)] = mx;
+            }
+        outMin = std::numeric_limits<float>::max();
+        outMax = -std::numeric_limits<float>::max();
+        for(size_t i = 0; i < colMin.size(); i++)
+        {
+            if(colMin[i] < outMin)
+                outMin = colMin[i];
+            if(colMax[i] > outMax)
+                outMax = colMax[i];
+        }
+    }
+
+    // MinMaxParticleKernel (lib/isaac/isaac_min_max_kernel.hpp:60-97): per (x, y)
+    // column of the particle grid, z ascending, the particles of each cell in list
+    // order (here: ascending particle index); host reduction lib/isaac.hpp:434-497.
+    inline void min_max_particles(const ParticleSource& ps, const Chain& chain, const Sizes& sz, float& outMin, float& outMax)
+    {
+        const CellList cl = bin_particles(ps, sz);
+        const int64_t gx = sz.localParticleSize[0], gy = sz.localParticleSize[1], gz = sz.localParticleSize[2];
+        std::vector<float> colMin(size_t(gx * gy)), colMax(size_t(gx * gy));
+        for(int64_t y = 0; y < gy; y++)
+            for(int64_t x = 0; x < gx; x++)
+            {
+                float mn = std::numeric_limits<float>::max();
+                float mx = -std::numeric_limits<float>::max();
+                for(int64_t z = 0; z < gz; z++)
+                {
+                    const int64_t c = x + y * gx + z * gx * gy;
+                    for(int64_t k = cl.offset[size_t(c)]; k < cl.offset[size_t(c) + 1]; k++)
+                    {
+                        const int64_t i = cl.perm[size_t(k)];
+                        const float v = apply_chain<float>(chain, ps.attr + i * ps.dim, ps.dim);
+                        mn = glm_min(mn, v);
+                        mx = glm_max(mx, v);
+                    }
+                }
+                colMin[size_t(x + y * gx)] = mn;
+                colMax[size_t(x + y * gx)] = mx;
+            }
+        outMin = std::numeric_limits<float>::max();
+        outMax = -std::numeric_limits<float>::max();
+        for(size_t i = 0; i < colMin.size(); i++)
+        {
+            if(colMin[i] < outMin)
+                outMin = colMin[i];
+            if(colMax[i] > outMax)
+                outMax = colMax[i];
+        }
+    }
+
     // --------------------------------------------------------------- G-buffer
     template<class R>
     struct GBuffer

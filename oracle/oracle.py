@@ -405,6 +405,19 @@ def render(spec, precision: int = 32, threads: int = 0, want_textures: bool = Fa
     return res
 
 
+def min_max(spec, prepared: Optional[PreparedScene] = None) -> Tuple[np.ndarray, np.ndarray]:
+    """The min/max request: per source (volume sources first, then particle sources)
+    the min and max of its functor-chain values, float32 (oracle_min_max)."""
+    ps = prepared or PreparedScene(spec)
+    n = int(ps.sc.n_volume) + int(ps.sc.n_particle)
+    mins = np.zeros(max(n, 1), np.float32)
+    maxs = np.zeros(max(n, 1), np.float32)
+    rc = lib().oracle_min_max(ctypes.byref(ps.sc), mins.ctypes.data_as(ctypes.c_void_p), maxs.ctypes.data_as(ctypes.c_void_p))
+    if rc != 0:
+        raise RuntimeError(f"oracle min_max failed rc={rc}")
+    return mins[:n], maxs[:n]
+
+
 BRICK_RAW, BRICK_F32, BRICK_F16, BRICK_BF16 = -1, 0, 1, 2
 
 

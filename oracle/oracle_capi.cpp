@@ -341,6 +341,19 @@ extern "C"
         return 0;
     }
 
+    // min/max request: per source, volume sources first (min_max_volume /
+    // min_max_particles); mins/maxs hold n_volume + n_particle floats
+    int oracle_min_max(const OracleScene* sc, float* mins, float* maxs)
+    {
+        State st = make_state(sc);
+        const int nv = sc->n_volume;
+        for(int s = 0; s < nv; s++)
+            min_max_volume(st.vol[s], st.chain[s], st.sizes, mins[s], maxs[s]);
+        for(int s = 0; s < sc->n_particle; s++)
+            min_max_particles(st.part[s], st.chain[nv + s], st.sizes, mins[nv + s], maxs[nv + s]);
+        return 0;
+    }
+
     // Stage-level entry points used by known-answer tests.
     void oracle_inverse_f32(const float* m, float* out)
     {
