@@ -154,7 +154,9 @@ extern "C"
     int isaac_hip_set_iso_threshold(isaac_hip_ctx* ctx, int idx, float threshold);
     int isaac_hip_set_step(isaac_hip_ctx* ctx, float step); /* clamped to >= 0.01 */
     int isaac_hip_set_interpolation(isaac_hip_ctx* ctx, int trilinear);
-    int isaac_hip_set_dither_mode(isaac_hip_ctx* ctx, int mode); /* 0 round, 1 ordered 4^3 */
+    /* applyDither3D (lib/isaac/isaac_dither_kernel.hpp:7-328) of the classification:
+     * 0 round, 1 ordered 4^3, 2 blue noise 4^3, 3 blue noise 8^3, others blue noise 16^3 */
+    int isaac_hip_set_dither_mode(isaac_hip_ctx* ctx, int mode);
     int isaac_hip_set_ao(isaac_hip_ctx* ctx, int enabled, float weight);
     int isaac_hip_set_render_mode(isaac_hip_ctx* ctx, int mode); /* 0..8 */
     /* clipping planes (lib/isaac.hpp:958-1017): add/edit return 1 on success, 0 as the reference's false */

@@ -99,6 +99,20 @@ namespace isaac
         bool isEnabled = false;
     };
 
+    // MinMax / MinMaxArray (lib/isaac/isaac_types.hpp:223-234)
+    struct MinMax
+    {
+        isaac_float min;
+        isaac_float max;
+    };
+
+    template<int T_n>
+    struct MinMaxArray
+    {
+        isaac_float min[T_n > 0 ? T_n : 1];
+        isaac_float max[T_n > 0 ? T_n : 1];
+    };
+
     // DefaultController (lib/isaac/isaac_controllers.hpp:23-43)
     struct DefaultController
     {
@@ -301,6 +315,14 @@ namespace isaac
                 bindSources(pointer);
                 check(isaac_hip_update_buffers(ctx), "update_buffers");
             }
+            // the min/max request (lib/isaac.hpp:1951-2006, one rank)
+            if(sendMinMax)
+            {
+                check(
+                    isaac_hip_min_max(ctx, minMaxArray.min, minMaxArray.max, combinedSourceListSize > 0 ? combinedSourceListSize : 1),
+                    "min_max");
+                sendMinMax = false;
+            }
             const int32_t vp[4] = {0, 0, int32_t(framebufferSize.x), int32_t(framebufferSize.y)};
             check(isaac_hip_render(ctx, projection, modelview, backgroundColor, vp, frame.data(), 0), "render");
             isaac_hip_timings t{};
@@ -328,6 +350,10 @@ namespace isaac
         isaac_float sourceWeight[combinedSourceListSize > 0 ? combinedSourceListSize : 1];
         isaac_float isoThreshold[combinedSourceListSize > 0 ? combinedSourceListSize : 1];
         isaac_float step = 0.5f;
+        // set to request min/max of every source with the next doVisualization; the
+        // result lands in minMaxArray (combined source order)
+        bool sendMinMax = false;
+        MinMaxArray<combinedSourceListSize> minMaxArray;
         bool interpolation = false;
         isaac_int ditherMode = 1;
         isaac_int renderMode = 0;

@@ -964,8 +964,7 @@ extern "C"
     {
         if(!c)
             return ISAAC_HIP_ERR_ARG;
-        if(mode != 0 && mode != 1)
-            return fail(c, ISAAC_HIP_ERR_UNSUPPORTED, "dither modes 0 (round) and 1 (ordered 4^3) are implemented");
+        // every value is valid: as in applyDither3D, modes other than 0-3 use the 16^3 mask
         c->ditherMode = mode;
         return 0;
     }

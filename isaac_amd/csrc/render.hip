@@ -21,10 +21,9 @@
 
 namespace isaac_hip
 {
-    __constant__ uint8_t kDitherMask4[64]
-        = {17, 46, 55, 1,  57, 7,  38, 31, 9,  52, 2,  54, 36, 26, 19, 44, 53, 20, 47, 22, 29, 60,
-           62, 12, 45, 14, 26, 33, 8,  39, 11, 63, 13, 56, 6,  50, 40, 32, 23, 48, 21, 42, 51, 5,
-           61, 3,  34, 27, 41, 10, 30, 37, 4,  35, 15, 59, 49, 24, 43, 18, 25, 64, 58, 16};
+#define DITHER_TABLE(type, name, n) __constant__ type name[n]
+#include "dither_tables.inc"
+#undef DITHER_TABLE
 
     // ------------------------------------------------------------------ functor chain
     // f2(f1(f0(v,p0),p1),p2).x with the feature dimension propagating
@@ -104,7 +103,17 @@ namespace isaac_hip
             uint32_t q3 = (uint32_t) roundf(gclamp(a, 0.0f, 1.0f) * 255.0f);
             return q0 | (q1 << 8) | (q2 << 16) | (q3 << 24);
         }
-        const float d = float(kDitherMask4[(x & 3) + (y & 3) * 4 + (z & 3) * 16]) / 64.0f;
+        // threshold of the mode's mask (applyDither3D, lib/isaac/isaac_dither_kernel.hpp:20-323):
+        // 1 ordered 4^3, 2 blue noise 4^3, 3 blue noise 8^3, any other mode blue noise 16^3
+        float d;
+        if(mode == 1)
+            d = float(kDitherOrdered4[(x & 3) + (y & 3) * 4 + (z & 3) * 16]) / 64.0f;
+        else if(mode == 2)
+            d = float(kDitherBlue4[(x & 3) + (y & 3) * 4 + (z & 3) * 16]) / 64.0f;
+        else if(mode == 3)
+            d = float(kDitherBlue8[(x & 7) + (y & 7) * 8 + (z & 7) * 64]) / 512.0f;
+        else
+            d = float(kDitherBlue16[(x & 15) + (y & 15) * 16 + (z & 15) * 256]) / 4096.0f;
         auto q = [d](float v) -> uint32_t
         {
             float s = v * 255.0f;
@@ -1023,32 +1032,6 @@ namespace isaac_hip
         return __half2float(__ushort_as_half((unsigned short) (v >> 16)));
     }
 
-    // pair (v[x], v[x+1]) of a brick at pair index idx, still divided by the brick scale
-    template<int DT>
-    __device__ inline void load_pair(const void* data, uint32_t idx, float& a, float& b)
-    {
-        if(DT == kDtF32)
-        {
-            const float2 v = reinterpret_cast<const float2*>(data)[idx];
-            a = v.x;
-            b = v.y;
-        }
-        else
-        {
-            const uint32_t v = reinterpret_cast<const uint32_t*>(data)[idx];
-            if(DT == kDtF16)
-            {
-                a = half_lo(v);
-                b = half_hi(v);
-            }
-            else
-            {
-                a = __uint_as_float(v << 16);
-                b = __uint_as_float(v & 0xffff0000u);
-            }
-        }
-    }
-
     // Tap geometry of one sample position, shared by every source (all sources
     // of a rank have the local box's brick grid).  Sampler LINEAR|NEAREST with
     // CLAMP addressing (lib/isaac/isaac_texture.hpp:182-259): the taps of a sample
@@ -1098,45 +1081,98 @@ namespace isaac_hip
         return t;
     }
 
-    // raw sample of every component of one source (v[c] for c < S.dim).  Pair rows
-    // hold the components of a source side by side, so the row loads of all
-    // components are adjacent and the compiler merges them into wide loads.
-    template<int DT, bool LINEAR>
+    // the DIM pairs of one pair row (all components of a source side by side):
+    // lo[c] = v_c[x], hi[c] = v_c[x+1], still divided by the brick scale.  One
+    // struct load per row, which the compiler emits as one wide global load.
+    template<int N>
+    struct Words
+    {
+        uint32_t w[N];
+    };
+
+    template<int DT, int DIM>
+    __device__ inline void load_row(const void* data, uint32_t row, float lo[DIM], float hi[DIM])
+    {
+        if constexpr(DT == kDtF32)
+        {
+            const Words<2 * DIM> r = reinterpret_cast<const Words<2 * DIM>*>(data)[row];
+#pragma unroll
+            for(int c = 0; c < DIM; c++)
+            {
+                lo[c] = __uint_as_float(r.w[2 * c]);
+                hi[c] = __uint_as_float(r.w[2 * c + 1]);
+            }
+        }
+        else
+        {
+            const Words<DIM> r = reinterpret_cast<const Words<DIM>*>(data)[row];
+#pragma unroll
+            for(int c = 0; c < DIM; c++)
+            {
+                if constexpr(DT == kDtF16)
+                {
+                    lo[c] = half_lo(r.w[c]);
+                    hi[c] = half_hi(r.w[c]);
+                }
+                else
+                {
+                    lo[c] = __uint_as_float(r.w[c] << 16);
+                    hi[c] = __uint_as_float(r.w[c] & 0xffff0000u);
+                }
+            }
+        }
+    }
+
+    // raw sample of every component of one source with DIM components (v[c], c < DIM)
+    template<int DT, bool LINEAR, int DIM>
     __device__ inline void brick_sample(const BrickSource& S, const BrickTaps& t, float v[4])
     {
-        const uint32_t dim = uint32_t(S.dim);
         const uint32_t row0 = t.brick * uint32_t(kBrickPairs);
-        const float oz = 1.0f - t.fz, oy = 1.0f - t.fy, ox = 1.0f - t.fx;
-        const float* sc = S.scale + t.brick * dim;
-#pragma unroll
-        for(int c = 0; c < 4; c++)
+        const Words<DIM> scw = reinterpret_cast<const Words<DIM>*>(S.scale)[t.brick];
+        if(!LINEAR)
         {
-            if(c >= S.dim)
-                break;
-            if(!LINEAR)
-            {
-                float a, b;
-                load_pair<DT>(S.data, (row0 + t.i00) * dim + c, a, b);
-                v[c] = a * sc[c];
-                continue;
-            }
-            float d000, d100, d010, d110, d001, d101, d011, d111;
-            load_pair<DT>(S.data, (row0 + t.i00) * dim + c, d000, d100);
-            load_pair<DT>(S.data, (row0 + t.i10) * dim + c, d010, d110);
-            load_pair<DT>(S.data, (row0 + t.i01) * dim + c, d001, d101);
-            load_pair<DT>(S.data, (row0 + t.i11) * dim + c, d011, d111);
+            float a[DIM], b[DIM];
+            load_row<DT, DIM>(S.data, row0 + t.i00, a, b);
+#pragma unroll
+            for(int c = 0; c < DIM; c++)
+                v[c] = a[c] * __uint_as_float(scw.w[c]);
+            return;
+        }
+        float d000[DIM], d100[DIM], d010[DIM], d110[DIM], d001[DIM], d101[DIM], d011[DIM], d111[DIM];
+        load_row<DT, DIM>(S.data, row0 + t.i00, d000, d100);
+        load_row<DT, DIM>(S.data, row0 + t.i10, d010, d110);
+        load_row<DT, DIM>(S.data, row0 + t.i01, d001, d101);
+        load_row<DT, DIM>(S.data, row0 + t.i11, d011, d111);
+        const float oz = 1.0f - t.fz, oy = 1.0f - t.fy, ox = 1.0f - t.fx;
+#pragma unroll
+        for(int c = 0; c < DIM; c++)
+        {
             if(t.dup)
             {
-                d100 = d000;
-                d110 = d010;
-                d101 = d001;
-                d111 = d011;
+                d100[c] = d000[c];
+                d110[c] = d010[c];
+                d101[c] = d001[c];
+                d111[c] = d011[c];
             }
-            const float a0 = d000 * oz + d001 * t.fz, a1 = d010 * oz + d011 * t.fz;
+            const float a0 = d000[c] * oz + d001[c] * t.fz, a1 = d010[c] * oz + d011[c] * t.fz;
             const float A = a0 * oy + a1 * t.fy;
-            const float b0 = d100 * oz + d101 * t.fz, b1 = d110 * oz + d111 * t.fz;
+            const float b0 = d100[c] * oz + d101[c] * t.fz, b1 = d110[c] * oz + d111[c] * t.fz;
             const float B = b0 * oy + b1 * t.fy;
-            v[c] = (A * ox + B * t.fx) * sc[c];
+            v[c] = (A * ox + B * t.fx) * __uint_as_float(scw.w[c]);
+        }
+    }
+
+    // the functor-chain value of one source's sample, dispatched on its component count
+    template<int DT, bool LINEAR>
+    __device__ __attribute__((always_inline)) inline float brick_value(const BrickSource& S, const BrickTaps& t)
+    {
+        float v[4] = {0.f, 0.f, 0.f, 0.f};
+        switch(S.dim)
+        {
+        case 1: brick_sample<DT, LINEAR, 1>(S, t, v); return apply_chain(S.chain, v[0], 0.f, 0.f, 0.f, 1);
+        case 2: brick_sample<DT, LINEAR, 2>(S, t, v); return apply_chain(S.chain, v[0], v[1], 0.f, 0.f, 2);
+        case 3: brick_sample<DT, LINEAR, 3>(S, t, v); return apply_chain(S.chain, v[0], v[1], v[2], 0.f, 3);
+        default: brick_sample<DT, LINEAR, 4>(S, t, v); return apply_chain(S.chain, v[0], v[1], v[2], v[3], 4);
         }
     }
 
@@ -1158,9 +1194,7 @@ namespace isaac_hip
             if(s >= Q.nsrc)
                 break;
             const BrickSource& S = Q.src[s];
-            float v[4] = {0.f, 0.f, 0.f, 0.f};
-            brick_sample<DT, LINEAR>(S, t, v);
-            const float tv = apply_chain(S.chain, v[0], v[1], v[2], v[3], S.dim);
+            const float tv = brick_value<DT, LINEAR>(S, t);
             const float4 col = lut[s * Q.T + tf_index(tv, Q.T)];
             if(S.weight > 0.0f)
             {

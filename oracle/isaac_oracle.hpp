@@ -443,13 +443,14 @@ namespace isaac_oracle
         return {R(b.x) / R(255), R(b.y) / R(255), R(b.z) / R(255), R(b.w) / R(255)};
     }
 
-    // dither mode 1 mask (reference data, lib/isaac/isaac_dither_kernel.hpp:25-28, LGPL-3.0)
-    static const uint8_t kDitherMask4[64]
-        = {17, 46, 55, 1,  57, 7,  38, 31, 9,  52, 2,  54, 36, 26, 19, 44, 53, 20, 47, 22, 29, 60,
-           62, 12, 45, 14, 26, 33, 8,  39, 11, 63, 13, 56, 6,  50, 40, 32, 23, 48, 21, 42, 51, 5,
-           61, 3,  34, 27, 41, 10, 30, 37, 4,  35, 15, 59, 49, 24, 43, 18, 25, 64, 58, 16};
+    // dither masks (reference data, lib/isaac/isaac_dither_kernel.hpp:22-323, LGPL-3.0)
+#define DITHER_TABLE(type, name, n) static const type name[n]
+#include "dither_tables.inc"
+#undef DITHER_TABLE
 
-    // applyDither3D modes 0 and 1 (lib/isaac/isaac_dither_kernel.hpp:7-30,325-328)
+    // applyDither3D (lib/isaac/isaac_dither_kernel.hpp:7-328): mode 0 rounds; modes 1
+    // (ordered 4^3), 2 (blue noise 4^3), 3 (blue noise 8^3) and every other mode
+    // (blue noise 16^3) add one where the fraction exceeds the mask threshold
     template<class R>
     inline B4 dither(int mode, int x, int y, int z, V4<R> c)
     {
@@ -458,7 +459,15 @@ namespace isaac_oracle
             auto q = [](R v) { return (uint8_t) std::round(gclamp(v, R(0), R(1)) * R(255)); };
             return {q(c.x), q(c.y), q(c.z), q(c.w)};
         }
-        R d = R(kDitherMask4[x % 4 + (y % 4) * 4 + (z % 4) * 16]) / R(64);
+        R d;
+        if(mode == 1)
+            d = R(kDitherOrdered4[x % 4 + (y % 4) * 4 + (z % 4) * 16]) / R(64);
+        else if(mode == 2)
+            d = R(kDitherBlue4[x % 4 + (y % 4) * 4 + (z % 4) * 16]) / R(64);
+        else if(mode == 3)
+            d = R(kDitherBlue8[x % 8 + (y % 8) * 8 + (z % 8) * 64]) / R(512);
+        else
+            d = R(kDitherBlue16[x % 16 + (y % 16) * 16 + (z % 16) * 256]) / R(4096);
         auto q = [d](R v)
         {
             R s = v * R(255);

@@ -84,11 +84,13 @@ int main(int argc, char** argv)
         vis.step = 0.5f;
         for(float& c : vis.backgroundColor)
             c = 0.0f;
+        vis.sendMinMax = true;
         vis.doVisualization(isaac::META_MASTER, nullptr, true);
         std::ofstream out(argv[5], std::ios::binary);
         out.write(reinterpret_cast<const char*>(vis.image().data()), std::streamsize(vis.image().size()));
         std::printf("shim_example: rendered %lldx%lld, kernel %llu us\n", (long long) W, (long long) H,
                     (unsigned long long) vis.kernelTime);
+        std::printf("minmax %.9g %.9g\n", double(vis.minMaxArray.min[0]), double(vis.minMaxArray.max[0]));
     }
     (void) hipFree(src.ptr);
     return 0;

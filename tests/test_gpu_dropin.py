@@ -33,3 +33,7 @@ def test_cpp_dropin_example_matches_oracle(tmp_path, oracle_lib, hip_lib):
     d = np.abs(img.astype(int) - ora.rgba.astype(int)).max(axis=2)
     print(f"c++ drop-in: {(d <= 2).mean() * 100:.4f}% within 2/255, max {d.max()}")
     assert (d <= 2).mean() >= 0.999
+    # the min/max request through the drop-in (sendMinMax -> minMaxArray)
+    line = [ln for ln in r.stdout.splitlines() if ln.startswith("minmax ")][0].split()
+    mins, maxs = oracle_lib.min_max(spec)
+    assert np.float32(line[1]) == mins[0] and np.float32(line[2]) == maxs[0], (line, mins, maxs)

@@ -132,6 +132,21 @@ def test_render_modes(gpu, oracle_lib, mode):
     compare(f"mode{mode}", spec, img, ora)
 
 
+@pytest.mark.parametrize("mode", [2, 3, 4, 5])
+def test_dither_modes(gpu, oracle_lib, mode):
+    """Blue-noise dither masks 4^3, 8^3, 16^3 (and a mode past 4, which uses the
+    16^3 mask): combined textures bit-exact, frame within the pixel bar."""
+    from oracle import oracle
+
+    spec = dataclasses.replace(_reduced_prod(n=40, fb=(160, 120), particles=2000), dither_mode=mode)
+    vis, img = gpu_frame(spec, gpu)
+    ora = oracle.render(spec, 32, want_textures=True)
+    vol, iso = vis.combined_textures()
+    assert np.array_equal(vol, ora.vol_tex), "classification volume texture differs"
+    assert np.array_equal(iso, ora.iso_tex), "classification iso texture differs"
+    compare(f"dither{mode}", spec, img, ora)
+
+
 @pytest.mark.parametrize(
     "label,changes",
     [

@@ -136,19 +136,51 @@ def test_dither_mode0_rounds_half_away(oracle_lib):
     assert _dither(oracle_lib, 0, 0, 0, 0, c) == [1, 255, 0, 64]
 
 
-def test_dither_mode1_against_fixture_table(oracle_lib):
-    mask = json.load(open(os.path.join(GOLDEN, "dither_mask4.json")))["mask"]
-    for z in range(4):
-        for y in range(4):
-            for x in range(4):
-                m = mask[x + 4 * y + 16 * z]
-                below = (10.0 + m / 64.0 - 1 / 128) / 255.0
-                got = _dither(oracle_lib, 1, x + 4, y + 8, z, (below, below, below, below))
-                assert got == [10] * 4, (x, y, z, m, got)
-                above = (10.0 + m / 64.0 + 1 / 128) / 255.0
-                got = _dither(oracle_lib, 1, x, y, z + 4, (above, 0.0, 1.0, 2.0))
-                assert got[0] == (11 if m < 64 else 11), (x, y, z, m, got)
-                assert got[1:] == [0, 255, 255]
+def _dither_tables():
+    return json.load(open(os.path.join(GOLDEN, "dither_tables.json")))["tables"]
+
+
+def test_dither_tables_are_intact():
+    """The committed masks hash to what extraction recorded; the blue-noise masks
+    (modes 2-4) are permutations of 0..n-1, so a mistyped entry shows up as a
+    duplicate; the generated include files carry the same numbers."""
+    import hashlib
+    import re
+
+    tables = _dither_tables()
+    for mode, t in tables.items():
+        vals = t["mask"]
+        assert len(vals) == t["side"] ** 3
+        assert hashlib.sha256(",".join(map(str, vals)).encode()).hexdigest() == t["sha256"], mode
+        if mode != "1":
+            assert sorted(vals) == list(range(len(vals))), mode
+    assert sorted(tables["1"]["mask"])[0] == 1 and max(tables["1"]["mask"]) == 64
+    root = os.path.dirname(os.path.dirname(GOLDEN))
+    for inc in ("isaac_amd/csrc/dither_tables.inc", "oracle/dither_tables.inc"):
+        txt = open(os.path.join(root, inc)).read()
+        for mode, t in tables.items():
+            body = re.search(r"DITHER_TABLE\(\w+, " + t["name"] + r", \d+\) = \{([^}]*)\}", txt).group(1)
+            assert [int(v) for v in body.replace("\n", " ").split(",") if v.strip()] == t["mask"], (inc, mode)
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3, 4, 7])
+def test_dither_modes_against_fixture_tables(oracle_lib, mode):
+    """Each texel rounds down just below its mask threshold and up just above it;
+    the mask index is x % s + (y % s) * s + (z % s) * s^2 (s = 4, 4, 8, 16; modes
+    other than 0-3 use the 16^3 mask)."""
+    t = _dither_tables()[str(min(mode, 4))]
+    side, div, mask = t["side"], t["divisor"], t["mask"]
+    rng = np.random.default_rng(mode)
+    for _ in range(300):
+        x, y, z = (int(v) for v in rng.integers(0, 3 * side, size=3))
+        m = mask[x % side + (y % side) * side + (z % side) * side * side]
+        below = (10.0 + m / div - 0.25 / div) / 255.0
+        got = _dither(oracle_lib, mode, x, y, z, (below, below, below, below))
+        assert got == [10] * 4, (x, y, z, m, got)
+        above = (10.0 + m / div + 0.25 / div) / 255.0
+        got = _dither(oracle_lib, mode, x, y, z, (above, 0.0, 1.0, 2.0))
+        assert got[0] == 11, (x, y, z, m, got)
+        assert got[1:] == [0, 255, 255]
 
 
 # --------------------------------------------------------------------------- sampler
