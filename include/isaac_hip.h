@@ -82,7 +82,8 @@ extern "C"
         float scale[3]; /* scale */
         int32_t transfer_size; /* T_transferSize */
         int32_t n_volume_sources; /* size of T_VolumeSourceList */
-        int32_t n_field_sources; /* size of T_FieldSourceList (rendered as volume sources, no advection) */
+        int32_t n_field_sources; /* size of T_FieldSourceList: float3 vector fields whose alpha follows the
+                                    advection texture (combined form) */
         int32_t n_particle_sources; /* size of T_ParticleList */
     } isaac_hip_config;
 
@@ -154,6 +155,22 @@ extern "C"
     int isaac_hip_set_iso_threshold(isaac_hip_ctx* ctx, int idx, float threshold);
     int isaac_hip_set_step(isaac_hip_ctx* ctx, float step); /* clamped to >= 0.01 */
     int isaac_hip_set_interpolation(isaac_hip_ctx* ctx, int trilinear);
+    /* Field-source advection (the JSON keys "advection step", "advection weight",
+     * "advection seeding period", "advection seeding duration", lib/isaac.hpp:1643-1661;
+     * defaults 1, 0.95, 1, 1).  Each isaac_hip_update_buffers swaps every field source's
+     * advection texture with its history, advances the time step and, per enabled field
+     * source, writes GenerateAdvectionTextureKernel (lib/isaac/isaac_common_kernel.hpp:500-550):
+     * history sampled at voxel centre - unit field direction * step_factor * 3 / scale,
+     * noise where time_step % seeding_period < seeding_duration, dithered to a byte.
+     * The classification multiplies the field source's TF alpha by texture / 255
+     * (lib/isaac/isaac_common_kernel.hpp:655-656).  Field sources with weight or iso
+     * are rejected by the brick form. */
+    int isaac_hip_set_advection(isaac_hip_ctx* ctx, float step_factor, float history_weight, int32_t seeding_period,
+                                int32_t seeding_duration);
+    /* "seed points" (lib/isaac.hpp:1636-1642): Halton points of the noise texture
+     * (updateNoiseTexture, lib/isaac.hpp:499-562), clamped to >= 0, default 1000;
+     * the noise is rebuilt at the next buffer update when the count changes */
+    int isaac_hip_set_seed_points(isaac_hip_ctx* ctx, int32_t seed_points);
     /* applyDither3D (lib/isaac/isaac_dither_kernel.hpp:7-328) of the classification:
      * 0 round, 1 ordered 4^3, 2 blue noise 4^3, 3 blue noise 8^3, others blue noise 16^3 */
     int isaac_hip_set_dither_mode(isaac_hip_ctx* ctx, int mode);
@@ -356,6 +373,15 @@ extern "C"
      * local voxel, x fastest; each texel is *texel_bytes (8) bytes: the combined volume
      * RGBA8 followed by the combined iso RGBA8 */
     int isaac_hip_get_combined_textures(const isaac_hip_ctx* ctx, const void** texels, int32_t* texel_bytes);
+    /* device pointers of field source field_idx's current advection texture (local
+     * voxels, x fastest, one byte each) and of the noise texture (floats), and the
+     * time step of the last buffer update */
+    int isaac_hip_get_advection(
+        const isaac_hip_ctx* ctx,
+        int field_idx,
+        const void** texture,
+        const void** noise,
+        int32_t* time_step);
     /* device pointers of a particle source's exact cell list: per-cell count
      * (int32) and exclusive offset (int32, ncells+1), and the stable
      * permutation of particle indices (int32) */

@@ -309,6 +309,7 @@ namespace isaac
         void doVisualization(IsaacVisualizationMetaEnum metaTargets = META_MASTER, void* pointer = nullptr, bool redraw = true)
         {
             (void) metaTargets;
+            redraw = redraw || advectionOnPause; // lib/isaac.hpp:1303
             pushState();
             if(redraw)
             {
@@ -350,6 +351,13 @@ namespace isaac
         isaac_float sourceWeight[combinedSourceListSize > 0 ? combinedSourceListSize : 1];
         isaac_float isoThreshold[combinedSourceListSize > 0 ? combinedSourceListSize : 1];
         isaac_float step = 0.5f;
+        // field-source advection steering (lib/isaac.hpp:610,639-643)
+        isaac_int seedPoints = 1000;
+        isaac_float advectionStepFactor = 1.0f;
+        isaac_float advectionHistoryWeight = 0.95f;
+        isaac_int advectionSeedingPeriod = 1;
+        isaac_int advectionSeedingTime = 1;
+        bool advectionOnPause = false;
         // set to request min/max of every source with the next doVisualization; the
         // result lands in minMaxArray (combined source order)
         bool sendMinMax = false;
@@ -391,6 +399,10 @@ namespace isaac
             check(isaac_hip_set_dither_mode(ctx, ditherMode), "set_dither_mode");
             check(isaac_hip_set_ao(ctx, ambientOcclusion.isEnabled ? 1 : 0, ambientOcclusion.weight), "set_ao");
             check(isaac_hip_set_render_mode(ctx, renderMode), "set_render_mode");
+            check(isaac_hip_set_advection(ctx, advectionStepFactor, advectionHistoryWeight, advectionSeedingPeriod,
+                                          advectionSeedingTime),
+                  "set_advection");
+            check(isaac_hip_set_seed_points(ctx, seedPoints), "set_seed_points");
         }
 
         template<typename T_Source>

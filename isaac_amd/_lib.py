@@ -111,6 +111,9 @@ _SIGS = {
     "isaac_hip_render": (ctypes.c_int, [P, P, P, P, P, P, ctypes.c_int]),
     "isaac_hip_synchronize": (ctypes.c_int, [P]),
     "isaac_hip_min_max": (ctypes.c_int, [P, P, P, ctypes.c_int32]),
+    "isaac_hip_set_advection": (ctypes.c_int, [P, ctypes.c_float, ctypes.c_float, ctypes.c_int32, ctypes.c_int32]),
+    "isaac_hip_set_seed_points": (ctypes.c_int, [P, ctypes.c_int32]),
+    "isaac_hip_get_advection": (ctypes.c_int, [P, ctypes.c_int, P, P, P]),
     "isaac_hip_get_timings": (ctypes.c_int, [P, P]),
     "isaac_hip_set_counting": (ctypes.c_int, [P, ctypes.c_int]),
     "isaac_hip_set_tuning": (ctypes.c_int, [P, I32, I32]),

@@ -15,6 +15,7 @@
 namespace isaac_hip
 {
     constexpr int kMaxSources = 16;
+    constexpr int kMaxAdvectionStep = 3; // ISAAC_MAX_ADVECTION_STEP_SIZE (lib/isaac/isaac_defines.hpp:21)
     constexpr int kMaxFunctors = 3;
     constexpr int kMaxClipping = 16;
     constexpr float kZNear = 0.002f; // ISAAC_Z_NEAR lib/isaac/isaac_defines.hpp:53
@@ -46,6 +47,7 @@ namespace isaac_hip
         float iso;
         ChainDesc chain;
         int32_t lutSlot; // index of the LUT in the staged table
+        const uint8_t* adv; // field source: advection texture (alpha *= adv / 255), else nullptr
     };
 
     struct ClassifyParams

@@ -25,6 +25,9 @@ namespace isaac_hip
         int64_t nbricks = 0;
         int brickDim = 0;
         bool bricked = false; // bricks hold the source as of the last update_buffers
+        uint8_t* adv[2] = {nullptr, nullptr}; // field source: advection textures (current, history)
+        int advCur = 0;
+        size_t advVox = 0;
     };
 
     struct PartSlot
@@ -91,6 +94,15 @@ struct isaac_hip_ctx
     float4* outFloat = nullptr;
     int aoMode = ISAAC_HIP_AO_PLAIN;
     isaac_hip::AoNet* aonet = nullptr;
+    // field sources: advection state (lib/isaac.hpp:606-644 defaults)
+    float* noise = nullptr;
+    float* noiseTmp = nullptr;
+    size_t noiseVox = 0;
+    bool noiseDirty = true;
+    int32_t seedPoints = 1000;
+    float advStepFactor = 1.0f, advHistoryWeight = 0.95f;
+    int32_t advSeedPeriod = 1, advSeedTime = 1;
+    int32_t timeStep = 0;
     // min/max request: per-workgroup partials and one (min, max) per source
     isaac_hip::MinMaxPartial* mmPartials = nullptr;
     float2* mmResult = nullptr;

@@ -334,6 +334,56 @@ namespace
         return 0;
     }
 
+    // the advection textures of a field source (zero-filled: the history of the
+    // first frame and the region outside the local box read as 0)
+    int ensure_advection(isaac_hip_ctx* c, VolSlot& v)
+    {
+        const size_t n = size_t(c->local[0]) * size_t(c->local[1]) * size_t(c->local[2]);
+        if(v.advVox != n)
+            for(uint8_t*& a : v.adv)
+                if(a)
+                {
+                    (void) hipFree(a);
+                    a = nullptr;
+                }
+        v.advVox = n;
+        for(uint8_t*& a : v.adv)
+        {
+            if(a)
+                continue;
+            HIPCHK(c, hipMalloc(&a, (n + 3) / 4 * 4));
+            HIPCHK(c, launch_fill_u32(reinterpret_cast<uint32_t*>(a), 0u, (n + 3) / 4, c->stream));
+        }
+        return 0;
+    }
+
+    // the noise of the advection seeds (updateNoiseTexture, lib/isaac.hpp:499-562),
+    // rebuilt when the seed count or the local size changes
+    int ensure_noise(isaac_hip_ctx* c)
+    {
+        const size_t n = size_t(c->local[0]) * size_t(c->local[1]) * size_t(c->local[2]);
+        if(c->noise && c->noiseVox != n)
+        {
+            (void) hipFree(c->noise);
+            (void) hipFree(c->noiseTmp);
+            c->noise = c->noiseTmp = nullptr;
+        }
+        if(!c->noise)
+        {
+            HIPCHK(c, hipMalloc(&c->noise, n * 4));
+            HIPCHK(c, hipMalloc(&c->noiseTmp, n * 4));
+            c->noiseVox = n;
+            c->noiseDirty = true;
+        }
+        if(c->noiseDirty)
+        {
+            HIPCHK(c, launch_noise(c->noise, c->noiseTmp, int(c->local[0]), int(c->local[1]), int(c->local[2]), c->scale,
+                                   uint32_t(c->seedPoints), c->stream));
+            c->noiseDirty = false;
+        }
+        return 0;
+    }
+
     int ensure_bricks(isaac_hip_ctx* c, VolSlot& v, int dim)
     {
         const int64_t nbx = (c->local[0] + 7) / 8, nby = (c->local[1] + 7) / 8, nbz = (c->local[2] + 7) / 8;
@@ -815,12 +865,15 @@ extern "C"
                 (void) hipFree(v.brick);
             if(v.brickScale)
                 (void) hipFree(v.brickScale);
+            for(uint8_t* a : v.adv)
+                if(a)
+                    (void) hipFree(a);
         }
         for(auto& p : c->parts)
             free_part(p);
         aonet_destroy(c);
         void* bufs[] = {c->lutDev, c->tex,      c->color,      c->depth,   c->normal,  c->ao,
-                        c->volColor, c->counters, c->outFloat, c->mmPartials, c->mmResult};
+                        c->volColor, c->counters, c->outFloat, c->mmPartials, c->mmResult, c->noise, c->noiseTmp};
         for(void* b : bufs)
             if(b)
                 (void) hipFree(b);
@@ -957,6 +1010,44 @@ extern "C"
         if(!c)
             return ISAAC_HIP_ERR_ARG;
         c->interpolation = on != 0;
+        return 0;
+    }
+
+    int isaac_hip_set_advection(isaac_hip_ctx* c, float step_factor, float history_weight, int32_t seeding_period, int32_t seeding_duration)
+    {
+        if(!c)
+            return ISAAC_HIP_ERR_ARG;
+        if(seeding_period < 1 || !(step_factor == step_factor) || !(history_weight == history_weight))
+            return fail(c, ISAAC_HIP_ERR_ARG, "advection: seeding period >= 1 and finite factors expected");
+        c->advStepFactor = step_factor;
+        c->advHistoryWeight = history_weight;
+        c->advSeedPeriod = seeding_period;
+        c->advSeedTime = seeding_duration;
+        return 0;
+    }
+
+    int isaac_hip_set_seed_points(isaac_hip_ctx* c, int32_t seed_points)
+    {
+        if(!c)
+            return ISAAC_HIP_ERR_ARG;
+        const int32_t n = seed_points < 0 ? 0 : seed_points;
+        if(n != c->seedPoints)
+            c->noiseDirty = true;
+        c->seedPoints = n;
+        return 0;
+    }
+
+    int isaac_hip_get_advection(const isaac_hip_ctx* c, int field_idx, const void** texture, const void** noise, int32_t* time_step)
+    {
+        if(!c || field_idx < 0 || field_idx >= c->nf)
+            return ISAAC_HIP_ERR_ARG;
+        const VolSlot& v = c->vols[size_t(c->nv + field_idx)];
+        if(texture)
+            *texture = v.adv[v.advCur];
+        if(noise)
+            *noise = c->noise;
+        if(time_step)
+            *time_step = c->timeStep;
         return 0;
     }
 
@@ -1115,6 +1206,49 @@ extern "C"
                     (const float*) v.desc.data, v.desc.stride, v.desc.feature_dim, c->chain[s], int(c->local[0] + 2 * g),
                     int(c->local[1] + 2 * g), int(c->local[2] + 2 * g), g, v.npTex, c->stream));
         }
+        // field sources (UpdateAdvectionTextureIterator, lib/isaac.hpp:292-370): every
+        // buffer update swaps the advection textures and advances the time step
+        // (lib/isaac.hpp:1821-1826), then each enabled field source advects its
+        // history along its unit field direction and reseeds from the noise
+        if(c->nf > 0)
+        {
+            int rc = ensure_noise(c);
+            if(rc)
+                return rc;
+            for(int s = c->nv; s < nvf; s++)
+                c->vols[s].advCur ^= 1;
+            c->timeStep++;
+            const bool seeding = (c->timeStep % c->advSeedPeriod) < c->advSeedTime;
+            for(int s = c->nv; s < nvf; s++)
+            {
+                VolSlot& v = c->vols[s];
+                const bool enabled = c->weight[s] != 0.0f || c->iso[s] != 0.0f;
+                if(!v.set || !enabled)
+                    continue;
+                if(v.desc.feature_dim != 3)
+                    return fail(c, ISAAC_HIP_ERR_UNSUPPORTED, "field sources are float3 vector fields");
+                rc = ensure_advection(c, v);
+                if(rc)
+                    return rc;
+                AdvectParams A{};
+                A.field = (const float*) v.desc.data;
+                for(int k = 0; k < 3; k++)
+                {
+                    A.stride[k] = v.desc.stride[k];
+                    A.scale[k] = c->scale[k];
+                }
+                A.noise = c->noise;
+                A.history = v.adv[v.advCur ^ 1];
+                A.out = v.adv[v.advCur];
+                A.X = int32_t(c->local[0]);
+                A.Y = int32_t(c->local[1]);
+                A.Z = int32_t(c->local[2]);
+                A.stepSize = c->advStepFactor * float(kMaxAdvectionStep);
+                A.historyWeight = c->advHistoryWeight;
+                A.seeding = seeding ? 1 : 0;
+                HIPCHK(c, launch_advect(A, c->stream));
+            }
+        }
         HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
         // particle cell lists
         for(int s = 0; s < c->np; s++)
@@ -1155,6 +1289,8 @@ extern "C"
                 v.bricked = false;
                 if(!v.set || c->weight[s] + c->iso[s] <= 0.0f)
                     continue;
+                if(s >= c->nv)
+                    return fail(c, ISAAC_HIP_ERR_UNSUPPORTED, "brick form: field sources (advection) render in the combined form");
                 if(++active > kMaxBrickSources)
                     return fail(c, ISAAC_HIP_ERR_UNSUPPORTED, "brick form: at most 4 enabled volume/field sources");
                 const float* src;
@@ -1247,6 +1383,7 @@ extern "C"
                 if(!S.data)
                     S.active = 0;
             }
+            S.adv = (s >= c->nv && S.active) ? v.adv[v.advCur] : nullptr;
         }
         HIPCHK(c, launch_classify(P, c->stream));
         HIPCHK(c, hipEventRecord(c->ev[3], c->stream));

@@ -75,6 +75,27 @@ namespace isaac_hip
         unsigned long long* scanTmp; // scan workspace (bin_scan_workspace_elems entries)
     };
 
+    // ---- field sources: advection noise (updateNoiseTexture, lib/isaac.hpp:499-562) and
+    // the per-frame advection texture (GenerateAdvectionTextureKernel,
+    // lib/isaac/isaac_common_kernel.hpp:500-550)
+    // noise (X*Y*Z floats, x fastest) = the seeded, blurred, scaled noise; tmp: scratch of the same size
+    hipError_t launch_noise(float* noise, float* tmp, int X, int Y, int Z, const float scale[3], uint32_t seedPoints, hipStream_t s);
+
+    struct AdvectParams
+    {
+        const float* field; // local voxel (0,0,0) of the float3 field source
+        int64_t stride[3]; // element strides (elements of 3 floats)
+        const float* noise; // X*Y*Z
+        const uint8_t* history; // last frame's advection texture, X*Y*Z
+        uint8_t* out; // this frame's, X*Y*Z
+        int32_t X, Y, Z;
+        float stepSize; // advectionStepFactor * ISAAC_MAX_ADVECTION_STEP_SIZE
+        float scale[3];
+        float historyWeight;
+        int32_t seeding; // 1: the noise seeds this frame (timeStep % period < seedingTime)
+    };
+    hipError_t launch_advect(const AdvectParams& P, hipStream_t s);
+
     // ---- min/max request (MinMaxKernel / MinMaxParticleKernel,
     // lib/isaac/isaac_min_max_kernel.hpp:22-97).  Every value carries the position
     // of its element in the reference's visiting order (column x + y*X, then z, then

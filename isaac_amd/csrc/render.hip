@@ -157,9 +157,14 @@ namespace isaac_hip
                 const float tv = apply_chain(S.chain, c0, c1, c2, c3, S.dim);
                 const int li = tf_index(tv, T);
                 const float4 col = (S.lutSlot < P.nLut) ? lds_lut[S.lutSlot * T + li] : P.luts[S.lutSlot * T + li];
+                // field sources: alpha blended with the advection texture
+                // (MergeAdvectionToCombinedTextureIterator, lib/isaac/isaac_common_kernel.hpp:655-656)
+                float ca = col.w;
+                if(S.adv)
+                    ca = ca * (float(S.adv[idx]) / 255.0f);
                 if(S.weight > 0.0f)
                 {
-                    float a = col.w * S.weight;
+                    float a = ca * S.weight;
                     vr = vr + col.x * a;
                     vg = vg + col.y * a;
                     vb = vb + col.z * a;
@@ -167,7 +172,7 @@ namespace isaac_hip
                 }
                 if(S.iso > 0.0f)
                 {
-                    float a = col.w / S.iso * 0.5f;
+                    float a = ca / S.iso * 0.5f;
                     if(ia < a)
                     {
                         ir = col.x;
@@ -1690,6 +1695,162 @@ namespace isaac_hip
         if(blocks > 4096)
             blocks = 4096;
         hipLaunchKernelGGL(classify_kernel, dim3(unsigned(blocks)), dim3(256), lds, s, P);
+        return hipGetLastError();
+    }
+
+    // ------------------------------------------------------------------ field-source advection
+    // halton (lib/isaac/isaac_helper.hpp:57-69)
+    __device__ inline float halton(uint32_t index, uint32_t base)
+    {
+        float result = 0.0f, f = 1.0f;
+        while(index > 0)
+        {
+            f = f / float(base);
+            result = result + f * float(index % base);
+            index = index / base;
+        }
+        return result;
+    }
+
+    // HaltonSeedingKernel (lib/isaac/isaac_common_kernel.hpp:334-348): point i = 1..count
+    // at (halton(i,3), halton(i,5), halton(i,7)) * size, truncated, set to 1 (one
+    // thread per point; points that land on one voxel all store 1)
+    __global__ __launch_bounds__(256) void halton_seed_kernel(float* noise, int X, int Y, int Z, uint32_t count)
+    {
+        for(uint32_t i = blockIdx.x * blockDim.x + threadIdx.x + 1; i <= count; i += gridDim.x * blockDim.x)
+        {
+            const int x = int(float(X) * halton(i, 3)), y = int(float(Y) * halton(i, 5)), z = int(float(Z) * halton(i, 7));
+            noise[(size_t(z) * Y + y) * X + x] = 1.0f;
+        }
+    }
+
+    // Sampler<LINEAR, REPEAT> on a guard-less float texture (lib/isaac/isaac_texture.hpp:182-259)
+    __device__ inline int wrap(int c, int n)
+    {
+        return (n + (c % n)) % n;
+    }
+
+    __device__ inline float sample_repeat(const float* t, int X, int Y, int Z, float px, float py, float pz)
+    {
+        const float cx = px - 0.5f, cy = py - 0.5f, cz = pz - 0.5f;
+        const float flx = floorf(cx), fly = floorf(cy), flz = floorf(cz);
+        const float fx = cx - flx, fy = cy - fly, fz = cz - flz;
+        const int x0 = int(flx), y0 = int(fly), z0 = int(flz);
+        auto v = [&](int dx, int dy, int dz) {
+            return t[(size_t(wrap(z0 + dz, Z)) * Y + wrap(y0 + dy, Y)) * X + wrap(x0 + dx, X)];
+        };
+        const float oz = 1.0f - fz, oy = 1.0f - fy, ox = 1.0f - fx;
+        const float a0 = v(0, 0, 0) * oz + v(0, 0, 1) * fz, a1 = v(0, 1, 0) * oz + v(0, 1, 1) * fz;
+        const float A = a0 * oy + a1 * fy;
+        const float b0 = v(1, 0, 0) * oz + v(1, 0, 1) * fz, b1 = v(1, 1, 0) * oz + v(1, 1, 1) * fz;
+        const float B = b0 * oy + b1 * fy;
+        return A * ox + B * fx;
+    }
+
+    // GaussBlur7Kernel (lib/isaac/isaac_common_kernel.hpp:381-411) along one axis
+    // sample position coord + mask * k / scale, as the reference evaluates it
+    __global__ __launch_bounds__(256) void gauss7_kernel(const float* src, float* dst, int X, int Y, int Z, float3 mask, float3 scale)
+    {
+        const float w[4] = {64.0f, 40.5f, 10.0f, 1.0f};
+        const uint32_t n = uint32_t(X) * uint32_t(Y) * uint32_t(Z);
+        for(uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        {
+            const int x = int(i % uint32_t(X)), y = int((i / uint32_t(X)) % uint32_t(Y)), z = int(i / (uint32_t(X) * uint32_t(Y)));
+            float r = 0.0f;
+            for(int k = -3; k < 4; k++)
+            {
+                const float px = float(x) + mask.x * float(k) / scale.x, py = float(y) + mask.y * float(k) / scale.y,
+                            pz = float(z) + mask.z * float(k) / scale.z;
+                r = r + sample_repeat(src, X, Y, Z, px, py, pz) * w[k < 0 ? -k : k];
+            }
+            dst[i] = r / 167.0f;
+        }
+    }
+
+    // MultiplyClampKernel (lib/isaac/isaac_common_kernel.hpp:413-429)
+    __global__ __launch_bounds__(256) void multiply_clamp_kernel(const float* src, float* dst, int64_t n, float value, float lo, float hi)
+    {
+        for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+            dst[i] = fminf(fmaxf(src[i] * value, lo), hi);
+    }
+
+    hipError_t launch_noise(float* noise, float* tmp, int X, int Y, int Z, const float scale[3], uint32_t seedPoints, hipStream_t s)
+    {
+        const int64_t n = int64_t(X) * Y * Z;
+        if(n <= 0)
+            return hipSuccess;
+        if(n >= (int64_t(1) << 32))
+            return hipErrorInvalidValue;
+        const unsigned grid = unsigned(std::min<int64_t>((n + 255) / 256, 8192));
+        const float3 sc = make_float3(scale[0], scale[1], scale[2]);
+        hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(noise), 0u, size_t(n), s);
+        if(e != hipSuccess)
+            return e;
+        if(seedPoints > 0)
+            hipLaunchKernelGGL(halton_seed_kernel, dim3(std::min<uint32_t>((seedPoints + 255) / 256, 4096)), dim3(256), 0, s,
+                               noise, X, Y, Z, seedPoints);
+        // three separable passes noise -> tmp (x) -> noise (y) -> tmp (z), then noise = clamp(tmp * 30, 0, 1)
+        hipLaunchKernelGGL(gauss7_kernel, dim3(grid), dim3(256), 0, s, (const float*) noise, tmp, X, Y, Z, make_float3(1.0f, 0.0f, 0.0f), sc);
+        hipLaunchKernelGGL(gauss7_kernel, dim3(grid), dim3(256), 0, s, (const float*) tmp, noise, X, Y, Z, make_float3(0.0f, 1.0f, 0.0f), sc);
+        hipLaunchKernelGGL(gauss7_kernel, dim3(grid), dim3(256), 0, s, (const float*) noise, tmp, X, Y, Z, make_float3(0.0f, 0.0f, 1.0f), sc);
+        hipLaunchKernelGGL(multiply_clamp_kernel, dim3(grid), dim3(256), 0, s, (const float*) tmp, noise, n, 30.0f, 0.0f, 1.0f);
+        return hipGetLastError();
+    }
+
+    // Sampler<LINEAR, VALUE> on a byte texture whose guard (and beyond) reads 0
+    __device__ inline float sample_bytes_zero(const uint8_t* t, int X, int Y, int Z, float px, float py, float pz)
+    {
+        const float cx = px - 0.5f, cy = py - 0.5f, cz = pz - 0.5f;
+        const float flx = floorf(cx), fly = floorf(cy), flz = floorf(cz);
+        const float fx = cx - flx, fy = cy - fly, fz = cz - flz;
+        const int x0 = int(flx), y0 = int(fly), z0 = int(flz);
+        auto v = [&](int dx, int dy, int dz) {
+            const int x = x0 + dx, y = y0 + dy, z = z0 + dz;
+            if(x < 0 || y < 0 || z < 0 || x >= X || y >= Y || z >= Z)
+                return 0.0f;
+            return float(t[(size_t(z) * Y + y) * X + x]);
+        };
+        const float oz = 1.0f - fz, oy = 1.0f - fy, ox = 1.0f - fx;
+        const float a0 = v(0, 0, 0) * oz + v(0, 0, 1) * fz, a1 = v(0, 1, 0) * oz + v(0, 1, 1) * fz;
+        const float A = a0 * oy + a1 * fy;
+        const float b0 = v(1, 0, 0) * oz + v(1, 0, 1) * fz, b1 = v(1, 1, 0) * oz + v(1, 1, 1) * fz;
+        const float B = b0 * oy + b1 * fy;
+        return A * ox + B * fx;
+    }
+
+    // GenerateAdvectionTextureKernel (lib/isaac/isaac_common_kernel.hpp:500-550): the
+    // unit field direction moves the history back by stepSize / scale; the noise seeds
+    // where the seeding window is open; blended value dithered with the 16^3 mask
+    __global__ __launch_bounds__(256) void advect_kernel(const AdvectParams P)
+    {
+        const uint32_t n = uint32_t(P.X) * uint32_t(P.Y) * uint32_t(P.Z);
+        for(uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        {
+            const int x = int(i % uint32_t(P.X)), y = int((i / uint32_t(P.X)) % uint32_t(P.Y)), z = int(i / (uint32_t(P.X) * uint32_t(P.Y)));
+            const float* e = P.field + (int64_t(x) * P.stride[0] + int64_t(y) * P.stride[1] + int64_t(z) * P.stride[2]) * 3;
+            float vx = e[0], vy = e[1], vz = e[2];
+            float len = sqrtf((vx * vx + vy * vy) + vz * vz);
+            len = (len < FLT_MIN) ? FLT_MIN : len;
+            vx = vx / len;
+            vy = vy / len;
+            vz = vz / len;
+            const float ox = vx * P.stepSize / P.scale[0], oy = vy * P.stepSize / P.scale[1], oz = vz * P.stepSize / P.scale[2];
+            const float px = (float(x) + 0.5f) - ox, py = (float(y) + 0.5f) - oy, pz = (float(z) + 0.5f) - oz;
+            const float history = sample_bytes_zero(P.history, P.X, P.Y, P.Z, px, py, pz) / 255.0f;
+            const float noise = P.seeding ? P.noise[i] : 0.0f;
+            const float v = noise + history * P.historyWeight * (1.0f - noise);
+            P.out[i] = uint8_t(dither_b4(5, x, y, z, v, 0.f, 0.f, 0.f) & 0xffu);
+        }
+    }
+
+    hipError_t launch_advect(const AdvectParams& P, hipStream_t s)
+    {
+        const int64_t n = int64_t(P.X) * P.Y * P.Z;
+        if(n <= 0)
+            return hipSuccess;
+        if(n >= (int64_t(1) << 32))
+            return hipErrorInvalidValue;
+        hipLaunchKernelGGL(advect_kernel, dim3(unsigned(std::min<int64_t>((n + 255) / 256, 8192))), dim3(256), 0, s, P);
         return hipGetLastError();
     }
 

@@ -78,6 +78,10 @@ class SceneSpec:
     transfer_size: int = 1024
     volumes: List[VolumeSourceSpec] = field(default_factory=list)
     particles: List[ParticleSourceSpec] = field(default_factory=list)
+    # field sources (float3 vector fields with advection) and the advection steering
+    # (step factor, history weight, seeding period, seeding duration, seed points)
+    fields: List[VolumeSourceSpec] = field(default_factory=list)
+    advection: Optional[Tuple[float, float, int, int, int]] = None
     step: float = 0.5
     interpolation: bool = True
     dither_mode: int = 1

@@ -192,6 +192,12 @@ class IsaacVisualization:
         self.dither_mode = 1
         self.render_mode = 0
         self.ambient_occlusion = {"isEnabled": False, "weight": 0.5}
+        # field-source advection (lib/isaac.hpp:610,639-642)
+        self.seed_points = 1000
+        self.advection_step = 1.0
+        self.advection_weight = 0.95
+        self.advection_seeding_period = 1
+        self.advection_seeding_duration = 1
         self.background_color = [0.0, 0.0, 0.0, 1.0]
         self.look_at = np.zeros(3)
         self.rotation = np.eye(3)
@@ -322,6 +328,11 @@ class IsaacVisualization:
         ao = self.ambient_occlusion
         check(L.isaac_hip_set_ao(c, int(bool(ao["isEnabled"])), float(ao["weight"])), c, "set_ao")
         check(L.isaac_hip_set_render_mode(c, int(self.render_mode)), c, "set_render_mode")
+        if self.field_sources:
+            check(L.isaac_hip_set_advection(c, float(self.advection_step), float(self.advection_weight),
+                                            int(self.advection_seeding_period), int(self.advection_seeding_duration)),
+                  c, "set_advection")
+            check(L.isaac_hip_set_seed_points(c, int(self.seed_points)), c, "set_seed_points")
 
     def _bind_sources(self, pointer=None) -> None:
         c = self._ctx
@@ -377,6 +388,22 @@ class IsaacVisualization:
 
     def synchronize(self) -> None:
         check(lib().isaac_hip_synchronize(self._ctx), self._ctx, "synchronize")
+
+    def advection(self, field_idx: int = 0):
+        """Host copies of field source field_idx's current advection texture (Z, Y, X)
+        uint8 and of the noise texture (Z, Y, X) float32, and the time step."""
+        X, Y, Z = self.local_size
+        tex, noise, ts = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_int32()
+        check(lib().isaac_hip_get_advection(self._ctx, int(field_idx), ctypes.byref(tex), ctypes.byref(noise),
+                                            ctypes.byref(ts)), self._ctx, "get_advection")
+        self.synchronize()
+        a = np.zeros((Z, Y, X), np.uint8)
+        n = np.zeros((Z, Y, X), np.float32)
+        if tex.value:
+            self._d2h(a, tex.value)
+        if noise.value:
+            self._d2h(n, noise.value)
+        return a, n, int(ts.value)
 
     def min_max(self) -> Tuple[np.ndarray, np.ndarray]:
         """The min/max request (sendMinMax, lib/isaac.hpp:1951-2006): per source in the
@@ -557,6 +584,8 @@ def from_scene(spec, device: int = 0) -> IsaacVisualization:
 
     dev = torch.device("cuda", device)
     vols = [VolumeSource(v.name, torch.from_numpy(np.ascontiguousarray(v.data)).to(dev), v.persistent) for v in spec.volumes]
+    fields = [VolumeSource(v.name, torch.from_numpy(np.ascontiguousarray(v.data)).to(dev), v.persistent)
+              for v in getattr(spec, "fields", [])]
     parts = []
     for p in spec.particles:
         rad = torch.from_numpy(p.radius).to(dev) if p.radius is not None else None
@@ -566,20 +595,26 @@ def from_scene(spec, device: int = 0) -> IsaacVisualization:
     torch.cuda.synchronize(dev)
     vis = IsaacVisualization(
         spec.name, spec.framebuffer, spec.global_size, spec.local_size, spec.particle_grid, spec.position,
-        vols, [], parts, spec.scale, spec.transfer_size, device,
+        vols, fields, parts, spec.scale, spec.transfer_size, device,
     )
     apply_scene(vis, spec)
     return vis
 
 
 def apply_scene(vis: IsaacVisualization, spec) -> None:
-    srcs = list(spec.volumes) + list(spec.particles)
+    fields = list(getattr(spec, "fields", []))
+    srcs = list(spec.volumes) + fields + list(spec.particles)
+    nvf = len(spec.volumes) + len(fields)
     vis.functions = [s.chain for s in srcs]
     vis.update_functions()
     for i, s in enumerate(srcs):
         vis.set_transfer_points(i, s.tf_points)
         vis.source_weight[i] = s.weight
-        vis.iso_threshold[i] = getattr(s, "iso", 0.0) if i < len(spec.volumes) else 0.0
+        vis.iso_threshold[i] = getattr(s, "iso", 0.0) if i < nvf else 0.0
+    adv = getattr(spec, "advection", None)
+    if adv is not None:
+        (vis.advection_step, vis.advection_weight, vis.advection_seeding_period,
+         vis.advection_seeding_duration, vis.seed_points) = adv
     vis.update_transfer()
     vis.step = spec.step
     vis.interpolation = spec.interpolation

@@ -307,6 +307,7 @@ namespace isaac_oracle
         std::vector<Chain> chain;
         std::vector<float> weight;
         std::vector<float> iso;
+        std::vector<const uint8_t*> adv; // per volume source: advection texture of a field source, else null
         float step = 0.5f;
         bool interpolation = false;
         int ditherMode = 1;
@@ -672,6 +673,10 @@ namespace isaac_oracle
                         const float* e = src.data + (x * src.stride[0] + y * src.stride[1] + z * src.stride[2]) * src.dim;
                         R tv = apply_chain<R>(st.chain[s], e, src.dim);
                         V4<R> color = lut_at<R>(st.lut[s], tf_index(tv, st.T));
+                        // field source: alpha * advection / 255
+                        // (MergeAdvectionToCombinedTextureIterator, lib/isaac/isaac_common_kernel.hpp:655-656)
+                        if(size_t(s) < st.adv.size() && st.adv[size_t(s)])
+                            color.w = color.w * (R(st.adv[size_t(s)][x + y * X + z * X * Y]) / R(255));
                         if(w > R(0))
                         {
                             V4<R> c = color;
@@ -762,7 +767,7 @@ namespace isaac_oracle
         return cl;
     }
 
-    // --------------------------------------------------------------- min/max request
+    // --------------------------------------------------------------- field-source advection
     // glm 0.9.9 scalar min/max: min(x, y) = (y < x) ? y : x, max(x, y) = (x < y) ? y : x
     // (a NaN y never replaces x; on ties the running value stays)
     inline float glm_min(float x, float y)
@@ -775,6 +780,143 @@ namespace isaac_oracle
         return (x < y) ? y : x;
     }
 
+    // halton (lib/isaac/isaac_helper.hpp:57-69)
+    inline float halton(uint32_t index, uint32_t base)
+    {
+        float result = 0.0f;
+        float f = 1.0f;
+        while(index > 0)
+        {
+            f /= float(base);
+            result += f * float(index % base);
+            index /= base;
+        }
+        return result;
+    }
+
+    // linear / bilinear / trilinear of the 2x2x2 taps v[x][y][z] (lib/isaac/isaac_helper.hpp:124-144)
+    inline float lin(float t, float a, float b)
+    {
+        return a * (1.0f - t) + b * t;
+    }
+
+    inline float trilin(float fx, float fy, float fz, const float v[2][2][2])
+    {
+        const float a = lin(fy, lin(fz, v[0][0][0], v[0][0][1]), lin(fz, v[0][1][0], v[0][1][1]));
+        const float b = lin(fy, lin(fz, v[1][0][0], v[1][0][1]), lin(fz, v[1][1][0], v[1][1][1]));
+        return lin(fx, a, b);
+    }
+
+    // Sampler<LINEAR, border>::sample on a guard-less texture (lib/isaac/isaac_texture.hpp:182-259):
+    // coord - 0.5, taps at floor + {0,1}^3; REPEAT wraps with the reference's modulo,
+    // VALUE reads 0 outside
+    template<class T, bool REPEAT>
+    inline float sample_linear(const T* t, int64_t X, int64_t Y, int64_t Z, float px, float py, float pz)
+    {
+        const float cx = px - 0.5f, cy = py - 0.5f, cz = pz - 0.5f;
+        const int64_t x0 = (int64_t) std::floor(cx), y0 = (int64_t) std::floor(cy), z0 = (int64_t) std::floor(cz);
+        float v[2][2][2];
+        for(int dz = 0; dz < 2; dz++)
+            for(int dy = 0; dy < 2; dy++)
+                for(int dx = 0; dx < 2; dx++)
+                {
+                    int64_t x = x0 + dx, y = y0 + dy, z = z0 + dz;
+                    if(REPEAT)
+                    {
+                        x = (X + (x % X)) % X;
+                        y = (Y + (y % Y)) % Y;
+                        z = (Z + (z % Z)) % Z;
+                        v[dx][dy][dz] = float(t[x + y * X + z * X * Y]);
+                    }
+                    else
+                    {
+                        const bool in = x >= 0 && y >= 0 && z >= 0 && x < X && y < Y && z < Z;
+                        v[dx][dy][dz] = in ? float(t[x + y * X + z * X * Y]) : 0.0f;
+                    }
+                }
+        return trilin(cx - std::floor(cx), cy - std::floor(cy), cz - std::floor(cz), v);
+    }
+
+    // updateNoiseTexture (lib/isaac.hpp:499-562): clear, Halton points set to 1
+    // (HaltonSeedingKernel, lib/isaac/isaac_common_kernel.hpp:334-348), GaussBlur7 along
+    // x, y, z (lib/isaac/isaac_common_kernel.hpp:381-411), then clamp(v * 30, 0, 1)
+    inline std::vector<float> make_noise(int64_t X, int64_t Y, int64_t Z, const float scale[3], uint32_t seedPoints)
+    {
+        const size_t n = size_t(X * Y * Z);
+        std::vector<float> a(n, 0.0f), b(n, 0.0f);
+        for(uint32_t i = 1; i <= seedPoints; ++i)
+        {
+            const int64_t x = (int64_t) (float(X) * halton(i, 3));
+            const int64_t y = (int64_t) (float(Y) * halton(i, 5));
+            const int64_t z = (int64_t) (float(Z) * halton(i, 7));
+            a[size_t(x + y * X + z * X * Y)] = 1.0f;
+        }
+        const float g7[4] = {64.0f, 40.5f, 10.0f, 1.0f};
+        auto blur = [&](const std::vector<float>& src, std::vector<float>& dst, float mx, float my, float mz)
+        {
+            for(int64_t z = 0; z < Z; z++)
+                for(int64_t y = 0; y < Y; y++)
+                    for(int64_t x = 0; x < X; x++)
+                    {
+                        float r = 0.0f;
+                        for(int k = -3; k < 4; k++)
+                        {
+                            const float px = float(x) + mx * float(k) / scale[0];
+                            const float py = float(y) + my * float(k) / scale[1];
+                            const float pz = float(z) + mz * float(k) / scale[2];
+                            r += sample_linear<float, true>(src.data(), X, Y, Z, px, py, pz) * g7[std::abs(k)];
+                        }
+                        dst[size_t(x + y * X + z * X * Y)] = r / 167.0f;
+                    }
+        };
+        blur(a, b, 1, 0, 0);
+        blur(b, a, 0, 1, 0);
+        blur(a, b, 0, 0, 1);
+        for(size_t i = 0; i < n; i++)
+            a[i] = std::min(std::max(b[i] * 30.0f, 0.0f), 1.0f);
+        return a;
+    }
+
+    // GenerateAdvectionTextureKernel (lib/isaac/isaac_common_kernel.hpp:500-550) over the
+    // local voxels; the advection textures' guard is never written on one rank, so the
+    // history reads 0 outside the local box
+    inline std::vector<uint8_t> advect(
+        const VolumeSource& field,
+        const float* noise,
+        const uint8_t* history,
+        int64_t X,
+        int64_t Y,
+        int64_t Z,
+        const float scale[3],
+        float stepSize,
+        float historyWeight,
+        bool seeding)
+    {
+        std::vector<uint8_t> out(size_t(X * Y * Z));
+        for(int64_t z = 0; z < Z; z++)
+            for(int64_t y = 0; y < Y; y++)
+                for(int64_t x = 0; x < X; x++)
+                {
+                    const float* e = field.data + (x * field.stride[0] + y * field.stride[1] + z * field.stride[2]) * 3;
+                    float v[3] = {e[0], e[1], e[2]};
+                    float len = std::sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+                    len = glm_max(len, std::numeric_limits<float>::min());
+                    for(int k = 0; k < 3; k++)
+                        v[k] /= len;
+                    float off[3];
+                    for(int k = 0; k < 3; k++)
+                        off[k] = v[k] * stepSize / scale[k];
+                    const float px = (float(x) + 0.5f) - off[0], py = (float(y) + 0.5f) - off[1], pz = (float(z) + 0.5f) - off[2];
+                    const float hist = sample_linear<uint8_t, false>(history, X, Y, Z, px, py, pz) / 255.0f;
+                    const size_t i = size_t(x + y * X + z * X * Y);
+                    const float nz = seeding ? noise[i] : 0.0f;
+                    const V4<float> c{nz + hist * historyWeight * (1.0f - nz), 0.0f, 0.0f, 0.0f};
+                    out[i] = dither<float>(5, (int) x, (int) y, (int) z, c).x;
+                }
+        return out;
+    }
+
+    // --------------------------------------------------------------- min/max request
     // MinMaxKernel (lib/isaac/isaac_min_max_kernel.hpp:22-57): one thread per (x, y)
     // column walks z over the functor-chain values; the host then reduces the columns
     // in order x + y*X with strict comparisons (CalcMinMaxIterator,
@@ -1200,18 +1342,6 @@ namespace isaac_oracle
                     }
                 g.ao[p] = gclamp(occ / R(42), R(0), R(1));
             });
-    }
-
-    inline float halton(uint32_t index, uint32_t base)
-    {
-        float result = 0, f = 1;
-        while(index > 0)
-        {
-            f /= float(base);
-            result += f * float(index % base);
-            index /= base;
-        }
-        return result;
     }
 
     // getHSVA (lib/isaac/isaac_helper.hpp:221-264); h converted to float by the caller

@@ -246,6 +246,7 @@ class OracleScene(ctypes.Structure):
         ("bg", ctypes.c_float * 4),
         ("threads", ctypes.c_int32),
         ("vol_nonpersistent", ctypes.c_int32 * MAX_SOURCES),
+        ("vol_adv", ctypes.c_void_p * MAX_SOURCES),
     ]
 
 
@@ -300,11 +301,12 @@ class PreparedScene:
         sc.viewport[:] = spec.vp()
         T = spec.transfer_size
         sc.transfer_size = T
-        sc.n_volume = len(spec.volumes)
+        vols = list(spec.volumes) + list(getattr(spec, "fields", []))
+        sc.n_volume = len(vols)
         sc.n_particle = len(spec.particles)
-        nv = len(spec.volumes)
-        sources = list(spec.volumes) + list(spec.particles)
-        for s, v in enumerate(spec.volumes):
+        nv = len(vols)
+        sources = vols + list(spec.particles)
+        for s, v in enumerate(vols):
             data = np.ascontiguousarray(v.data, dtype=np.float32)
             self.keep.append(data)
             sc.vol_data[s] = data.ctypes.data
@@ -403,6 +405,41 @@ def render(spec, precision: int = 32, threads: int = 0, want_textures: bool = Fa
     res.seconds = {"classify": out.seconds_classify, "bin": out.seconds_bin, "render": out.seconds_render}
     res.prepared = ps
     return res
+
+
+def set_advection(ps: PreparedScene, textures: dict) -> None:
+    """Attach advection textures {source index: (Z, Y, X) uint8} of field sources to a
+    prepared scene (the classification multiplies their alpha by texture / 255)."""
+    for s, t in textures.items():
+        a = np.ascontiguousarray(t, np.uint8)
+        ps.keep.append(a)
+        ps.sc.vol_adv[s] = a.ctypes.data
+
+
+def noise(local_size, scale, seed_points: int) -> np.ndarray:
+    """updateNoiseTexture: (Z, Y, X) float32."""
+    X, Y, Z = local_size
+    out = np.zeros((Z, Y, X), np.float32)
+    loc = (ctypes.c_int64 * 3)(*local_size)
+    sc = (ctypes.c_float * 3)(*scale)
+    lib().oracle_noise(loc, sc, ctypes.c_uint32(seed_points), out.ctypes.data_as(ctypes.c_void_p))
+    return out
+
+
+def advect(field: np.ndarray, noise_tex: np.ndarray, history: np.ndarray, scale, step_size: float,
+           history_weight: float, seeding: bool) -> np.ndarray:
+    """One GenerateAdvectionTextureKernel pass: field (Z, Y, X, 3) float32 -> (Z, Y, X) uint8."""
+    Z, Y, X = history.shape
+    f = np.ascontiguousarray(field, np.float32)
+    nz = np.ascontiguousarray(noise_tex, np.float32)
+    h = np.ascontiguousarray(history, np.uint8)
+    out = np.zeros((Z, Y, X), np.uint8)
+    loc = (ctypes.c_int64 * 3)(X, Y, Z)
+    sc = (ctypes.c_float * 3)(*scale)
+    lib().oracle_advect(f.ctypes.data_as(ctypes.c_void_p), loc, sc, ctypes.c_float(step_size), ctypes.c_float(history_weight),
+                        ctypes.c_int(int(seeding)), nz.ctypes.data_as(ctypes.c_void_p), h.ctypes.data_as(ctypes.c_void_p),
+                        out.ctypes.data_as(ctypes.c_void_p))
+    return out
 
 
 def min_max(spec, prepared: Optional[PreparedScene] = None) -> Tuple[np.ndarray, np.ndarray]:

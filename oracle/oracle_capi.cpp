@@ -49,6 +49,7 @@ struct OracleScene
     float bg[4];
     int32_t threads; // 0 = OpenMP default
     int32_t vol_nonpersistent[MAX_SOURCES]; // brick form: 1 = sample the functor-applied field
+    const uint8_t* vol_adv[MAX_SOURCES]; // field sources: advection texture (X*Y*Z bytes), else null
 };
 
 struct OraclePostOutputs
@@ -104,6 +105,9 @@ namespace
             for(int k = 0; k < 3; k++)
                 st.vol[s].stride[k] = sc->vol_stride[s][k];
         }
+        st.adv.assign(size_t(nv), nullptr);
+        for(int s = 0; s < nv; s++)
+            st.adv[size_t(s)] = sc->vol_adv[s];
         st.part.resize(np);
         for(int s = 0; s < np; s++)
         {
@@ -351,6 +355,37 @@ extern "C"
             min_max_volume(st.vol[s], st.chain[s], st.sizes, mins[s], maxs[s]);
         for(int s = 0; s < sc->n_particle; s++)
             min_max_particles(st.part[s], st.chain[nv + s], st.sizes, mins[nv + s], maxs[nv + s]);
+        return 0;
+    }
+
+    // field-source advection: the noise texture and one advection step
+    int oracle_noise(const int64_t local[3], const float scale[3], uint32_t seed_points, float* out)
+    {
+        const std::vector<float> n = make_noise(local[0], local[1], local[2], scale, seed_points);
+        std::memcpy(out, n.data(), n.size() * sizeof(float));
+        return 0;
+    }
+
+    int oracle_advect(
+        const float* field,
+        const int64_t local[3],
+        const float scale[3],
+        float step_size,
+        float history_weight,
+        int seeding,
+        const float* noise,
+        const uint8_t* history,
+        uint8_t* out)
+    {
+        VolumeSource f;
+        f.data = field;
+        f.dim = 3;
+        f.stride[0] = 1;
+        f.stride[1] = local[0];
+        f.stride[2] = local[0] * local[1];
+        const std::vector<uint8_t> a
+            = advect(f, noise, history, local[0], local[1], local[2], scale, step_size, history_weight, seeding != 0);
+        std::memcpy(out, a.data(), a.size());
         return 0;
     }
 
