@@ -100,6 +100,7 @@ def test_render_path_occlusion_matches_network(gpu):
     p = aonet.init_params(5)
     vis.aonet_set_params(p)
     vis.set_ao_mode(vis.AO_LEARNED)
+    vis.update_buffers()
     vis.draw()
     _, depth, normal, ao = vis.gbuffer()
     scale = max(int(np.float32(g) * np.float32(s)) for g, s in zip(spec.global_size, spec.scale))
