@@ -6,6 +6,8 @@
   * the trained pass is gated on the final image: >= 40 dB PSNR against the
     plain pass on held-out poses, and well above the image without occlusion
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -95,7 +97,11 @@ def test_render_path_occlusion_matches_network(gpu):
     from isaac_amd import aonet
     from oracle import aonet_oracle as A
 
-    spec = scenes.prod_scene(n=32, fb=(100, 60), n_particles=2000)
+    # close pose, opaque particles: about a third of the 100 x 60 frame is surface
+    spec = scenes.prod_scene(n=48, fb=(100, 60), n_particles=50_000, pose=scenes.FILL_POSE)
+    p0 = spec.particles[0]
+    spec.particles[0] = dataclasses.replace(p0, tf_points=[(0, (1.0, 1.0, 1.0, 1.0)), (1024, (1.0, 0.8, 0.2, 1.0))],
+                                            constant_radius=1.0)
     vis = _net_vis(gpu, spec)
     p = aonet.init_params(5)
     vis.aonet_set_params(p)
