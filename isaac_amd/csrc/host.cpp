@@ -763,8 +763,6 @@ extern "C"
         if(cfg->n_volume_sources + cfg->n_field_sources > ISAAC_HIP_MAX_SOURCES
            || cfg->n_particle_sources > ISAAC_HIP_MAX_SOURCES)
             return ISAAC_HIP_ERR_ARG;
-        if(cfg->n_field_sources > 0)
-            return ISAAC_HIP_ERR_UNSUPPORTED; // vector-field advection is not part of this library
         if(cfg->framebuffer_size[0] <= 0 || cfg->framebuffer_size[1] <= 0 || cfg->transfer_size <= 0)
             return ISAAC_HIP_ERR_ARG;
         for(int i = 0; i < 3; i++)

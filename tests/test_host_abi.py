@@ -47,7 +47,8 @@ def _cfg(**kw):
 @pytest.mark.parametrize(
     "kw,rc",
     [
-        (dict(n_field_sources=1), -4),  # advection of vector-field sources is not built
+        (dict(n_field_sources=-1), -1),
+        (dict(n_volume_sources=10, n_field_sources=7), -1),  # volume + field sources above 16
         (dict(transfer_size=0), -1),
         (dict(framebuffer_size=(0, 10)), -1),
         (dict(local_size=(0, 16, 16)), -1),
