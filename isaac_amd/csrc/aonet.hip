@@ -192,8 +192,11 @@ namespace isaac_hip
         return ((CIN / 8) % 2) ? CIN : CIN + 8;
     }
 
-    // All of a thread's 16-byte loads are issued before the first LDS write, so the
-    // workgroup waits for one global-load latency, not one per loop trip.
+    // A thread owns one 16-byte channel unit of one patch column (of RU = 66*CIN/8
+    // per row) and walks rows; when a row has fewer than 256 units, G = 256/RU
+    // thread groups split the rows.  The column's address is formed once, each row
+    // adds a stride; all loads are issued before the first LDS write, so the
+    // workgroup waits for one global-load latency.
     template<int CIN, int CA, int PROWS>
     __device__ inline void stage_patch8(
         _Float16* patch,
@@ -205,190 +208,252 @@ namespace isaac_hip
         int y0,
         int x0)
     {
-        constexpr int CS = CIN - CA, U = CIN / 8, PS = patch_stride<CIN>();
-        constexpr int TOTAL = PROWS * 66 * U, ITER = (TOTAL + 255) / 256;
-        h8 v[ITER];
+        constexpr int CS = CIN - CA, U = CIN / 8, PS = patch_stride<CIN>(), RU = 66 * U;
+        constexpr int G = RU >= 256 ? 1 : 256 / RU; // row groups
+        constexpr int CI = (RU + 255) / 256; // unit iterations per thread
+        constexpr int RPG = (PROWS + G - 1) / G; // rows per group
+        const int t = int(threadIdx.x);
+        const int grp = RU >= 256 ? 0 : t / RU;
+        const int u0 = RU >= 256 ? t : t - grp * RU;
+        h8 v[CI][RPG];
+        int col[CI], ch[CI];
+        bool uok[CI];
 #pragma unroll
-        for(int it = 0; it < ITER; it++)
+        for(int ci = 0; ci < CI; ci++)
         {
-            const int u = int(threadIdx.x) + it * 256;
-            v[it] = h8{0, 0, 0, 0, 0, 0, 0, 0};
-            if(TOTAL % 256 == 0 || u < TOTAL)
+            const int u = u0 + ci * 256;
+            uok[ci] = (RU >= 256) ? (u < RU) : (grp < G);
+            col[ci] = u / U;
+            ch[ci] = (u - col[ci] * U) * 8;
+            const int xx = x0 + col[ci] - 1;
+            const bool xin = uok[ci] && xx >= 0 && xx < w;
+            const bool fromUp = CA > 0 && ch[ci] < CA;
+            const _Float16* base;
+            size_t rs;
+            if(fromUp)
             {
-                const int pix = u / U, c = (u - pix * U) * 8;
-                const int r = pix / 66, col = pix - r * 66;
-                const int yy = y0 + r - 1, xx = x0 + col - 1;
-                if(yy >= 0 && yy < h && xx >= 0 && xx < w)
-                {
-                    if(CA > 0 && c < CA)
-                        v[it] = *reinterpret_cast<const h8*>(up + ((size_t(n) * (h / 2) + yy / 2) * (w / 2) + xx / 2) * CA + c);
-                    else
-                        v[it] = *reinterpret_cast<const h8*>(x + ((size_t(n) * h + yy) * w + xx) * CS + (c - CA));
-                }
+                base = up + (size_t(n) * size_t(h / 2) * size_t(w / 2) + size_t(xx >= 0 ? xx / 2 : 0)) * CA + ch[ci];
+                rs = size_t(w / 2) * CA;
+            }
+            else
+            {
+                base = x + (size_t(n) * size_t(h) * size_t(w) + size_t(xx >= 0 ? xx : 0)) * CS + (ch[ci] - CA);
+                rs = size_t(w) * CS;
+            }
+#pragma unroll
+            for(int k = 0; k < RPG; k++)
+            {
+                const int r = grp + k * G;
+                const int yy = y0 + r - 1;
+                v[ci][k] = h8{0, 0, 0, 0, 0, 0, 0, 0};
+                if(xin && r < PROWS && yy >= 0 && yy < h)
+                    v[ci][k] = *reinterpret_cast<const h8*>(base + size_t(fromUp ? yy / 2 : yy) * rs);
             }
         }
 #pragma unroll
-        for(int it = 0; it < ITER; it++)
-        {
-            const int u = int(threadIdx.x) + it * 256;
-            if(TOTAL % 256 == 0 || u < TOTAL)
+        for(int ci = 0; ci < CI; ci++)
+#pragma unroll
+            for(int k = 0; k < RPG; k++)
             {
-                const int pix = u / U, c = (u - pix * U) * 8;
-                *reinterpret_cast<h8*>(patch + pix * PS + c) = v[it];
+                const int r = grp + k * G;
+                if(uok[ci] && r < PROWS)
+                    *reinterpret_cast<h8*>(patch + (r * 66 + col[ci]) * PS + ch[ci]) = v[ci][k];
             }
-        }
     }
 
-    // Workgroup = ROWS rows x 64 pixels, wave = a 16-pixel column in all ROWS rows
-    // and all COUT channels: each A fragment feeds ROWS MFMAs, each B fragment
-    // COUT / 16.
+    // LDS layout of a forward layer: the input patch, then (16-channel layers) the
+    // weights, one row per output channel padded to an odd number of 16-byte units
+    template<int CIN, int COUT>
+    struct FwdLds
+    {
+        static constexpr int KP = (9 * CIN + 31) / 32 * 32;
+        static constexpr bool WLDS = COUT == 16;
+        static constexpr int WS = KP + 8; // weight row stride (halves)
+        static constexpr size_t bytes(int rows)
+        {
+            return size_t((rows + 2) * 66) * patch_stride<CIN>() * sizeof(_Float16)
+                + (WLDS ? size_t(16) * WS * sizeof(_Float16) : 0);
+        }
+    };
+
+    // Workgroup = ROWS rows x 64 pixels.  The four waves split the 16-channel output
+    // blocks first (COUT 64: one block per wave; COUT 32: two blocks x two 32-pixel
+    // halves; COUT 16: four 16-pixel columns), so a workgroup fetches each weight
+    // once: a wave owns one block in NPC 16-pixel columns x ROWS rows, and each
+    // weight fragment feeds NPC * ROWS MFMAs.  16-channel layers read their weights
+    // from LDS (staged with the patch), wider ones stream them from global memory
+    // PD chunks ahead of the MFMAs.
     template<int CIN, int COUT, int CA, int ROWS>
     __global__ __launch_bounds__(256) void conv_fwd_kernel(const ConvArgs a)
     {
         constexpr int K = 9 * CIN, KP = (K + 31) / 32 * 32, NCH = KP / 32, NCB = COUT / 16;
         constexpr int PS = patch_stride<CIN>();
+        constexpr int WCB = NCB < 4 ? NCB : 4; // waves along output-channel blocks
+        constexpr int NPC = 4 / (4 / WCB); // 16-pixel columns per wave
+        static_assert(NCB == WCB, "one output-channel block per wave");
+        using L = FwdLds<CIN, COUT>;
         extern __shared__ _Float16 patch[];
+        _Float16* wl = patch + (ROWS + 2) * 66 * PS;
         const int x0 = blockIdx.x * 64, y0 = blockIdx.y * ROWS, n = blockIdx.z;
-        stage_patch8<CIN, CA, ROWS + 2>(patch, a.x, a.up, n, a.h, a.w_, y0, x0);
-        __syncthreads();
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        const int pl = wave * 16 + (lane & 15);
+        const int cb = wave % WCB, pc0 = (wave / WCB) * NPC;
         const int g = lane >> 4;
-        f4 acc[ROWS][NCB];
+        constexpr int PD = L::WLDS ? 1 : (NCH < 8 ? NCH : 8);
+        h8 wq[PD];
+        const _Float16* wrow = a.w + size_t(cb * 16 + (lane & 15)) * KP + 8 * g;
+        if constexpr(L::WLDS)
+        {
+            // the 16 x KP weights into LDS (issued before the patch loads)
+            constexpr int WU = 16 * KP / 8, WIT = (WU + 255) / 256;
+            h8 wv[WIT];
+#pragma unroll
+            for(int it = 0; it < WIT; it++)
+            {
+                const int u = int(threadIdx.x) + it * 256;
+                if(WU % 256 == 0 || u < WU)
+                    wv[it] = *reinterpret_cast<const h8*>(a.w + size_t(u) * 8);
+            }
+            stage_patch8<CIN, CA, ROWS + 2>(patch, a.x, a.up, n, a.h, a.w_, y0, x0);
+#pragma unroll
+            for(int it = 0; it < WIT; it++)
+            {
+                const int u = int(threadIdx.x) + it * 256;
+                if(WU % 256 == 0 || u < WU)
+                {
+                    const int row = u / (KP / 8), k = (u - row * (KP / 8)) * 8;
+                    *reinterpret_cast<h8*>(wl + row * L::WS + k) = wv[it];
+                }
+            }
+        }
+        else
+        {
+            // the first PD chunks are requested before the patch, so both latencies overlap
+#pragma unroll
+            for(int p = 0; p < PD; p++)
+                wq[p] = *reinterpret_cast<const h8*>(wrow + 32 * p);
+            stage_patch8<CIN, CA, ROWS + 2>(patch, a.x, a.up, n, a.h, a.w_, y0, x0);
+        }
+        __syncthreads();
+        f4 acc[ROWS][NPC];
 #pragma unroll
         for(int r = 0; r < ROWS; r++)
 #pragma unroll
-            for(int cb = 0; cb < NCB; cb++)
-                acc[r][cb] = f4{0.f, 0.f, 0.f, 0.f};
-        // weight fragments run PD chunks ahead of the MFMAs (a ring of PD x NCB)
-        constexpr int PD = NCH < 16 / NCB ? NCH : 16 / NCB;
-        const _Float16* wrow = a.w + size_t(lane & 15) * KP + 8 * g;
-        h8 wq[PD][NCB];
-#pragma unroll
-        for(int p = 0; p < PD; p++)
-#pragma unroll
-            for(int cb = 0; cb < NCB; cb++)
-                wq[p][cb] = *reinterpret_cast<const h8*>(wrow + size_t(cb * 16) * KP + 32 * p);
+            for(int q = 0; q < NPC; q++)
+                acc[r][q] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for(int j = 0; j < NCH; j++)
         {
-            h8 wa[NCB];
-#pragma unroll
-            for(int cb = 0; cb < NCB; cb++)
-                wa[cb] = wq[j % PD][cb];
-            if(j + PD < NCH)
+            h8 wa;
+            if constexpr(L::WLDS)
+                wa = *reinterpret_cast<const h8*>(wl + (lane & 15) * L::WS + 32 * j + 8 * g);
+            else
             {
-#pragma unroll
-                for(int cb = 0; cb < NCB; cb++)
-                    wq[j % PD][cb] = *reinterpret_cast<const h8*>(wrow + size_t(cb * 16) * KP + 32 * (j + PD));
+                wa = wq[j % PD];
+                if(j + PD < NCH)
+                    wq[j % PD] = *reinterpret_cast<const h8*>(wrow + 32 * (j + PD));
             }
             const int k0 = 32 * j + 8 * g;
             const int tap = k0 / CIN, ci = k0 - tap * CIN;
             const bool valid = k0 < K;
             const int ky = valid ? tap / 3 : 0, kx = valid ? tap - 3 * (tap / 3) : 0;
-            const _Float16* pb = patch + (ky * 66 + pl + kx) * PS + ci;
+            const _Float16* pb = patch + (ky * 66 + pc0 * 16 + (lane & 15) + kx) * PS + ci;
+#pragma unroll
+            for(int r = 0; r < ROWS; r++)
+#pragma unroll
+                for(int q = 0; q < NPC; q++)
+                {
+                    h8 b = {0, 0, 0, 0, 0, 0, 0, 0};
+                    if(valid)
+                        b = *reinterpret_cast<const h8*>(pb + (r * 66 + q * 16) * PS);
+                    acc[r][q] = mfma32(wa, b, acc[r][q]);
+                }
+        }
+        // epilogue per 16-pixel column q: bias + ReLU rounded to FP16 (the activation),
+        // then optionally its 2x2 average pool (rows r-1, r of this wave; x and x+1 in
+        // lanes l, l^1) and the 1x1 head (COUT 16: channels of lane groups 1..3
+        // gathered into group 0)
+        _Float16* y = reinterpret_cast<_Float16*>(a.y);
+        const int co = cb * 16 + 4 * g;
+        float bias[4];
+#pragma unroll
+        for(int e = 0; e < 4; e++)
+            bias[e] = a.bias[co + e];
+#pragma unroll
+        for(int q = 0; q < NPC; q++)
+        {
+            const int x = x0 + (pc0 + q) * 16 + (lane & 15);
+            const bool xin = x < a.w_;
+            h4 prev;
 #pragma unroll
             for(int r = 0; r < ROWS; r++)
             {
-                h8 b = {0, 0, 0, 0, 0, 0, 0, 0};
-                if(valid)
-                    b = *reinterpret_cast<const h8*>(pb + r * 66 * PS);
+                const int yy = y0 + r;
+                const bool in = xin && yy < a.h;
+                const size_t p = (size_t(n) * a.h + yy) * a.w_ + x;
+                h4 o;
 #pragma unroll
-                for(int cb = 0; cb < NCB; cb++)
-                    acc[r][cb] = mfma32(wa[cb], b, acc[r][cb]);
-            }
-        }
-        // epilogue: bias + ReLU rounded to FP16 (the activation), then optionally its
-        // 2x2 average pool (rows r-1, r of this wave; x and x+1 in lanes l, l^1) and
-        // the 1x1 head (channels of lane groups 1..3 gathered into group 0)
-        const int x = x0 + pl;
-        const bool xin = x < a.w_;
-        _Float16* y = reinterpret_cast<_Float16*>(a.y);
-        h4 prev[NCB];
-#pragma unroll
-        for(int r = 0; r < ROWS; r++)
-        {
-            const int yy = y0 + r;
-            const bool in = xin && yy < a.h;
-            const size_t p = (size_t(n) * a.h + yy) * a.w_ + x;
-            h4 o[NCB];
-#pragma unroll
-            for(int cb = 0; cb < NCB; cb++)
-            {
-                const int co = cb * 16 + 4 * g;
-#pragma unroll
-                for(int q = 0; q < 4; q++)
+                for(int e = 0; e < 4; e++)
                 {
-                    const float v = acc[r][cb][q] + a.bias[co + q];
-                    o[cb][q] = _Float16(v > 0.0f ? v : 0.0f);
+                    const float v = acc[r][q][e] + bias[e];
+                    o[e] = _Float16(v > 0.0f ? v : 0.0f);
                 }
                 if(in && y)
-                    *reinterpret_cast<h4*>(y + p * COUT + co) = o[cb];
-            }
-            if((ROWS % 2) == 0 && a.pool && (r & 1))
-            {
-                const size_t p2 = (size_t(n) * (a.h / 2) + yy / 2) * (a.w_ / 2) + x / 2;
-#pragma unroll
-                for(int cb = 0; cb < NCB; cb++)
+                    *reinterpret_cast<h4*>(y + p * COUT + co) = o;
+                if((ROWS % 2) == 0 && a.pool && (r & 1))
                 {
+                    const size_t p2 = (size_t(n) * (a.h / 2) + yy / 2) * (a.w_ / 2) + x / 2;
                     h4 m;
 #pragma unroll
-                    for(int q = 0; q < 4; q++)
+                    for(int e = 0; e < 4; e++)
                     {
-                        const float v00 = float(prev[cb][q]), v10 = float(o[cb][q]);
+                        const float v00 = float(prev[e]), v10 = float(o[e]);
                         const float v01 = __shfl_xor(v00, 1), v11 = __shfl_xor(v10, 1);
-                        m[q] = _Float16(((v00 + v01) + (v10 + v11)) * 0.25f);
+                        m[e] = _Float16(((v00 + v01) + (v10 + v11)) * 0.25f);
                     }
                     if(in && (lane & 1) == 0)
-                        *reinterpret_cast<h4*>(a.pool + p2 * COUT + cb * 16 + 4 * g) = m;
+                        *reinterpret_cast<h4*>(a.pool + p2 * COUT + co) = m;
                 }
-            }
-            if constexpr(COUT == 16)
-            {
-                if(a.head)
+                if constexpr(COUT == 16)
                 {
-                    // channels 4g'..4g'+3 of this pixel from lane l + 16 g'
-                    typedef uint32_t u2 __attribute__((ext_vector_type(2)));
-                    const u2 bits = __builtin_bit_cast(u2, o[0]);
-                    h4 ch[4];
-                    ch[0] = o[0];
-#pragma unroll
-                    for(int gg = 1; gg < 4; gg++)
+                    if(a.head)
                     {
-                        u2 t;
-                        t.x = __shfl_down(bits.x, 16 * gg);
-                        t.y = __shfl_down(bits.y, 16 * gg);
-                        ch[gg] = __builtin_bit_cast(h4, t);
-                    }
-                    if(g == 0 && x < a.imgW && yy < a.imgH)
-                    {
-                        float sum = a.headW[16];
+                        // channels 4g'..4g'+3 of this pixel from lane l + 16 g'
+                        typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+                        const u2 bits = __builtin_bit_cast(u2, o);
+                        h4 ch[4];
+                        ch[0] = o;
 #pragma unroll
-                        for(int q = 0; q < 4; q++)
+                        for(int gg = 1; gg < 4; gg++)
+                        {
+                            u2 t;
+                            t.x = __shfl_down(bits.x, 16 * gg);
+                            t.y = __shfl_down(bits.y, 16 * gg);
+                            ch[gg] = __builtin_bit_cast(h4, t);
+                        }
+                        if(g == 0 && x < a.imgW && yy < a.imgH)
+                        {
+                            float sum = a.headW[16];
 #pragma unroll
-                            for(int e = 0; e < 4; e++)
-                                sum += float(ch[q][e]) * a.headW[q * 4 + e];
-                        a.head[(size_t(n) * a.imgH + yy) * a.imgW + x] = sum < 0.0f ? 0.0f : (sum > 1.0f ? 1.0f : sum);
+                            for(int c4 = 0; c4 < 4; c4++)
+#pragma unroll
+                                for(int e = 0; e < 4; e++)
+                                    sum += float(ch[c4][e]) * a.headW[c4 * 4 + e];
+                            a.head[(size_t(n) * a.imgH + yy) * a.imgW + x] = sum < 0.0f ? 0.0f : (sum > 1.0f ? 1.0f : sum);
+                        }
                     }
                 }
+                prev = o;
             }
-#pragma unroll
-            for(int cb = 0; cb < NCB; cb++)
-                prev[cb] = o[cb];
         }
     }
 
-    template<int CIN>
-    constexpr size_t patch_bytes(int rows)
-    {
-        return size_t((rows + 2) * 66) * patch_stride<CIN>() * sizeof(_Float16);
-    }
 
     template<int CIN, int COUT, int CA, int ROWS>
     hipError_t conv_fwd_launch(const ConvArgs& a, hipStream_t s)
     {
         const dim3 grid(unsigned((a.w_ + 63) / 64), unsigned((a.h + ROWS - 1) / ROWS), unsigned(a.n));
-        hipLaunchKernelGGL((conv_fwd_kernel<CIN, COUT, CA, ROWS>), grid, dim3(256), patch_bytes<CIN>(ROWS), s, a);
+        const size_t lds = FwdLds<CIN, COUT>::bytes(ROWS);
+        hipLaunchKernelGGL((conv_fwd_kernel<CIN, COUT, CA, ROWS>), grid, dim3(256), lds, s, a);
         return hipGetLastError();
     }
 
@@ -405,16 +470,16 @@ namespace isaac_hip
         // a fused pool needs both rows of each pair in one workgroup
         const bool pool = a.pool != nullptr;
         const auto groups = [&](int rows) { return int64_t((a.w_ + 63) / 64) * ((a.h + rows - 1) / rows) * a.n; };
-        if constexpr(patch_bytes<CIN>(8) <= kLdsCap)
-            if(patch_bytes<CIN>(8) <= max_lds && groups(8) >= 1024)
+        if constexpr(FwdLds<CIN, COUT>::bytes(8) <= kLdsCap)
+            if(FwdLds<CIN, COUT>::bytes(8) <= max_lds && groups(8) >= 1024)
                 return conv_fwd_launch<CIN, COUT, CA, 8>(a, s);
-        if constexpr(patch_bytes<CIN>(4) <= kLdsCap)
-            if(patch_bytes<CIN>(4) <= max_lds && groups(4) >= 1024)
+        if constexpr(FwdLds<CIN, COUT>::bytes(4) <= kLdsCap)
+            if(FwdLds<CIN, COUT>::bytes(4) <= max_lds && groups(4) >= 1024)
                 return conv_fwd_launch<CIN, COUT, CA, 4>(a, s);
-        if constexpr(patch_bytes<CIN>(2) <= kLdsCap)
-            if(patch_bytes<CIN>(2) <= max_lds && (pool || groups(2) >= 1024))
+        if constexpr(FwdLds<CIN, COUT>::bytes(2) <= kLdsCap)
+            if(FwdLds<CIN, COUT>::bytes(2) <= max_lds && (pool || groups(2) >= 1024))
                 return conv_fwd_launch<CIN, COUT, CA, 2>(a, s);
-        if(!pool && patch_bytes<CIN>(1) <= max_lds)
+        if(!pool && FwdLds<CIN, COUT>::bytes(1) <= max_lds)
             return conv_fwd_launch<CIN, COUT, CA, 1>(a, s);
         return hipErrorInvalidValue;
     }

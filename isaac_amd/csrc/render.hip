@@ -1494,42 +1494,6 @@ namespace isaac_hip
 
 
     // per brick and component: scale = max |v| over the 9^3 region (1 if all zero)
-    __global__ __launch_bounds__(256) void brick_scale_kernel(
-        const float* src,
-        int64_t sx,
-        int64_t sy,
-        int64_t sz,
-        int dim,
-        int X,
-        int Y,
-        int Z,
-        int nbx,
-        int nby,
-        float* scale)
-    {
-        __shared__ float red[256];
-        const int bc = blockIdx.x;
-        const int brick = bc / dim, c = bc % dim;
-        const int Bx = brick % nbx, By = (brick / nbx) % nby, Bz = brick / (nbx * nby);
-        float m = 0.0f;
-        for(int k = threadIdx.x; k < 729; k += 256)
-        {
-            const int lx = k % 9, ly = (k / 9) % 9, lz = k / 81;
-            const int x = min(Bx * 8 + lx, X - 1), y = min(By * 8 + ly, Y - 1), z = min(Bz * 8 + lz, Z - 1);
-            m = fmaxf(m, fabsf(src[(int64_t(x) * sx + int64_t(y) * sy + int64_t(z) * sz) * dim + c]));
-        }
-        red[threadIdx.x] = m;
-        __syncthreads();
-        for(int off = 128; off > 0; off >>= 1)
-        {
-            if(int(threadIdx.x) < off)
-                red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + off]);
-            __syncthreads();
-        }
-        if(threadIdx.x == 0)
-            scale[bc] = red[0] > 0.0f ? red[0] : 1.0f;
-    }
-
     __device__ inline uint32_t to_bf16_rne(float f)
     {
         const uint32_t u = __float_as_uint(f);
@@ -1538,43 +1502,66 @@ namespace isaac_hip
         return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
     }
 
-    template<int DT>
-    __global__ __launch_bounds__(256) void brick_encode_kernel(
+    // One workgroup per brick: the brick's 9^3 region (x, y, z clamped to the local
+    // box) of all DIM components is read once into LDS; wave c < DIM reduces the
+    // max |v| of component c (the brick scale: 1 when the region is all zero or the
+    // form is FP32); then every thread writes pairs (v[x], v[x+1]) of the region,
+    // divided by the scale and rounded for 16-bit forms.  Output element
+    // (brick*648 + pair)*DIM + c, pair = (lz*9 + ly)*8 + lx.
+    template<int DT, int DIM>
+    __global__ __launch_bounds__(256) void brick_build_kernel(
         const float* src,
         int64_t sx,
         int64_t sy,
         int64_t sz,
-        int dim,
         int X,
         int Y,
         int Z,
         int nbx,
         int nby,
-        int64_t nbricks,
-        const float* scale,
+        float* scale,
         void* out)
     {
-        // element i = (brick*648 + pair)*dim + component
-        const int64_t total = nbricks * dim * kBrickPairs;
-        for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += int64_t(gridDim.x) * blockDim.x)
+        __shared__ float reg[729 * DIM];
+        __shared__ float sc[DIM];
+        const int64_t brick = blockIdx.x;
+        const int Bx = int(brick % nbx), By = int((brick / nbx) % nby), Bz = int(brick / (int64_t(nbx) * nby));
+        for(int i = threadIdx.x; i < 729 * DIM; i += 256)
         {
-            const int64_t row = i / dim;
-            const int c = int(i - row * dim);
-            const int64_t brick = row / kBrickPairs;
-            const int pi = int(row - brick * kBrickPairs);
-            const int64_t bc = brick * dim + c;
-            const int lx = pi % 8, ly = (pi / 8) % 9, lz = pi / 72;
-            const int Bx = int(brick % nbx), By = int((brick / nbx) % nby), Bz = int(brick / (int64_t(nbx) * nby));
-            const int x0 = min(Bx * 8 + lx, X - 1), x1 = min(Bx * 8 + lx + 1, X - 1);
-            const int y = min(By * 8 + ly, Y - 1), z = min(Bz * 8 + lz, Z - 1);
-            const int64_t yz = int64_t(y) * sy + int64_t(z) * sz;
-            const float v0 = src[(int64_t(x0) * sx + yz) * dim + c];
-            const float v1 = src[(int64_t(x1) * sx + yz) * dim + c];
+            const int vox = i / DIM, c = i - vox * DIM;
+            const int lz = vox / 81, ly = (vox / 9) % 9, lx = vox % 9;
+            const int x = min(Bx * 8 + lx, X - 1), y = min(By * 8 + ly, Y - 1), z = min(Bz * 8 + lz, Z - 1);
+            reg[i] = src[(int64_t(x) * sx + int64_t(y) * sy + int64_t(z) * sz) * DIM + c];
+        }
+        __syncthreads();
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        if(wave < DIM)
+        {
+            float m = 0.0f;
+            if(DT != kDtF32)
+                for(int vox = lane; vox < 729; vox += 64)
+                    m = fmaxf(m, fabsf(reg[vox * DIM + wave]));
+            for(int off = 32; off > 0; off >>= 1)
+                m = fmaxf(m, __shfl_xor(m, off));
+            const float s = m > 0.0f ? m : 1.0f;
+            if(lane == 0)
+            {
+                sc[wave] = s;
+                scale[brick * DIM + wave] = s;
+            }
+        }
+        __syncthreads();
+        for(int e = threadIdx.x; e < kBrickPairs * DIM; e += 256)
+        {
+            const int pair = e / DIM, c = e - pair * DIM;
+            const int vox0 = (pair >> 3) * 9 + (pair & 7);
+            const float v0 = reg[vox0 * DIM + c], v1 = reg[(vox0 + 1) * DIM + c];
+            const int64_t i = brick * (kBrickPairs * DIM) + e;
             if(DT == kDtF32)
                 reinterpret_cast<float2*>(out)[i] = make_float2(v0, v1);
             else
             {
-                const float s = scale[bc];
+                const float s = sc[c];
                 const float q0 = v0 / s, q1 = v1 / s;
                 uint32_t w;
                 if(DT == kDtF16)
@@ -1584,6 +1571,23 @@ namespace isaac_hip
                 reinterpret_cast<uint32_t*>(out)[i] = w;
             }
         }
+    }
+
+    template<int DT>
+    hipError_t launch_brick_build(
+        const float* src, const int64_t stride[3], int dim, int X, int Y, int Z, int nbx, int nby, int64_t nbricks, float* scale,
+        void* out, hipStream_t s)
+    {
+        const dim3 grid = dim3(static_cast<unsigned>(nbricks)), block = dim3(256);
+        switch(dim)
+        {
+        case 1: hipLaunchKernelGGL((brick_build_kernel<DT, 1>), grid, block, 0, s, src, stride[0], stride[1], stride[2], X, Y, Z, nbx, nby, scale, out); break;
+        case 2: hipLaunchKernelGGL((brick_build_kernel<DT, 2>), grid, block, 0, s, src, stride[0], stride[1], stride[2], X, Y, Z, nbx, nby, scale, out); break;
+        case 3: hipLaunchKernelGGL((brick_build_kernel<DT, 3>), grid, block, 0, s, src, stride[0], stride[1], stride[2], X, Y, Z, nbx, nby, scale, out); break;
+        case 4: hipLaunchKernelGGL((brick_build_kernel<DT, 4>), grid, block, 0, s, src, stride[0], stride[1], stride[2], X, Y, Z, nbx, nby, scale, out); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
     }
 
     hipError_t launch_bricks(
@@ -1600,28 +1604,15 @@ namespace isaac_hip
     {
         const int nbx = (X + 7) / 8, nby = (Y + 7) / 8, nbz = (Z + 7) / 8;
         const int64_t nbricks = int64_t(nbx) * nby * nbz;
+        if(nbricks == 0)
+            return hipSuccess;
+        if(nbricks > int64_t(0x7fffffff))
+            return hipErrorInvalidValue;
         if(dtype == kDtF32)
-        {
-            const hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(scale), 0x3f800000u, size_t(nbricks * dim), s);
-            if(e != hipSuccess)
-                return e;
-        }
-        else
-            hipLaunchKernelGGL(brick_scale_kernel, dim3(unsigned(nbricks * dim)), dim3(256), 0, s, src, stride[0], stride[1],
-                               stride[2], dim, X, Y, Z, nbx, nby, scale);
-        int64_t blocks = (nbricks * dim * kBrickPairs + 255) / 256;
-        if(blocks > 16384)
-            blocks = 16384;
-        if(dtype == kDtF32)
-            hipLaunchKernelGGL((brick_encode_kernel<kDtF32>), dim3(unsigned(blocks)), dim3(256), 0, s, src, stride[0],
-                               stride[1], stride[2], dim, X, Y, Z, nbx, nby, nbricks, scale, out);
-        else if(dtype == kDtF16)
-            hipLaunchKernelGGL((brick_encode_kernel<kDtF16>), dim3(unsigned(blocks)), dim3(256), 0, s, src, stride[0],
-                               stride[1], stride[2], dim, X, Y, Z, nbx, nby, nbricks, scale, out);
-        else
-            hipLaunchKernelGGL((brick_encode_kernel<kDtBf16>), dim3(unsigned(blocks)), dim3(256), 0, s, src, stride[0],
-                               stride[1], stride[2], dim, X, Y, Z, nbx, nby, nbricks, scale, out);
-        return hipGetLastError();
+            return launch_brick_build<kDtF32>(src, stride, dim, X, Y, Z, nbx, nby, nbricks, scale, out, s);
+        if(dtype == kDtF16)
+            return launch_brick_build<kDtF16>(src, stride, dim, X, Y, Z, nbx, nby, nbricks, scale, out, s);
+        return launch_brick_build<kDtBf16>(src, stride, dim, X, Y, Z, nbx, nby, nbricks, scale, out, s);
     }
 
     static_assert(sizeof(FrameParams) + sizeof(PostParams) + sizeof(ParticleParams) <= 4096, "kernel argument block");
