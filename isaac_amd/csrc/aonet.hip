@@ -192,119 +192,145 @@ namespace isaac_hip
         return ((CIN / 8) % 2) ? CIN : CIN + 8;
     }
 
-    // A thread owns one 16-byte channel unit of one patch column (of RU = 66*CIN/8
-    // per row) and walks rows; when a row has fewer than 256 units, G = 256/RU
-    // thread groups split the rows.  The column's address is formed once, each row
-    // adds a stride; all loads are issued before the first LDS write, so the
-    // workgroup waits for one global-load latency.
+    // Patch staging in two halves, global loads into registers and LDS stores, so
+    // a persistent workgroup loads tile t+1 while it computes tile t.  A thread
+    // owns one 16-byte channel unit of one patch column (of RU = 66*CIN/8 per row)
+    // and walks rows; when a row has fewer than 256 units, G = 256/RU thread groups
+    // split the rows.
     template<int CIN, int CA, int PROWS>
-    __device__ inline void stage_patch8(
-        _Float16* patch,
-        const _Float16* __restrict__ x,
-        const _Float16* __restrict__ up,
-        int n,
-        int h,
-        int w,
-        int y0,
-        int x0)
+    struct PatchStager
     {
-        constexpr int CS = CIN - CA, U = CIN / 8, PS = patch_stride<CIN>(), RU = 66 * U;
-        constexpr int G = RU >= 256 ? 1 : 256 / RU; // row groups
-        constexpr int CI = (RU + 255) / 256; // unit iterations per thread
-        constexpr int RPG = (PROWS + G - 1) / G; // rows per group
-        const int t = int(threadIdx.x);
-        const int grp = RU >= 256 ? 0 : t / RU;
-        const int u0 = RU >= 256 ? t : t - grp * RU;
-        h8 v[CI][RPG];
+        static constexpr int CS = CIN - CA, U = CIN / 8, PS = patch_stride<CIN>(), RU = 66 * U;
+        static constexpr int G = RU >= 256 ? 1 : 256 / RU; // row groups
+        static constexpr int CI = (RU + 255) / 256; // unit iterations per thread
+        static constexpr int RPG = (PROWS + G - 1) / G; // rows per group
+        int grp;
         int col[CI], ch[CI];
         bool uok[CI];
-#pragma unroll
-        for(int ci = 0; ci < CI; ci++)
+        h8 v[CI][RPG];
+
+        __device__ inline void init()
         {
-            const int u = u0 + ci * 256;
-            uok[ci] = (RU >= 256) ? (u < RU) : (grp < G);
-            col[ci] = u / U;
-            ch[ci] = (u - col[ci] * U) * 8;
-            const int xx = x0 + col[ci] - 1;
-            const bool xin = uok[ci] && xx >= 0 && xx < w;
-            const bool fromUp = CA > 0 && ch[ci] < CA;
-            const _Float16* base;
-            size_t rs;
-            if(fromUp)
-            {
-                base = up + (size_t(n) * size_t(h / 2) * size_t(w / 2) + size_t(xx >= 0 ? xx / 2 : 0)) * CA + ch[ci];
-                rs = size_t(w / 2) * CA;
-            }
-            else
-            {
-                base = x + (size_t(n) * size_t(h) * size_t(w) + size_t(xx >= 0 ? xx : 0)) * CS + (ch[ci] - CA);
-                rs = size_t(w) * CS;
-            }
+            const int t = int(threadIdx.x);
+            grp = RU >= 256 ? 0 : t / RU;
+            const int u0 = RU >= 256 ? t : t - grp * RU;
 #pragma unroll
-            for(int k = 0; k < RPG; k++)
+            for(int ci = 0; ci < CI; ci++)
             {
-                const int r = grp + k * G;
-                const int yy = y0 + r - 1;
-                v[ci][k] = h8{0, 0, 0, 0, 0, 0, 0, 0};
-                if(xin && r < PROWS && yy >= 0 && yy < h)
-                    v[ci][k] = *reinterpret_cast<const h8*>(base + size_t(fromUp ? yy / 2 : yy) * rs);
+                const int u = u0 + ci * 256;
+                uok[ci] = (RU >= 256) ? (u < RU) : (grp < G);
+                col[ci] = u / U;
+                ch[ci] = (u - col[ci] * U) * 8;
             }
         }
-#pragma unroll
-        for(int ci = 0; ci < CI; ci++)
-#pragma unroll
-            for(int k = 0; k < RPG; k++)
-            {
-                const int r = grp + k * G;
-                if(uok[ci] && r < PROWS)
-                    *reinterpret_cast<h8*>(patch + (r * 66 + col[ci]) * PS + ch[ci]) = v[ci][k];
-            }
-    }
 
-    // LDS layout of a forward layer: the input patch, then (16-channel layers) the
-    // weights, one row per output channel padded to an odd number of 16-byte units
+        __device__ inline void load(const _Float16* __restrict__ x, const _Float16* __restrict__ up, int n, int h, int w, int y0, int x0)
+        {
+#pragma unroll
+            for(int ci = 0; ci < CI; ci++)
+            {
+                const int xx = x0 + col[ci] - 1;
+                const bool xin = uok[ci] && xx >= 0 && xx < w;
+                const bool fromUp = CA > 0 && ch[ci] < CA;
+                const _Float16* base;
+                size_t rs;
+                if(fromUp)
+                {
+                    base = up + (size_t(n) * size_t(h / 2) * size_t(w / 2) + size_t(xx >= 0 ? xx / 2 : 0)) * CA + ch[ci];
+                    rs = size_t(w / 2) * CA;
+                }
+                else
+                {
+                    base = x + (size_t(n) * size_t(h) * size_t(w) + size_t(xx >= 0 ? xx : 0)) * CS + (ch[ci] - CA);
+                    rs = size_t(w) * CS;
+                }
+#pragma unroll
+                for(int k = 0; k < RPG; k++)
+                {
+                    const int r = grp + k * G;
+                    const int yy = y0 + r - 1;
+                    v[ci][k] = h8{0, 0, 0, 0, 0, 0, 0, 0};
+                    if(xin && r < PROWS && yy >= 0 && yy < h)
+                        v[ci][k] = *reinterpret_cast<const h8*>(base + size_t(fromUp ? yy / 2 : yy) * rs);
+                }
+            }
+        }
+
+        __device__ inline void store(_Float16* patch) const
+        {
+#pragma unroll
+            for(int ci = 0; ci < CI; ci++)
+#pragma unroll
+                for(int k = 0; k < RPG; k++)
+                {
+                    const int r = grp + k * G;
+                    if(uok[ci] && r < PROWS)
+                        *reinterpret_cast<h8*>(patch + (r * 66 + col[ci]) * PS + ch[ci]) = v[ci][k];
+                }
+        }
+    };
+
+    // LDS of a forward layer: the weights when all of them fit beside a one-row
+    // patch in the 160 KiB of a CU (one row of KP + 8 halves per output channel: an
+    // odd number of 16-byte units, so the 16 rows of a fragment read hit 16 bank
+    // groups), then the (ROWS + 2) x 66-pixel input patch
     template<int CIN, int COUT>
     struct FwdLds
     {
         static constexpr int KP = (9 * CIN + 31) / 32 * 32;
-        static constexpr bool WLDS = COUT == 16;
-        static constexpr int WS = KP + 8; // weight row stride (halves)
+        static constexpr int WS = KP + 8;
+        static constexpr size_t patch(int rows)
+        {
+            return size_t((rows + 2) * 66) * patch_stride<CIN>() * sizeof(_Float16);
+        }
+        static constexpr size_t weights = size_t(COUT) * WS * sizeof(_Float16);
+        static constexpr bool WLDS = weights + patch(1) <= size_t(160) * 1024;
         static constexpr size_t bytes(int rows)
         {
-            return size_t((rows + 2) * 66) * patch_stride<CIN>() * sizeof(_Float16)
-                + (WLDS ? size_t(16) * WS * sizeof(_Float16) : 0);
+            return patch(rows) + (WLDS ? weights : 0);
         }
     };
 
-    // Workgroup = ROWS rows x 64 pixels.  The four waves split the 16-channel output
-    // blocks first (COUT 64: one block per wave; COUT 32: two blocks x two 32-pixel
-    // halves; COUT 16: four 16-pixel columns), so a workgroup fetches each weight
-    // once: a wave owns one block in NPC 16-pixel columns x ROWS rows, and each
-    // weight fragment feeds NPC * ROWS MFMAs.  16-channel layers read their weights
-    // from LDS (staged with the patch), wider ones stream them from global memory
+    // Persistent forward layer: workgroup b computes tiles b, b + grid, ... of ROWS
+    // rows x 64 pixels.  The four waves split the 16-channel output blocks first
+    // (COUT 64: one block per wave; COUT 32: two blocks x two 32-pixel halves;
+    // COUT 16: four 16-pixel columns): a wave owns one block in NPC 16-pixel columns
+    // x ROWS rows, so each weight fragment feeds NPC * ROWS MFMAs.  Weights come from
+    // LDS (loaded once per workgroup) or, when they do not fit, from global memory
     // PD chunks ahead of the MFMAs.
     template<int CIN, int COUT, int CA, int ROWS>
-    __global__ __launch_bounds__(256) void conv_fwd_kernel(const ConvArgs a)
+    __global__ __launch_bounds__(256) void conv_fwd_kernel(const ConvArgs a, int tilesX, int tilesY, int numTiles)
     {
-        constexpr int K = 9 * CIN, KP = (K + 31) / 32 * 32, NCH = KP / 32, NCB = COUT / 16;
+        using L = FwdLds<CIN, COUT>;
+        constexpr int K = 9 * CIN, KP = L::KP, NCH = KP / 32, NCB = COUT / 16;
         constexpr int PS = patch_stride<CIN>();
         constexpr int WCB = NCB < 4 ? NCB : 4; // waves along output-channel blocks
-        constexpr int NPC = 4 / (4 / WCB); // 16-pixel columns per wave
+        constexpr int NPC = WCB; // 16-pixel columns per wave
         static_assert(NCB == WCB, "one output-channel block per wave");
-        using L = FwdLds<CIN, COUT>;
-        extern __shared__ _Float16 patch[];
-        _Float16* wl = patch + (ROWS + 2) * 66 * PS;
-        const int x0 = blockIdx.x * 64, y0 = blockIdx.y * ROWS, n = blockIdx.z;
+        constexpr bool WLDS = L::WLDS;
+        constexpr int PD = WLDS ? 1 : (NCH < 8 ? NCH : 8);
+        extern __shared__ _Float16 lds[];
+        _Float16* wl = lds;
+        _Float16* patch = lds + (WLDS ? COUT * L::WS : 0);
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
         const int cb = wave % WCB, pc0 = (wave / WCB) * NPC;
         const int g = lane >> 4;
-        constexpr int PD = L::WLDS ? 1 : (NCH < 8 ? NCH : 8);
-        h8 wq[PD];
-        const _Float16* wrow = a.w + size_t(cb * 16 + (lane & 15)) * KP + 8 * g;
-        if constexpr(L::WLDS)
+        const int tpi = tilesX * tilesY;
+        auto coords = [&](int t, int& n, int& y0, int& x0) {
+            n = t / tpi;
+            const int r = t - n * tpi;
+            const int by = r / tilesX;
+            y0 = by * ROWS;
+            x0 = (r - by * tilesX) * 64;
+        };
+        PatchStager<CIN, CA, ROWS + 2> st;
+        st.init();
+        int t = int(blockIdx.x), n, y0, x0;
+        coords(t, n, y0, x0);
+        if constexpr(WLDS)
         {
-            // the 16 x KP weights into LDS (issued before the patch loads)
-            constexpr int WU = 16 * KP / 8, WIT = (WU + 255) / 256;
+            // all weights into LDS once (requested together with the first patch)
+            constexpr int WU = COUT * KP / 8, WIT = (WU + 255) / 256;
             h8 wv[WIT];
 #pragma unroll
             for(int it = 0; it < WIT; it++)
@@ -313,7 +339,7 @@ namespace isaac_hip
                 if(WU % 256 == 0 || u < WU)
                     wv[it] = *reinterpret_cast<const h8*>(a.w + size_t(u) * 8);
             }
-            stage_patch8<CIN, CA, ROWS + 2>(patch, a.x, a.up, n, a.h, a.w_, y0, x0);
+            st.load(a.x, a.up, n, a.h, a.w_, y0, x0);
 #pragma unroll
             for(int it = 0; it < WIT; it++)
             {
@@ -326,161 +352,214 @@ namespace isaac_hip
             }
         }
         else
-        {
-            // the first PD chunks are requested before the patch, so both latencies overlap
-#pragma unroll
-            for(int p = 0; p < PD; p++)
-                wq[p] = *reinterpret_cast<const h8*>(wrow + 32 * p);
-            stage_patch8<CIN, CA, ROWS + 2>(patch, a.x, a.up, n, a.h, a.w_, y0, x0);
-        }
-        __syncthreads();
-        f4 acc[ROWS][NPC];
-#pragma unroll
-        for(int r = 0; r < ROWS; r++)
-#pragma unroll
-            for(int q = 0; q < NPC; q++)
-                acc[r][q] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for(int j = 0; j < NCH; j++)
-        {
-            h8 wa;
-            if constexpr(L::WLDS)
-                wa = *reinterpret_cast<const h8*>(wl + (lane & 15) * L::WS + 32 * j + 8 * g);
-            else
-            {
-                wa = wq[j % PD];
-                if(j + PD < NCH)
-                    wq[j % PD] = *reinterpret_cast<const h8*>(wrow + 32 * (j + PD));
-            }
-            const int k0 = 32 * j + 8 * g;
-            const int tap = k0 / CIN, ci = k0 - tap * CIN;
-            const bool valid = k0 < K;
-            const int ky = valid ? tap / 3 : 0, kx = valid ? tap - 3 * (tap / 3) : 0;
-            const _Float16* pb = patch + (ky * 66 + pc0 * 16 + (lane & 15) + kx) * PS + ci;
-#pragma unroll
-            for(int r = 0; r < ROWS; r++)
-#pragma unroll
-                for(int q = 0; q < NPC; q++)
-                {
-                    h8 b = {0, 0, 0, 0, 0, 0, 0, 0};
-                    if(valid)
-                        b = *reinterpret_cast<const h8*>(pb + (r * 66 + q * 16) * PS);
-                    acc[r][q] = mfma32(wa, b, acc[r][q]);
-                }
-        }
-        // epilogue per 16-pixel column q: bias + ReLU rounded to FP16 (the activation),
-        // then optionally its 2x2 average pool (rows r-1, r of this wave; x and x+1 in
-        // lanes l, l^1) and the 1x1 head (COUT 16: channels of lane groups 1..3
-        // gathered into group 0)
+            st.load(a.x, a.up, n, a.h, a.w_, y0, x0);
+        const _Float16* wrow = a.w + size_t(cb * 16 + (lane & 15)) * KP + 8 * g;
         _Float16* y = reinterpret_cast<_Float16*>(a.y);
         const int co = cb * 16 + 4 * g;
         float bias[4];
 #pragma unroll
         for(int e = 0; e < 4; e++)
             bias[e] = a.bias[co + e];
-#pragma unroll
-        for(int q = 0; q < NPC; q++)
+        for(; t < numTiles; t += int(gridDim.x))
         {
-            const int x = x0 + (pc0 + q) * 16 + (lane & 15);
-            const bool xin = x < a.w_;
-            h4 prev;
+            st.store(patch);
+            __syncthreads();
+            const int cn = n, cy0 = y0, cx0 = x0;
+            if(t + int(gridDim.x) < numTiles)
+            {
+                coords(t + int(gridDim.x), n, y0, x0);
+                st.load(a.x, a.up, n, a.h, a.w_, y0, x0);
+            }
+            h8 wq[PD];
+            if constexpr(!WLDS)
+            {
+#pragma unroll
+                for(int p = 0; p < PD; p++)
+                    wq[p] = *reinterpret_cast<const h8*>(wrow + 32 * p);
+            }
+            f4 acc[ROWS][NPC];
 #pragma unroll
             for(int r = 0; r < ROWS; r++)
-            {
-                const int yy = y0 + r;
-                const bool in = xin && yy < a.h;
-                const size_t p = (size_t(n) * a.h + yy) * a.w_ + x;
-                h4 o;
 #pragma unroll
-                for(int e = 0; e < 4; e++)
+                for(int q = 0; q < NPC; q++)
+                    acc[r][q] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for(int j = 0; j < NCH; j++)
+            {
+                h8 wa;
+                if constexpr(WLDS)
+                    wa = *reinterpret_cast<const h8*>(wl + (cb * 16 + (lane & 15)) * L::WS + 32 * j + 8 * g);
+                else
                 {
-                    const float v = acc[r][q][e] + bias[e];
-                    o[e] = _Float16(v > 0.0f ? v : 0.0f);
+                    wa = wq[j % PD];
+                    if(j + PD < NCH)
+                        wq[j % PD] = *reinterpret_cast<const h8*>(wrow + 32 * (j + PD));
                 }
-                if(in && y)
-                    *reinterpret_cast<h4*>(y + p * COUT + co) = o;
-                if((ROWS % 2) == 0 && a.pool && (r & 1))
+                const int k0 = 32 * j + 8 * g;
+                const int tap = k0 / CIN, ci = k0 - tap * CIN;
+                const bool valid = k0 < K;
+                const int ky = valid ? tap / 3 : 0, kx = valid ? tap - 3 * (tap / 3) : 0;
+                const _Float16* pb = patch + (ky * 66 + pc0 * 16 + (lane & 15) + kx) * PS + ci;
+#pragma unroll
+                for(int r = 0; r < ROWS; r++)
+#pragma unroll
+                    for(int q = 0; q < NPC; q++)
+                    {
+                        h8 b = {0, 0, 0, 0, 0, 0, 0, 0};
+                        if(valid)
+                            b = *reinterpret_cast<const h8*>(pb + (r * 66 + q * 16) * PS);
+                        acc[r][q] = mfma32(wa, b, acc[r][q]);
+                    }
+            }
+            // epilogue per 16-pixel column q: bias + ReLU rounded to FP16 (the
+            // activation), then optionally its 2x2 average pool (rows r-1, r of this
+            // wave; x and x+1 in lanes l, l^1) and the 1x1 head (COUT 16: channels of
+            // lane groups 1..3 gathered into group 0)
+#pragma unroll
+            for(int q = 0; q < NPC; q++)
+            {
+                const int x = cx0 + (pc0 + q) * 16 + (lane & 15);
+                const bool xin = x < a.w_;
+                h4 prev;
+#pragma unroll
+                for(int r = 0; r < ROWS; r++)
                 {
-                    const size_t p2 = (size_t(n) * (a.h / 2) + yy / 2) * (a.w_ / 2) + x / 2;
-                    h4 m;
+                    const int yy = cy0 + r;
+                    const bool in = xin && yy < a.h;
+                    const size_t p = (size_t(cn) * a.h + yy) * a.w_ + x;
+                    h4 o;
 #pragma unroll
                     for(int e = 0; e < 4; e++)
                     {
-                        const float v00 = float(prev[e]), v10 = float(o[e]);
-                        const float v01 = __shfl_xor(v00, 1), v11 = __shfl_xor(v10, 1);
-                        m[e] = _Float16(((v00 + v01) + (v10 + v11)) * 0.25f);
+                        const float v = acc[r][q][e] + bias[e];
+                        o[e] = _Float16(v > 0.0f ? v : 0.0f);
                     }
-                    if(in && (lane & 1) == 0)
-                        *reinterpret_cast<h4*>(a.pool + p2 * COUT + co) = m;
-                }
-                if constexpr(COUT == 16)
-                {
-                    if(a.head)
+                    if(in && y)
+                        *reinterpret_cast<h4*>(y + p * COUT + co) = o;
+                    if((ROWS % 2) == 0 && a.pool && (r & 1))
                     {
-                        // channels 4g'..4g'+3 of this pixel from lane l + 16 g'
-                        typedef uint32_t u2 __attribute__((ext_vector_type(2)));
-                        const u2 bits = __builtin_bit_cast(u2, o);
-                        h4 ch[4];
-                        ch[0] = o;
+                        const size_t p2 = (size_t(cn) * (a.h / 2) + yy / 2) * (a.w_ / 2) + x / 2;
+                        h4 m;
 #pragma unroll
-                        for(int gg = 1; gg < 4; gg++)
+                        for(int e = 0; e < 4; e++)
                         {
-                            u2 t;
-                            t.x = __shfl_down(bits.x, 16 * gg);
-                            t.y = __shfl_down(bits.y, 16 * gg);
-                            ch[gg] = __builtin_bit_cast(h4, t);
+                            const float v00 = float(prev[e]), v10 = float(o[e]);
+                            const float v01 = __shfl_xor(v00, 1), v11 = __shfl_xor(v10, 1);
+                            m[e] = _Float16(((v00 + v01) + (v10 + v11)) * 0.25f);
                         }
-                        if(g == 0 && x < a.imgW && yy < a.imgH)
+                        if(in && (lane & 1) == 0)
+                            *reinterpret_cast<h4*>(a.pool + p2 * COUT + co) = m;
+                    }
+                    if constexpr(COUT == 16)
+                    {
+                        if(a.head)
                         {
-                            float sum = a.headW[16];
+                            // channels 4g'..4g'+3 of this pixel from lane l + 16 g'
+                            typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+                            const u2 bits = __builtin_bit_cast(u2, o);
+                            h4 chv[4];
+                            chv[0] = o;
 #pragma unroll
-                            for(int c4 = 0; c4 < 4; c4++)
+                            for(int gg = 1; gg < 4; gg++)
+                            {
+                                u2 tt;
+                                tt.x = __shfl_down(bits.x, 16 * gg);
+                                tt.y = __shfl_down(bits.y, 16 * gg);
+                                chv[gg] = __builtin_bit_cast(h4, tt);
+                            }
+                            if(g == 0 && x < a.imgW && yy < a.imgH)
+                            {
+                                float sum = a.headW[16];
 #pragma unroll
-                                for(int e = 0; e < 4; e++)
-                                    sum += float(ch[c4][e]) * a.headW[c4 * 4 + e];
-                            a.head[(size_t(n) * a.imgH + yy) * a.imgW + x] = sum < 0.0f ? 0.0f : (sum > 1.0f ? 1.0f : sum);
+                                for(int c4 = 0; c4 < 4; c4++)
+#pragma unroll
+                                    for(int e = 0; e < 4; e++)
+                                        sum += float(chv[c4][e]) * a.headW[c4 * 4 + e];
+                                a.head[(size_t(cn) * a.imgH + yy) * a.imgW + x] = sum < 0.0f ? 0.0f : (sum > 1.0f ? 1.0f : sum);
+                            }
                         }
                     }
+                    prev = o;
                 }
-                prev = o;
             }
+            __syncthreads();
         }
     }
 
+    // compute units of the current device (cached per device index)
+    static int device_cus()
+    {
+        static int cus[16] = {};
+        int dev = 0;
+        if(hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16)
+            return 256;
+        if(cus[dev] == 0)
+        {
+            int v = 0;
+            if(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+                v = 256;
+            cus[dev] = v;
+        }
+        return cus[dev];
+    }
 
     template<int CIN, int COUT, int CA, int ROWS>
     hipError_t conv_fwd_launch(const ConvArgs& a, hipStream_t s)
     {
-        const dim3 grid(unsigned((a.w_ + 63) / 64), unsigned((a.h + ROWS - 1) / ROWS), unsigned(a.n));
+        const int tilesX = (a.w_ + 63) / 64, tilesY = (a.h + ROWS - 1) / ROWS;
+        const int64_t numTiles = int64_t(tilesX) * tilesY * a.n;
+        if(numTiles > int64_t(0x7fffffff))
+            return hipErrorInvalidValue;
         const size_t lds = FwdLds<CIN, COUT>::bytes(ROWS);
-        hipLaunchKernelGGL((conv_fwd_kernel<CIN, COUT, CA, ROWS>), grid, dim3(256), lds, s, a);
+        static int perCu = 0; // resident workgroups per CU (same for every gfx950 device)
+        if(perCu == 0)
+        {
+            int v = 0;
+            if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, conv_fwd_kernel<CIN, COUT, CA, ROWS>, 256, lds) != hipSuccess || v <= 0)
+                v = 1;
+            perCu = v;
+        }
+        const int64_t grid = std::min<int64_t>(numTiles, int64_t(device_cus()) * perCu);
+        hipLaunchKernelGGL((conv_fwd_kernel<CIN, COUT, CA, ROWS>), dim3(unsigned(grid)), dim3(256), lds, s, a, tilesX, tilesY,
+                           int(numTiles));
         return hipGetLastError();
     }
 
-    // rows per workgroup: the most (of 8, 4, 2, 1) whose patch fits max_lds and whose
-    // grid still has >= 1024 workgroups; else the fewest that fits (variants whose
-    // patch exceeds 64 KiB are not built)
+    // rows per tile: the most (of 8, 4, 2, 1) whose LDS leaves room for two
+    // workgroups per CU and whose tiles cover every CU twice; else the most that fit
+    // with a tile per CU; else the fewest that fit (a fused pool needs an even count).
+    // A wave keeps at most 8 accumulator tiles (rows x its 16-pixel columns).
     template<int CIN, int COUT, int CA>
     hipError_t conv_fwd_rows(const ConvArgs& a, size_t max_lds, hipStream_t s)
     {
-        constexpr int KP = (9 * CIN + 31) / 32 * 32;
-        constexpr size_t kLdsCap = 64 * 1024;
-        if(a.cin != CIN || a.ca != CA || a.kp != KP || (a.head && COUT != 16))
+        using L = FwdLds<CIN, COUT>;
+        constexpr int NPC = COUT / 16 < 4 ? COUT / 16 : 4;
+        constexpr auto fits = [](int rows) { return L::bytes(rows) <= size_t(160) * 1024 && rows * NPC <= 8; };
+        if(a.cin != CIN || a.ca != CA || a.kp != L::KP || (a.head && COUT != 16))
             return hipErrorInvalidValue;
-        // a fused pool needs both rows of each pair in one workgroup
         const bool pool = a.pool != nullptr;
-        const auto groups = [&](int rows) { return int64_t((a.w_ + 63) / 64) * ((a.h + rows - 1) / rows) * a.n; };
-        if constexpr(FwdLds<CIN, COUT>::bytes(8) <= kLdsCap)
-            if(FwdLds<CIN, COUT>::bytes(8) <= max_lds && groups(8) >= 1024)
-                return conv_fwd_launch<CIN, COUT, CA, 8>(a, s);
-        if constexpr(FwdLds<CIN, COUT>::bytes(4) <= kLdsCap)
-            if(FwdLds<CIN, COUT>::bytes(4) <= max_lds && groups(4) >= 1024)
-                return conv_fwd_launch<CIN, COUT, CA, 4>(a, s);
-        if constexpr(FwdLds<CIN, COUT>::bytes(2) <= kLdsCap)
-            if(FwdLds<CIN, COUT>::bytes(2) <= max_lds && (pool || groups(2) >= 1024))
-                return conv_fwd_launch<CIN, COUT, CA, 2>(a, s);
-        if(!pool && FwdLds<CIN, COUT>::bytes(1) <= max_lds)
-            return conv_fwd_launch<CIN, COUT, CA, 1>(a, s);
+        const int64_t cus = device_cus();
+        const auto tiles = [&](int rows) { return int64_t((a.w_ + 63) / 64) * ((a.h + rows - 1) / rows) * a.n; };
+        const auto ok = [&](int rows, size_t lim, int64_t minTiles) {
+            return L::bytes(rows) <= lim && tiles(rows) >= minTiles && (!pool || rows % 2 == 0);
+        };
+        for(int pass = 0; pass < 3; pass++)
+        {
+            const size_t lim = pass == 0 ? max_lds / 2 : max_lds;
+            const int64_t minTiles = pass == 0 ? 2 * cus : (pass == 1 ? cus : 0);
+            if constexpr(fits(8))
+                if(pass < 2 && ok(8, lim, minTiles))
+                    return conv_fwd_launch<CIN, COUT, CA, 8>(a, s);
+            if constexpr(fits(4))
+                if(pass < 2 && ok(4, lim, minTiles))
+                    return conv_fwd_launch<CIN, COUT, CA, 4>(a, s);
+            if constexpr(fits(2))
+                if(ok(2, lim, minTiles) && (pass < 2 || !ok(1, lim, 0)))
+                    return conv_fwd_launch<CIN, COUT, CA, 2>(a, s);
+            if constexpr(fits(1))
+                if(ok(1, lim, minTiles))
+                    return conv_fwd_launch<CIN, COUT, CA, 1>(a, s);
+        }
         return hipErrorInvalidValue;
     }
 

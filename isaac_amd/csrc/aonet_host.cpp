@@ -179,9 +179,9 @@ namespace isaac_hip
             net->headOff = off;
             net->nparams = off + kHeadIn + 1;
             int lds = 0;
-            // dynamic LDS beyond 64 KiB per workgroup is not requested
+            // the device's LDS per workgroup (160 KiB on gfx950), at most 160 KiB
             if(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) == hipSuccess && lds > 0)
-                net->maxLds = std::min<size_t>(size_t(lds), 64 * 1024);
+                net->maxLds = std::min<size_t>(size_t(lds), 160 * 1024);
             hipError_t e = hipSuccess;
             auto chk = [&](hipError_t r) {
                 if(e == hipSuccess)
