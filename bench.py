@@ -190,6 +190,13 @@ def learned_ao(vis, poses, out, args):
     }
     if "bricks-f16/learned" in timing:
         res["speedup_learned_f16_vs_plain_f32"] = timing["combined/plain"]["ms_per_step"] / timing["bricks-f16/learned"]["ms_per_step"]
+    if "combined/learned" in timing:
+        # the network's dense roofline: FP16 MFMA, 2.5 PFLOP/s (MI355X dense peak)
+        W, H = vis.framebuffer_size
+        gf = aonet.flops_per_frame(W, H) / 1e9
+        tf = gf / timing["combined/learned"]["occlusion_ms"]
+        res["network"] = {"gflop_per_frame": gf, "tflops": tf, "bound": "mfma", "peak_tflops": 2500.0,
+                          "frac": tf / 2500.0}
     return res
 
 

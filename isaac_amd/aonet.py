@@ -31,8 +31,20 @@ LAYERS = [
 HEAD_IN = 16
 
 
+# resolution level (divisor 2^level) of each layer
+LEVELS = [0, 0, 1, 1, 2, 2, 3, 3, 2, 1, 0]
+
+
 def param_count() -> int:
     return sum(co * 9 * ci + co for _, ci, co in LAYERS) + HEAD_IN + 1
+
+
+def flops_per_frame(width: int, height: int) -> float:
+    """Multiply-add FLOPs (2 per MAC) of one inference over a width x height frame
+    (padded to multiples of 8, as the library runs it): the 3x3 layers and the head."""
+    wp, hp = (width + 7) // 8 * 8, (height + 7) // 8 * 8
+    macs = sum((wp >> lv) * (hp >> lv) * co * 9 * ci for (_, ci, co), lv in zip(LAYERS, LEVELS))
+    return 2.0 * (macs + wp * hp * HEAD_IN)
 
 
 def init_params(seed: int) -> np.ndarray:
