@@ -26,6 +26,14 @@ namespace isaac_hip
 #undef DITHER_TABLE
 
     // ------------------------------------------------------------------ functor chain
+    // pow stays out of line: inlined, its ~70 instructions per component and
+    // functor slot made up most of the code of the marches that evaluate the
+    // chain per sample (rarely used, and the call is only taken for it).
+    __device__ __noinline__ float chain_pow(float a, float b)
+    {
+        return powf(a, b);
+    }
+
     // f2(f1(f0(v,p0),p1),p2).x with the feature dimension propagating
     // (lib/isaac/isaac_functor_chain.hpp:96-108, lib/isaac/isaac_functors.hpp:42-183).
     __device__ inline float apply_chain(const ChainDesc& c, float v0, float v1, float v2, float v3, int n)
@@ -68,7 +76,7 @@ namespace isaac_hip
 #pragma unroll
                 for(int i = 0; i < 4; i++)
                     if(i < n)
-                        v[i] = powf(v[i], c.p[k][i]);
+                        v[i] = chain_pow(v[i], c.p[k][i]);
             }
             else if(op == OP_SUM)
             {
@@ -1095,59 +1103,99 @@ namespace isaac_hip
         uint32_t w[N];
     };
 
+    // 32-bit words of one pair row: 2 per component in FP32, 1 (two 16-bit halves) otherwise
     template<int DT, int DIM>
-    __device__ inline void load_row(const void* data, uint32_t row, float lo[DIM], float hi[DIM])
+    constexpr int row_words()
     {
-        if constexpr(DT == kDtF32)
-        {
-            const Words<2 * DIM> r = reinterpret_cast<const Words<2 * DIM>*>(data)[row];
+        return DT == kDtF32 ? 2 * DIM : DIM;
+    }
+
+    // The raw words of one source's sample: its 1 (NEAREST) or 4 (LINEAR) pair
+    // rows and its brick scales, sized for 4 components.  Fetching is split
+    // from interpolation so that the rows of every source of a sample are
+    // requested before the first of them is used.
+    template<int DT>
+    struct SlotRaw
+    {
+        uint32_t r[4][row_words<DT, 4>()];
+        uint32_t sc[4];
+    };
+
+    template<int DT, bool LINEAR, int DIM>
+    __device__ inline void slot_fetch_dim(const BrickSource& S, const BrickTaps& t, SlotRaw<DT>& R)
+    {
+        constexpr int W = row_words<DT, DIM>();
+        const Words<W>* rows = reinterpret_cast<const Words<W>*>(S.data) + t.brick * uint32_t(kBrickPairs);
+        const uint32_t idx[4] = {t.i00, t.i10, t.i01, t.i11};
 #pragma unroll
-            for(int c = 0; c < DIM; c++)
-            {
-                lo[c] = __uint_as_float(r.w[2 * c]);
-                hi[c] = __uint_as_float(r.w[2 * c + 1]);
-            }
+        for(int k = 0; k < (LINEAR ? 4 : 1); k++)
+        {
+            const Words<W> a = rows[idx[k]];
+#pragma unroll
+            for(int w = 0; w < W; w++)
+                R.r[k][w] = a.w[w];
         }
-        else
-        {
-            const Words<DIM> r = reinterpret_cast<const Words<DIM>*>(data)[row];
+        const Words<DIM> sc = reinterpret_cast<const Words<DIM>*>(S.scale)[t.brick];
 #pragma unroll
-            for(int c = 0; c < DIM; c++)
+        for(int c = 0; c < DIM; c++)
+            R.sc[c] = sc.w[c];
+    }
+
+    template<int DT, bool LINEAR>
+    __device__ __attribute__((always_inline)) inline void slot_fetch(const BrickSource& S, const BrickTaps& t, SlotRaw<DT>& R)
+    {
+        switch(S.dim)
+        {
+        case 1: slot_fetch_dim<DT, LINEAR, 1>(S, t, R); break;
+        case 2: slot_fetch_dim<DT, LINEAR, 2>(S, t, R); break;
+        case 3: slot_fetch_dim<DT, LINEAR, 3>(S, t, R); break;
+        default: slot_fetch_dim<DT, LINEAR, 4>(S, t, R); break;
+        }
+    }
+
+    // lo[c] = v_c[x], hi[c] = v_c[x+1] of one fetched row (still divided by the brick scale)
+    template<int DT, int DIM>
+    __device__ inline void decode_row(const uint32_t* w, float lo[DIM], float hi[DIM])
+    {
+#pragma unroll
+        for(int c = 0; c < DIM; c++)
+        {
+            if constexpr(DT == kDtF32)
             {
-                if constexpr(DT == kDtF16)
-                {
-                    lo[c] = half_lo(r.w[c]);
-                    hi[c] = half_hi(r.w[c]);
-                }
-                else
-                {
-                    lo[c] = __uint_as_float(r.w[c] << 16);
-                    hi[c] = __uint_as_float(r.w[c] & 0xffff0000u);
-                }
+                lo[c] = __uint_as_float(w[2 * c]);
+                hi[c] = __uint_as_float(w[2 * c + 1]);
+            }
+            else if constexpr(DT == kDtF16)
+            {
+                lo[c] = half_lo(w[c]);
+                hi[c] = half_hi(w[c]);
+            }
+            else
+            {
+                lo[c] = __uint_as_float(w[c] << 16);
+                hi[c] = __uint_as_float(w[c] & 0xffff0000u);
             }
         }
     }
 
     // raw sample of every component of one source with DIM components (v[c], c < DIM)
     template<int DT, bool LINEAR, int DIM>
-    __device__ inline void brick_sample(const BrickSource& S, const BrickTaps& t, float v[4])
+    __device__ inline void slot_interp(const SlotRaw<DT>& R, const BrickTaps& t, float v[4])
     {
-        const uint32_t row0 = t.brick * uint32_t(kBrickPairs);
-        const Words<DIM> scw = reinterpret_cast<const Words<DIM>*>(S.scale)[t.brick];
         if(!LINEAR)
         {
             float a[DIM], b[DIM];
-            load_row<DT, DIM>(S.data, row0 + t.i00, a, b);
+            decode_row<DT, DIM>(R.r[0], a, b);
 #pragma unroll
             for(int c = 0; c < DIM; c++)
-                v[c] = a[c] * __uint_as_float(scw.w[c]);
+                v[c] = a[c] * __uint_as_float(R.sc[c]);
             return;
         }
         float d000[DIM], d100[DIM], d010[DIM], d110[DIM], d001[DIM], d101[DIM], d011[DIM], d111[DIM];
-        load_row<DT, DIM>(S.data, row0 + t.i00, d000, d100);
-        load_row<DT, DIM>(S.data, row0 + t.i10, d010, d110);
-        load_row<DT, DIM>(S.data, row0 + t.i01, d001, d101);
-        load_row<DT, DIM>(S.data, row0 + t.i11, d011, d111);
+        decode_row<DT, DIM>(R.r[0], d000, d100);
+        decode_row<DT, DIM>(R.r[1], d010, d110);
+        decode_row<DT, DIM>(R.r[2], d001, d101);
+        decode_row<DT, DIM>(R.r[3], d011, d111);
         const float oz = 1.0f - t.fz, oy = 1.0f - t.fy, ox = 1.0f - t.fx;
 #pragma unroll
         for(int c = 0; c < DIM; c++)
@@ -1163,21 +1211,61 @@ namespace isaac_hip
             const float A = a0 * oy + a1 * t.fy;
             const float b0 = d100[c] * oz + d101[c] * t.fz, b1 = d110[c] * oz + d111[c] * t.fz;
             const float B = b0 * oy + b1 * t.fy;
-            v[c] = (A * ox + B * t.fx) * __uint_as_float(scw.w[c]);
+            v[c] = (A * ox + B * t.fx) * __uint_as_float(R.sc[c]);
         }
     }
 
-    // the functor-chain value of one source's sample, dispatched on its component count
+    // the functor-chain value of one source's sample, dispatched on its component
+    // count (fetch and interpolation in the same case: one source at a time)
+    template<int DT, bool LINEAR, int DIM>
+    __device__ inline void slot_sample(const BrickSource& S, const BrickTaps& t, float v[4])
+    {
+        SlotRaw<DT> R;
+        slot_fetch_dim<DT, LINEAR, DIM>(S, t, R);
+        slot_interp<DT, LINEAR, DIM>(R, t, v);
+    }
+
     template<int DT, bool LINEAR>
     __device__ __attribute__((always_inline)) inline float brick_value(const BrickSource& S, const BrickTaps& t)
     {
         float v[4] = {0.f, 0.f, 0.f, 0.f};
         switch(S.dim)
         {
-        case 1: brick_sample<DT, LINEAR, 1>(S, t, v); return apply_chain(S.chain, v[0], 0.f, 0.f, 0.f, 1);
-        case 2: brick_sample<DT, LINEAR, 2>(S, t, v); return apply_chain(S.chain, v[0], v[1], 0.f, 0.f, 2);
-        case 3: brick_sample<DT, LINEAR, 3>(S, t, v); return apply_chain(S.chain, v[0], v[1], v[2], 0.f, 3);
-        default: brick_sample<DT, LINEAR, 4>(S, t, v); return apply_chain(S.chain, v[0], v[1], v[2], v[3], 4);
+        case 1: slot_sample<DT, LINEAR, 1>(S, t, v); break;
+        case 2: slot_sample<DT, LINEAR, 2>(S, t, v); break;
+        case 3: slot_sample<DT, LINEAR, 3>(S, t, v); break;
+        default: slot_sample<DT, LINEAR, 4>(S, t, v); break;
+        }
+        // one chain per source slot, with the component count as a value
+        return apply_chain(S.chain, v[0], v[1], v[2], v[3], S.dim);
+    }
+
+    // Source signature: the component counts of the enabled sources in slot
+    // order, as base-5 digits (slot 0 lowest), 0 = resolved at run time.  A
+    // march compiled for one signature fetches the rows of all its sources
+    // before interpolating any of them, so their latencies overlap.
+    constexpr int sig_count(int sig)
+    {
+        int n = 0;
+        for(; sig > 0; sig /= 5)
+            n++;
+        return n;
+    }
+
+    constexpr int sig_dim(int sig, int slot)
+    {
+        for(int i = 0; i < slot; i++)
+            sig /= 5;
+        return sig % 5;
+    }
+
+    template<int DT, bool LINEAR, int SIG, int S>
+    __device__ __attribute__((always_inline)) inline void sig_fetch(const PostParams& Q, const BrickTaps& t, SlotRaw<DT>* R)
+    {
+        if constexpr(S < sig_count(SIG))
+        {
+            slot_fetch_dim<DT, LINEAR, sig_dim(SIG, S)>(Q.src[S], t, R[S]);
+            sig_fetch<DT, LINEAR, SIG, S + 1>(Q, t, R);
         }
     }
 
@@ -1187,38 +1275,67 @@ namespace isaac_hip
         float ir, ig, ib, ia; // iso colour and alpha (alpha / threshold * 0.5, maximum over sources)
     };
 
-    template<int DT, bool LINEAR>
+    // one source's classified sample into the running sums (weights in slot order)
+    __device__ __attribute__((always_inline)) inline void post_accumulate(PostSample& o, const BrickSource& S, const float4 col)
+    {
+        if(S.weight > 0.0f)
+        {
+            const float a = col.w * S.weight;
+            o.r = o.r + col.x * a;
+            o.g = o.g + col.y * a;
+            o.b = o.b + col.z * a;
+            o.a = o.a + a;
+        }
+        if(S.iso > 0.0f)
+        {
+            const float a = col.w * S.isoScale;
+            if(o.ia < a)
+            {
+                o.ir = col.x;
+                o.ig = col.y;
+                o.ib = col.z;
+                o.ia = a;
+            }
+        }
+    }
+
+    template<int DT, bool LINEAR, int SIG, int S>
+    __device__ __attribute__((always_inline)) inline void sig_classify(
+        const PostParams& Q, const float4* lut, const BrickTaps& t, const SlotRaw<DT>* R, PostSample& o)
+    {
+        if constexpr(S < sig_count(SIG))
+        {
+            constexpr int D = sig_dim(SIG, S);
+            float v[4] = {0.f, 0.f, 0.f, 0.f};
+            slot_interp<DT, LINEAR, D>(R[S], t, v);
+            const float tv = apply_chain(Q.src[S].chain, v[0], v[1], v[2], v[3], D);
+            post_accumulate(o, Q.src[S], lut[S * Q.T + tf_index(tv, Q.T)]);
+            sig_classify<DT, LINEAR, SIG, S + 1>(Q, lut, t, R, o);
+        }
+    }
+
+    template<int DT, bool LINEAR, int SIG>
     __device__ __attribute__((always_inline)) inline PostSample post_classify(const PostParams& Q, const float4* lut, int X, int Y, int Z, float px, float py, float pz)
     {
         const BrickTaps t = brick_taps<LINEAR>(Q, X, Y, Z, px, py, pz);
         PostSample o{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        // unrolled over the source slots so every parameter-block index is constant
-#pragma unroll
-        for(int s = 0; s < kMaxBrickSources; s++)
+        if constexpr(SIG != 0)
         {
-            if(s >= Q.nsrc)
-                break;
-            const BrickSource& S = Q.src[s];
-            const float tv = brick_value<DT, LINEAR>(S, t);
-            const float4 col = lut[s * Q.T + tf_index(tv, Q.T)];
-            if(S.weight > 0.0f)
+            SlotRaw<DT> R[sig_count(SIG)];
+            sig_fetch<DT, LINEAR, SIG, 0>(Q, t, R);
+            sig_classify<DT, LINEAR, SIG, 0>(Q, lut, t, R, o);
+        }
+        else
+        {
+            // unrolled over the source slots so every parameter-block index is constant
+#pragma unroll
+            for(int s = 0; s < kMaxBrickSources; s++)
             {
-                const float a = col.w * S.weight;
-                o.r = o.r + col.x * a;
-                o.g = o.g + col.y * a;
-                o.b = o.b + col.z * a;
-                o.a = o.a + a;
-            }
-            if(S.iso > 0.0f)
-            {
-                const float a = col.w * S.isoScale;
-                if(o.ia < a)
-                {
-                    o.ir = col.x;
-                    o.ig = col.y;
-                    o.ib = col.z;
-                    o.ia = a;
-                }
+                if(s >= Q.nsrc)
+                    break;
+                const BrickSource& S = Q.src[s];
+                const float tv = brick_value<DT, LINEAR>(S, t);
+                post_accumulate(o, S, lut[s * Q.T + tf_index(tv, Q.T)]);
             }
         }
         // invTotalWeight = 1 / (sum of volume-source weights), 0 when that sum is 0
@@ -1247,7 +1364,7 @@ namespace isaac_hip
         }
     }
 
-    template<int DT, bool LINEAR, bool COUNT>
+    template<int DT, bool LINEAR, bool COUNT, int SIG = 0>
     __device__ __attribute__((always_inline)) inline void brick_march_body(
         const FrameParams& F,
         const PostParams& Q,
@@ -1316,7 +1433,7 @@ namespace isaac_hip
                         if(COUNT)
                             nSteps++;
                         const float fi = float(i);
-                        const PostSample sm = post_classify<DT, LINEAR>(
+                        const PostSample sm = post_classify<DT, LINEAR, SIG>(
                             Q, lut, F.X, F.Y, F.Z, startU.x + stepVec.x * fi, startU.y + stepVec.y * fi, startU.z + stepVec.z * fi);
                         if(isoOn)
                         {
@@ -1366,7 +1483,7 @@ namespace isaac_hip
                             const float sx = d == 0 ? 1.0f : (d == 1 ? -1.0f : 0.0f);
                             const float sy = d == 2 ? 1.0f : (d == 3 ? -1.0f : 0.0f);
                             const float sz = d == 4 ? 1.0f : (d == 5 ? -1.0f : 0.0f);
-                            const PostSample h = post_classify<DT, LINEAR>(Q, lut, F.X, F.Y, F.Z, pu.x + sx, pu.y + sy, pu.z + sz);
+                            const PostSample h = post_classify<DT, LINEAR, SIG>(Q, lut, F.X, F.Y, F.Z, pu.x + sx, pu.y + sy, pu.z + sz);
                             const float v = gmin(h.ia, 1.0f);
                             if(d < 0)
                             {
@@ -1433,6 +1550,16 @@ namespace isaac_hip
         stage_luts(lds_lut, Q);
         __syncthreads();
         brick_march_body<DT, LINEAR, COUNT>(F, Q, PP, lds_lut, xcd_tile(F));
+    }
+
+    // trilinear march compiled for one source signature (see sig_count)
+    template<int DT, int SIG>
+    __global__ __launch_bounds__(256) void brick_march_sig_kernel(const FrameParams F, const PostParams Q, const ParticleParams PP)
+    {
+        extern __shared__ float4 lds_lut[];
+        stage_luts(lds_lut, Q);
+        __syncthreads();
+        brick_march_body<DT, true, false, SIG>(F, Q, PP, lds_lut, xcd_tile(F));
     }
 
     // ------------------------------------------------------------------ batched form
@@ -1617,6 +1744,14 @@ namespace isaac_hip
 
     static_assert(sizeof(FrameParams) + sizeof(PostParams) + sizeof(ParticleParams) <= 4096, "kernel argument block");
 
+    int post_signature(const PostParams& Q)
+    {
+        int sig = 0;
+        for(int i = Q.nsrc - 1; i >= 0; i--)
+            sig = sig * 5 + Q.src[i].dim;
+        return Q.nsrc > 0 ? sig : -1;
+    }
+
     hipError_t launch_brick_march(const FrameParams& F, const PostParams& Q, const ParticleParams& PP, bool counting, hipStream_t s)
     {
         const unsigned nb = unsigned(F.tilesX) * unsigned(F.tilesY);
@@ -1624,6 +1759,34 @@ namespace isaac_hip
             return hipSuccess;
         const dim3 grid(nb), block(256);
         const size_t lds = size_t(Q.nsrc) * Q.T * sizeof(float4);
+        if(F.linear && !counting)
+        {
+            // signatures with a compiled march: density (1), vector (3), and pairs of them
+            const int sig = post_signature(Q);
+            const int v = Q.dtype * 8 + (sig == 1 ? 0 : sig == 3 ? 1 : sig == 6 ? 2 : sig == 8 ? 3 : sig == 16 ? 4 : 7);
+            switch(v)
+            {
+#define ISAAC_SIG_CASE(dt, k, sg)                                                                                       \
+    case dt * 8 + k: hipLaunchKernelGGL((brick_march_sig_kernel<dt, sg>), grid, block, lds, s, F, Q, PP); return hipGetLastError();
+                ISAAC_SIG_CASE(0, 0, 1)
+                ISAAC_SIG_CASE(0, 1, 3)
+                ISAAC_SIG_CASE(0, 2, 6)
+                ISAAC_SIG_CASE(0, 3, 8)
+                ISAAC_SIG_CASE(0, 4, 16)
+                ISAAC_SIG_CASE(1, 0, 1)
+                ISAAC_SIG_CASE(1, 1, 3)
+                ISAAC_SIG_CASE(1, 2, 6)
+                ISAAC_SIG_CASE(1, 3, 8)
+                ISAAC_SIG_CASE(1, 4, 16)
+                ISAAC_SIG_CASE(2, 0, 1)
+                ISAAC_SIG_CASE(2, 1, 3)
+                ISAAC_SIG_CASE(2, 2, 6)
+                ISAAC_SIG_CASE(2, 3, 8)
+                ISAAC_SIG_CASE(2, 4, 16)
+#undef ISAAC_SIG_CASE
+            default: break;
+            }
+        }
         const int v = (Q.dtype * 4) | (F.linear ? 2 : 0) | (counting ? 1 : 0);
         switch(v)
         {
