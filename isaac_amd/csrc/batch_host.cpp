@@ -45,6 +45,7 @@ struct isaac_hip_batch
     bool aoOn = false;
     int dtype = 0;
     bool linear = true;
+    int sig = 0; // source signature of every frame (0: mixed)
     size_t lutLds = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -136,7 +137,7 @@ namespace
         B.chunk = b->chunk;
         B.counter = b->dCounter;
         const int grid = std::max(1, b->numCU * b->groupsPerCU);
-        BCHK(b, launch_batch_march(B, b->dtype, b->linear, b->lutLds, grid, b->stream));
+        BCHK(b, launch_batch_march(B, b->dtype, b->linear, b->sig, b->lutLds, grid, b->stream));
         if(b->aoOn)
         {
             if(b->learned)
@@ -302,9 +303,19 @@ extern "C"
             b->graphStale = true;
         b->nframes = nframes;
         b->nviews = nviews;
+        // the source signature selects a march compiled for it when every frame shares it
+        int sig = post_signature(b->hQ[0]);
+        for(int f = 1; f < nframes; f++)
+            if(post_signature(b->hQ[f]) != sig)
+                sig = 0;
+        const size_t lutLds = size_t(b->hQ[0].nsrc) * size_t(b->hQ[0].T) * sizeof(float4);
+        // launch arguments the captured graph holds
+        if(c0->brickDtype != b->dtype || c0->interpolation != b->linear || sig != b->sig || lutLds != b->lutLds)
+            b->graphStale = true;
         b->dtype = c0->brickDtype;
         b->linear = c0->interpolation;
-        b->lutLds = size_t(b->hQ[0].nsrc) * size_t(b->hQ[0].T) * sizeof(float4);
+        b->sig = sig;
+        b->lutLds = lutLds;
         const bool aoOn = b->hF[0].aoEnabled != 0;
         const bool learned = aoOn && c0->aoMode == ISAAC_HIP_AO_LEARNED;
         if(aoOn != b->aoOn || learned != b->learned)

@@ -39,7 +39,9 @@ namespace isaac_hip
         hipStream_t s);
     // batched form: the persistent march over all frames (grid = persistent workgroups),
     // then SSAO and shading over a (maxTiles, nframes) grid
-    hipError_t launch_batch_march(const BatchParams& B, int dtype, bool linear, size_t lds, int grid, hipStream_t s);
+    hipError_t launch_batch_march(const BatchParams& B, int dtype, bool linear, int sig, size_t lds, int grid, hipStream_t s);
+    // component counts of the enabled sources as base-5 digits, slot 0 lowest (-1: no source)
+    int post_signature(const PostParams& Q);
     hipError_t launch_batch_ssao(const FrameParams* frames, int nframes, int maxTiles, hipStream_t s);
     hipError_t launch_batch_shade(const FrameParams* frames, int nframes, int maxTiles, hipStream_t s);
     hipError_t launch_brick_march(const FrameParams& F, const PostParams& Q, const ParticleParams& PP, bool counting, hipStream_t s);

@@ -1567,7 +1567,7 @@ namespace isaac_hip
     // chunks of 16x16 tiles from a global counter until the work is exhausted
     // (every workgroup reaches the exit when the counter passes the last chunk).
     // Frames share the transfer functions (staged once per workgroup).
-    template<int DT, bool LINEAR>
+    template<int DT, bool LINEAR, int SIG = 0>
     __global__ __launch_bounds__(256) void batch_march_kernel(const BatchParams B)
     {
         extern __shared__ float4 lds_lut[];
@@ -1601,7 +1601,7 @@ namespace isaac_hip
                 while(t >= B.tileStart[item + 1])
                     item++;
                 item = __builtin_amdgcn_readfirstlane(item);
-                brick_march_body<DT, LINEAR, false>(B.frames[item], B.posts[item], B.parts[item], lds_lut, t - B.tileStart[item]);
+                brick_march_body<DT, LINEAR, false, SIG>(B.frames[item], B.posts[item], B.parts[item], lds_lut, t - B.tileStart[item]);
             }
         }
     }
@@ -1806,10 +1806,30 @@ namespace isaac_hip
         return hipGetLastError();
     }
 
-    hipError_t launch_batch_march(const BatchParams& B, int dtype, bool linear, size_t lds, int grid, hipStream_t s)
+    hipError_t launch_batch_march(const BatchParams& B, int dtype, bool linear, int sig, size_t lds, int grid, hipStream_t s)
     {
         if(B.nframes <= 0 || grid <= 0)
             return hipSuccess;
+        if(linear && (sig == 1 || sig == 3 || sig == 16))
+        {
+            const int k = dtype * 3 + (sig == 1 ? 0 : sig == 3 ? 1 : 2);
+            switch(k)
+            {
+#define ISAAC_SIG_CASE(dt, j, sg)                                                                                       \
+    case dt * 3 + j: hipLaunchKernelGGL((batch_march_kernel<dt, true, sg>), dim3(grid), dim3(256), lds, s, B); return hipGetLastError();
+                ISAAC_SIG_CASE(0, 0, 1)
+                ISAAC_SIG_CASE(0, 1, 3)
+                ISAAC_SIG_CASE(0, 2, 16)
+                ISAAC_SIG_CASE(1, 0, 1)
+                ISAAC_SIG_CASE(1, 1, 3)
+                ISAAC_SIG_CASE(1, 2, 16)
+                ISAAC_SIG_CASE(2, 0, 1)
+                ISAAC_SIG_CASE(2, 1, 3)
+                ISAAC_SIG_CASE(2, 2, 16)
+#undef ISAAC_SIG_CASE
+            default: break;
+            }
+        }
         const int v = dtype * 2 + (linear ? 1 : 0);
         switch(v)
         {
