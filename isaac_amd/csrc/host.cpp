@@ -313,8 +313,10 @@ namespace
         if(c->tex)
             (void) hipFree(c->tex);
         c->tex = nullptr;
-        HIPCHK(c, hipMalloc(&c->tex, std::max<size_t>(nvox, 1) * 8));
-        HIPCHK(c, hipMemsetAsync(c->tex, 0, std::max<size_t>(nvox, 1) * 8, c->stream));
+        // one texel of padding: the march reads texels x and x+1 with one 16-byte load,
+        // also at the last texel (whose x+1 it then ignores)
+        HIPCHK(c, hipMalloc(&c->tex, (nvox + 1) * 8));
+        HIPCHK(c, hipMemsetAsync(c->tex, 0, (nvox + 1) * 8, c->stream));
         c->texVox = nvox;
         c->classified = false;
         return 0;

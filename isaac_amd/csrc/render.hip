@@ -367,6 +367,12 @@ namespace isaac_hip
         float fx, fy, fz;
     };
 
+    // two adjacent texels (8-byte aligned; one 16-byte load)
+    struct TexPair
+    {
+        uint2 a, b;
+    };
+
     // Sampler<LINEAR|NEAREST, CLAMP> addressing (lib/isaac/isaac_texture.hpp:182-259).  Every
     // coordinate is clamped into the texture, so a fetch at any position is in bounds.
     template<bool LINEAR>
@@ -390,14 +396,22 @@ namespace isaac_hip
         const uint32_t z0 = iclamp(bz, 0, Z - 1), z1 = iclamp(bz + 1, 0, Z - 1);
         const uint32_t r00 = uint32_t(X) * (y0 + uint32_t(Y) * z0), r01 = uint32_t(X) * (y0 + uint32_t(Y) * z1);
         const uint32_t r10 = uint32_t(X) * (y1 + uint32_t(Y) * z0), r11 = uint32_t(X) * (y1 + uint32_t(Y) * z1);
-        T.t[0] = tex[x0 + r00];
-        T.t[1] = tex[x0 + r01];
-        T.t[2] = tex[x0 + r10];
-        T.t[3] = tex[x0 + r11];
-        T.t[4] = tex[x1 + r00];
-        T.t[5] = tex[x1 + r01];
-        T.t[6] = tex[x1 + r10];
-        T.t[7] = tex[x1 + r11];
+        // texels x0 and x0+1 are adjacent: one 16-byte load per (y, z) row; at the
+        // x edge (x1 == x0) the second texel (the next row's first, or the padding
+        // texel after the last) is replaced by the first
+        const TexPair p00 = *reinterpret_cast<const TexPair*>(tex + (x0 + r00));
+        const TexPair p01 = *reinterpret_cast<const TexPair*>(tex + (x0 + r01));
+        const TexPair p10 = *reinterpret_cast<const TexPair*>(tex + (x0 + r10));
+        const TexPair p11 = *reinterpret_cast<const TexPair*>(tex + (x0 + r11));
+        const bool dup = x1 == x0;
+        T.t[0] = p00.a;
+        T.t[1] = p01.a;
+        T.t[2] = p10.a;
+        T.t[3] = p11.a;
+        T.t[4] = dup ? p00.a : p00.b;
+        T.t[5] = dup ? p01.a : p01.b;
+        T.t[6] = dup ? p10.a : p10.b;
+        T.t[7] = dup ? p11.a : p11.b;
     }
 
     // one normalised channel: trilinear of the bytes (z innermost, x outermost,
