@@ -64,7 +64,9 @@ extern "C"
         ISAAC_HIP_CNT_ISO_RAYS = 6,
         ISAAC_HIP_CNT_PART_RAYS = 7,
         ISAAC_HIP_CNT_MARCH_STEPS = 8, /* fused iso+volume traversal steps (one gather of the taps each) */
-        ISAAC_HIP_CNT_COUNT = 9
+        ISAAC_HIP_CNT_MARCH_WAVE_STEPS = 9, /* sum over waves of their longest lane's traversal steps:
+                                               MARCH_STEPS / (64 * this) is the march's lane utilisation */
+        ISAAC_HIP_CNT_COUNT = 10
     };
 
     typedef struct isaac_hip_ctx isaac_hip_ctx;

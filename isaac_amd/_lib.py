@@ -17,7 +17,7 @@ HEADER = os.path.join(ROOT, "include", "isaac_hip.h")
 
 MAX_SOURCES = 16
 CNT_NAMES = ["vol_samples", "iso_samples", "iso_hits", "cells_visited", "sphere_tests", "vol_rays", "iso_rays", "part_rays",
-             "march_steps"]
+             "march_steps", "march_wave_steps"]
 
 ERRORS = {0: "ok", -1: "bad argument", -2: "device error", -3: "bad state", -4: "unsupported", -5: "allocation failed"}
 

@@ -805,6 +805,11 @@ namespace isaac_hip
             block_count_add(F.counters + 6, nIsoRays);
             block_count_add(F.counters + 7, nPartRays);
             block_count_add(F.counters + 8, nSteps);
+            // the wave runs the traversal loop as long as its longest lane
+            int wm = int(nSteps);
+            for(int off = 32; off > 0; off >>= 1)
+                wm = max(wm, __shfl_xor(wm, off, 64));
+            block_count_add(F.counters + 9, (threadIdx.x & 63u) == 0 ? (unsigned long long) wm : 0ull);
         }
     }
 
@@ -1554,6 +1559,11 @@ namespace isaac_hip
             block_count_add(F.counters + 6, nIsoRays);
             block_count_add(F.counters + 7, nPartRays);
             block_count_add(F.counters + 8, nSteps);
+            // the wave runs the traversal loop as long as its longest lane
+            int wm = int(nSteps);
+            for(int off = 32; off > 0; off >>= 1)
+                wm = max(wm, __shfl_xor(wm, off, 64));
+            block_count_add(F.counters + 9, (threadIdx.x & 63u) == 0 ? (unsigned long long) wm : 0ull);
         }
     }
 
