@@ -162,10 +162,10 @@ def learned_ao(vis, poses, out, args):
     held_out = [scenes.DEFAULT_POSE, scenes.FILL_POSE, scenes.random_pose(np.random.default_rng(99))]
     gate = aonet.gate(vis, held_out, floor_db=40.0)
     timing = {}
-    for name in ("combined", "bricks-f16"):
+    for name in ("combined", "bricks-f32", "bricks-f16"):
         vis.set_render_form(*FORMS[name])
         for mode, label in ((vis.AO_PLAIN, "plain"), (vis.AO_LEARNED, "learned")):
-            if mode == vis.AO_LEARNED and not gate["passed"]:
+            if mode == vis.AO_LEARNED and (not gate["passed"] or name == "bricks-f32"):
                 continue
             vis.set_ao_mode(mode)
             w2, s2, us2, c2, _ = measure(vis, poses, out, args.form_steps, 2)
@@ -189,7 +189,10 @@ def learned_ao(vis, poses, out, args):
         "frames": timing,
     }
     if "bricks-f16/learned" in timing:
+        # the plain FP32 form: the combined (pre-classified) form, and the FP32 brick form
         res["speedup_learned_f16_vs_plain_f32"] = timing["combined/plain"]["ms_per_step"] / timing["bricks-f16/learned"]["ms_per_step"]
+        res["speedup_learned_f16_vs_plain_f32_bricks"] = (timing["bricks-f32/plain"]["ms_per_step"]
+                                                          / timing["bricks-f16/learned"]["ms_per_step"])
     if "combined/learned" in timing:
         # the network's dense roofline: FP16 MFMA, 2.5 PFLOP/s (MI355X dense peak)
         W, H = vis.framebuffer_size
