@@ -203,6 +203,34 @@ def learned_ao(vis, poses, out, args):
     return res
 
 
+def compositing(n_images: int = 8, width: int = 1920, height: int = 1080, reps: int = 20):
+    """The per-GPU compositing step (isaac_hip_composite): n_images RGBA8 frames of
+    the sub-volumes one GPU renders, blended in composite order into one; timed with
+    HIP events on the stream it runs on.  Algorithmic bytes: every input pixel read
+    once, the output written once."""
+    import torch
+
+    from isaac_amd import composite as K
+
+    px = width * height
+    g = torch.Generator(device="cuda").manual_seed(5)
+    imgs = [torch.randint(0, 256, (px, 4), dtype=torch.uint8, device="cuda", generator=g) for _ in range(n_images)]
+    order = list(range(n_images))
+    out = torch.empty_like(imgs[0])
+    for _ in range(3):
+        K.composite(imgs, order, out)
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(reps):
+        K.composite(imgs, order, out)
+    stop.record()
+    torch.cuda.synchronize()
+    us = start.elapsed_time(stop) * 1e3 / reps
+    nbytes = (n_images + 1) * px * 4
+    return {"images": n_images, "framebuffer": [width, height], "us": us, "algorithmic_bytes": nbytes,
+            "GBs": nbytes / (us * 1e-6) / 1e9, "hbm_frac": nbytes / (us * 1e-6) / 8.0e12}
+
+
 def batched_workload(args):
     """48 sub-volumes of side U{32..256} (FP16 bricks: Gaussian-mixture density with an
     iso-surface + float3 noise via length) x 8 seeded views at 960x540 with learned
@@ -406,6 +434,7 @@ def main() -> int:
     vis.close()
     if not args.no_batch:
         line["batched"] = batched_workload(args)
+    line["compositing"] = compositing()
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
         spec0 = spec.with_camera(poses[0])

@@ -320,6 +320,29 @@ extern "C"
     /* device pointer of the frames' RGBA8 images, frame-major (nframes * width * height * 4 bytes) */
     int isaac_hip_batch_get_images(const isaac_hip_batch* batch, const void** rgba, int32_t* nframes);
 
+    /* ---------------------------------------------------------------- sort-last compositing
+     * The per-GPU part of the reference's IceT step (lib/isaac.hpp:800-831, 2015-2056) for
+     * several sub-volumes rendered on one GPU; no exchange between processes.
+     *
+     * isaac_hip_composite_order replaces the distance sort feeding icetCompositeOrder
+     * (lib/isaac.hpp:2021-2048): for sub-volume r (position / local_size: n x 3 int32,
+     * global_size, scale and the modelview as every context of the set has them) the distance
+     * float(|modelview * p|) of p = (positionScaled + (localSizeScaled - globalSizeScaled) / 2)
+     * / (maxSizeScaled / 2) (integer division), w = 1; order[0..n) = sub-volumes by ascending
+     * distance, equal distances in index order.  Host only.
+     *
+     * isaac_hip_composite replaces icetDrawFrame's compositing (ICET_COMPOSITE_MODE_BLEND,
+     * RGBA_UBYTE, ordered; lib/isaac.hpp:811-823): out = images[order[0]] over images[order[1]]
+     * over ..., IceT's byte "over" dest = (dest * (255 - src.a)) / 255 + src per channel
+     * (integer division, result taken mod 256), associated as a pairwise tree over the order
+     * (binary swap's association for power-of-two counts).  images: host array of n device
+     * pointers to RGBA8 images of `pixels` pixels, 16-byte aligned; out: device, 16-byte
+     * aligned; 1 <= n <= 64; enqueued on `stream` (0: the null stream). */
+    int isaac_hip_composite_order(int32_t n, const int32_t* position, const int32_t* local_size, const int32_t global_size[3],
+                                  const float scale[3], const double modelview[16], int32_t* order);
+    int isaac_hip_composite(int32_t n, const uint8_t* const* images, const int32_t* order, int64_t pixels, uint8_t* out,
+                            void* stream);
+
     /* ---------------------------------------------------------------- learned occlusion
      * No counterpart in the reference.  A U-Net over the G-buffer (input: depth /
      * maxGlobalSizeScaled, background flag, normal; 3x3 convolutions with 16/32/64/64
