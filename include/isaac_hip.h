@@ -238,7 +238,10 @@ extern "C"
     enum isaac_hip_tuning_key
     {
         ISAAC_HIP_TUNE_MARCH_PREFETCH = 0, /* 1: fetch step i+1 before compositing step i (default 0) */
-        ISAAC_HIP_TUNE_MARCH_WAVES = 1 /* march occupancy target: 0 (compiler default), 6 or 8 waves per SIMD */
+        ISAAC_HIP_TUNE_MARCH_WAVES = 1, /* march occupancy target: 0 (compiler default), 6 or 8 waves per SIMD */
+        ISAAC_HIP_TUNE_MARCH_TILES = 2 /* march tile order: 0 one band of tile rows per XCD; 2, 4 (default), 8 or
+                                          16: super-tiles of that many 16x16-pixel tiles a side, dealt
+                                          round-robin to the XCDs */
     };
     int isaac_hip_set_tuning(isaac_hip_ctx* ctx, int32_t key, int32_t value);
 

@@ -1,8 +1,8 @@
 """Launch-schedule autotuner.
 
 Times the march kernel of a visualization under each candidate schedule
-(include/isaac_hip.h, "tuning": march occupancy target, next-step prefetch) on
-the caller's own frames and keeps the fastest.  Results do not depend on the
+(include/isaac_hip.h, "tuning": march occupancy target, next-step prefetch,
+tile order over the XCDs) on the caller's own frames and keeps the fastest.  Results do not depend on the
 schedule, only speed does; choices are cached per workload key.
 """
 from __future__ import annotations
@@ -17,6 +17,7 @@ _CACHE: Dict[Tuple, Dict[str, int]] = {}
 CANDIDATES = {
     "march_waves": (0, 6, 8),
     "march_prefetch": (0, 1),
+    "march_tiles": (0, 4, 8),
 }
 
 
@@ -28,6 +29,7 @@ def _key(vis) -> Tuple:
 def _apply(vis, choice: Dict[str, int]) -> None:
     vis.set_tuning(vis.TUNE_MARCH_WAVES, choice["march_waves"])
     vis.set_tuning(vis.TUNE_MARCH_PREFETCH, choice["march_prefetch"])
+    vis.set_tuning(vis.TUNE_MARCH_TILES, choice.get("march_tiles", 4))
 
 
 def tune_march(vis, poses: Sequence, frames: int = 4, use_cache: bool = True) -> Dict:
@@ -38,10 +40,11 @@ def tune_march(vis, poses: Sequence, frames: int = 4, use_cache: bool = True) ->
         _apply(vis, _CACHE[key])
         return {"choice": dict(_CACHE[key]), "march_us": {}, "cached": True}
     results = {}
-    for waves, pf in itertools.product(CANDIDATES["march_waves"], CANDIDATES["march_prefetch"]):
+    for waves, pf, tiles in itertools.product(CANDIDATES["march_waves"], CANDIDATES["march_prefetch"],
+                                              CANDIDATES["march_tiles"]):
         if pf and waves:
             continue  # the occupancy-targeted kernels have no prefetch variant
-        choice = {"march_waves": waves, "march_prefetch": pf}
+        choice = {"march_waves": waves, "march_prefetch": pf, "march_tiles": tiles}
         _apply(vis, choice)
         times = []
         for p in poses:
@@ -53,9 +56,10 @@ def tune_march(vis, poses: Sequence, frames: int = 4, use_cache: bool = True) ->
             for _ in range(frames):
                 vis.draw(out=False)
                 times.append(vis.timings()["march_us"])
-        results[(waves, pf)] = float(np.median(times))
+        results[(waves, pf, tiles)] = float(np.median(times))
     best = min(results, key=results.get)
-    choice = {"march_waves": best[0], "march_prefetch": best[1]}
+    choice = {"march_waves": best[0], "march_prefetch": best[1], "march_tiles": best[2]}
     _CACHE[key] = choice
     _apply(vis, choice)
-    return {"choice": choice, "march_us": {f"waves={w},prefetch={p}": t for (w, p), t in results.items()}, "cached": False}
+    return {"choice": choice, "cached": False,
+            "march_us": {f"waves={w},prefetch={p},tiles={s}": t for (w, p, s), t in results.items()}}

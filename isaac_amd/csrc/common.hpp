@@ -86,6 +86,7 @@ namespace isaac_hip
         int32_t linear;
         int32_t prefetch; // fetch the next step's taps before compositing the current one
         int32_t waves; // march occupancy target: 0 compiler default, 6 or 8 waves per SIMD
+        int32_t tileOrder; // march tile order: 0 one band of tile rows per XCD, S: SxS-tile super-tiles dealt to the XCDs
         // textures
         int32_t X, Y, Z;
         const uint2* tex; // interleaved combined textures, x fastest

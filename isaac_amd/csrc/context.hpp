@@ -87,6 +87,7 @@ struct isaac_hip_ctx
     bool classified = false;
     int tunePrefetch = 0;
     int tuneWaves = 0;
+    int tuneTiles = 4; // super-tiles of 4x4 tiles (measured fastest at 1080p)
     int form = ISAAC_HIP_FORM_COMBINED;
     int brickDtype = ISAAC_HIP_F32;
     int builtForm = -1; // form prepared by the last update_buffers

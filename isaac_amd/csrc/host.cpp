@@ -555,6 +555,7 @@ namespace isaac_hip
         F.linear = c->interpolation ? 1 : 0;
         F.prefetch = c->tunePrefetch;
         F.waves = c->tuneWaves;
+        F.tileOrder = c->tuneTiles;
         F.X = int32_t(c->local[0]);
         F.Y = int32_t(c->local[1]);
         F.Z = int32_t(c->local[2]);
@@ -1507,6 +1508,11 @@ extern "C"
             if(value != 0 && value != 6 && value != 8)
                 return fail(c, ISAAC_HIP_ERR_ARG, "march waves: 0, 6 or 8");
             c->tuneWaves = value;
+            return 0;
+        case ISAAC_HIP_TUNE_MARCH_TILES:
+            if(value != 0 && value != 2 && value != 4 && value != 8 && value != 16)
+                return fail(c, ISAAC_HIP_ERR_ARG, "march tile order: 0 (XCD bands) or a super-tile side of 2, 4, 8 or 16 tiles");
+            c->tuneTiles = value;
             return 0;
         default:
             return fail(c, ISAAC_HIP_ERR_ARG, "unknown tuning key");

@@ -479,6 +479,32 @@ namespace isaac_hip
         return b;
     }
 
+    // Super-tiles of SxS tiles (S = F.tileOrder: 2, 4, 8 or 16; 4 is 64x64 pixels)
+    // dealt round-robin to the XCDs: workgroup b runs on XCD b % 8 and takes tile
+    // (b / 8) % S^2 of that XCD's super-tile number (b / 8) / S^2.  Every XCD gets
+    // super-tiles from all over the frame (balanced when the work is concentrated,
+    // e.g. a volume in the middle of the frame) while the rays of one super-tile
+    // share its L2.  Grid: march_grid workgroups; padding workgroups map to no tile.
+    __device__ inline uint32_t supertile_tile(const FrameParams& F)
+    {
+        const uint32_t S = uint32_t(F.tileOrder), sh = uint32_t(__ffs(int(S)) - 1);
+        const uint32_t tX = uint32_t(F.tilesX), tY = uint32_t(F.tilesY);
+        const uint32_t sx = (tX + S - 1u) >> sh, sy = (tY + S - 1u) >> sh;
+        const uint32_t b = blockIdx.x, i = b >> 3;
+        const uint32_t s = (b & 7u) + 8u * (i >> (2u * sh)), w = i & (S * S - 1u);
+        if(s >= sx * sy)
+            return 0xffffffffu;
+        const uint32_t tx = ((s % sx) << sh) + (w & (S - 1u)), ty = ((s / sx) << sh) + (w >> sh);
+        if(tx >= tX || ty >= tY)
+            return 0xffffffffu;
+        return ty * tX + tx;
+    }
+
+    __device__ inline uint32_t march_tile(const FrameParams& F)
+    {
+        return F.tileOrder != 0 ? supertile_tile(F) : xcd_tile(F);
+    }
+
     __device__ inline bool tile_pixel(const FrameParams& F, int& px, int& py)
     {
         return tile_pixel_at(F, xcd_tile(F), px, py);
@@ -628,7 +654,7 @@ namespace isaac_hip
     __device__ __attribute__((always_inline)) inline void march_body(const FrameParams& F, const ParticleParams& PP)
     {
         int px, py;
-        const bool valid = tile_pixel(F, px, py);
+        const bool valid = tile_pixel_at(F, march_tile(F), px, py);
         unsigned long long nCells = 0, nTests = 0, nIso = 0, nIsoHits = 0, nIsoRays = 0, nPartRays = 0;
         unsigned long long nVol = 0, nVolRays = 0, nSteps = 0;
         if(valid)
@@ -1573,7 +1599,7 @@ namespace isaac_hip
         extern __shared__ float4 lds_lut[];
         stage_luts(lds_lut, Q);
         __syncthreads();
-        brick_march_body<DT, LINEAR, COUNT>(F, Q, PP, lds_lut, xcd_tile(F));
+        brick_march_body<DT, LINEAR, COUNT>(F, Q, PP, lds_lut, march_tile(F));
     }
 
     // trilinear march compiled for one source signature (see sig_count)
@@ -1583,7 +1609,7 @@ namespace isaac_hip
         extern __shared__ float4 lds_lut[];
         stage_luts(lds_lut, Q);
         __syncthreads();
-        brick_march_body<DT, true, false, SIG>(F, Q, PP, lds_lut, xcd_tile(F));
+        brick_march_body<DT, true, false, SIG>(F, Q, PP, lds_lut, march_tile(F));
     }
 
     // ------------------------------------------------------------------ batched form
@@ -1768,6 +1794,17 @@ namespace isaac_hip
 
     static_assert(sizeof(FrameParams) + sizeof(PostParams) + sizeof(ParticleParams) <= 4096, "kernel argument block");
 
+    // workgroups of a march launch: one per tile, or the padded super-tile grid
+    unsigned march_grid(const FrameParams& F)
+    {
+        const unsigned nb = unsigned(F.tilesX) * unsigned(F.tilesY);
+        const unsigned S = unsigned(F.tileOrder);
+        if(S == 0 || nb == 0)
+            return nb;
+        const unsigned ns = ((unsigned(F.tilesX) + S - 1u) / S) * ((unsigned(F.tilesY) + S - 1u) / S);
+        return (ns + 7u) / 8u * 8u * S * S;
+    }
+
     int post_signature(const PostParams& Q)
     {
         int sig = 0;
@@ -1778,7 +1815,7 @@ namespace isaac_hip
 
     hipError_t launch_brick_march(const FrameParams& F, const PostParams& Q, const ParticleParams& PP, bool counting, hipStream_t s)
     {
-        const unsigned nb = unsigned(F.tilesX) * unsigned(F.tilesY);
+        const unsigned nb = march_grid(F);
         if(nb == 0)
             return hipSuccess;
         const dim3 grid(nb), block(256);
@@ -2282,7 +2319,7 @@ namespace isaac_hip
 
     hipError_t launch_march(const FrameParams& F, const ParticleParams& PP, bool counting, hipStream_t s)
     {
-        const unsigned nb = unsigned(F.tilesX) * unsigned(F.tilesY);
+        const unsigned nb = march_grid(F);
         if(nb == 0)
             return hipSuccess;
         const dim3 grid(nb), block(256);

@@ -424,6 +424,7 @@ class IsaacVisualization:
 
     TUNE_MARCH_PREFETCH = 0
     TUNE_MARCH_WAVES = 1
+    TUNE_MARCH_TILES = 2
 
     def set_tuning(self, key: int, value: int) -> None:
         check(lib().isaac_hip_set_tuning(self._ctx, int(key), int(value)), self._ctx, "set_tuning")

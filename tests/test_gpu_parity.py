@@ -197,3 +197,33 @@ def test_production_full_size(gpu, oracle_lib):
     c = vis.counters()
     print("gpu counters", c, "oracle", ora.counters, flush=True)
     assert abs(c["vol_samples"] - ora.counters["vol_samples"]) <= 1e-3 * ora.counters["vol_samples"]
+
+
+@pytest.mark.parametrize("form", ["combined", "bricks-f16"])
+def test_schedules_do_not_change_the_frame(gpu, oracle_lib, form):
+    """Every march schedule (tile order over the XCDs, occupancy target, prefetch)
+    renders the same frame; the counting march executes the same steps in every
+    tile order."""
+    spec = _reduced_prod(n=64, fb=(330, 250), particles=20000, visible_tf=True)
+    vis = _vis(spec, gpu)
+    if form != "combined":
+        vis.set_render_form(vis.FORM_BRICKS, vis.F16)
+    vis.update_buffers()
+    ref = ref_counts = None
+    for tiles in (0, 2, 4, 8, 16):
+        vis.set_tuning(vis.TUNE_MARCH_TILES, tiles)
+        for waves, prefetch in ((0, 0), (6, 0), (8, 0), (0, 1)):
+            vis.set_tuning(vis.TUNE_MARCH_WAVES, waves)
+            vis.set_tuning(vis.TUNE_MARCH_PREFETCH, prefetch)
+            img = vis.draw()
+            if ref is None:
+                ref = img
+            assert np.array_equal(img, ref), (tiles, waves, prefetch)
+        vis.set_counting(True)
+        assert np.array_equal(vis.draw(), ref), tiles
+        c = {k: v for k, v in vis.counters().items() if k != "march_wave_steps"}
+        vis.set_counting(False)
+        if ref_counts is None:
+            ref_counts = c
+        assert c == ref_counts, tiles
+    vis.close()
