@@ -346,6 +346,24 @@ extern "C"
     int isaac_hip_composite(int32_t n, const uint8_t* const* images, const int32_t* order, int64_t pixels, uint8_t* out,
                             void* stream);
 
+    /* ---------------------------------------------------------------- frame message
+     * The video frame serverSendFrame sends without libjpeg (ISAAC_JPEG off,
+     * lib/isaac/isaac_communicator.hpp:217-287; the JPEG path is not built).
+     *
+     * isaac_hip_base64: n bytes (device, 4-byte aligned) -> ceil(8n/6) characters (device,
+     * 16-byte aligned): boost base64_from_binary<transform_width<., 6, 8>> as the reference
+     * uses it (isaac_communicator.hpp:251-263): standard alphabet, most significant bits
+     * first, the last partial 6-bit group zero-filled, no '=' padding.  Enqueued on stream.
+     *
+     * isaac_hip_frame_message: msg (host) = {"payload": "data:image/raw-rgba;base64,<base64 of
+     * the width*height RGBA8 device image>"} with a terminating 0; *length = its length
+     * without the 0 (also set when capacity is too small, which returns ISAAC_HIP_ERR_ARG);
+     * scratch: device buffer of ceil(8n/6) characters, n = 4*width*height.  Blocks until
+     * the message is on the host. */
+    int isaac_hip_base64(const uint8_t* in, int64_t n, char* out, void* stream);
+    int isaac_hip_frame_message(const uint8_t* rgba, int32_t width, int32_t height, char* scratch, char* msg, int64_t capacity,
+                                int64_t* length, void* stream);
+
     /* ---------------------------------------------------------------- learned occlusion
      * No counterpart in the reference.  A U-Net over the G-buffer (input: depth /
      * maxGlobalSizeScaled, background flag, normal; 3x3 convolutions with 16/32/64/64

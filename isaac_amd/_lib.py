@@ -133,6 +133,8 @@ _SIGS = {
     "isaac_hip_batch_get_images": (ctypes.c_int, [P, P, P]),
     "isaac_hip_composite_order": (ctypes.c_int, [ctypes.c_int32, P, P, P, P, P, P]),
     "isaac_hip_composite": (ctypes.c_int, [ctypes.c_int32, P, P, ctypes.c_int64, P, P]),
+    "isaac_hip_base64": (ctypes.c_int, [P, ctypes.c_int64, P, P]),
+    "isaac_hip_frame_message": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, P, P, ctypes.c_int64, P, P]),
     "isaac_hip_aonet_param_count": (ctypes.c_int64, []),
     "isaac_hip_aonet_set_params": (ctypes.c_int, [P, P, ctypes.c_int64]),
     "isaac_hip_aonet_get_params": (ctypes.c_int, [P, P, ctypes.c_int64]),
