@@ -1921,107 +1921,149 @@ namespace isaac_hip
     }
 
     // ------------------------------------------------------------------ launchers
-    // Classification compiled for the signature of its active sources (component
-    // counts 1 or 3, at most two sources, all transfer functions in LDS, x-contiguous
-    // sources with rows of a multiple of 4 voxels, no advection texture): each
-    // thread classifies 4 consecutive voxels of a row, loads every source's values
-    // with 16-byte loads before classifying any, and stores the 4 texels with two
-    // 16-byte stores.  Same arithmetic, in the same order, as classify_kernel.
+    // the 4 voxels of quad q (x, y, z of its first voxel): every source's components,
+    // f[s][k * D + c] for voxel k, component c (16-byte loads)
     template<int SIG>
-    __global__ __launch_bounds__(256) void classify_sig_kernel(const ClassifyParams P)
+    __device__ __attribute__((always_inline)) inline void cls_fetch(
+        const ClassifyParams& P, int64_t q, float (&f)[sig_count(SIG)][16], int& x, int& y, int& z)
     {
-        extern __shared__ float4 lds_lut[];
-        const int T = P.T;
-        for(int i = threadIdx.x; i < P.nLut * T; i += blockDim.x)
-            lds_lut[i] = P.luts[i];
-        __syncthreads();
         constexpr int N = sig_count(SIG);
-        const int64_t nq = P.nvox >> 2;
-        const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-        for(int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < nq; q += stride)
+        const uint32_t u = uint32_t(q) << 2;
+        const uint32_t zq = fastdiv(u, P.divXY_mul, P.divXY_shift);
+        const uint32_t rem = u - zq * uint32_t(P.X * P.Y);
+        const uint32_t yq = fastdiv(rem, P.divX_mul, P.divX_shift);
+        x = int(rem - yq * uint32_t(P.X));
+        y = int(yq);
+        z = int(zq);
+#pragma unroll
+        for(int s = 0; s < N; s++)
         {
-            const uint32_t u = uint32_t(q) << 2;
-            const uint32_t zq = fastdiv(u, P.divXY_mul, P.divXY_shift);
-            const uint32_t rem = u - zq * uint32_t(P.X * P.Y);
-            const uint32_t yq = fastdiv(rem, P.divX_mul, P.divX_shift);
-            const int x = int(rem - yq * uint32_t(P.X)), y = int(yq), z = int(zq);
-            // the 4 voxels' components of every source: f[s][4 * D] (voxel k, component c at k * D + c)
-            float f[N][16];
+            const int D = sig_dim(SIG, s);
+            const ClassifySource& S = P.src[s];
+            const float4* e = reinterpret_cast<const float4*>(
+                S.data + (int64_t(x) + int64_t(y) * S.stride[1] + int64_t(z) * S.stride[2]) * D);
+#pragma unroll
+            for(int j = 0; j < D; j++)
+            {
+                const float4 w = e[j];
+                f[s][4 * j + 0] = w.x;
+                f[s][4 * j + 1] = w.y;
+                f[s][4 * j + 2] = w.z;
+                f[s][4 * j + 3] = w.w;
+            }
+        }
+    }
+
+    // classification of one fetched quad, its 4 texels stored with two 16-byte stores
+    template<int SIG>
+    __device__ __attribute__((always_inline)) inline void cls_emit(
+        const ClassifyParams& P, const float4* lds_lut, const float (&f)[sig_count(SIG)][16], int x, int y, int z, int64_t q)
+    {
+        constexpr int N = sig_count(SIG);
+        const int T = P.T;
+        const uint32_t u = uint32_t(q) << 2;
+        uint32_t texel[8];
+#pragma unroll
+        for(int k = 0; k < 4; k++)
+        {
+            float vr = 0.f, vg = 0.f, vb = 0.f, va = 0.f;
+            float ir = 0.f, ig = 0.f, ib = 0.f, ia = 0.f;
 #pragma unroll
             for(int s = 0; s < N; s++)
             {
                 const int D = sig_dim(SIG, s);
                 const ClassifySource& S = P.src[s];
-                const float4* e = reinterpret_cast<const float4*>(
-                    S.data + (int64_t(x) + int64_t(y) * S.stride[1] + int64_t(z) * S.stride[2]) * D);
-#pragma unroll
-                for(int j = 0; j < D; j++)
+                const float c0 = f[s][k * D];
+                const float c1 = D > 1 ? f[s][k * D + 1] : 0.f;
+                const float c2 = D > 2 ? f[s][k * D + 2] : 0.f;
+                const float tv = apply_chain(S.chain, c0, c1, c2, 0.f, D);
+                const float4 col = lds_lut[S.lutSlot * T + tf_index(tv, T)];
+                const float ca = col.w;
+                if(S.weight > 0.0f)
                 {
-                    const float4 w = e[j];
-                    f[s][4 * j + 0] = w.x;
-                    f[s][4 * j + 1] = w.y;
-                    f[s][4 * j + 2] = w.z;
-                    f[s][4 * j + 3] = w.w;
+                    float a = ca * S.weight;
+                    vr = vr + col.x * a;
+                    vg = vg + col.y * a;
+                    vb = vb + col.z * a;
+                    va = va + a;
+                }
+                if(S.iso > 0.0f)
+                {
+                    float a = ca / S.iso * 0.5f;
+                    if(ia < a)
+                    {
+                        ir = col.x;
+                        ig = col.y;
+                        ib = col.z;
+                        ia = a;
+                    }
                 }
             }
-            uint32_t texel[8];
-#pragma unroll
-            for(int k = 0; k < 4; k++)
+            if(P.totalWeight != 0.0f)
             {
-                float vr = 0.f, vg = 0.f, vb = 0.f, va = 0.f;
-                float ir = 0.f, ig = 0.f, ib = 0.f, ia = 0.f;
-#pragma unroll
-                for(int s = 0; s < N; s++)
-                {
-                    const int D = sig_dim(SIG, s);
-                    const ClassifySource& S = P.src[s];
-                    const float c0 = f[s][k * D];
-                    const float c1 = D > 1 ? f[s][k * D + 1] : 0.f;
-                    const float c2 = D > 2 ? f[s][k * D + 2] : 0.f;
-                    const float tv = apply_chain(S.chain, c0, c1, c2, 0.f, D);
-                    const float4 col = lds_lut[S.lutSlot * T + tf_index(tv, T)];
-                    const float ca = col.w;
-                    if(S.weight > 0.0f)
-                    {
-                        float a = ca * S.weight;
-                        vr = vr + col.x * a;
-                        vg = vg + col.y * a;
-                        vb = vb + col.z * a;
-                        va = va + a;
-                    }
-                    if(S.iso > 0.0f)
-                    {
-                        float a = ca / S.iso * 0.5f;
-                        if(ia < a)
-                        {
-                            ir = col.x;
-                            ig = col.y;
-                            ib = col.z;
-                            ia = a;
-                        }
-                    }
-                }
-                if(P.totalWeight != 0.0f)
-                {
-                    vr = vr / P.totalWeight;
-                    vg = vg / P.totalWeight;
-                    vb = vb / P.totalWeight;
-                    va = va / P.totalWeight;
-                }
-                else
-                {
-                    vr = vg = vb = va = 0.0f;
-                }
-                texel[2 * k] = dither_b4(P.ditherMode, x + k, y, z, vr, vg, vb, va);
-                const uint32_t q0 = (uint32_t) roundf(gclamp(ir, 0.0f, 1.0f) * 255.0f);
-                const uint32_t q1 = (uint32_t) roundf(gclamp(ig, 0.0f, 1.0f) * 255.0f);
-                const uint32_t q2 = (uint32_t) roundf(gclamp(ib, 0.0f, 1.0f) * 255.0f);
-                const uint32_t q3 = (uint32_t) roundf(gclamp(ia, 0.0f, 1.0f) * 255.0f);
-                texel[2 * k + 1] = q0 | (q1 << 8) | (q2 << 16) | (q3 << 24);
+                vr = vr / P.totalWeight;
+                vg = vg / P.totalWeight;
+                vb = vb / P.totalWeight;
+                va = va / P.totalWeight;
             }
-            uint4* o = reinterpret_cast<uint4*>(P.tex + u);
-            o[0] = make_uint4(texel[0], texel[1], texel[2], texel[3]);
-            o[1] = make_uint4(texel[4], texel[5], texel[6], texel[7]);
+            else
+            {
+                vr = vg = vb = va = 0.0f;
+            }
+            texel[2 * k] = dither_b4(P.ditherMode, x + k, y, z, vr, vg, vb, va);
+            const uint32_t q0 = (uint32_t) roundf(gclamp(ir, 0.0f, 1.0f) * 255.0f);
+            const uint32_t q1 = (uint32_t) roundf(gclamp(ig, 0.0f, 1.0f) * 255.0f);
+            const uint32_t q2 = (uint32_t) roundf(gclamp(ib, 0.0f, 1.0f) * 255.0f);
+            const uint32_t q3 = (uint32_t) roundf(gclamp(ia, 0.0f, 1.0f) * 255.0f);
+            texel[2 * k + 1] = q0 | (q1 << 8) | (q2 << 16) | (q3 << 24);
+        }
+        uint4* o = reinterpret_cast<uint4*>(P.tex + u);
+        o[0] = make_uint4(texel[0], texel[1], texel[2], texel[3]);
+        o[1] = make_uint4(texel[4], texel[5], texel[6], texel[7]);
+    }
+
+    // Classification compiled for the signature of its active sources (component
+    // counts 1 or 3, at most two sources, all transfer functions in LDS, x-contiguous
+    // sources with rows of a multiple of 4 voxels, no advection texture): each
+    // thread classifies 4 consecutive voxels of a row per iteration, with every
+    // source's values of its next quad requested (16-byte loads) before the current
+    // quad is classified, and stores the 4 texels with two 16-byte stores.  Same
+    // arithmetic, in the same order, as classify_kernel.
+    template<int SIG>
+    __global__ __launch_bounds__(256) void classify_sig_kernel(const ClassifyParams P)
+    {
+        extern __shared__ float4 lds_lut[];
+        for(int i = threadIdx.x; i < P.nLut * P.T; i += blockDim.x)
+            lds_lut[i] = P.luts[i];
+        __syncthreads();
+        constexpr int N = sig_count(SIG);
+        const int64_t nq = P.nvox >> 2;
+        const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+        int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+        if(q >= nq)
+            return;
+        float f[N][16];
+        int x, y, z;
+        cls_fetch<SIG>(P, q, f, x, y, z);
+        for(;;)
+        {
+            const int64_t qn = q + stride;
+            float g[N][16];
+            int xn = 0, yn = 0, zn = 0;
+            if(qn < nq)
+                cls_fetch<SIG>(P, qn, g, xn, yn, zn);
+            cls_emit<SIG>(P, lds_lut, f, x, y, z, q);
+            if(qn >= nq)
+                break;
+            q = qn;
+            x = xn;
+            y = yn;
+            z = zn;
+#pragma unroll
+            for(int s = 0; s < N; s++)
+#pragma unroll
+                for(int j = 0; j < 16; j++)
+                    f[s][j] = g[s][j];
         }
     }
 
