@@ -231,6 +231,25 @@ def compositing(n_images: int = 8, width: int = 1920, height: int = 1080, reps: 
             "GBs": nbytes / (us * 1e-6) / 1e9, "hbm_frac": nbytes / (us * 1e-6) / 8.0e12}
 
 
+def jpeg_frames(vis_frame, quality: int = 90, reps: int = 10):
+    """The default video-frame path (isaac_hip_jpeg_encode, libjpeg-turbo's baseline
+    JPEG reproduced on the GPU) on one rendered 1080p frame: wall time per frame
+    including the copy of the file to the host, and the file size."""
+    import torch
+
+    from isaac_amd import frame as K
+
+    dev = torch.from_numpy(vis_frame).cuda()
+    jpg = K.jpeg(dev, quality)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        jpg = K.jpeg(dev, quality)
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    return {"framebuffer": [int(vis_frame.shape[1]), int(vis_frame.shape[0])], "quality": quality, "ms": ms,
+            "bytes": len(jpg)}
+
+
 def batched_workload(args):
     """48 sub-volumes of side U{32..256} (FP16 bricks: Gaussian-mixture density with an
     iso-surface + float3 noise via length) x 8 seeded views at 960x540 with learned
@@ -431,6 +450,8 @@ def main() -> int:
         line["forms"] = forms
     if not args.no_learned_ao:
         line["learned_ao"] = learned_ao(vis, poses, out, args)
+    vis.set_render_form(*FORMS["combined"])
+    line["jpeg_frame"] = jpeg_frames(vis.draw())
     vis.close()
     if not args.no_batch:
         line["batched"] = batched_workload(args)

@@ -364,6 +364,23 @@ extern "C"
     int isaac_hip_frame_message(const uint8_t* rgba, int32_t width, int32_t height, char* scratch, char* msg, int64_t capacity,
                                 int64_t* length, void* stream);
 
+    /* The JPEG path of serverSendFrame (the reference's default): libjpeg-turbo with
+     * JCS_EXT_RGBX input, jpeg_set_defaults and jpeg_set_quality(quality, FALSE)
+     * (isaac_communicator.hpp:221-250), reproduced byte for byte on the GPU: baseline
+     * 4:2:0 JPEG of the width x height RGBX frame (device, 4-byte aligned; the 4th byte
+     * is ignored), quality 1..100 (out-of-range values clamp, as jpeg_set_quality does).
+     * workspace: device, 256-byte aligned, isaac_hip_jpeg_workspace_bytes(width, height)
+     * bytes (-1 for sizes outside 1..65535); the file goes to jpeg (host, capacity bytes),
+     * *length its size (also set when capacity is too small, which returns
+     * ISAAC_HIP_ERR_ARG).  Blocks until the file is on the host.
+     * isaac_hip_jpeg_frame_message: {"payload": "data:image/jpeg;base64,<file>"} with a
+     * terminating 0 into msg (host), base64 as for isaac_hip_base64. */
+    int64_t isaac_hip_jpeg_workspace_bytes(int32_t width, int32_t height);
+    int isaac_hip_jpeg_encode(const uint8_t* rgbx, int32_t width, int32_t height, int32_t quality, void* workspace,
+                              int64_t workspace_bytes, uint8_t* jpeg, int64_t capacity, int64_t* length, void* stream);
+    int isaac_hip_jpeg_frame_message(const uint8_t* rgbx, int32_t width, int32_t height, int32_t quality, void* workspace,
+                                     int64_t workspace_bytes, char* msg, int64_t capacity, int64_t* length, void* stream);
+
     /* ---------------------------------------------------------------- learned occlusion
      * No counterpart in the reference.  A U-Net over the G-buffer (input: depth /
      * maxGlobalSizeScaled, background flag, normal; 3x3 convolutions with 16/32/64/64

@@ -135,6 +135,11 @@ _SIGS = {
     "isaac_hip_composite": (ctypes.c_int, [ctypes.c_int32, P, P, ctypes.c_int64, P, P]),
     "isaac_hip_base64": (ctypes.c_int, [P, ctypes.c_int64, P, P]),
     "isaac_hip_frame_message": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, P, P, ctypes.c_int64, P, P]),
+    "isaac_hip_jpeg_workspace_bytes": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32]),
+    "isaac_hip_jpeg_encode": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, P, ctypes.c_int64, P,
+                                             ctypes.c_int64, P, P]),
+    "isaac_hip_jpeg_frame_message": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, P, ctypes.c_int64, P,
+                                                    ctypes.c_int64, P, P]),
     "isaac_hip_aonet_param_count": (ctypes.c_int64, []),
     "isaac_hip_aonet_set_params": (ctypes.c_int, [P, P, ctypes.c_int64]),
     "isaac_hip_aonet_get_params": (ctypes.c_int, [P, P, ctypes.c_int64]),

@@ -1,6 +1,7 @@
 """The reference's video frame message built from a frame in device memory
-(serverSendFrame without libjpeg, lib/isaac/isaac_communicator.hpp:217-287),
-through the C-ABI (isaac_hip_base64, isaac_hip_frame_message)."""
+(serverSendFrame, lib/isaac/isaac_communicator.hpp:217-287): the raw-RGBA
+payload (isaac_hip_base64, isaac_hip_frame_message) and the default JPEG path
+(isaac_hip_jpeg_encode, isaac_hip_jpeg_frame_message), through the C-ABI."""
 from __future__ import annotations
 
 import ctypes
@@ -45,4 +46,42 @@ def frame_message(rgba) -> bytes:
     return buf.raw[:length.value]
 
 
-__all__ = ["base64", "base64_length", "frame_message"]
+def _frame_args(rgba):
+    import torch
+
+    if not rgba.is_cuda or rgba.dtype != torch.uint8 or not rgba.is_contiguous() or rgba.dim() != 3 or rgba.shape[2] != 4:
+        raise TypeError("rgba must be a contiguous CUDA uint8 tensor of shape (H, W, 4)")
+    H, W = int(rgba.shape[0]), int(rgba.shape[1])
+    nbytes = int(lib().isaac_hip_jpeg_workspace_bytes(W, H))
+    if nbytes < 0:
+        raise ValueError("JPEG frames are 1..65535 pixels a side")
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=rgba.device)
+    stream = torch.cuda.current_stream(rgba.device).cuda_stream
+    return W, H, ws, stream
+
+
+def jpeg(rgba, quality: int = 90) -> bytes:
+    """The JPEG file libjpeg-turbo writes for the frame (JCS_EXT_RGBX, jpeg_set_defaults,
+    jpeg_set_quality(quality, FALSE)), encoded on the GPU; rgba: (H, W, 4) CUDA uint8."""
+    W, H, ws, stream = _frame_args(rgba)
+    cap = ws.numel() + 4096
+    buf = ctypes.create_string_buffer(cap)
+    n = ctypes.c_int64(0)
+    check(lib().isaac_hip_jpeg_encode(ctypes.c_void_p(rgba.data_ptr()), W, H, int(quality), ctypes.c_void_p(ws.data_ptr()),
+                                      ws.numel(), buf, cap, ctypes.byref(n), ctypes.c_void_p(stream)), None, "jpeg_encode")
+    return buf.raw[:n.value]
+
+
+def jpeg_frame_message(rgba, quality: int = 90) -> bytes:
+    """{"payload": "data:image/jpeg;base64,..."} for an (H, W, 4) CUDA uint8 frame."""
+    W, H, ws, stream = _frame_args(rgba)
+    cap = 2 * ws.numel() + 8192
+    buf = ctypes.create_string_buffer(cap)
+    n = ctypes.c_int64(0)
+    check(lib().isaac_hip_jpeg_frame_message(ctypes.c_void_p(rgba.data_ptr()), W, H, int(quality),
+                                             ctypes.c_void_p(ws.data_ptr()), ws.numel(), buf, cap, ctypes.byref(n),
+                                             ctypes.c_void_p(stream)), None, "jpeg_frame_message")
+    return buf.raw[:n.value]
+
+
+__all__ = ["base64", "base64_length", "frame_message", "jpeg", "jpeg_frame_message"]

@@ -227,3 +227,15 @@ def test_schedules_do_not_change_the_frame(gpu, oracle_lib, form):
             ref_counts = c
         assert c == ref_counts, tiles
     vis.close()
+
+
+def test_tuning_rejects_unknown_values(gpu):
+    from isaac_amd._lib import IsaacHipError
+
+    vis = _vis(_reduced_prod(n=16, fb=(32, 32), particles=100), gpu)
+    for key, bad in ((vis.TUNE_MARCH_TILES, 3), (vis.TUNE_MARCH_TILES, 32), (vis.TUNE_MARCH_WAVES, 5), (99, 0)):
+        with pytest.raises(IsaacHipError):
+            vis.set_tuning(key, bad)
+    for tiles in (0, 2, 4, 8, 16):
+        vis.set_tuning(vis.TUNE_MARCH_TILES, tiles)
+    vis.close()
