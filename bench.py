@@ -451,6 +451,7 @@ def main() -> int:
     if not args.no_learned_ao:
         line["learned_ao"] = learned_ao(vis, poses, out, args)
     vis.set_render_form(*FORMS["combined"])
+    vis.update_buffers()
     line["jpeg_frame"] = jpeg_frames(vis.draw())
     vis.close()
     if not args.no_batch:

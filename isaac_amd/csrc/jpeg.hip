@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "isaac_hip.h"
@@ -712,9 +713,10 @@ extern "C"
         if(!msg || !length || width < 1 || height < 1)
             return ISAAC_HIP_ERR_ARG;
         const JpegLayout L = jpeg_layout(width, height);
-        std::vector<uint8_t> jpg(size_t(L.maxBytes) * 2 + 4096);
+        const size_t cap = size_t(L.maxBytes) * 2 + 4096;
+        std::unique_ptr<uint8_t[]> jpg(new uint8_t[cap]); // no zero fill: the encoder writes what is read
         int64_t n = 0;
-        int rc = isaac_hip_jpeg_encode(rgbx, width, height, quality, workspace, workspace_bytes, jpg.data(), int64_t(jpg.size()), &n, stream);
+        int rc = isaac_hip_jpeg_encode(rgbx, width, height, quality, workspace, workspace_bytes, jpg.get(), int64_t(cap), &n, stream);
         if(rc)
             return rc;
         static const char header[] = "{\"payload\": \"data:image/jpeg;base64,";

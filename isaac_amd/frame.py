@@ -46,42 +46,48 @@ def frame_message(rgba) -> bytes:
     return buf.raw[:length.value]
 
 
+_SCRATCH = {}
+
+
 def _frame_args(rgba):
+    """frame size, the encoder's device workspace and a host buffer for the file
+    (both kept per device and frame size), and torch's current stream"""
     import torch
 
     if not rgba.is_cuda or rgba.dtype != torch.uint8 or not rgba.is_contiguous() or rgba.dim() != 3 or rgba.shape[2] != 4:
         raise TypeError("rgba must be a contiguous CUDA uint8 tensor of shape (H, W, 4)")
     H, W = int(rgba.shape[0]), int(rgba.shape[1])
-    nbytes = int(lib().isaac_hip_jpeg_workspace_bytes(W, H))
-    if nbytes < 0:
-        raise ValueError("JPEG frames are 1..65535 pixels a side")
-    ws = torch.empty(nbytes, dtype=torch.uint8, device=rgba.device)
+    key = (rgba.device.index, W, H)
+    if key not in _SCRATCH:
+        nbytes = int(lib().isaac_hip_jpeg_workspace_bytes(W, H))
+        if nbytes < 0:
+            raise ValueError("JPEG frames are 1..65535 pixels a side")
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=rgba.device)
+        _SCRATCH[key] = (ws, ctypes.create_string_buffer(2 * nbytes + 8192))
+    ws, host = _SCRATCH[key]
     stream = torch.cuda.current_stream(rgba.device).cuda_stream
-    return W, H, ws, stream
+    return W, H, ws, host, stream
 
 
 def jpeg(rgba, quality: int = 90) -> bytes:
     """The JPEG file libjpeg-turbo writes for the frame (JCS_EXT_RGBX, jpeg_set_defaults,
     jpeg_set_quality(quality, FALSE)), encoded on the GPU; rgba: (H, W, 4) CUDA uint8."""
-    W, H, ws, stream = _frame_args(rgba)
-    cap = ws.numel() + 4096
-    buf = ctypes.create_string_buffer(cap)
+    W, H, ws, host, stream = _frame_args(rgba)
     n = ctypes.c_int64(0)
     check(lib().isaac_hip_jpeg_encode(ctypes.c_void_p(rgba.data_ptr()), W, H, int(quality), ctypes.c_void_p(ws.data_ptr()),
-                                      ws.numel(), buf, cap, ctypes.byref(n), ctypes.c_void_p(stream)), None, "jpeg_encode")
-    return buf.raw[:n.value]
+                                      ws.numel(), host, len(host), ctypes.byref(n), ctypes.c_void_p(stream)), None,
+          "jpeg_encode")
+    return ctypes.string_at(host, n.value)
 
 
 def jpeg_frame_message(rgba, quality: int = 90) -> bytes:
     """{"payload": "data:image/jpeg;base64,..."} for an (H, W, 4) CUDA uint8 frame."""
-    W, H, ws, stream = _frame_args(rgba)
-    cap = 2 * ws.numel() + 8192
-    buf = ctypes.create_string_buffer(cap)
+    W, H, ws, host, stream = _frame_args(rgba)
     n = ctypes.c_int64(0)
     check(lib().isaac_hip_jpeg_frame_message(ctypes.c_void_p(rgba.data_ptr()), W, H, int(quality),
-                                             ctypes.c_void_p(ws.data_ptr()), ws.numel(), buf, cap, ctypes.byref(n),
+                                             ctypes.c_void_p(ws.data_ptr()), ws.numel(), host, len(host), ctypes.byref(n),
                                              ctypes.c_void_p(stream)), None, "jpeg_frame_message")
-    return buf.raw[:n.value]
+    return ctypes.string_at(host, n.value)
 
 
 __all__ = ["base64", "base64_length", "frame_message", "jpeg", "jpeg_frame_message"]
