@@ -338,8 +338,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(
-            f"bench.py --gpus {args.gpus}: this build renders one rank's sub-volume on one GPU; "
-            "multi-GPU sort-last compositing is not part of it, so only --gpus 1 is measured.",
+            f"bench.py --gpus {args.gpus}: the library renders one rank's sub-volume on one GPU; "
+            "the exchange of multi-GPU sort-last compositing is not implemented, so only --gpus 1 is measured.",
             file=sys.stderr,
         )
         return 2

@@ -1,6 +1,6 @@
 """GPU parity: the gfx950 render path against the CPU oracle on identical seeded inputs.
 
-Bars (BASELINE north star):
+Bars:
   * combined RGBA8 textures (classification) ........ bit-exact
   * particle cell lists (counts, offsets, perm) ..... bit-exact
   * RGBA8 framebuffer ................................ |diff| <= 2/255 per channel on >= 99.9 % of pixels
