@@ -443,6 +443,7 @@ namespace
         p.B.rank = (uint32_t*) p.blocks[1];
         p.B.rel = (float4*) p.blocks[2];
         p.B.count64 = (unsigned long long*) p.blocks[3];
+        p.B.ncells = nc;
         p.B.count = (int32_t*) p.blocks[4];
         p.B.offset = (int32_t*) p.blocks[5];
         p.B.voffset = (int32_t*) p.blocks[6];

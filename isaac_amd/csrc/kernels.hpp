@@ -67,7 +67,8 @@ namespace isaac_hip
         int32_t* cellOf; // n
         uint32_t* rank; // n: arrival rank of the particle inside its cell
         float4* rel; // n: rel.xyz, lut index (or -1 if invisible) in .w
-        unsigned long long* count64; // ncells: all particles (low 32 bits), visible ones (high 32 bits)
+        unsigned long long* count64; // 2 * ncells uint32: all particles per cell, then visible ones per cell
+        int64_t ncells;
         int32_t* count; // ncells (all particles)
         int32_t* offset; // ncells + 1
         int32_t* voffset; // ncells + 1

@@ -96,8 +96,12 @@ namespace isaac_hip
             const bool visible = !P.filter || P.lut[li].w >= 0.5f;
             B.cellOf[i] = c;
             B.rel[i] = make_float4(qx - float(cx), qy - float(cy), qz - float(cz), __int_as_float(visible ? li : -1));
-            const unsigned long long inc = 1ull | (visible ? (1ull << 32) : 0ull);
-            B.rank[i] = uint32_t(atomicAdd(&B.count64[c], inc));
+            // all particles: a returning 32-bit add (the arrival rank); visible ones: a second
+            // count in the upper half of the counter buffer (no return needed)
+            uint32_t* cnt = reinterpret_cast<uint32_t*>(B.count64);
+            B.rank[i] = atomicAdd(cnt + c, 1u);
+            if(visible)
+                (void) atomicAdd(cnt + B.ncells + c, 1u);
         }
     }
 
@@ -156,11 +160,18 @@ namespace isaac_hip
         return sh[tid];
     }
 
+    // cell i's counts packed for the scan: visible (high 32 bits), all (low 32 bits)
+    __device__ inline unsigned long long cell_counts(const unsigned long long* in, int64_t n, int64_t i)
+    {
+        const uint32_t* c = reinterpret_cast<const uint32_t*>(in);
+        return i < n ? (static_cast<unsigned long long>(c[n + i]) << 32) | c[i] : 0ull;
+    }
+
     __global__ __launch_bounds__(kScanBlock) void scan_reduce_kernel(const unsigned long long* in, int64_t n, unsigned long long* blockSums)
     {
         __shared__ unsigned long long sh[kScanBlock];
         const int64_t i = int64_t(blockIdx.x) * kScanBlock + threadIdx.x;
-        const unsigned long long v = i < n ? in[i] : 0ull;
+        const unsigned long long v = cell_counts(in, n, i);
         const unsigned long long s = block_inclusive_scan(v, sh);
         if(threadIdx.x == kScanBlock - 1)
             blockSums[blockIdx.x] = s;
@@ -195,7 +206,7 @@ namespace isaac_hip
     {
         __shared__ unsigned long long sh[kScanBlock];
         const int64_t i = int64_t(blockIdx.x) * kScanBlock + threadIdx.x;
-        const unsigned long long v = i < n ? in[i] : 0ull;
+        const unsigned long long v = cell_counts(in, n, i);
         const unsigned long long s = block_inclusive_scan(v, sh);
         const unsigned long long base = blockSums[blockIdx.x];
         if(i < n)
