@@ -375,6 +375,13 @@ extern "C"
      * ISAAC_HIP_ERR_ARG).  Blocks until the file is on the host.
      * isaac_hip_jpeg_frame_message: {"payload": "data:image/jpeg;base64,<file>"} with a
      * terminating 0 into msg (host), base64 as for isaac_hip_base64. */
+    /* The video frame of the context's last render (its framebuffer, as isaac_hip_render left
+     * it), the message serverSendFrame sends after each doVisualization
+     * (lib/isaac.hpp:2845-2861): quality 1..100 the JPEG path, 0 the raw-RGBA path.  The
+     * device workspace is the context's own (allocated on first use, sized for the frame).
+     * isaac_hip_video_frame_capacity: a msg size that always suffices.  Blocks until done. */
+    int isaac_hip_video_frame_message(isaac_hip_ctx* ctx, int32_t quality, char* msg, int64_t capacity, int64_t* length);
+    int64_t isaac_hip_video_frame_capacity(const isaac_hip_ctx* ctx);
     int64_t isaac_hip_jpeg_workspace_bytes(int32_t width, int32_t height);
     int isaac_hip_jpeg_encode(const uint8_t* rgbx, int32_t width, int32_t height, int32_t quality, void* workspace,
                               int64_t workspace_bytes, uint8_t* jpeg, int64_t capacity, int64_t* length, void* stream);

@@ -340,6 +340,20 @@ namespace isaac
             return frame;
         }
 
+        // the message serverSendFrame sends for the last frame (lib/isaac.hpp:2845-2861):
+        // {"payload": "data:image/jpeg;base64,..."} (quality 1..100, the communicator's default
+        // 90) or the raw-RGBA payload (quality 0); JPEG and base64 computed on the GPU
+        std::string videoFrameMessage(int quality = 90)
+        {
+            const int64_t cap = isaac_hip_video_frame_capacity(ctx);
+            if(cap <= 0)
+                throw std::runtime_error("isaac: video frame capacity");
+            message.resize(size_t(cap));
+            int64_t n = 0;
+            check(isaac_hip_video_frame_message(ctx, quality, message.data(), cap, &n), "video_frame_message");
+            return std::string(message.data(), size_t(n));
+        }
+
         isaac_hip_ctx* handle() const
         {
             return ctx;
@@ -468,5 +482,6 @@ namespace isaac
         T_ParticleList& particleSources;
         isaac_hip_ctx* ctx = nullptr;
         std::vector<uint8_t> frame;
+        std::vector<char> message;
     };
 } // namespace isaac

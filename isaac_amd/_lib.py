@@ -136,6 +136,8 @@ _SIGS = {
     "isaac_hip_base64": (ctypes.c_int, [P, ctypes.c_int64, P, P]),
     "isaac_hip_frame_message": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, P, P, ctypes.c_int64, P, P]),
     "isaac_hip_jpeg_workspace_bytes": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32]),
+    "isaac_hip_video_frame_message": (ctypes.c_int, [P, ctypes.c_int32, P, ctypes.c_int64, P]),
+    "isaac_hip_video_frame_capacity": (ctypes.c_int64, [P]),
     "isaac_hip_jpeg_encode": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, P, ctypes.c_int64, P,
                                              ctypes.c_int64, P, P]),
     "isaac_hip_jpeg_frame_message": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, P, ctypes.c_int64, P,

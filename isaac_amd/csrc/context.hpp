@@ -48,6 +48,10 @@ struct isaac_hip_ctx
     int device = 0;
     hipStream_t stream = nullptr;
     bool ownStream = false;
+    // video frame encoding (isaac_hip_video_frame_message): device workspace, lazily sized
+    void* videoWs = nullptr;
+    int64_t videoWsBytes = 0;
+    bool rendered = false; // a frame is in the colour buffer
     std::string err;
     int nv = 0, nf = 0, np = 0, nsrc = 0;
     std::vector<VolSlot> vols;

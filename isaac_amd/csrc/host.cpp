@@ -882,6 +882,8 @@ extern "C"
         for(auto& ev : c->ev)
             if(ev)
                 (void) hipEventDestroy(ev);
+        if(c->videoWs)
+            (void) hipFree(c->videoWs);
         if(c->ownStream && c->stream)
             (void) hipStreamDestroy(c->stream);
         delete c;
@@ -1450,9 +1452,46 @@ extern "C"
         }
         HIPCHK(c, hipEventRecord(c->ev[8], c->stream));
         c->evRecorded[1] = true;
+        c->rendered = true;
         if(out_rgba && !out_is_device)
             HIPCHK(c, hipStreamSynchronize(c->stream));
         return 0;
+    }
+
+    int isaac_hip_video_frame_message(isaac_hip_ctx* c, int32_t quality, char* msg, int64_t capacity, int64_t* length)
+    {
+        if(!c || !msg || !length || quality < 0)
+            return ISAAC_HIP_ERR_ARG;
+        if(!c->rendered)
+            return fail(c, ISAAC_HIP_ERR_STATE, "no frame rendered yet");
+        HIPCHK(c, hipSetDevice(c->device));
+        const int W = c->cfg.framebuffer_size[0], H = c->cfg.framebuffer_size[1];
+        const uint8_t* rgba = static_cast<const uint8_t*>(static_cast<void*>(c->color));
+        const int64_t need = quality > 0 ? isaac_hip_jpeg_workspace_bytes(W, H) : (8 * int64_t(W) * H * 4 + 5) / 6 + 16;
+        if(need < 0)
+            return fail(c, ISAAC_HIP_ERR_UNSUPPORTED, "video frames are 1..65535 pixels a side");
+        if(c->videoWsBytes < need)
+        {
+            if(c->videoWs)
+                (void) hipFree(c->videoWs);
+            c->videoWs = nullptr;
+            c->videoWsBytes = 0;
+            HIPCHK(c, hipMalloc(&c->videoWs, size_t(need)));
+            c->videoWsBytes = need;
+        }
+        const int rc = quality > 0
+            ? isaac_hip_jpeg_frame_message(rgba, W, H, quality, c->videoWs, c->videoWsBytes, msg, capacity, length, c->stream)
+            : isaac_hip_frame_message(rgba, W, H, static_cast<char*>(c->videoWs), msg, capacity, length, c->stream);
+        return rc ? fail(c, rc, rc == ISAAC_HIP_ERR_ARG ? "message buffer too small" : "video frame encoding failed") : 0;
+    }
+
+    int64_t isaac_hip_video_frame_capacity(const isaac_hip_ctx* c)
+    {
+        if(!c)
+            return -1;
+        const int64_t n = int64_t(c->cfg.framebuffer_size[0]) * c->cfg.framebuffer_size[1] * 4;
+        // raw RGBA base64, or a JPEG (at most twice the raw size with its markers) in base64
+        return (8 * (2 * n + 4096) + 5) / 6 + 64;
     }
 
     int isaac_hip_synchronize(isaac_hip_ctx* c)

@@ -405,6 +405,17 @@ class IsaacVisualization:
             self._d2h(n, noise.value)
         return a, n, int(ts.value)
 
+    def video_frame_message(self, quality: int = 90) -> bytes:
+        """The message serverSendFrame sends for the last rendered frame
+        (lib/isaac.hpp:2845-2861): {"payload": "data:image/jpeg;base64,..."} for quality
+        1..100 (libjpeg-turbo's bytes, encoded on the GPU), the raw-RGBA payload for 0."""
+        cap = int(lib().isaac_hip_video_frame_capacity(self._ctx))
+        buf = ctypes.create_string_buffer(cap)
+        n = ctypes.c_int64(0)
+        check(lib().isaac_hip_video_frame_message(self._ctx, int(quality), buf, cap, ctypes.byref(n)), self._ctx,
+              "video_frame_message")
+        return ctypes.string_at(buf, n.value)
+
     def min_max(self) -> Tuple[np.ndarray, np.ndarray]:
         """The min/max request (sendMinMax, lib/isaac.hpp:1951-2006): per source in the
         combined order, min and max of its functor-chain values (float32 arrays).

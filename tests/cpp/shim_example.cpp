@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -91,6 +92,11 @@ int main(int argc, char** argv)
         std::printf("shim_example: rendered %lldx%lld, kernel %llu us\n", (long long) W, (long long) H,
                     (unsigned long long) vis.kernelTime);
         std::printf("minmax %.9g %.9g\n", double(vis.minMaxArray.min[0]), double(vis.minMaxArray.max[0]));
+        // the video frame message the reference's communicator would send (JPEG, quality 90)
+        const std::string msg = vis.videoFrameMessage(90);
+        std::ofstream mout(std::string(argv[5]) + ".msg", std::ios::binary);
+        mout.write(msg.data(), std::streamsize(msg.size()));
+        std::printf("video frame message %zu bytes\n", msg.size());
     }
     (void) hipFree(src.ptr);
     return 0;

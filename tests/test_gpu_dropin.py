@@ -37,3 +37,8 @@ def test_cpp_dropin_example_matches_oracle(tmp_path, oracle_lib, hip_lib):
     line = [ln for ln in r.stdout.splitlines() if ln.startswith("minmax ")][0].split()
     mins, maxs = oracle_lib.min_max(spec)
     assert np.float32(line[1]) == mins[0] and np.float32(line[2]) == maxs[0], (line, mins, maxs)
+    # the video frame message (videoFrameMessage -> the JPEG path of serverSendFrame) of that frame
+    from oracle import jpeg_oracle
+
+    msg = (tmp_path / "frame.rgba.msg").read_bytes()
+    assert msg == jpeg_oracle.frame_message(img, 90)

@@ -67,3 +67,20 @@ def test_rendered_frame_and_message(gpu, oracle_lib):
     dev = torch.from_numpy(img).to(gpu)
     assert K.jpeg(dev, 90) == _pil(np.ascontiguousarray(img[..., :3]), 90)
     assert K.jpeg_frame_message(dev, 90) == J.frame_message(img, 90)
+
+
+def test_video_frame_message_of_the_context(gpu, oracle_lib):
+    from isaac_amd import visualization as V
+    from isaac_amd._lib import IsaacHipError
+    from oracle import frame_oracle as F
+    from oracle import jpeg_oracle as J
+
+    spec = scenes.small_scene(n=32, fb=(250, 130))
+    vis = V.from_scene(spec, gpu.index or 0)
+    with pytest.raises(IsaacHipError):
+        vis.video_frame_message(90)  # nothing rendered yet
+    vis.update_buffers()
+    img = vis.draw()
+    assert vis.video_frame_message(90) == J.frame_message(img, 90)
+    assert vis.video_frame_message(0) == F.frame_message(img.tobytes())
+    vis.close()
