@@ -239,3 +239,24 @@ def test_tuning_rejects_unknown_values(gpu):
     for tiles in (0, 2, 4, 8, 16):
         vis.set_tuning(vis.TUNE_MARCH_TILES, tiles)
     vis.close()
+
+
+@pytest.mark.parametrize("fb", [(1, 1), (17, 9), (300, 1), (1, 200), (129, 65)])
+def test_odd_framebuffers(gpu, oracle_lib, fb):
+    """Framebuffers of one pixel, one row or column, and sizes that leave partial
+    tiles and super-tiles, in both render forms."""
+    from oracle import oracle
+
+    spec = _reduced_prod(n=32, fb=fb, particles=3000, visible_tf=True)
+    for form in ("combined", "bricks"):
+        vis = _vis(spec, gpu)
+        if form == "bricks":
+            vis.set_render_form(vis.FORM_BRICKS, vis.F32)
+        vis.update_buffers()
+        img = vis.draw()
+        vis.close()
+        assert img.shape == (fb[1], fb[0], 4)
+        if form == "combined":
+            compare(f"fb {fb}", spec, img, oracle.render(spec, 32))
+        else:
+            compare(f"fb {fb} bricks", spec, img, oracle.render_post(spec, 32, dtype=oracle.BRICK_F32))
