@@ -274,6 +274,8 @@ namespace isaac_hip
                 }
             }
             int w = B.voffset[c];
+            if(B.voffset[c + 1] == w)
+                continue; // no visible particle in this cell: no records to gather
             for(int k = 0; k < len; k++)
             {
                 const int32_t i = a[k];
