@@ -2028,9 +2028,12 @@ namespace isaac_hip
     // thread classifies 4 consecutive voxels of a row per iteration, with every
     // source's values of its next quad requested (16-byte loads) before the current
     // quad is classified, and stores the 4 texels with two 16-byte stores.  Same
-    // arithmetic, in the same order, as classify_kernel.
+    // arithmetic, in the same order, as classify_kernel.  Workgroups of 512 threads:
+    // eight waves share one LDS copy of the transfer functions, which halves the
+    // staging reads against 256-thread workgroups and lets 8 waves per SIMD be
+    // resident with two 1024-entry functions (32 KB per workgroup).
     template<int SIG>
-    __global__ __launch_bounds__(256) void classify_sig_kernel(const ClassifyParams P)
+    __global__ __launch_bounds__(512) void classify_sig_kernel(const ClassifyParams P)
     {
         extern __shared__ float4 lds_lut[];
         for(int i = threadIdx.x; i < P.nLut * P.T; i += blockDim.x)
@@ -2100,10 +2103,10 @@ namespace isaac_hip
         const int sig = classify_signature(P, Pc);
         if(sig)
         {
-            int64_t blocks = (P.nvox / 4 + 255) / 256;
-            if(blocks > 4096)
-                blocks = 4096;
-            const dim3 grid = dim3(unsigned(blocks)), block(256);
+            int64_t blocks = (P.nvox / 4 + 511) / 512;
+            if(blocks > 2048)
+                blocks = 2048;
+            const dim3 grid = dim3(unsigned(blocks)), block(512);
             switch(sig)
             {
             case 1: hipLaunchKernelGGL((classify_sig_kernel<1>), grid, block, lds, s, Pc); return hipGetLastError();
