@@ -169,7 +169,7 @@ namespace isaac_hip
 
     struct ParticleSourceDev
     {
-        const int32_t* voff; // ncells + 1 exclusive offsets into recs (visible particles)
+        const int2* vrange; // ncells: [start, end) of the cell's visible records in recs
         const float4* recs; // rel.xyz (cell units), lut index bits in .w
         const float* radius; // per rec or nullptr
         float constRadius;

@@ -434,7 +434,7 @@ namespace
             return 0;
         free_part(p);
         const size_t sizes[11] = {size_t(n) * 4, size_t(n) * 4, size_t(n) * 16, size_t(nc) * 8, size_t(nc) * 4,
-                                  size_t(nc + 1) * 4, size_t(nc + 1) * 4, size_t(n) * 4, size_t(n) * 16,
+                                  size_t(nc + 1) * 4, size_t(nc) * 8, size_t(n) * 4, size_t(n) * 16,
                                   needRad ? size_t(n) * 4 : 0, bin_scan_workspace_elems(nc) * 8};
         for(int i = 0; i < 11; i++)
             if(sizes[i])
@@ -446,7 +446,7 @@ namespace
         p.B.ncells = nc;
         p.B.count = (int32_t*) p.blocks[4];
         p.B.offset = (int32_t*) p.blocks[5];
-        p.B.voffset = (int32_t*) p.blocks[6];
+        p.B.vrange = (int2*) p.blocks[6];
         p.B.perm = (int32_t*) p.blocks[7];
         p.B.recs = (float4*) p.blocks[8];
         p.B.recRadius = (float*) p.blocks[9];
@@ -612,7 +612,7 @@ namespace isaac_hip
             if(PP.nsrc >= 4)
                 return fail(c, ISAAC_HIP_ERR_UNSUPPORTED, "at most 4 active particle sources per frame");
             ParticleSourceDev& S = PP.src[PP.nsrc++];
-            S.voff = p.B.voffset;
+            S.vrange = p.B.vrange;
             S.recs = p.B.recs;
             S.radius = p.B.recRadius;
             S.constRadius = p.desc.constant_radius;

@@ -67,13 +67,13 @@ namespace isaac_hip
         int32_t* cellOf; // n
         uint32_t* rank; // n: arrival rank of the particle inside its cell
         float4* rel; // n: rel.xyz, lut index (or -1 if invisible) in .w
-        unsigned long long* count64; // 2 * ncells uint32: all particles per cell, then visible ones per cell
+        unsigned long long* count64; // 2 * ncells uint32: all particles per cell, then a flag per cell (1: holds visible particles)
         int64_t ncells;
         int32_t* count; // ncells (all particles)
         int32_t* offset; // ncells + 1
-        int32_t* voffset; // ncells + 1
+        int2* vrange; // ncells: [start, end) of the cell's visible records in recs (start = offset)
         int32_t* perm; // n
-        float4* recs; // visible records in cell order
+        float4* recs; // n: each cell's visible records from its offset on, ascending particle index
         float* recRadius; // per record radius or nullptr
         unsigned long long* scanTmp; // scan workspace (bin_scan_workspace_elems entries)
     };

@@ -6,17 +6,17 @@
 // library builds those lists itself from raw particles:
 //
 //   1. bin_count   per particle: cell = clamp(floor(p / cw)), rel = p / cw - cell,
-//                  TF index of the functor-applied attribute; ONE 64-bit atomicAdd per
-//                  particle counts it (low 32 bits) and, when its TF alpha is >= 0.5,
-//                  counts it as visible (high 32 bits).  The returned low half is the
-//                  particle's arrival rank inside its cell.
-//   2. scan        exclusive prefix sum of the packed 64-bit counts: the low half of
-//                  the sum is the cell offset, the high half the visible-record offset
-//                  (the low half cannot carry into the high half below 2^32 particles)
+//                  TF index of the functor-applied attribute; ONE returning 32-bit
+//                  atomicAdd per particle counts it, and its return value is the
+//                  particle's arrival rank inside its cell; a particle whose TF alpha
+//                  is >= 0.5 (visible) also flags its cell with a plain store of 1
+//   2. scan        exclusive prefix sum of the counts: the cell offsets
 //   3. bin_scatter perm[offset[cell] + rank] = i, plain stores
 //   4. bin_sort    per cell: sort the cell's particle indices ascending, which makes
 //                  the permutation equal to a stable counting sort by cell
-//                  (bit-exact), then write the visible render records in that order
+//                  (bit-exact), then, in flagged cells, write the visible render
+//                  records in that order from the cell's offset on; the cell's
+//                  record range [offset, offset + visible) goes to vrange
 //
 // A particle whose TF alpha is below 0.5 can never produce a hit in the render
 // kernel (the alpha test follows the geometric test and changes nothing when
@@ -96,12 +96,13 @@ namespace isaac_hip
             const bool visible = !P.filter || P.lut[li].w >= 0.5f;
             B.cellOf[i] = c;
             B.rel[i] = make_float4(qx - float(cx), qy - float(cy), qz - float(cz), __int_as_float(visible ? li : -1));
-            // all particles: a returning 32-bit add (the arrival rank); visible ones: a second
-            // count in the upper half of the counter buffer (no return needed)
+            // all particles: a returning 32-bit add (the arrival rank); visible ones flag their
+            // cell in the upper half of the counter buffer with a plain store of 1 (every
+            // writer stores the same value, so no read-modify-write is needed)
             uint32_t* cnt = reinterpret_cast<uint32_t*>(B.count64);
             B.rank[i] = atomicAdd(cnt + c, 1u);
             if(visible)
-                (void) atomicAdd(cnt + B.ncells + c, 1u);
+                __hip_atomic_store(cnt + B.ncells + c, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 
@@ -142,7 +143,7 @@ namespace isaac_hip
         return e != hipSuccess ? e : launch_min_max_final(partials, groups, out, s);
     }
 
-    // ---- exclusive scan of packed 64-bit counts (3-phase, 1024-wide blocks)
+    // ---- exclusive scan of the per-cell counts (64-bit sums, 3-phase, 1024-wide blocks)
     constexpr int kScanBlock = 1024;
 
     __device__ inline unsigned long long block_inclusive_scan(unsigned long long v, unsigned long long* sh)
@@ -160,11 +161,11 @@ namespace isaac_hip
         return sh[tid];
     }
 
-    // cell i's counts packed for the scan: visible (high 32 bits), all (low 32 bits)
+    // cell i's particle count (the first half of the counter buffer)
     __device__ inline unsigned long long cell_counts(const unsigned long long* in, int64_t n, int64_t i)
     {
         const uint32_t* c = reinterpret_cast<const uint32_t*>(in);
-        return i < n ? (static_cast<unsigned long long>(c[n + i]) << 32) | c[i] : 0ull;
+        return i < n ? static_cast<unsigned long long>(c[i]) : 0ull;
     }
 
     __global__ __launch_bounds__(kScanBlock) void scan_reduce_kernel(const unsigned long long* in, int64_t n, unsigned long long* blockSums)
@@ -195,14 +196,13 @@ namespace isaac_hip
         }
     }
 
-    // writes per-cell counts, cell offsets and visible-record offsets (ncells + 1 each)
+    // writes per-cell counts (ncells) and cell offsets (ncells + 1)
     __global__ __launch_bounds__(kScanBlock) void scan_apply_kernel(
         const unsigned long long* in,
         int64_t n,
         const unsigned long long* blockSums,
         int32_t* count,
-        int32_t* offset,
-        int32_t* voffset)
+        int32_t* offset)
     {
         __shared__ unsigned long long sh[kScanBlock];
         const int64_t i = int64_t(blockIdx.x) * kScanBlock + threadIdx.x;
@@ -214,13 +214,11 @@ namespace isaac_hip
             const unsigned long long ex = base + s - v;
             count[i] = int32_t(uint32_t(v));
             offset[i] = int32_t(uint32_t(ex));
-            voffset[i] = int32_t(uint32_t(ex >> 32));
         }
         if(i == n - 1)
         {
             const unsigned long long tot = base + s;
             offset[n] = int32_t(uint32_t(tot));
-            voffset[n] = int32_t(uint32_t(tot >> 32));
         }
     }
 
@@ -231,7 +229,7 @@ namespace isaac_hip
     }
 
     // one thread per cell: ascending sort of the cell's particle indices, then
-    // the visible render records in that order
+    // the visible render records in that order and the cell's record range
     __global__ __launch_bounds__(256) void bin_sort_kernel(const BinParams P, const BinBuffers B, int64_t ncells)
     {
         for(int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; c < ncells; c += int64_t(gridDim.x) * blockDim.x)
@@ -273,21 +271,24 @@ namespace isaac_hip
                     }
                 }
             }
-            int w = B.voffset[c];
-            if(B.voffset[c + 1] == w)
-                continue; // no visible particle in this cell: no records to gather
-            for(int k = 0; k < len; k++)
+            int w = k0;
+            // cells without a visible particle (flag 0) gather no records
+            if(reinterpret_cast<const uint32_t*>(B.count64)[B.ncells + c] != 0u)
             {
-                const int32_t i = a[k];
-                const float4 r = B.rel[i];
-                if(__float_as_int(r.w) >= 0)
+                for(int k = 0; k < len; k++)
                 {
-                    B.recs[w] = r;
-                    if(B.recRadius)
-                        B.recRadius[w] = P.radius[i];
-                    w++;
+                    const int32_t i = a[k];
+                    const float4 r = B.rel[i];
+                    if(__float_as_int(r.w) >= 0)
+                    {
+                        B.recs[w] = r;
+                        if(B.recRadius)
+                            B.recRadius[w] = P.radius[i];
+                        w++;
+                    }
                 }
             }
+            B.vrange[c] = make_int2(k0, w);
         }
     }
 
@@ -328,7 +329,7 @@ namespace isaac_hip
         hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(kScanBlock), 0, s, B.count64, ncells, B.scanTmp);
         hipLaunchKernelGGL(scan_blocksums_kernel, dim3(1), dim3(kScanBlock), 0, s, B.scanTmp, nb);
         hipLaunchKernelGGL(scan_apply_kernel, dim3(nb), dim3(kScanBlock), 0, s, B.count64, ncells, B.scanTmp, B.count,
-                           B.offset, B.voffset);
+                           B.offset);
         if(P.n > 0)
             hipLaunchKernelGGL(bin_scatter_kernel, dim3(unsigned(pb)), dim3(256), 0, s, P, B);
         int64_t cb = (ncells + 255) / 256;

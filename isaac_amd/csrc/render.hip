@@ -578,8 +578,8 @@ namespace isaac_hip
             for(int s = 0; s < PP.nsrc; s++)
             {
                 const ParticleSourceDev& S = PP.src[s];
-                const int k0 = S.voff[cid], k1 = S.voff[cid + 1];
-                for(int k = k0; k < k1; k++)
+                const int2 kr = S.vrange[cid];
+                for(int k = kr.x; k < kr.y; k++)
                 {
                     if(COUNT)
                         nTests++;
