@@ -9,7 +9,7 @@ from isaac_amd import scenes
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 8, 11, 12, 13, 23, 24, 25, 4096, 1000003])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 8, 9, 10, 11, 12, 13, 23, 24, 25, 4096, 1000003])
 def test_base64_bit_exact(gpu, n):
     import torch
 
