@@ -9,7 +9,8 @@ plane, SSAO, trilinear sampling at step 0.5, 1920x1080.  A step is one
 doVisualization(redraw): particle cell lists + classification + the draw
 callback, with the camera alternating between the default pose and a
 volume-filling pose.  Inputs are resident in HBM before timing starts and the
-framebuffer stays on the device (no PCIe in the timed region).
+framebuffer stays on the device (no PCIe in the timed region); fps_host_output
+repeats the frames with the framebuffer copied to pinned host memory each step.
 
 Metric: trilinear samples per second (volume + iso march samples actually
 executed, after early ray termination), with frames/s alongside.  The same run
@@ -72,26 +73,82 @@ def kernel_bytes(counts, spec, n_vox, n_particles, n_visible, form="combined"):
     return out
 
 
-def cpu_baseline(spec, threads):
+def host_cpus():
+    """What the host offers this process: `nproc` (the CPUs it may use; GNU nproc also
+    honours OMP_NUM_THREADS), `nproc --all`, and lscpu's model name."""
+    import subprocess
+
+    def run(cmd):
+        try:
+            return subprocess.run(cmd, capture_output=True, text=True, timeout=20).stdout
+        except (OSError, subprocess.SubprocessError):
+            return ""
+
+    try:
+        nproc = int(run(["nproc"]).strip())
+    except ValueError:
+        nproc = len(os.sched_getaffinity(0))
+    try:
+        nall = int(run(["nproc", "--all"]).strip())
+    except ValueError:
+        nall = os.cpu_count() or nproc
+    model = ""
+    for ln in run(["lscpu"]).splitlines():
+        if ln.lower().startswith("model name"):
+            model = ln.split(":", 1)[1].strip()
+            break
+    return {"nproc": nproc, "nproc_all": nall, "affinity": len(os.sched_getaffinity(0)), "model": model}
+
+
+def cpu_baseline(spec, poses, threads, frames=3):
+    """The CPU restatement of the reference path (classification + cell lists + draw
+    of the full frame) at each pose, like the GPU line: with `threads` OpenMP threads
+    (the reference's AccCpuOmp2Blocks backend: one pixel or voxel per block), the
+    median of `frames` frames per pose; and with one thread (AccCpuSerial), one frame
+    per pose.  Frames/s and samples/s average the poses as the GPU line does."""
+    import numpy as np
+
     from oracle import oracle
 
-    ps = oracle.PreparedScene(spec, threads=threads)
-    t0 = time.perf_counter()
-    res = oracle.render(spec, 32, threads=threads, prepared=ps)
-    wall = time.perf_counter() - t0
-    sec = res.seconds["classify"] + res.seconds["bin"] + res.seconds["render"]
-    samples = res.counters["vol_samples"] + res.counters["iso_samples"]
+    def run(th, reps):
+        per = []
+        for p in poses:
+            s = spec.with_camera(p)
+            ps = oracle.PreparedScene(s, threads=th)
+            secs, samples = [], 0
+            for _ in range(reps):
+                res = oracle.render(s, 32, threads=th, prepared=ps)
+                secs.append(res.seconds["classify"] + res.seconds["bin"] + res.seconds["render"])
+                samples = res.counters["vol_samples"] + res.counters["iso_samples"]
+            per.append((float(np.median(secs)), samples, res.seconds))
+        t = sum(x[0] for x in per)
+        smp = sum(x[1] for x in per)
+        return {
+            "value": smp / t,
+            "fps": len(poses) / t,
+            "frame_s": {name: x[0] for name, x in zip(("default", "fill"), per)},
+            "phase_s": {name: {k: float(v) for k, v in x[2].items()} for name, x in zip(("default", "fill"), per)},
+            "samples": {name: x[1] for name, x in zip(("default", "fill"), per)},
+        }
+
+    host = host_cpus()
+    allc = run(threads, frames)
+    serial = run(1, 1)
     return {
-        "value": samples / sec,
+        "value": allc["value"],
         "unit": "samples/s",
         "cores": threads,
         "kind": "port",
-        "sample": "one full 1920x1080 frame at the default pose (classification + cell lists + draw), "
-        "FP32 C++ restatement of the reference CPU path, OpenMP",
-        "frame_s": sec,
-        "fps": 1.0 / sec,
-        "wall_s_incl_setup": wall,
-        "samples": samples,
+        "sample": f"full 1920x1080 frames (classification + cell lists + draw) at the default and the fill pose; "
+        f"{threads} threads: median of {frames} frames per pose; 1 thread: one frame per pose; "
+        "FP32 C++ restatement of the reference CPU path (OpenMP, one pixel or voxel per iteration)",
+        "fps": allc["fps"],
+        "frame_s": allc["frame_s"],
+        "phase_s": allc["phase_s"],
+        "samples": allc["samples"],
+        "serial": {"value": serial["value"], "unit": "samples/s", "cores": 1, "fps": serial["fps"],
+                   "frame_s": serial["frame_s"]},
+        "host": host,
     }
 
 
@@ -380,6 +437,10 @@ def main() -> int:
     wall, total_samples, avg_us, avg_counts, counts = measure(vis, poses, out, args.steps, args.warmup)
     ms_per_step = wall / args.steps * 1e3
     value = total_samples / wall
+    # the same frames with the framebuffer copied to (pinned) host memory each step, as
+    # the reference's drawCallBack copies it into IceT's image (lib/isaac.hpp:2530-2539)
+    host_out = torch.empty((spec.framebuffer[1], spec.framebuffer[0], 4), dtype=torch.uint8, pin_memory=True)
+    wall_h, _, _, _, _ = measure(vis, poses, host_out, args.steps, 2)
     kb = kernel_bytes(avg_counts, spec, n_vox, n_particles, n_visible, args.form)
     dom = max(avg_us, key=lambda k: avg_us[k])
     achieved = kb[dom] / (avg_us[dom] * 1e-6) / 1e9
@@ -413,6 +474,8 @@ def main() -> int:
             "form": args.form,
         },
         "fps": args.steps / wall,
+        "fps_host_output": args.steps / wall_h,
+        "ms_per_step_host_output": wall_h / args.steps * 1e3,
         "samples_per_frame": {"default": counts[0]["vol_samples"] + counts[0]["iso_samples"],
                               "fill": counts[1]["vol_samples"] + counts[1]["iso_samples"]},
         "kernel_ms": {k: v / 1e3 for k, v in avg_us.items()},
@@ -458,10 +521,11 @@ def main() -> int:
         line["batched"] = batched_workload(args)
     line["compositing"] = compositing()
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
-        spec0 = spec.with_camera(poses[0])
-        line["cpu_baseline"] = cpu_baseline(spec0, threads)
-        line["gpu_vs_cpu_samples"] = value / line["cpu_baseline"]["value"]
+        threads = args.cpu_threads or host_cpus()["nproc"]
+        cb = cpu_baseline(spec, poses, threads)
+        line["cpu_baseline"] = cb
+        line["gpu_vs_cpu"] = {"samples_per_s": value / cb["value"], "fps": line["fps"] / cb["fps"],
+                              "fps_vs_serial": line["fps"] / cb["serial"]["fps"]}
     print(json.dumps(line), flush=True)
     return 0
 

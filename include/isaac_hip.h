@@ -293,8 +293,16 @@ extern "C"
      * over several frames at a time) and one shading launch.  Each context is one
      * sub-volume in the brick form with update_buffers done, and all contexts share
      * the steering state of context 0 (transfer functions, chains, weights, iso,
-     * interpolation, brick dtype, occlusion).  Contexts must outlive the batch, and
-     * the batch uses context 0's stream. */
+     * interpolation, brick dtype, occlusion) and bind the same sources.  Contexts must
+     * outlive the batch.  The batch renders on a stream of its own; each render waits
+     * for the work already queued on every context's stream (update_buffers, network
+     * training), so bricks rebuilt in place are read after their rebuild.  set_views
+     * snapshots the contexts' buffers and steering: after any context reallocates a
+     * buffer (update_buffers with more particles or another local size, brick dtype
+     * change), changes stream or steering, render returns ISAAC_HIP_ERR_STATE until
+     * set_views runs again.  The learned occlusion pass uses context 0's network with
+     * activations owned by the batch, so batches of different sizes can share
+     * contexts. */
     typedef struct isaac_hip_batch isaac_hip_batch;
     enum isaac_hip_batch_mode
     {

@@ -23,7 +23,13 @@
 //     returned in the reference);
 //   - source lists are std::tuple instead of boost::fusion::list;
 //   - there is no server, JSON, MPI or IceT: doVisualization renders the
-//     local sub-volume and keeps the RGBA8 frame in `image()`.
+//     local sub-volume and keeps the RGBA8 frame in `image()`.  It still
+//     returns json_t* (always null: no client metadata arrives without a
+//     server) and getJsonMetaRoot() returns a null json_t*, so the
+//     reference's `json_decref(vis->doVisualization(META_MASTER))` and
+//     `json_object_set_new(vis->getJsonMetaRoot(), ...)` compile and run
+//     unchanged: jansson accepts a null json_t* in both; without jansson,
+//     include isaac_json_stub.h for no-op stand-ins.
 #pragma once
 
 #include "isaac_hip.h"
@@ -37,6 +43,11 @@
 #include <tuple>
 #include <utility>
 #include <vector>
+
+// jansson's root type (struct json_t, typedef json_t); declared here so the
+// signatures match the reference whether or not jansson.h is included
+struct json_t;
+typedef struct json_t json_t;
 
 namespace isaac
 {
@@ -304,9 +315,19 @@ namespace isaac
             updateModelview();
         }
 
+        // getJsonMetaRoot (lib/isaac.hpp:1269-1273): the metadata root the reference
+        // sends to the server; there is no server here, so it is null (jansson's
+        // json_object_set_new on it returns -1 and releases the value)
+        json_t* getJsonMetaRoot()
+        {
+            return nullptr;
+        }
+
         // doVisualization (lib/isaac.hpp:1298-2104) for this rank: buffers when
-        // redrawing, then the draw callback; the frame lands in image()
-        void doVisualization(IsaacVisualizationMetaEnum metaTargets = META_MASTER, void* pointer = nullptr, bool redraw = true)
+        // redrawing, then the draw callback; the frame lands in image().  Returns the
+        // client metadata the reference forwards (lib/isaac/mainpage.dox:113-117):
+        // always null here, and json_decref(nullptr) is a no-op
+        json_t* doVisualization(IsaacVisualizationMetaEnum metaTargets = META_MASTER, void* pointer = nullptr, bool redraw = true)
         {
             (void) metaTargets;
             redraw = redraw || advectionOnPause; // lib/isaac.hpp:1303
@@ -332,6 +353,7 @@ namespace isaac
             copyTime = uint64_t(t.copy_us);
             bufferTime = uint64_t(t.buffer_us);
             optimizationBufferTime = uint64_t(t.optimization_buffer_us);
+            return nullptr;
         }
 
         // RGBA8, premultiplied, row 0 = bottom (the IceT image of lib/isaac.hpp:2533-2539)

@@ -131,14 +131,13 @@ namespace isaac_hip
         int64_t step = 0;
         bool ready = false;
         size_t maxLds = 64 * 1024; // LDS bytes per workgroup on this device
-        AoWorkspace infer, train, batch;
+        AoWorkspace infer, train;
         int batchCount = 0, batchSide = 0;
 
         ~AoNet()
         {
             infer.release();
             train.release();
-            batch.release();
             for(void* p : {(void*) master, (void*) grad, (void*) m, (void*) v})
                 if(p)
                     (void) hipFree(p);
@@ -431,13 +430,31 @@ namespace isaac_hip
         return 0;
     }
 
-    int aonet_reserve_frames(isaac_hip_ctx* c, int chunk, int W, int H)
+    AoWorkspace* aonet_workspace_create()
+    {
+        return new AoWorkspace();
+    }
+
+    void aonet_workspace_destroy(AoWorkspace* ws)
+    {
+        if(!ws)
+            return;
+        ws->release();
+        delete ws;
+    }
+
+    int aonet_reserve_frames(isaac_hip_ctx* c, AoWorkspace* ws, int chunk, int W, int H, bool* reallocated)
     {
         AoNet* net = c->aonet;
         if(!net || !net->ready)
             return fail(c, ISAAC_HIP_ERR_STATE, "learned occlusion selected without network parameters");
+        if(!ws)
+            return fail(c, ISAAC_HIP_ERR_ARG, "no occlusion workspace");
         const int Wp = (W + 7) / 8 * 8, Hp = (H + 7) / 8 * 8;
-        HIPCHK(c, net->batch.ensure(chunk, Hp, Wp, false));
+        const _Float16* before = ws->x0;
+        HIPCHK(c, ws->ensure(chunk, Hp, Wp, false));
+        if(reallocated)
+            *reallocated = ws->x0 != before;
         return 0;
     }
 
@@ -445,6 +462,7 @@ namespace isaac_hip
     // per network evaluation; allocates nothing once aonet_reserve_frames has run
     int aonet_occlusion_frames(
         isaac_hip_ctx* c,
+        AoWorkspace* ws,
         const float* depth,
         const float* normal,
         float* ao,
@@ -459,7 +477,7 @@ namespace isaac_hip
         if(!net || !net->ready)
             return fail(c, ISAAC_HIP_ERR_STATE, "learned occlusion selected without network parameters");
         const int Wp = (W + 7) / 8 * 8, Hp = (H + 7) / 8 * 8;
-        if(net->batch.n < chunk || net->batch.h != Hp || net->batch.w != Wp)
+        if(!ws || ws->n < chunk || ws->h != Hp || ws->w != Wp)
             return fail(c, ISAAC_HIP_ERR_STATE, "batch occlusion workspace not reserved");
         const size_t img = size_t(W) * H;
         for(int f0 = 0; f0 < nframes; f0 += chunk)
@@ -473,12 +491,12 @@ namespace isaac_hip
             fa.invScale = 1.0f;
             fa.invScaleItem = invScaleDev + f0;
             fa.imgStride = int64_t(img);
-            fa.x = net->batch.x0;
+            fa.x = ws->x0;
             fa.n = n;
             fa.h = Hp;
             fa.w = Wp;
             HIPCHK(c, launch_features(fa, s));
-            HIPCHK(c, forward(net, net->batch, n, Hp, Wp, s, ao + img * f0, W, H));
+            HIPCHK(c, forward(net, *ws, n, Hp, Wp, s, ao + img * f0, W, H));
         }
         return 0;
     }
@@ -507,6 +525,7 @@ extern "C"
             return fail(c, ISAAC_HIP_ERR_ARG, "unknown occlusion mode");
         if(mode == ISAAC_HIP_AO_LEARNED && !(c->aonet && c->aonet->ready))
             return fail(c, ISAAC_HIP_ERR_STATE, "learned occlusion needs network parameters (isaac_hip_aonet_set_params)");
+        c->generation++;
         c->aoMode = mode;
         return 0;
     }

@@ -16,9 +16,13 @@ using namespace isaac_hip;
 struct isaac_hip_batch
 {
     int device = 0;
+    // the batch's own stream, ordered after each context's stream at every render
     hipStream_t stream = nullptr;
     std::string err;
     std::vector<isaac_hip_ctx*> ctx;
+    std::vector<hipEvent_t> ctxEv; // one per context: its stream's position at the render
+    std::vector<uint64_t> gen; // each context's generation when the views were set
+    isaac_hip::AoWorkspace* aoWs = nullptr; // learned occlusion activations of this batch
     int W = 0, H = 0, nviews = 0, nframes = 0;
     int tilesPerFrame = 0;
     int chunk = 4, groupsPerCU = 4, aoChunk = 16;
@@ -142,8 +146,8 @@ namespace
         {
             if(b->learned)
             {
-                int rc = aonet_occlusion_frames(b->ctx[0], b->depth, b->normal, b->ao, b->nframes, b->W, b->H, b->dInvScale,
-                                                b->aoChunk, b->stream);
+                int rc = aonet_occlusion_frames(b->ctx[0], b->aoWs, b->depth, b->normal, b->ao, b->nframes, b->W, b->H,
+                                                b->dInvScale, b->aoChunk, b->stream);
                 if(rc)
                     return bfail(b, rc, b->ctx[0]->err);
             }
@@ -168,12 +172,17 @@ extern "C"
         isaac_hip_batch* b = new isaac_hip_batch();
         b->ctx.assign(ctxs, ctxs + n);
         b->device = ctxs[0]->device;
-        b->stream = ctxs[0]->stream;
         b->W = width;
         b->H = height;
         b->tilesPerFrame = ((width + 15) / 16) * ((height + 15) / 16);
-        if(hipSetDevice(b->device) != hipSuccess || hipMalloc(&b->dCounter, 64) != hipSuccess
-           || hipEventCreate(&b->ev[0]) != hipSuccess || hipEventCreate(&b->ev[1]) != hipSuccess)
+        b->ctxEv.assign(size_t(n), nullptr);
+        b->gen.assign(size_t(n), 0);
+        bool ok = hipSetDevice(b->device) == hipSuccess && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess
+                  && hipMalloc(&b->dCounter, 64) == hipSuccess && hipEventCreate(&b->ev[0]) == hipSuccess
+                  && hipEventCreate(&b->ev[1]) == hipSuccess;
+        for(int i = 0; ok && i < n; i++)
+            ok = hipEventCreateWithFlags(&b->ctxEv[i], hipEventDisableTiming) == hipSuccess;
+        if(!ok)
         {
             isaac_hip_batch_destroy(b);
             return ISAAC_HIP_ERR_DEVICE;
@@ -194,11 +203,17 @@ extern "C"
             (void) hipStreamSynchronize(b->stream);
         drop_graph(b);
         free_frames(b);
+        aonet_workspace_destroy(b->aoWs);
         if(b->dCounter)
             (void) hipFree(b->dCounter);
         for(hipEvent_t e : b->ev)
             if(e)
                 (void) hipEventDestroy(e);
+        for(hipEvent_t e : b->ctxEv)
+            if(e)
+                (void) hipEventDestroy(e);
+        if(b->stream)
+            (void) hipStreamDestroy(b->stream);
         delete b;
     }
 
@@ -289,6 +304,17 @@ extern "C"
             b->hF[f].waves = 0;
             b->hF[f].prefetch = 0;
             b->hInvScale[f] = 1.0f / float(c->maxGsScaled);
+            // the persistent march stages frame 0's transfer functions for every frame:
+            // every frame must bind the same sources in the same slots
+            const PostParams& q0 = b->hQ[0];
+            const PostParams& q = b->hQ[f];
+            bool same = q.nsrc == q0.nsrc && q.dtype == q0.dtype && q.T == q0.T;
+            for(int s = 0; same && s < q.nsrc; s++)
+                same = q.src[s].active == q0.src[s].active && q.src[s].dim == q0.src[s].dim
+                       && q.src[s].lutSlot == q0.src[s].lutSlot;
+            if(!same)
+                return bfail(b, ISAAC_HIP_ERR_ARG,
+                             "context " + std::to_string(f / nviews) + ": bound sources differ from context 0's");
         }
         std::vector<uint32_t> ts(nframes + 1);
         for(int f = 0; f <= nframes; f++)
@@ -324,10 +350,17 @@ extern "C"
         b->learned = learned;
         if(learned)
         {
-            rc = aonet_reserve_frames(c0, b->aoChunk, b->W, b->H);
+            if(!b->aoWs)
+                b->aoWs = aonet_workspace_create();
+            bool moved = false;
+            rc = aonet_reserve_frames(c0, b->aoWs, b->aoChunk, b->W, b->H, &moved);
             if(rc)
                 return bfail(b, rc, c0->err);
+            if(moved)
+                drop_graph(b);
         }
+        for(int i = 0; i < n; i++)
+            b->gen[i] = b->ctx[i]->generation;
         return 0;
     }
 
@@ -336,11 +369,28 @@ extern "C"
         if(!b || b->nframes == 0)
             return b ? bfail(b, ISAAC_HIP_ERR_STATE, "no views set (isaac_hip_batch_set_views)") : ISAAC_HIP_ERR_ARG;
         BCHK(b, hipSetDevice(b->device));
+        // the frames' parameter blocks point into the contexts' buffers as they were
+        // when the views were set: refuse if any was reallocated or re-steered since
+        for(size_t i = 0; i < b->ctx.size(); i++)
+            if(b->ctx[i]->generation != b->gen[i])
+                return bfail(b, ISAAC_HIP_ERR_STATE,
+                             "context " + std::to_string(i)
+                                 + " changed since isaac_hip_batch_set_views (buffers reallocated, stream or steering changed)");
         if(b->learned)
         {
-            int rc = aonet_reserve_frames(b->ctx[0], b->aoChunk, b->W, b->H);
+            bool moved = false;
+            int rc = aonet_reserve_frames(b->ctx[0], b->aoWs, b->aoChunk, b->W, b->H, &moved);
             if(rc)
                 return bfail(b, rc, b->ctx[0]->err);
+            if(moved)
+                drop_graph(b);
+        }
+        // after everything already queued on the contexts' streams (brick and cell-list
+        // builds of update_buffers, network training on context 0)
+        for(size_t i = 0; i < b->ctx.size(); i++)
+        {
+            BCHK(b, hipEventRecord(b->ctxEv[i], b->ctx[i]->stream));
+            BCHK(b, hipStreamWaitEvent(b->stream, b->ctxEv[i], 0));
         }
         BCHK(b, hipEventRecord(b->ev[0], b->stream));
         if(mode == ISAAC_HIP_BATCH_PERSISTENT)
@@ -384,8 +434,8 @@ extern "C"
                     if(b->learned)
                     {
                         const size_t img = size_t(b->W) * b->H;
-                        int rc = aonet_occlusion_frames(b->ctx[0], b->depth + img * f, b->normal + img * 3 * f, b->ao + img * f, 1,
-                                                        b->W, b->H, b->dInvScale + f, 1, b->stream);
+                        int rc = aonet_occlusion_frames(b->ctx[0], b->aoWs, b->depth + img * f, b->normal + img * 3 * f,
+                                                        b->ao + img * f, 1, b->W, b->H, b->dInvScale + f, 1, b->stream);
                         if(rc)
                             return bfail(b, rc, b->ctx[0]->err);
                     }

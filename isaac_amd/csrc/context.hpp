@@ -12,6 +12,7 @@
 namespace isaac_hip
 {
     struct AoNet; // learned occlusion network (aonet_host.cpp)
+    struct AoWorkspace; // activations of one set of network evaluations (aonet_host.cpp)
 
     struct VolSlot
     {
@@ -52,6 +53,11 @@ struct isaac_hip_ctx
     void* videoWs = nullptr;
     int64_t videoWsBytes = 0;
     bool rendered = false; // a frame is in the colour buffer
+    // bumped whenever a device buffer that a frame's parameter blocks point to is
+    // (re)allocated or freed, the stream changes, or steering state changes: a batch
+    // (batch_host.cpp) that snapshotted this context's frames refuses to render
+    // until its views are set again
+    uint64_t generation = 0;
     std::string err;
     int nv = 0, nf = 0, np = 0, nsrc = 0;
     std::vector<VolSlot> vols;
@@ -153,9 +159,14 @@ namespace isaac_hip
 
     // learned occlusion pass (aonet_host.cpp)
     int aonet_occlusion(isaac_hip_ctx* c);
-    int aonet_reserve_frames(isaac_hip_ctx* c, int chunk, int W, int H);
+    // a batch's own activation workspace, evaluated with context c's network
+    AoWorkspace* aonet_workspace_create();
+    void aonet_workspace_destroy(AoWorkspace* ws);
+    // sizes ws for `chunk` frames of W x H; *reallocated tells whether its buffers moved
+    int aonet_reserve_frames(isaac_hip_ctx* c, AoWorkspace* ws, int chunk, int W, int H, bool* reallocated);
     int aonet_occlusion_frames(
         isaac_hip_ctx* c,
+        AoWorkspace* ws,
         const float* depth,
         const float* normal,
         float* ao,

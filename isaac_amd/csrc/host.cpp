@@ -307,6 +307,15 @@ namespace
         c->maxGsScaled = std::max(c->gsScaled[0], std::max(c->gsScaled[1], c->gsScaled[2]));
     }
 
+    // assigns v to dst and bumps the context's generation when the value changes
+    template<class T>
+    void set_tracked(isaac_hip_ctx* c, T& dst, const T& v)
+    {
+        if(std::memcmp(&dst, &v, sizeof(T)) != 0)
+            c->generation++;
+        dst = v;
+    }
+
     int alloc_textures(isaac_hip_ctx* c)
     {
         const size_t nvox = size_t(c->local[0]) * size_t(c->local[1]) * size_t(c->local[2]);
@@ -315,6 +324,7 @@ namespace
         c->tex = nullptr;
         // one texel of padding: the march reads texels x and x+1 with one 16-byte load,
         // also at the last texel (whose x+1 it then ignores)
+        c->generation++;
         HIPCHK(c, hipMalloc(&c->tex, (nvox + 1) * 8));
         HIPCHK(c, hipMemsetAsync(c->tex, 0, (nvox + 1) * 8, c->stream));
         c->texVox = nvox;
@@ -331,6 +341,7 @@ namespace
         if(v.npTex)
             (void) hipFree(v.npTex);
         v.npTex = nullptr;
+        c->generation++;
         HIPCHK(c, hipMalloc(&v.npTex, n * 4));
         v.npBytes = n * 4;
         return 0;
@@ -353,6 +364,7 @@ namespace
         {
             if(a)
                 continue;
+            c->generation++;
             HIPCHK(c, hipMalloc(&a, (n + 3) / 4 * 4));
             HIPCHK(c, launch_fill_u32(reinterpret_cast<uint32_t*>(a), 0u, (n + 3) / 4, c->stream));
         }
@@ -372,6 +384,7 @@ namespace
         }
         if(!c->noise)
         {
+            c->generation++;
             HIPCHK(c, hipMalloc(&c->noise, n * 4));
             HIPCHK(c, hipMalloc(&c->noiseTmp, n * 4));
             c->noiseVox = n;
@@ -404,6 +417,7 @@ namespace
         v.brick = nullptr;
         v.brickScale = nullptr;
         v.brickBytes = v.scaleCount = 0;
+        c->generation++;
         HIPCHK(c, hipMalloc(&v.brick, std::max<size_t>(bytes, 8)));
         HIPCHK(c, hipMalloc(&v.brickScale, std::max<size_t>(ns, 1) * 4));
         v.brickBytes = bytes;
@@ -433,6 +447,7 @@ namespace
         if(p.capN >= n && p.capCells == nc && (p.B.recRadius != nullptr) == needRad)
             return 0;
         free_part(p);
+        c->generation++;
         const size_t sizes[11] = {size_t(n) * 4, size_t(n) * 4, size_t(n) * 16, size_t(nc) * 8, size_t(nc) * 4,
                                   size_t(nc + 1) * 4, size_t(nc) * 8, size_t(n) * 4, size_t(n) * 16,
                                   needRad ? size_t(n) * 4 : 0, bin_scan_workspace_elems(nc) * 8};
@@ -899,6 +914,7 @@ extern "C"
         if(!c)
             return ISAAC_HIP_ERR_ARG;
         (void) hipStreamSynchronize(c->stream);
+        c->generation++;
         if(c->ownStream && c->stream)
             (void) hipStreamDestroy(c->stream);
         if(s)
@@ -925,6 +941,8 @@ extern "C"
             return fail(c, ISAAC_HIP_ERR_UNSUPPORTED, "the plain render path reads float32 sources");
         if(d->feature_dim < 1 || d->feature_dim > 4 || !d->data || d->guard_size < 0)
             return fail(c, ISAAC_HIP_ERR_ARG, "bad volume source descriptor");
+        if(!c->vols[idx].set || std::memcmp(&c->vols[idx].desc, d, sizeof(*d)) != 0)
+            c->generation++;
         c->vols[idx].desc = *d;
         c->vols[idx].set = true;
         return 0;
@@ -939,6 +957,8 @@ extern "C"
         if(d->feature_dim < 1 || d->feature_dim > 4 || d->count < 0 || d->count > INT32_MAX
            || (d->count > 0 && (!d->position || !d->attribute)))
             return fail(c, ISAAC_HIP_ERR_ARG, "bad particle source descriptor");
+        if(!c->parts[idx].set || std::memcmp(&c->parts[idx].desc, d, sizeof(*d)) != 0)
+            c->generation++;
         c->parts[idx].desc = *d;
         c->parts[idx].set = true;
         return 0;
@@ -952,8 +972,8 @@ extern "C"
         int32_t bc[3];
         float pr[12];
         const int32_t e = parse_chain(chain ? std::string(chain) : std::string(), bc, pr);
-        c->chain[idx] = to_chain(bc, pr);
-        c->chainErr[idx] = e;
+        set_tracked(c, c->chain[idx], to_chain(bc, pr));
+        set_tracked(c, c->chainErr[idx], e);
         if(error_code)
             *error_code = e;
         return 0;
@@ -966,8 +986,14 @@ extern "C"
             return rc;
         if(n < 0 || (n > 0 && (!positions || !rgba)))
             return fail(c, ISAAC_HIP_ERR_ARG, "bad transfer points");
-        build_lut(c->cfg.transfer_size, n, positions, rgba, c->lutHost[idx].data());
-        c->lutDirty = true;
+        std::vector<float> lut(c->lutHost[idx].size());
+        build_lut(c->cfg.transfer_size, n, positions, rgba, lut.data());
+        if(lut != c->lutHost[idx])
+        {
+            c->lutHost[idx] = lut;
+            c->lutDirty = true;
+            c->generation++;
+        }
         return 0;
     }
 
@@ -978,8 +1004,12 @@ extern "C"
             return rc;
         if(!lut)
             return fail(c, ISAAC_HIP_ERR_ARG, "null lut");
-        std::memcpy(c->lutHost[idx].data(), lut, size_t(c->cfg.transfer_size) * 16);
-        c->lutDirty = true;
+        if(std::memcmp(c->lutHost[idx].data(), lut, size_t(c->cfg.transfer_size) * 16) != 0)
+        {
+            std::memcpy(c->lutHost[idx].data(), lut, size_t(c->cfg.transfer_size) * 16);
+            c->lutDirty = true;
+            c->generation++;
+        }
         return 0;
     }
 
@@ -988,7 +1018,7 @@ extern "C"
         int rc = check_idx(c, idx);
         if(rc)
             return rc;
-        c->weight[idx] = w;
+        set_tracked(c, c->weight[idx], w);
         return 0;
     }
 
@@ -997,7 +1027,7 @@ extern "C"
         int rc = check_idx(c, idx);
         if(rc)
             return rc;
-        c->iso[idx] = t;
+        set_tracked(c, c->iso[idx], t);
         return 0;
     }
 
@@ -1005,7 +1035,7 @@ extern "C"
     {
         if(!c)
             return ISAAC_HIP_ERR_ARG;
-        c->step = step < 0.01f ? 0.01f : step; // lib/isaac.hpp:1629-1633
+        set_tracked(c, c->step, step < 0.01f ? 0.01f : step); // lib/isaac.hpp:1629-1633
         return 0;
     }
 
@@ -1013,7 +1043,7 @@ extern "C"
     {
         if(!c)
             return ISAAC_HIP_ERR_ARG;
-        c->interpolation = on != 0;
+        set_tracked(c, c->interpolation, on != 0);
         return 0;
     }
 
@@ -1023,10 +1053,10 @@ extern "C"
             return ISAAC_HIP_ERR_ARG;
         if(seeding_period < 1 || !(step_factor == step_factor) || !(history_weight == history_weight))
             return fail(c, ISAAC_HIP_ERR_ARG, "advection: seeding period >= 1 and finite factors expected");
-        c->advStepFactor = step_factor;
-        c->advHistoryWeight = history_weight;
-        c->advSeedPeriod = seeding_period;
-        c->advSeedTime = seeding_duration;
+        set_tracked(c, c->advStepFactor, step_factor);
+        set_tracked(c, c->advHistoryWeight, history_weight);
+        set_tracked(c, c->advSeedPeriod, seeding_period);
+        set_tracked(c, c->advSeedTime, seeding_duration);
         return 0;
     }
 
@@ -1037,7 +1067,7 @@ extern "C"
         const int32_t n = seed_points < 0 ? 0 : seed_points;
         if(n != c->seedPoints)
             c->noiseDirty = true;
-        c->seedPoints = n;
+        set_tracked(c, c->seedPoints, n);
         return 0;
     }
 
@@ -1060,7 +1090,7 @@ extern "C"
         if(!c)
             return ISAAC_HIP_ERR_ARG;
         // every value is valid: as in applyDither3D, modes other than 0-3 use the 16^3 mask
-        c->ditherMode = mode;
+        set_tracked(c, c->ditherMode, mode);
         return 0;
     }
 
@@ -1068,8 +1098,8 @@ extern "C"
     {
         if(!c)
             return ISAAC_HIP_ERR_ARG;
-        c->aoEnabled = enabled != 0;
-        c->aoWeight = weight;
+        set_tracked(c, c->aoEnabled, enabled != 0);
+        set_tracked(c, c->aoWeight, weight);
         return 0;
     }
 
@@ -1077,7 +1107,7 @@ extern "C"
     {
         if(!c)
             return ISAAC_HIP_ERR_ARG;
-        c->renderMode = mode;
+        set_tracked(c, c->renderMode, mode);
         return 0;
     }
 
@@ -1092,12 +1122,12 @@ extern "C"
         const float l = sqrtf(sx * sx + sy * sy + sz * sz);
         if(l == 0.0f)
             return 0;
-        c->clipPos[nr][0] = px;
-        c->clipPos[nr][1] = py;
-        c->clipPos[nr][2] = pz;
-        c->clipNormal[nr][0] = sx / l;
-        c->clipNormal[nr][1] = sy / l;
-        c->clipNormal[nr][2] = sz / l;
+        set_tracked(c, c->clipPos[nr][0], px);
+        set_tracked(c, c->clipPos[nr][1], py);
+        set_tracked(c, c->clipPos[nr][2], pz);
+        set_tracked(c, c->clipNormal[nr][0], sx / l);
+        set_tracked(c, c->clipNormal[nr][1], sy / l);
+        set_tracked(c, c->clipNormal[nr][2], sz / l);
         c->clipSaved[nr][0] = nx;
         c->clipSaved[nr][1] = ny;
         c->clipSaved[nr][2] = nz;
@@ -1111,6 +1141,7 @@ extern "C"
         if(isaac_hip_edit_clipping(c, uint32_t(c->nclip), px, py, pz, nx, ny, nz) == 1)
         {
             c->nclip++;
+            c->generation++;
             return 1;
         }
         return 0;
@@ -1122,6 +1153,7 @@ extern "C"
             return ISAAC_HIP_ERR_ARG;
         if(nr >= uint32_t(c->nclip))
             return 0;
+        c->generation++;
         c->nclip--;
         for(int i = int(nr); i < c->nclip; i++)
             for(int k = 0; k < 3; k++)
@@ -1143,7 +1175,7 @@ extern "C"
         if(!c || !position)
             return ISAAC_HIP_ERR_ARG;
         for(int i = 0; i < 3; i++)
-            c->position[i] = position[i];
+            set_tracked(c, c->position[i], position[i]);
         derive_sizes(c);
         return 0;
     }
@@ -1156,7 +1188,7 @@ extern "C"
             if(ls[i] <= 0 || ls[i] > (1 << 20))
                 return fail(c, ISAAC_HIP_ERR_ARG, "bad local size");
         for(int i = 0; i < 3; i++)
-            c->local[i] = ls[i];
+            set_tracked(c, c->local[i], ls[i]);
         derive_sizes(c);
         return alloc_textures(c);
     }
@@ -1169,7 +1201,7 @@ extern "C"
             if(ps[i] <= 0)
                 return fail(c, ISAAC_HIP_ERR_ARG, "bad particle grid");
         for(int i = 0; i < 3; i++)
-            c->pgrid[i] = ps[i];
+            set_tracked(c, c->pgrid[i], ps[i]);
         return 0;
     }
 
@@ -1567,8 +1599,8 @@ extern "C"
             return fail(c, ISAAC_HIP_ERR_ARG, "unknown render form");
         if(brick_dtype != ISAAC_HIP_F32 && brick_dtype != ISAAC_HIP_F16 && brick_dtype != ISAAC_HIP_BF16)
             return fail(c, ISAAC_HIP_ERR_ARG, "brick dtype must be ISAAC_HIP_F32, ISAAC_HIP_F16 or ISAAC_HIP_BF16");
-        c->form = form;
-        c->brickDtype = brick_dtype;
+        set_tracked(c, c->form, int(form));
+        set_tracked(c, c->brickDtype, int(brick_dtype));
         return 0;
     }
 

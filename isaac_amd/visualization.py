@@ -356,7 +356,8 @@ class IsaacVisualization:
 
     def draw(self, out=None, viewport=None, projection=None, modelview_=None):
         """drawCallBack (lib/isaac.hpp:2140-2543).  `out`: None -> host numpy
-        RGBA8 (H, W, 4); a CUDA uint8 tensor -> device copy; False -> no copy."""
+        RGBA8 (H, W, 4); a CUDA uint8 tensor -> device copy; a host numpy array or
+        (pinned) CPU tensor of H*W*4 bytes -> host copy into it; False -> no copy."""
         self._push_state()
         W, H = self.framebuffer_size
         proj = np.ascontiguousarray(projection if projection is not None else self.projection, np.float64)
@@ -371,6 +372,14 @@ class IsaacVisualization:
             ptr, dev = result.ctypes.data, 0
         elif out is False:
             ptr, dev = None, 0
+        elif isinstance(out, np.ndarray):
+            if out.nbytes != W * H * 4 or not out.flags["C_CONTIGUOUS"]:
+                raise ValueError("host framebuffer: a contiguous array of width*height*4 bytes")
+            ptr, dev = out.ctypes.data, 0
+        elif not out.is_cuda:
+            if out.numel() * out.element_size() != W * H * 4 or not out.is_contiguous():
+                raise ValueError("host framebuffer: a contiguous tensor of width*height*4 bytes")
+            ptr, dev = out.data_ptr(), 0
         else:
             ptr, dev = out.data_ptr(), 1
         check(

@@ -31,7 +31,19 @@
 //   camera inverse .............................. M * inv(M) = I, FP64 vs FP32
 // glm's exact FP32 rounding for inverse/normalize is "parity unpinned" (no
 // glm build is available); the restatement follows glm's published op order.
+//
+// ORACLE_MUTANT (default 0) plants one known mistake, for the tests that show the
+// closed-form checks reject it; the shipped liboracle.so is always built with 0:
+//   1 classification divides by the weights of ALL sources (particles included)
+//   2 the volume march multiplies by the weights of the volume sources only
+//   3 iso ties go to the later source (<= instead of <)
+//   4 iso texel alpha is not rescaled by the threshold
+//   5 render mode 7 keeps the texel colour instead of the rank hue
+//   6 render mode 2 keeps the depth (the volume is drawn over the solid)
 #pragma once
+#ifndef ORACLE_MUTANT
+#define ORACLE_MUTANT 0
+#endif
 
 #include <algorithm>
 #include <cmath>
@@ -655,6 +667,10 @@ namespace isaac_oracle
             float tw = 0.0f;
             for(int s = 0; s < nv; s++)
                 tw += st.weight[s];
+#if ORACLE_MUTANT == 1
+            for(size_t s = size_t(nv); s < st.weight.size(); s++)
+                tw += st.weight[s];
+#endif
             totalWeight = R(tw);
         }
 #pragma omp parallel for schedule(dynamic, 1) collapse(2)
@@ -689,8 +705,14 @@ namespace isaac_oracle
                         if(thr > R(0))
                         {
                             V4<R> c = color;
+#if ORACLE_MUTANT != 4
                             c.w = c.w / thr * R(0.5);
+#endif
+#if ORACLE_MUTANT == 3
+                            if(ic.w <= c.w)
+#else
                             if(ic.w < c.w)
+#endif
                                 ic = c;
                         }
                     }
@@ -1394,8 +1416,10 @@ namespace isaac_oracle
                 V4<R> color = from_byte<R>(g.color[p]);
                 V3<R> normal = g.normal[p];
                 R aoStrength = R(1) - g.ao[p];
+#if ORACLE_MUTANT != 5
                 if(mode == 7)
                     color = hsva<R>(R(halton(st.rank, 3) * float(2 * M_PI)), 1, 1, color.w);
+#endif
                 if(mode < 3 || mode == 7)
                 {
                     Ray<R> ray = pixel_to_ray(fr.invMVP, sz, R(x), R(y), R(g.W), R(g.H));
@@ -1414,7 +1438,11 @@ namespace isaac_oracle
                     V4<R> sc = color * lightFactor;
                     V3<R> shaded{gmin(sc.x + spec, R(1)), gmin(sc.y + spec, R(1)), gmin(sc.z + spec, R(1))};
                     g.color[p] = to_byte(V4<R>{shaded.x, shaded.y, shaded.z, color.w});
+#if ORACLE_MUTANT == 6
+                    if(mode == 7)
+#else
                     if(mode == 2 || mode == 7)
+#endif
                         g.depth[p] = R(0);
                 }
                 else if(mode == 3)
@@ -1457,8 +1485,13 @@ namespace isaac_oracle
     {
         const Sizes& sz = st.sizes;
         float twf = 0.0f;
+#if ORACLE_MUTANT == 2
+        for(size_t s = 0; s < st.vol.size(); s++)
+            twf += st.weight[s];
+#else
         for(float w : st.weight)
             twf += w;
+#endif
         if(!(twf > 0.0f))
             return;
         const R totalWeight = R(twf);

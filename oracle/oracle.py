@@ -43,17 +43,51 @@ def build(force: bool = False) -> str:
 _lib = None
 
 
+def _bind(path: str):
+    L = ctypes.CDLL(path)
+    L.oracle_render_f32.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.oracle_render_f64.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.oracle_apply_chain.restype = ctypes.c_float
+    L.oracle_ray.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_void_p]
+    L.oracle_perspective.argtypes = [ctypes.c_double] * 4 + [ctypes.c_void_p]
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         build()
-        _lib = ctypes.CDLL(LIB_PATH)
-        _lib.oracle_render_f32.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
-        _lib.oracle_render_f64.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
-        _lib.oracle_apply_chain.restype = ctypes.c_float
-        _lib.oracle_ray.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_void_p]
-        _lib.oracle_perspective.argtypes = [ctypes.c_double] * 4 + [ctypes.c_void_p]
+        _lib = _bind(LIB_PATH)
     return _lib
+
+
+def build_mutant(mutant: int, out_dir: str) -> str:
+    """Compile the restatement with ORACLE_MUTANT=<mutant> (one planted mistake, see
+    isaac_oracle.hpp) into out_dir; used only by the tests that show the closed-form
+    checks reject each planted mistake."""
+    path = os.path.join(out_dir, f"liboracle_mutant{int(mutant)}.so")
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-fast-math",
+                           "-shared", f"-DORACLE_MUTANT={int(mutant)}", "-o", path, os.path.join(HERE, "oracle_capi.cpp")])
+    return path
+
+
+class use_library:
+    """Context manager: route this module's calls through another build of the
+    restatement (a mutant from build_mutant) and restore the shipped one after."""
+
+    def __init__(self, path: str):
+        self.path = path
+
+    def __enter__(self):
+        global _lib
+        self.saved = lib()
+        _lib = _bind(self.path)
+        return _lib
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.saved
+        return False
 
 
 # --------------------------------------------------------------------------- host-side restatements

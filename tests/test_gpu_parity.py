@@ -184,16 +184,18 @@ def test_functor_chains_and_nonpersistent(gpu, oracle_lib):
     compare("chains", spec, img, ora)
 
 
-def test_production_full_size(gpu, oracle_lib):
-    """The benchmark workload itself (256^3 x 3 sources, 2M particles, 1920x1080)."""
+@pytest.mark.parametrize("pose", ["default", "fill"])
+def test_production_full_size(gpu, oracle_lib, pose):
+    """The benchmark workload itself (256^3 x 3 sources, 2M particles, 1920x1080)
+    at both of its poses (the fill pose executes 272 M samples per frame)."""
     from oracle import oracle
 
-    spec = scenes.prod_scene()
+    spec = scenes.prod_scene(pose=scenes.DEFAULT_POSE if pose == "default" else scenes.FILL_POSE)
     vis, img = gpu_frame(spec, gpu)
     img2 = vis.do_visualization()
     assert np.array_equal(img, img2), "two renders of the same frame differ"
     ora = oracle.render(spec, 32)
-    compare("prod256", spec, img, ora)
+    compare(f"prod256-{pose}", spec, img, ora)
     c = vis.counters()
     print("gpu counters", c, "oracle", ora.counters, flush=True)
     assert abs(c["vol_samples"] - ora.counters["vol_samples"]) <= 1e-3 * ora.counters["vol_samples"]
@@ -260,3 +262,32 @@ def test_odd_framebuffers(gpu, oracle_lib, fb):
             compare(f"fb {fb}", spec, img, oracle.render(spec, 32))
         else:
             compare(f"fb {fb} bricks", spec, img, oracle.render_post(spec, 32, dtype=oracle.BRICK_F32))
+
+
+def test_multi_source_weighting(gpu, oracle_lib):
+    """Three volume sources with different weights, transfer functions and iso
+    thresholds plus an invisible particle source: the classification divides by the
+    volume sources' weights, the march multiplies by all sources' weights, the iso
+    texel is the argmax of alpha / threshold / 2 (closed forms in
+    tests/test_oracle_weighting.py).  Textures bit-exact, frame within the bar."""
+    from oracle import oracle
+    from test_oracle_weighting import ISO_CASES, ISO_TEXEL, VOL_TEXEL, _iso_scene, _weighting_scene
+
+    for dither in (0, 1):
+        spec = _weighting_scene(dither)
+        vis, img = gpu_frame(spec, gpu)
+        ora = oracle.render(spec, 32, want_textures=True)
+        vol, iso = vis.combined_textures()
+        assert np.array_equal(vol, ora.vol_tex)
+        assert np.array_equal(iso, ora.iso_tex)
+        if dither == 0:
+            assert (vol == np.round(VOL_TEXEL * 255).astype(np.uint8)).all()
+            assert (iso == ISO_TEXEL).all()
+        compare(f"weights-dither{dither}", spec, img, ora)
+        vis.close()
+    for sources, expect in ISO_CASES:
+        spec = _iso_scene(sources)
+        vis, img = gpu_frame(spec, gpu)
+        vol, iso = vis.combined_textures()
+        assert (iso == np.array(expect, np.uint8)).all(), (sources, np.unique(iso.reshape(-1, 4), axis=0))
+        vis.close()
