@@ -296,7 +296,8 @@ extern "C"
      * interpolation, brick dtype, occlusion) and bind the same sources.  Contexts must
      * outlive the batch.  The batch renders on a stream of its own; each render waits
      * for the work already queued on every context's stream (update_buffers, network
-     * training), so bricks rebuilt in place are read after their rebuild.  set_views
+     * training), so bricks rebuilt in place are read after their rebuild, and the work
+     * queued on those streams afterwards waits for the render.  set_views
      * snapshots the contexts' buffers and steering: after any context reallocates a
      * buffer (update_buffers with more particles or another local size, brick dtype
      * change), changes stream or steering, render returns ISAAC_HIP_ERR_STATE until

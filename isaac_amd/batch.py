@@ -201,16 +201,21 @@ def make_subvolumes(device_index: int = 0, count: int = 48, seed: int = 5, fb=(9
     return vises, sides
 
 
-def views_for(vises: Sequence, nviews: int = 8, seed: int = 5):
-    """Seeded random poses per visualization; (projections, modelviews) frame-major."""
+def poses_for(count: int, nviews: int = 8, seed: int = 5):
+    """The seeded camera poses of `count` sub-volumes x `nviews` views, frame-major."""
     from . import scenes
-    from .visualization import modelview
 
     rng = np.random.Generator(np.random.PCG64(seed + 1000))
+    return [scenes.random_pose(rng) for _ in range(count * nviews)]
+
+
+def views_for(vises: Sequence, nviews: int = 8, seed: int = 5):
+    """Seeded random poses per visualization; (projections, modelviews) frame-major."""
+    from .visualization import modelview
+
+    poses = poses_for(len(vises), nviews, seed)
     projs, mvs = [], []
-    for v in vises:
-        for _ in range(nviews):
-            p = scenes.random_pose(rng)
-            projs.append(v.projection)
-            mvs.append(modelview(p.look_at, p.rotation, p.distance))
+    for f, p in enumerate(poses):
+        projs.append(vises[f // nviews].projection)
+        mvs.append(modelview(p.look_at, p.rotation, p.distance))
     return np.stack(projs), np.stack(mvs)

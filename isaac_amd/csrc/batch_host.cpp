@@ -448,6 +448,10 @@ extern "C"
         else
             return bfail(b, ISAAC_HIP_ERR_ARG, "unknown batch mode");
         BCHK(b, hipEventRecord(b->ev[1], b->stream));
+        // and the contexts' later work (a rebuild of their bricks or cell lists) after the
+        // frames that read their buffers
+        for(size_t i = 0; i < b->ctx.size(); i++)
+            BCHK(b, hipStreamWaitEvent(b->ctx[i]->stream, b->ev[1], 0));
         b->timed = true;
         return 0;
     }

@@ -499,7 +499,8 @@ def render_post(spec, precision: int = 64, dtype: int = BRICK_RAW, stride: int =
     precision 32: ray setup in binary64, sampling/compositing in binary32 (the
     library's arithmetic); 64: everything in binary64.  dtype BRICK_RAW samples the
     float sources (the FP64 reference image), BRICK_F32/F16/BF16 the bricks.
-    stride > 1 renders only pixels with x % stride == 0 and y % stride == 0."""
+    stride > 1 renders only the pixels of sampled_mask(W, H, stride, SSAO on): with
+    SSAO on the stride must divide 10 and the viewport be the whole framebuffer."""
     ps = prepared or PreparedScene(spec, threads=threads)
     ps.sc.threads = threads
     W, H = spec.framebuffer
@@ -524,6 +525,20 @@ def render_post(spec, precision: int = 64, dtype: int = BRICK_RAW, stride: int =
     res.seconds = out.seconds
     res.prepared = ps
     return res
+
+
+def sampled_mask(width: int, height: int, stride: int, ssao: bool) -> np.ndarray:
+    """(H, W) bool: the pixels render_post renders at `stride`.  x % stride == 0 and
+    y % stride == 0; with SSAO on also (W-1-x) % stride == 0 and (H-1-y) % stride == 0,
+    which closes the set under the SSAO taps x + 10i clamped to [0, W-1]."""
+    def kept(n):
+        v = np.arange(n)
+        k = v % stride == 0
+        if ssao:
+            k |= (n - 1 - v) % stride == 0
+        return k
+
+    return kept(height)[:, None] & kept(width)[None, :]
 
 
 def brick_scales(data: np.ndarray, dtype: int) -> np.ndarray:

@@ -326,7 +326,11 @@ namespace isaac_oracle
     };
 
     // The whole frame in the brick form.  `stride` > 1 renders only pixels with
-    // x % stride == 0 and y % stride == 0 (sampled error studies; SSAO must be off).
+    // x % stride == 0 and y % stride == 0 (sampled error studies).  With SSAO on, the
+    // stride must divide the SSAO tap spacing (10 pixels) and the viewport must be the
+    // whole framebuffer; then pixels with (W-1-x) % stride == 0 or (H-1-y) % stride == 0
+    // are rendered too, so every depth tap of a rendered pixel (x + 10i clamped to
+    // [0, W-1]) is itself a rendered pixel and its occlusion is exact.
     template<class R>
     inline void render_post(const PostScene& ps, const std::vector<CellList>& lists, int stride, PostOut<R>& out)
     {
@@ -346,6 +350,12 @@ namespace isaac_oracle
         Frame<double> frd = make_frame<double>(st);
         Frame<R> fr = make_frame<R>(st);
         const V3<R> scale{R(st.scale[0]), R(st.scale[1]), R(st.scale[2])};
+        const bool ssao = st.aoEnabled && st.aoWeight > 0.0f;
+        if(ssao && stride != 1
+           && (10 % stride != 0 || st.viewport[0] != 0 || st.viewport[1] != 0 || st.viewport[2] < st.fbW
+               || st.viewport[3] < st.fbH))
+            throw std::runtime_error("sampled rendering with SSAO needs a stride dividing 10 and the whole framebuffer");
+        auto kept = [&](int v, int n) { return v % stride == 0 || (ssao && (n - 1 - v) % stride == 0); };
         auto each = [&](auto&& fn)
         {
             const int x0 = st.viewport[0], y0 = st.viewport[1];
@@ -354,10 +364,10 @@ namespace isaac_oracle
 #pragma omp parallel for schedule(dynamic, 1)
             for(int y = y0; y < y1; y++)
             {
-                if(y % stride)
+                if(!kept(y, st.fbH))
                     continue;
                 for(int x = x0; x < x1; x++)
-                    if(x % stride == 0)
+                    if(kept(x, st.fbW))
                         fn(x, y);
             }
         };
@@ -479,12 +489,8 @@ namespace isaac_oracle
             out.counters.isoHits += hits;
             out.counters.isoRays += rays;
         }
-        if(st.aoEnabled && st.aoWeight > 0.0f)
-        {
-            if(stride != 1)
-                throw std::runtime_error("sampled rendering needs SSAO off");
+        if(ssao)
             pass_ssao(st, g);
-        }
         out.pre = g.color;
         pass_shading(st, fr, g);
         // volume

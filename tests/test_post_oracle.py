@@ -263,3 +263,29 @@ def test_sampled_rendering_matches_full(oracle_lib):
     full = oracle_lib.render_post(spec, 64, RAW)
     part = oracle_lib.render_post(spec, 64, RAW, stride=4)
     assert np.array_equal(full.rgba_f[::4, ::4], part.rgba_f[::4, ::4])
+
+
+@pytest.mark.parametrize("stride", [2, 5, 10])
+def test_sampled_rendering_with_ssao_matches_full(oracle_lib, stride):
+    """With SSAO on, the sampled pixel set is closed under the 7x7 taps (spacing 10,
+    clamped at the borders), so every sampled pixel equals the full render's."""
+    import dataclasses
+
+    spec = scenes.prod_scene(n=32, fb=(201, 133), n_particles=500, pose=scenes.FILL_POSE)
+    spec = dataclasses.replace(spec, ao_enabled=True, ao_weight=0.8)
+    full = oracle_lib.render_post(spec, 32, F16)
+    part = oracle_lib.render_post(spec, 32, F16, stride=stride)
+    m = oracle_lib.sampled_mask(201, 133, stride, True)
+    assert m[0].sum() > 1 and m[:, 0].sum() > 1 and m[-1, -1]
+    assert (full.ao[m] > 0).any(), "the scene should occlude somewhere"
+    assert np.array_equal(full.ao[m], part.ao[m])
+    assert np.array_equal(full.rgba[m], part.rgba[m])
+    assert np.array_equal(full.rgba_f[m], part.rgba_f[m])
+
+
+def test_sampled_rendering_with_ssao_rejects_bad_strides(oracle_lib):
+    import dataclasses
+
+    spec = dataclasses.replace(scenes.small_scene(n=16, fb=(40, 30)), ao_enabled=True)
+    with pytest.raises(RuntimeError):
+        oracle_lib.render_post(spec, 32, F16, stride=3)
