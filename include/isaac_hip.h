@@ -489,6 +489,132 @@ extern "C"
         int32_t T,
         uint32_t positions[2],
         float rgba[8]);
+    /* ---------------------------------------------------------------- server messages
+     * The JSON messages the master rank sends to the ISAAC server, as the reference
+     * builds them with jansson and writes them with json_dumps(root, 0): the register
+     * message of the constructor (lib/isaac.hpp:834-892, sent by init, 1276-1292), the
+     * periodic state message of each visualization step (lib/isaac.hpp:2546-2838) and
+     * the exit message of the destructor (lib/isaac.hpp:2110-2118); and the bytes
+     * serverSend puts on the socket (lib/isaac/isaac_communicator.hpp:125-190).  Host
+     * only: no GPU work, no context.  jansson's writer as restated here: objects keep
+     * insertion order, ", " between items and ": " after keys, no indentation;
+     * integers as %lld; reals as %.17g with ".0" appended when the text has neither
+     * '.' nor 'e', and a '+' and leading zeros removed from the exponent; strings with
+     * \" \\ \b \f \n \r \t and \u00XX (uppercase hex) for other control characters,
+     * '/' and non-ASCII bytes unescaped.  Every message function writes the text and a
+     * terminating 0 into msg (host, capacity bytes) and sets *length to the text's
+     * length (also when capacity is too small, which returns ISAAC_HIP_ERR_ARG). */
+    typedef struct isaac_hip_steering_state
+    {
+        int32_t n_sources; /* combined order: volume, field, particle sources */
+        int32_t n_volume_field; /* volume + field sources (those with an iso threshold) */
+        const char* const* source_names; /* getName() of each source */
+        const int32_t* feature_dims;
+        const char* const* functions; /* functor chain text of each source */
+        const int32_t* function_errors; /* its errorCode */
+        int32_t transfer_size; /* T */
+        const float* transfer_lut; /* n_sources * T RGBA floats */
+        const int32_t* transfer_point_counts; /* control points per source */
+        const uint32_t* transfer_point_values; /* all sources' points, concatenated, ascending per source */
+        const float* transfer_point_rgba; /* 4 per point */
+        const float* weights; /* n_sources */
+        const float* iso_thresholds; /* n_volume_field */
+        const float* min_max; /* (min, max) per source */
+        float step;
+        int32_t interpolation;
+        int32_t render_optimization;
+        int32_t seed_points;
+        float advection_step;
+        float advection_weight;
+        int32_t advection_seeding_period;
+        int32_t advection_seeding_duration;
+        int32_t advection_on_pause;
+        int32_t advection_border;
+        float background[3];
+        int32_t n_clipping;
+        const float* clipping_position; /* 3 per plane */
+        const float* clipping_normal; /* 3 per plane: the normals as set (clippingSavedNormals) */
+        int32_t ao_enabled;
+        float ao_weight;
+        int32_t render_mode;
+        int32_t dither_mode;
+        double projection[16]; /* column-major */
+        double look_at[3];
+        double rotation[9]; /* column-major */
+        double distance;
+        const char* metadata; /* JSON text of the "metadata" object; NULL: {} */
+    } isaac_hip_steering_state;
+
+    /* sections of the periodic message: the reference's send* flags */
+    enum isaac_hip_send_flag
+    {
+        ISAAC_HIP_SEND_PROJECTION = 1u << 0,
+        ISAAC_HIP_SEND_LOOK_AT = 1u << 1,
+        ISAAC_HIP_SEND_ROTATION = 1u << 2,
+        ISAAC_HIP_SEND_DISTANCE = 1u << 3,
+        ISAAC_HIP_SEND_TRANSFER = 1u << 4,
+        ISAAC_HIP_SEND_FUNCTIONS = 1u << 5,
+        ISAAC_HIP_SEND_WEIGHT = 1u << 6,
+        ISAAC_HIP_SEND_INTERPOLATION = 1u << 7,
+        ISAAC_HIP_SEND_RENDER_OPTIMIZATION = 1u << 8,
+        ISAAC_HIP_SEND_STEP = 1u << 9,
+        ISAAC_HIP_SEND_SEED_POINTS = 1u << 10,
+        ISAAC_HIP_SEND_ADVECTION_STEP = 1u << 11,
+        ISAAC_HIP_SEND_ADVECTION_WEIGHT = 1u << 12,
+        ISAAC_HIP_SEND_ADVECTION_SEEDING_PERIOD = 1u << 13,
+        ISAAC_HIP_SEND_ADVECTION_SEEDING_DURATION = 1u << 14,
+        ISAAC_HIP_SEND_ADVECTION_ON_PAUSE = 1u << 15,
+        ISAAC_HIP_SEND_ADVECTION_BORDER = 1u << 16,
+        ISAAC_HIP_SEND_ISO_THRESHOLD = 1u << 17,
+        ISAAC_HIP_SEND_MIN_MAX = 1u << 18,
+        ISAAC_HIP_SEND_BACKGROUND = 1u << 19,
+        ISAAC_HIP_SEND_CLIPPING = 1u << 20,
+        ISAAC_HIP_SEND_AO = 1u << 21,
+        ISAAC_HIP_SEND_RENDER_MODE = 1u << 22,
+        ISAAC_HIP_SEND_DITHER_MODE = 1u << 23,
+        ISAAC_HIP_SEND_INIT = 1u << 24 /* sendInitJson set by a request ("init" / "retry") */
+    };
+
+    /* The master rank's message state across steps: the init root (jsonInitRoot), which
+     * starts as the register message's root and takes the camera, interpolation, step,
+     * advection, background, occlusion and mode values of every periodic message that
+     * sends them; a periodic message that sends any of these (or SEND_INIT) carries it
+     * as "init". */
+    typedef struct isaac_hip_session isaac_hip_session;
+    int isaac_hip_session_create(isaac_hip_session** out);
+    void isaac_hip_session_destroy(isaac_hip_session* session);
+    /* the register message; it becomes the session's init root.  framebuffer: the
+     * compositing buffer size; global_size_scaled: globalSizeScaled */
+    int isaac_hip_register_message(
+        isaac_hip_session* session,
+        const isaac_hip_steering_state* state,
+        const char* name,
+        int32_t nodes,
+        const int32_t framebuffer[2],
+        const int32_t global_size_scaled[3],
+        char* msg,
+        int64_t capacity,
+        int64_t* length);
+    /* the periodic message with meta number meta_nr and the sections in send (flags);
+     * ISAAC_HIP_ERR_STATE before the register message */
+    int isaac_hip_period_message(
+        isaac_hip_session* session,
+        const isaac_hip_steering_state* state,
+        int64_t meta_nr,
+        uint32_t send,
+        char* msg,
+        int64_t capacity,
+        int64_t* length);
+    int isaac_hip_exit_message(char* msg, int64_t capacity, int64_t* length);
+    /* serverSend's bytes for one call: with uid >= 0 (starting) "{\"uid\": <uid>"; then,
+     * when content is not NULL, content with its first byte replaced by ',' and without
+     * its last byte; then "} " when finishing.  A periodic message goes out starting and
+     * not finishing, the video frame message after it not starting and finishing (or,
+     * without a frame, content NULL and finishing), so the server reads one object
+     * (lib/isaac.hpp:2836-2861). */
+    int isaac_hip_server_send_bytes(int32_t uid, const char* content, int32_t finishing, char* out, int64_t capacity,
+                                    int64_t* length);
+
     /* library version string */
     const char* isaac_hip_version(void);
     /* blocking copy of `bytes` from device memory to host memory on the context

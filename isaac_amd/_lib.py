@@ -161,7 +161,57 @@ _SIGS = {
     "isaac_hip_default_transfer": (ctypes.c_int, [I32, I32, I32, I32, P, P]),
     "isaac_hip_version": (ctypes.c_char_p, []),
     "isaac_hip_copy_to_host": (ctypes.c_int, [P, P, P, ctypes.c_size_t]),
+    "isaac_hip_session_create": (ctypes.c_int, [P]),
+    "isaac_hip_session_destroy": (None, [P]),
+    "isaac_hip_register_message": (ctypes.c_int, [P, P, ctypes.c_char_p, I32, P, P, P, ctypes.c_int64, P]),
+    "isaac_hip_period_message": (ctypes.c_int, [P, P, ctypes.c_int64, ctypes.c_uint32, P, ctypes.c_int64, P]),
+    "isaac_hip_exit_message": (ctypes.c_int, [P, ctypes.c_int64, P]),
+    "isaac_hip_server_send_bytes": (ctypes.c_int, [I32, ctypes.c_char_p, I32, P, ctypes.c_int64, P]),
 }
+
+
+class SteeringState(ctypes.Structure):
+    """isaac_hip_steering_state (include/isaac_hip.h)."""
+
+    _fields_ = [
+        ("n_sources", ctypes.c_int32),
+        ("n_volume_field", ctypes.c_int32),
+        ("source_names", ctypes.POINTER(ctypes.c_char_p)),
+        ("feature_dims", ctypes.POINTER(ctypes.c_int32)),
+        ("functions", ctypes.POINTER(ctypes.c_char_p)),
+        ("function_errors", ctypes.POINTER(ctypes.c_int32)),
+        ("transfer_size", ctypes.c_int32),
+        ("transfer_lut", ctypes.c_void_p),
+        ("transfer_point_counts", ctypes.POINTER(ctypes.c_int32)),
+        ("transfer_point_values", ctypes.POINTER(ctypes.c_uint32)),
+        ("transfer_point_rgba", ctypes.POINTER(ctypes.c_float)),
+        ("weights", ctypes.POINTER(ctypes.c_float)),
+        ("iso_thresholds", ctypes.POINTER(ctypes.c_float)),
+        ("min_max", ctypes.POINTER(ctypes.c_float)),
+        ("step", ctypes.c_float),
+        ("interpolation", ctypes.c_int32),
+        ("render_optimization", ctypes.c_int32),
+        ("seed_points", ctypes.c_int32),
+        ("advection_step", ctypes.c_float),
+        ("advection_weight", ctypes.c_float),
+        ("advection_seeding_period", ctypes.c_int32),
+        ("advection_seeding_duration", ctypes.c_int32),
+        ("advection_on_pause", ctypes.c_int32),
+        ("advection_border", ctypes.c_int32),
+        ("background", ctypes.c_float * 3),
+        ("n_clipping", ctypes.c_int32),
+        ("clipping_position", ctypes.POINTER(ctypes.c_float)),
+        ("clipping_normal", ctypes.POINTER(ctypes.c_float)),
+        ("ao_enabled", ctypes.c_int32),
+        ("ao_weight", ctypes.c_float),
+        ("render_mode", ctypes.c_int32),
+        ("dither_mode", ctypes.c_int32),
+        ("projection", ctypes.c_double * 16),
+        ("look_at", ctypes.c_double * 3),
+        ("rotation", ctypes.c_double * 9),
+        ("distance", ctypes.c_double),
+        ("metadata", ctypes.c_char_p),
+    ]
 
 
 def header_functions() -> List[str]:

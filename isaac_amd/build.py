@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libisaac_hip.so")
-SOURCES = ["host.cpp", "aonet_host.cpp", "batch_host.cpp", "render.hip", "particles.hip", "aonet.hip", "composite.hip", "frame.hip", "jpeg.hip"]
+SOURCES = ["host.cpp", "aonet_host.cpp", "batch_host.cpp", "render.hip", "particles.hip", "aonet.hip", "composite.hip", "frame.hip", "jpeg.hip", "messages.cpp"]
 
 
 def hipcc() -> str:
