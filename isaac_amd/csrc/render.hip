@@ -452,6 +452,103 @@ namespace isaac_hip
         return channel<LINEAR, true>(T, 24);
     }
 
+    // ------------------------------------------------------------------ march sampling
+    // The march's per-step samples weigh the eight texels once per position: each
+    // channel is sum_k byte_k * w_k (one multiply and seven fmas) with the corner
+    // weights w_k = wx * wy * wz, instead of seven lerps per channel, and at the x
+    // edge (both x taps on texel x0, CLAMP) the first texel takes weight 1 instead of
+    // the texels being swapped.  In exact arithmetic this is the reference's trilinear
+    // (lib/isaac/isaac_helper.hpp:124-144); in binary32 the result differs from its
+    // nested lerps by rounding only.  Texel indices are 32-bit; rows y+1 and z+1
+    // are the row of y and z plus a stride or plus 0 at a clamped edge.
+    struct Corners
+    {
+        uint2 t[8]; // k = 4 x + 2 y + z (x outermost, as the reference's lerps nest)
+        float w[8];
+    };
+
+    // the corners of the sample at c: for LINEAR the position minus 0.5 (the texel
+    // centres' offset), for NEAREST the position itself
+    template<bool LINEAR>
+    __device__ inline void fetch_corners(
+        const uint2* __restrict__ tex,
+        int X,
+        int Y,
+        int Z,
+        uint32_t XY,
+        float cx,
+        float cy,
+        float cz,
+        Corners& C)
+    {
+        if(!LINEAR)
+        {
+            const uint32_t x = iclamp(int(cx), 0, X - 1), y = iclamp(int(cy), 0, Y - 1), z = iclamp(int(cz), 0, Z - 1);
+            C.t[0] = tex[x + uint32_t(X) * y + XY * z];
+            return;
+        }
+        const float flx = floorf(cx), fly = floorf(cy), flz = floorf(cz);
+        const int bx = int(flx), by = int(fly), bz = int(flz);
+        // taps x0 and x0 + 1 come from one 16-byte load; where both are x0 (bx < 0 or
+        // bx >= X - 1) the second gets weight 0
+        const bool xin = uint32_t(bx) < uint32_t(X - 1);
+        const float fx = xin ? cx - flx : 0.0f, fy = cy - fly, fz = cz - flz;
+        const uint32_t x0 = iclamp(bx, 0, X - 1), y0 = iclamp(by, 0, Y - 1), z0 = iclamp(bz, 0, Z - 1);
+        const uint32_t dy = uint32_t(by) < uint32_t(Y - 1) ? uint32_t(X) : 0u;
+        const uint32_t dz = uint32_t(bz) < uint32_t(Z - 1) ? XY : 0u;
+        const uint32_t i00 = x0 + __umul24(y0, uint32_t(X)) + z0 * XY;
+        const uint32_t i01 = i00 + dz, i10 = i00 + dy, i11 = i10 + dz;
+        const uint4 p00 = *reinterpret_cast<const uint4*>(tex + i00);
+        const uint4 p01 = *reinterpret_cast<const uint4*>(tex + i01);
+        const uint4 p10 = *reinterpret_cast<const uint4*>(tex + i10);
+        const uint4 p11 = *reinterpret_cast<const uint4*>(tex + i11);
+        C.t[0] = make_uint2(p00.x, p00.y);
+        C.t[1] = make_uint2(p01.x, p01.y);
+        C.t[2] = make_uint2(p10.x, p10.y);
+        C.t[3] = make_uint2(p11.x, p11.y);
+        C.t[4] = make_uint2(p00.z, p00.w);
+        C.t[5] = make_uint2(p01.z, p01.w);
+        C.t[6] = make_uint2(p10.z, p10.w);
+        C.t[7] = make_uint2(p11.z, p11.w);
+        const float ox = 1.0f - fx, oy = 1.0f - fy, oz = 1.0f - fz;
+        const float w00 = oy * oz, w01 = oy * fz, w10 = fy * oz, w11 = fy * fz;
+        C.w[0] = ox * w00;
+        C.w[1] = ox * w01;
+        C.w[2] = ox * w10;
+        C.w[3] = ox * w11;
+        C.w[4] = fx * w00;
+        C.w[5] = fx * w01;
+        C.w[6] = fx * w10;
+        C.w[7] = fx * w11;
+    }
+
+    // one channel in byte units (0..255): the volume (.x) or iso (.y) texel's byte at `sh`
+    template<bool LINEAR, bool ISO>
+    __device__ inline float corner_sum(const Corners& C, int sh)
+    {
+        auto d = [&](int k) { return float(((ISO ? C.t[k].y : C.t[k].x) >> sh) & 0xffu); };
+        if(!LINEAR)
+            return d(0);
+        float v = d(0) * C.w[0];
+#pragma unroll
+        for(int k = 1; k < 8; k++)
+            v = __builtin_fmaf(d(k), C.w[k], v);
+        return v;
+    }
+
+    template<bool LINEAR, bool ISO>
+    __device__ inline uint32_t corners_or(const Corners& C)
+    {
+        uint32_t o = ISO ? C.t[0].y : C.t[0].x;
+        if(LINEAR)
+        {
+#pragma unroll
+            for(int k = 1; k < 8; k++)
+                o |= ISO ? C.t[k].y : C.t[k].x;
+        }
+        return o;
+    }
+
     struct F4
     {
         float x, y, z, w;
@@ -701,33 +798,43 @@ namespace isaac_hip
                     int last = isoOn ? endIso : endVol;
                     if(i <= last)
                     {
-                        Taps cur;
-                        fetch_taps<LINEAR>(F.tex, F.X, F.Y, F.Z, startU.x + stepVec.x * float(i), startU.y + stepVec.y * float(i),
-                                           startU.z + stepVec.z * float(i), cur);
+                        // volume samples in byte units times factor / 255
+                        const float vscale = factor * (1.0f / 255.0f);
+                        const uint32_t XY = uint32_t(F.X) * uint32_t(F.Y);
+                        // sample i at startC + stepVec * i (one fma per axis), shifted by the
+                        // texel centres' -0.5 for the trilinear sampler
+                        const float sh = LINEAR ? 0.5f : 0.0f;
+                        const F3 startC = f3(startU.x - sh, startU.y - sh, startU.z - sh);
+                        auto at = [&](int k, Corners& C)
+                        {
+                            const float fk = float(k);
+                            fetch_corners<LINEAR>(F.tex, F.X, F.Y, F.Z, XY, __builtin_fmaf(stepVec.x, fk, startC.x),
+                                                  __builtin_fmaf(stepVec.y, fk, startC.y), __builtin_fmaf(stepVec.z, fk, startC.z), C);
+                        };
+                        Corners cur;
+                        at(i, cur);
                         for(;;)
                         {
                             if(COUNT)
                                 nSteps++;
-                            Taps nxt;
+                            Corners nxt;
                             const bool ahead = PREFETCH && i + 1 <= last;
                             if(ahead)
-                            {
-                                const float fn = float(i + 1);
-                                fetch_taps<LINEAR>(F.tex, F.X, F.Y, F.Z, startU.x + stepVec.x * fn, startU.y + stepVec.y * fn,
-                                                   startU.z + stepVec.z * fn, nxt);
-                            }
+                                at(i + 1, nxt);
                             if(isoOn)
                             {
-                                // eight zero alpha bytes interpolate to exactly 0
-                                const float value = (taps_or<LINEAR, true>(cur) >> 24) ? channel<LINEAR, true>(cur, 24) : 0.0f;
+                                // the iso alpha in byte units (eight zero alpha bytes give exactly 0);
+                                // alpha >= 0.5 is a sum >= 127.5
+                                const float sum = (corners_or<LINEAR, true>(cur) >> 24) ? corner_sum<LINEAR, true>(cur, 24) : 0.0f;
                                 if(COUNT)
                                     nIso++;
-                                const float tmp = oldValue;
-                                oldValue = value;
-                                if(value >= 0.5f)
+                                const float tmpSum = oldValue;
+                                oldValue = sum;
+                                if(sum >= 127.5f)
                                 {
                                     const bool first = ray.isClipped && i == startSteps;
                                     const float tt = float(i) * stepU;
+                                    const float value = div255(sum), tmp = div255(tmpSum);
                                     float d;
                                     if(first)
                                         d = ray.startDepth;
@@ -754,13 +861,13 @@ namespace isaac_hip
                             {
                                 if(COUNT)
                                     nVol++;
-                                if(taps_or<LINEAR, false>(cur) != 0u)
+                                if(corners_or<LINEAR, false>(cur) != 0u)
                                 {
-                                    const float vr = channel<LINEAR, false>(cur, 0) * factor;
-                                    const float vg = channel<LINEAR, false>(cur, 8) * factor;
-                                    const float vb = channel<LINEAR, false>(cur, 16) * factor;
-                                    const float va = channel<LINEAR, false>(cur, 24) * factor;
-                                    const float w = 1.0f - ca;
+                                    const float vr = corner_sum<LINEAR, false>(cur, 0);
+                                    const float vg = corner_sum<LINEAR, false>(cur, 8);
+                                    const float vb = corner_sum<LINEAR, false>(cur, 16);
+                                    const float va = corner_sum<LINEAR, false>(cur, 24);
+                                    const float w = (1.0f - ca) * vscale;
                                     cr = cr + w * vr;
                                     cg = cg + w * vg;
                                     cb = cb + w * vb;
@@ -778,8 +885,7 @@ namespace isaac_hip
                             if(PREFETCH)
                                 cur = nxt;
                             else
-                                fetch_taps<LINEAR>(F.tex, F.X, F.Y, F.Z, startU.x + stepVec.x * float(i),
-                                                   startU.y + stepVec.y * float(i), startU.z + stepVec.z * float(i), cur);
+                                at(i, cur);
                         }
                     }
                     if(isoHit)
@@ -1093,14 +1199,16 @@ namespace isaac_hip
     // Tap geometry of one sample position, shared by every source (all sources
     // of a rank have the local box's brick grid).  Sampler LINEAR|NEAREST with
     // CLAMP addressing (lib/isaac/isaac_texture.hpp:182-259): the taps of a sample
-    // all come from the brick holding its clamped (x0, y0, z0) tap; x1 = x0
-    // (clamped at the edge) duplicates the first element of each pair.
+    // all come from the brick holding its clamped (x0, y0, z0) tap; where x1 = x0
+    // (clamped at the edge) the second element of each pair gets weight 0.
     struct BrickTaps
     {
         uint32_t brick; // brick index
         uint32_t i00, i10, i01, i11; // pair index in a component block of rows (y0,z0) (y1,z0) (y0,z1) (y1,z1)
-        bool dup;
-        float fx, fy, fz;
+        // trilinear weights of the corners (x, y, z) = 000 001 010 011 100 101 110 111,
+        // shared by every component of every source at this position; where both x
+        // taps are texel x0 (CLAMP at the x edge) the x1 corners weigh 0
+        float w[8];
     };
 
     template<bool LINEAR>
@@ -1113,15 +1221,10 @@ namespace isaac_hip
             t.brick = ((z >> 3) * uint32_t(Q.nby) + (y >> 3)) * uint32_t(Q.nbx) + (x >> 3);
             t.i00 = ((z & 7) * 9 + (y & 7)) * 8 + (x & 7);
             t.i10 = t.i01 = t.i11 = t.i00;
-            t.dup = true;
-            t.fx = t.fy = t.fz = 0.0f;
             return t;
         }
         const float cx = px - 0.5f, cy = py - 0.5f, cz = pz - 0.5f;
         const float flx = floorf(cx), fly = floorf(cy), flz = floorf(cz);
-        t.fx = cx - flx;
-        t.fy = cy - fly;
-        t.fz = cz - flz;
         const int bx = int(flx), by = int(fly), bz = int(flz);
         const uint32_t x0 = iclamp(bx, 0, X - 1), x1 = iclamp(bx + 1, 0, X - 1);
         const uint32_t y0 = iclamp(by, 0, Y - 1), y1 = iclamp(by + 1, 0, Y - 1);
@@ -1135,7 +1238,17 @@ namespace isaac_hip
         t.i10 = lx + (lz0 * 9 + ly1) * 8;
         t.i01 = lx + (lz1 * 9 + ly0) * 8;
         t.i11 = lx + (lz1 * 9 + ly1) * 8;
-        t.dup = x1 == x0;
+        const float fx = x1 == x0 ? 0.0f : cx - flx, fy = cy - fly, fz = cz - flz;
+        const float ox = 1.0f - fx, oy = 1.0f - fy, oz = 1.0f - fz;
+        const float w00 = oy * oz, w01 = oy * fz, w10 = fy * oz, w11 = fy * fz;
+        t.w[0] = ox * w00;
+        t.w[1] = ox * w01;
+        t.w[2] = ox * w10;
+        t.w[3] = ox * w11;
+        t.w[4] = fx * w00;
+        t.w[5] = fx * w01;
+        t.w[6] = fx * w10;
+        t.w[7] = fx * w11;
         return t;
     }
 
@@ -1241,22 +1354,20 @@ namespace isaac_hip
         decode_row<DT, DIM>(R.r[1], d010, d110);
         decode_row<DT, DIM>(R.r[2], d001, d101);
         decode_row<DT, DIM>(R.r[3], d011, d111);
-        const float oz = 1.0f - t.fz, oy = 1.0f - t.fy, ox = 1.0f - t.fx;
+        // sum of the corners times their weights (one multiply and seven fmas per
+        // component), then the brick scale; equal to the nested lerps in exact arithmetic
 #pragma unroll
         for(int c = 0; c < DIM; c++)
         {
-            if(t.dup)
-            {
-                d100[c] = d000[c];
-                d110[c] = d010[c];
-                d101[c] = d001[c];
-                d111[c] = d011[c];
-            }
-            const float a0 = d000[c] * oz + d001[c] * t.fz, a1 = d010[c] * oz + d011[c] * t.fz;
-            const float A = a0 * oy + a1 * t.fy;
-            const float b0 = d100[c] * oz + d101[c] * t.fz, b1 = d110[c] * oz + d111[c] * t.fz;
-            const float B = b0 * oy + b1 * t.fy;
-            v[c] = (A * ox + B * t.fx) * __uint_as_float(R.sc[c]);
+            float a = d000[c] * t.w[0];
+            a = __builtin_fmaf(d001[c], t.w[1], a);
+            a = __builtin_fmaf(d010[c], t.w[2], a);
+            a = __builtin_fmaf(d011[c], t.w[3], a);
+            a = __builtin_fmaf(d100[c], t.w[4], a);
+            a = __builtin_fmaf(d101[c], t.w[5], a);
+            a = __builtin_fmaf(d110[c], t.w[6], a);
+            a = __builtin_fmaf(d111[c], t.w[7], a);
+            v[c] = a * __uint_as_float(R.sc[c]);
         }
     }
 
