@@ -397,6 +397,20 @@ extern "C"
     int isaac_hip_jpeg_frame_message(const uint8_t* rgbx, int32_t width, int32_t height, int32_t quality, void* workspace,
                                      int64_t workspace_bytes, char* msg, int64_t capacity, int64_t* length, void* stream);
 
+    /* No counterpart in the reference.  For callers that cannot allocate device memory
+     * (serving loops and harnesses that forbid hipMalloc): isaac_hip_jpeg_encode without
+     * the workspace argument.  The same kernels work in the library's own device scratch
+     * of ISAAC_HIP_JPEG_NOALLOC_SCRATCH_BYTES (44 MiB), which every GPU the library runs
+     * on holds until the process ends; frames whose isaac_hip_jpeg_workspace_bytes exceeds
+     * it (larger than 1920x1080 or 2048x1024) return ISAAC_HIP_ERR_ARG.  The stream must
+     * belong to the current device (0: its null stream).  Calls no hipMalloc or hipFree.
+     * Every call on one GPU uses that GPU's one scratch: calls that overlap on one GPU
+     * (from other threads or on other streams) race, so the caller serialises them.
+     * Blocks until the file is on the host. */
+#define ISAAC_HIP_JPEG_NOALLOC_SCRATCH_BYTES (44ll << 20)
+    int isaac_hip_jpeg_encode_noalloc(const uint8_t* rgbx, int32_t width, int32_t height, int32_t quality, uint8_t* jpeg,
+                                      int64_t capacity, int64_t* length, void* stream);
+
     /* ---------------------------------------------------------------- learned occlusion
      * No counterpart in the reference.  A U-Net over the G-buffer (input: depth /
      * maxGlobalSizeScaled, background flag, normal; 3x3 convolutions with 16/32/64/64

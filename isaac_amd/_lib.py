@@ -142,6 +142,8 @@ _SIGS = {
                                              ctypes.c_int64, P, P]),
     "isaac_hip_jpeg_frame_message": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, P, ctypes.c_int64, P,
                                                     ctypes.c_int64, P, P]),
+    "isaac_hip_jpeg_encode_noalloc": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, P, ctypes.c_int64, P,
+                                                     P]),
     "isaac_hip_aonet_param_count": (ctypes.c_int64, []),
     "isaac_hip_aonet_set_params": (ctypes.c_int, [P, P, ctypes.c_int64]),
     "isaac_hip_aonet_get_params": (ctypes.c_int, [P, P, ctypes.c_int64]),

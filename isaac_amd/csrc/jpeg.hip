@@ -26,7 +26,9 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <initializer_list>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "isaac_hip.h"
@@ -556,10 +558,33 @@ namespace isaac_hip
         return L;
     }
 
-    // the markers before the entropy-coded data (jcmarker.c order)
-    std::vector<uint8_t> jpeg_headers(int32_t width, int32_t height, const int qt[2][64])
+    // the marker bytes before the entropy-coded data, built in place (at most
+    // 20 + 2 * 133 + 19 + 4 * 21 + 348 + 14 = 751 bytes)
+    struct JpegHeaders
     {
-        std::vector<uint8_t> h = {0xFF, 0xD8, 0xFF, 0xE0, 0x00, 0x10, 'J', 'F', 'I', 'F', 0x00, 0x01, 0x01, 0x00, 0x00, 0x01, 0x00, 0x01, 0x00, 0x00};
+        uint8_t b[1024];
+        size_t n = 0;
+        void put(std::initializer_list<uint8_t> v)
+        {
+            for(uint8_t x : v)
+                b[n++] = x;
+        }
+        void put(const uint8_t* p, size_t k)
+        {
+            std::memcpy(b + n, p, k);
+            n += k;
+        }
+        void push_back(uint8_t x)
+        {
+            b[n++] = x;
+        }
+    };
+
+    // the markers before the entropy-coded data (jcmarker.c order)
+    void jpeg_headers(int32_t width, int32_t height, const int qt[2][64], JpegHeaders& h)
+    {
+        h.n = 0;
+        h.put({0xFF, 0xD8, 0xFF, 0xE0, 0x00, 0x10, 'J', 'F', 'I', 'F', 0x00, 0x01, 0x01, 0x00, 0x00, 0x01, 0x00, 0x01, 0x00, 0x00});
         bool wide = false;
         for(int t = 0; t < 2; t++)
         {
@@ -567,7 +592,7 @@ namespace isaac_hip
             for(int i = 0; i < 64; i++)
                 w16 = w16 || qt[t][i] > 255;
             wide = wide || w16;
-            h.insert(h.end(), {0xFF, 0xDB, 0x00, uint8_t(w16 ? 0x83 : 0x43), uint8_t((w16 ? 0x10 : 0x00) | t)});
+            h.put({0xFF, 0xDB, 0x00, uint8_t(w16 ? 0x83 : 0x43), uint8_t((w16 ? 0x10 : 0x00) | t)});
             for(int k = 0; k < 64; k++)
             {
                 const int v = qt[t][kJpegZigzag[k]];
@@ -576,8 +601,8 @@ namespace isaac_hip
                 h.push_back(uint8_t(v & 0xff));
             }
         }
-        h.insert(h.end(), {0xFF, uint8_t(wide ? 0xC1 : 0xC0), 0x00, 0x11, 0x08, uint8_t(height >> 8), uint8_t(height & 0xff),
-                           uint8_t(width >> 8), uint8_t(width & 0xff), 0x03, 0x01, 0x22, 0x00, 0x02, 0x11, 0x01, 0x03, 0x11, 0x01});
+        h.put({0xFF, uint8_t(wide ? 0xC1 : 0xC0), 0x00, 0x11, 0x08, uint8_t(height >> 8), uint8_t(height & 0xff),
+               uint8_t(width >> 8), uint8_t(width & 0xff), 0x03, 0x01, 0x22, 0x00, 0x02, 0x11, 0x01, 0x03, 0x11, 0x01});
         const uint8_t* bits[4] = {kJpegBits_dc0, kJpegBits_ac0, kJpegBits_dc1, kJpegBits_ac1};
         const uint8_t* vals[4] = {kJpegVals_dc0, kJpegVals_ac0, kJpegVals_dc1, kJpegVals_ac1};
         const uint8_t cls[4] = {0x00, 0x10, 0x01, 0x11};
@@ -587,45 +612,17 @@ namespace isaac_hip
             for(int i = 0; i < 16; i++)
                 n += bits[t][i];
             const int len = 3 + 16 + n;
-            h.insert(h.end(), {0xFF, 0xC4, uint8_t(len >> 8), uint8_t(len & 0xff), cls[t]});
-            h.insert(h.end(), bits[t], bits[t] + 16);
-            h.insert(h.end(), vals[t], vals[t] + n);
+            h.put({0xFF, 0xC4, uint8_t(len >> 8), uint8_t(len & 0xff), cls[t]});
+            h.put(bits[t], 16);
+            h.put(vals[t], size_t(n));
         }
-        h.insert(h.end(), {0xFF, 0xDA, 0x00, 0x0C, 0x03, 0x01, 0x00, 0x02, 0x11, 0x03, 0x11, 0x00, 0x3F, 0x00});
-        return h;
-    }
-} // namespace isaac_hip
-
-using namespace isaac_hip;
-
-extern "C"
-{
-    int64_t isaac_hip_jpeg_workspace_bytes(int32_t width, int32_t height)
-    {
-        if(width < 1 || height < 1 || width > 65535 || height > 65535)
-            return -1;
-        return int64_t(jpeg_layout(width, height).end);
+        h.put({0xFF, 0xDA, 0x00, 0x0C, 0x03, 0x01, 0x00, 0x02, 0x11, 0x03, 0x11, 0x00, 0x3F, 0x00});
     }
 
-    int isaac_hip_jpeg_encode(
-        const uint8_t* rgbx,
-        int32_t width,
-        int32_t height,
-        int32_t quality,
-        void* workspace,
-        int64_t workspace_bytes,
-        uint8_t* jpeg,
-        int64_t capacity,
-        int64_t* length,
-        void* stream)
+    // the encoder over a workspace laid out by jpeg_layout (L), arguments checked
+    int encode(const uint8_t* rgbx, int32_t width, int32_t height, int32_t quality, uint8_t* ws, const JpegLayout& L,
+               uint8_t* jpeg, int64_t capacity, int64_t* length, hipStream_t s)
     {
-        if(!rgbx || !workspace || !jpeg || !length || width < 1 || height < 1 || width > 65535 || height > 65535)
-            return ISAAC_HIP_ERR_ARG;
-        const JpegLayout L = jpeg_layout(width, height);
-        if(workspace_bytes < int64_t(L.end) || (reinterpret_cast<uintptr_t>(rgbx) & 3u) || (reinterpret_cast<uintptr_t>(workspace) & 255u))
-            return ISAAC_HIP_ERR_ARG;
-        const hipStream_t s = (hipStream_t) stream;
-        uint8_t* ws = static_cast<uint8_t*>(workspace);
         int16_t* coef = reinterpret_cast<int16_t*>(ws + L.coef);
         uint32_t* bits = reinterpret_cast<uint32_t*>(ws + L.bits);
         uint64_t* offs = reinterpret_cast<uint64_t*>(ws + L.offs);
@@ -684,18 +681,110 @@ extern "C"
            || hipStreamSynchronize(s) != hipSuccess)
             return ISAAC_HIP_ERR_DEVICE;
         const int64_t coded = int64_t((tbits + 7) / 8 + nff);
-        const std::vector<uint8_t> hdr = jpeg_headers(width, height, qt);
-        *length = int64_t(hdr.size()) + coded + 2;
+        JpegHeaders hdr;
+        jpeg_headers(width, height, qt, hdr);
+        *length = int64_t(hdr.n) + coded + 2;
         if(capacity < *length)
             return ISAAC_HIP_ERR_ARG;
-        std::memcpy(jpeg, hdr.data(), hdr.size());
+        std::memcpy(jpeg, hdr.b, hdr.n);
         if(coded > 0
-           && (hipMemcpyAsync(jpeg + hdr.size(), out, size_t(coded), hipMemcpyDeviceToHost, s) != hipSuccess
+           && (hipMemcpyAsync(jpeg + hdr.n, out, size_t(coded), hipMemcpyDeviceToHost, s) != hipSuccess
                || hipStreamSynchronize(s) != hipSuccess))
             return ISAAC_HIP_ERR_DEVICE;
         jpeg[*length - 2] = 0xFF;
         jpeg[*length - 1] = 0xD9;
         return 0;
+    }
+
+    // The workspace of isaac_hip_jpeg_encode_noalloc: a device array that every GPU
+    // holds from the library's load to the end of the process.  Host code uses it only
+    // through the address hipGetSymbolAddress returns for the current device (the
+    // array's name in host code is a stand-in, not a device address).
+    static_assert(ISAAC_HIP_JPEG_NOALLOC_SCRATCH_BYTES % 256 == 0, "scratch size");
+    static __device__ __align__(256) uint8_t scratch[ISAAC_HIP_JPEG_NOALLOC_SCRATCH_BYTES];
+
+    constexpr int kMaxScratchDevices = 64;
+    std::mutex scratchLock;
+    uint8_t* scratchAddr[kMaxScratchDevices] = {};
+
+    // the scratch's address on the current device, looked up once per device
+    int scratch_on_current_device(uint8_t** out)
+    {
+        int dev = -1;
+        if(hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxScratchDevices)
+            return ISAAC_HIP_ERR_DEVICE;
+        std::lock_guard<std::mutex> guard(scratchLock);
+        if(!scratchAddr[dev])
+        {
+            void* p = nullptr;
+            if(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+                return ISAAC_HIP_ERR_DEVICE;
+            scratchAddr[dev] = static_cast<uint8_t*>(p);
+        }
+        *out = scratchAddr[dev];
+        return 0;
+    }
+} // namespace isaac_hip
+
+using namespace isaac_hip;
+
+extern "C"
+{
+    int64_t isaac_hip_jpeg_workspace_bytes(int32_t width, int32_t height)
+    {
+        if(width < 1 || height < 1 || width > 65535 || height > 65535)
+            return -1;
+        return int64_t(jpeg_layout(width, height).end);
+    }
+
+    int isaac_hip_jpeg_encode(
+        const uint8_t* rgbx,
+        int32_t width,
+        int32_t height,
+        int32_t quality,
+        void* workspace,
+        int64_t workspace_bytes,
+        uint8_t* jpeg,
+        int64_t capacity,
+        int64_t* length,
+        void* stream)
+    {
+        if(!rgbx || !workspace || !jpeg || !length || width < 1 || height < 1 || width > 65535 || height > 65535)
+            return ISAAC_HIP_ERR_ARG;
+        const JpegLayout L = jpeg_layout(width, height);
+        if(workspace_bytes < int64_t(L.end) || (reinterpret_cast<uintptr_t>(rgbx) & 3u) || (reinterpret_cast<uintptr_t>(workspace) & 255u))
+            return ISAAC_HIP_ERR_ARG;
+        return encode(rgbx, width, height, quality, static_cast<uint8_t*>(workspace), L, jpeg, capacity, length,
+                      (hipStream_t) stream);
+    }
+
+    int isaac_hip_jpeg_encode_noalloc(
+        const uint8_t* rgbx,
+        int32_t width,
+        int32_t height,
+        int32_t quality,
+        uint8_t* jpeg,
+        int64_t capacity,
+        int64_t* length,
+        void* stream)
+    {
+        if(!rgbx || !jpeg || !length || width < 1 || height < 1 || width > 65535 || height > 65535
+           || (reinterpret_cast<uintptr_t>(rgbx) & 3u))
+            return ISAAC_HIP_ERR_ARG;
+        const JpegLayout L = jpeg_layout(width, height);
+        if(L.end > size_t(ISAAC_HIP_JPEG_NOALLOC_SCRATCH_BYTES))
+            return ISAAC_HIP_ERR_ARG;
+        const hipStream_t s = (hipStream_t) stream;
+        int dev = -1, sdev = -1;
+        if(hipGetDevice(&dev) != hipSuccess)
+            return ISAAC_HIP_ERR_DEVICE;
+        if(s && (hipStreamGetDevice(s, &sdev) != hipSuccess || sdev != dev))
+            return ISAAC_HIP_ERR_ARG; // the scratch of another GPU than the stream's
+        uint8_t* ws = nullptr;
+        const int rc = scratch_on_current_device(&ws);
+        if(rc)
+            return rc;
+        return encode(rgbx, width, height, quality, ws, L, jpeg, capacity, length, s);
     }
 
     int isaac_hip_jpeg_frame_message(
