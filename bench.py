@@ -41,7 +41,11 @@ VALUE_BYTES = {0: 4, 1: 2, 2: 2}  # bytes per stored brick value
 def kernel_bytes(counts, spec, n_vox, n_particles, n_visible, form="combined"):
     """Algorithmic bytes per launch of each kernel (read + write), from the
     executed work counts of one frame."""
+    import numpy as np
+
     W, H = spec.framebuffer
+    attr_dim = spec.particles[0].feature_dim if spec.particles else 1
+    n_cells = int(np.prod(spec.particle_grid)) if spec.particles else 0
     npx = W * H
     src_bytes = sum(4 * v.feature_dim for v in spec.volumes)
     comps = sum(v.feature_dim for v in spec.volumes)
@@ -50,9 +54,12 @@ def kernel_bytes(counts, spec, n_vox, n_particles, n_visible, form="combined"):
         "ssao": npx * (49 * 4 + 4),
         # G-buffer read colour 4 + normal 12 + depth 4 + ao 4 + volume colour 16, colour write 4
         "shade": 44 * npx,
-        # pos 12 + attr 4 in; cell id 4 + record 16 written and read back; perm 4 written and read back;
-        # visible render record 16 written
-        "binning": n_particles * (12 + 4 + 2 * 4 + 2 * 16 + 2 * 4) + n_visible * 16,
+        # the visible-records build: every particle's attribute in; per visible particle its
+        # position 12 in, cell id, rank, slot index and permutation entry 4 each written and
+        # read back, its slot record 16 written and read back and its render record 16 written;
+        # per cell the count (fill 4, scan read 2 x 4), count and offset 8 written, offsets 8
+        # read and the record range 8 written
+        "binning": n_particles * 4 * attr_dim + n_visible * (12 + 4 * 2 * 4 + 2 * 16 + 16) + n_cells * (4 + 8 + 8 + 8 + 8),
     }
     particles = 8 * counts["cells_visited"] + 16 * counts["sphere_tests"]
     if form == "combined":
@@ -201,28 +208,39 @@ def measure(vis, poses, out, steps, warmup):
 
 
 def learned_ao(vis, poses, out, args):
-    """Train the occlusion network on the GPU (fresh frames of this scene at random
-    poses), gate it on held-out poses against the plain SSAO pass, and time the
-    frame with each occlusion pass in the combined and FP16-brick forms."""
+    """Train the occlusion network on the GPU from plain-SSAO triples of 16 scenes of
+    this workload's style and size with other seeds (fields, particles, clipping
+    planes) at random poses at the workload's frame size, crops x steps triples
+    generated on the fly; gate it on this scene's poses against the plain SSAO pass;
+    time the frame with each occlusion pass in the combined and FP16-brick forms."""
     import numpy as np
     import torch
 
     from isaac_amd import aonet, scenes
 
     vis.set_render_form(*FORMS["combined"])
+    vis.update_buffers()
     vis.set_ao_mode(vis.AO_PLAIN)
+    W, H = vis.framebuffer_size
     t0 = time.perf_counter()
-    vis.aonet_set_params(aonet.init_params(0))
-    losses = aonet.train(vis, steps=args.ao_train_steps, seed=1)
+    n = int(vis.global_size[0])
+    n_particles = int(vis.particle_sources[0].pos.shape[0]) if vis.particle_sources else 0
+    ts = aonet.TrainingSet(0, fb=(W, H), n=n, particles=max(1, n_particles))
+    ts.vis.aonet_set_params(aonet.init_params(0))
+    losses = aonet.train(ts.vis, steps=args.ao_train_steps, seed=1, crops=16, prepare=ts.prepare)
+    vis.aonet_set_params(ts.vis.aonet_get_params())
+    ts.close()
+    del ts
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     held_out = [scenes.DEFAULT_POSE, scenes.FILL_POSE, scenes.random_pose(np.random.default_rng(99))]
-    gate = aonet.gate(vis, held_out, floor_db=40.0)
+    gate = aonet.gate(vis, held_out, floor_db=40.0, min_gain_db=3.0)
     timing = {}
     for name in ("combined", "bricks-f32", "bricks-f16"):
         vis.set_render_form(*FORMS[name])
         for mode, label in ((vis.AO_PLAIN, "plain"), (vis.AO_LEARNED, "learned")):
-            if mode == vis.AO_LEARNED and (not gate["passed"] or name == "bricks-f32"):
+            # the learned pass is timed whether or not the gate kept it (the gate result is reported)
+            if mode == vis.AO_LEARNED and name == "bricks-f32":
                 continue
             vis.set_ao_mode(mode)
             w2, s2, us2, c2, _ = measure(vis, poses, out, args.form_steps, 2)
@@ -238,11 +256,17 @@ def learned_ao(vis, poses, out, args):
         "train_s": train_s,
         "train_mse_first20": float(np.mean(losses[:20])),
         "train_mse_last20": float(np.mean(losses[-20:])),
+        "training_triples": args.ao_train_steps * 16,
+        "training_set": f"16 seeded {n}^3 scenes of the workload's style and other seeds (Gaussian mixture + "
+                        f"value noise, iso 0.5, {n_particles} particles, random clipping plane) at random poses, "
+                        f"{W}x{H}, 16 crops of 128^2 per step",
         "gate_floor_db": 40.0,
+        "gate_min_gain_db": 3.0,
         "gate_passed": bool(gate["passed"]),
         "image_psnr_db": gate["image_psnr"],
         "occlusion_map_psnr_db": gate["ao_psnr"],
         "image_psnr_without_occlusion_db": gate["image_psnr_no_ao"],
+        "gain_over_no_occlusion_db": gate["gain_db"],
         "frames": timing,
     }
     if "bricks-f16/learned" in timing:
@@ -257,6 +281,44 @@ def learned_ao(vis, poses, out, args):
         tf = gf / timing["combined/learned"]["occlusion_ms"]
         res["network"] = {"gflop_per_frame": gf, "tflops": tf, "bound": "mfma", "peak_tflops": 2500.0,
                           "frac": tf / 2500.0}
+    return res
+
+
+def precision_workload(steps: int = 5):
+    """The precision form's own workload: the 512^3 Gaussian-mixture density at
+    3840x2160, trilinear, step 0.25, in the combined FP32 form and the FP32 / FP16 /
+    BF16 brick forms: frame and march times, and the march's algorithmic bytes per
+    launch against the HBM peak."""
+    import numpy as np
+    import torch
+
+    from isaac_amd import scenes
+    from isaac_amd import visualization as V
+
+    spec = scenes.precision_scene()
+    W, H = spec.framebuffer
+    n_vox = int(np.prod(spec.local_size))
+    vis = V.from_scene(spec, 0)
+    out = torch.empty((H, W, 4), dtype=torch.uint8, device="cuda")
+    res = {"workload": f"{spec.local_size[0]}^3 f32 Gaussian mixture, {W}x{H}, trilinear, step {spec.step}",
+           "forms": {}}
+    for name in ("combined", "bricks-f32", "bricks-f16", "bricks-bf16"):
+        f, d = FORMS[name]
+        vis.set_render_form(f, d)
+        wall, samples, us, counts, _ = measure(vis, [spec.camera, spec.camera], out, steps, 1)
+        kb = kernel_bytes(counts, spec, n_vox, 0, 0, name)
+        ach = kb["march"] / (us["march"] * 1e-6) / 1e9
+        res["forms"][name] = {
+            "ms_per_step": wall / steps * 1e3,
+            "samples_per_s": samples / wall,
+            "kernel_ms": {k: v / 1e3 for k, v in us.items()},
+            "march_steps_per_frame": counts["march_steps"],
+            "roofline": {"kernel": "march", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": kb["march"]},
+        }
+    vis.close()
+    del out
+    torch.cuda.synchronize()
     return res
 
 
@@ -324,12 +386,14 @@ def batched_workload(args):
     t_build = time.perf_counter() - t0
     projs, mvs = B.views_for(vises, args.batch_views, seed=5)
     nframes = len(vises) * args.batch_views
-    # occlusion network: trained on the largest sub-volume at random poses, shared by all frames
-    big = int(np.argmax(sides))
+    # occlusion network: trained at 960x540 on the same kind of triples as the 1080p one
     t1 = time.perf_counter()
-    vises[big].aonet_set_params(aonet.init_params(0))
-    losses = aonet.train(vises[big], steps=args.ao_train_steps, seed=2)
-    params = vises[big].aonet_get_params()
+    ts = aonet.TrainingSet(0, fb=(960, 540), n=128, particles=250_000)
+    ts.vis.aonet_set_params(aonet.init_params(0))
+    losses = aonet.train(ts.vis, steps=args.ao_train_steps, seed=2, crops=16, prepare=ts.prepare)
+    params = ts.vis.aonet_get_params()
+    ts.close()
+    del ts
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t1
     vises[0].aonet_set_params(params)
@@ -363,6 +427,38 @@ def batched_workload(args):
     res["frames_per_s"] = nframes / (times["graph"] * 1e-3)
     res["rays_per_s"] = res["rays"] / (times["graph"] * 1e-3)
     res["speedup_vs_per_frame_launches"] = times["per_frame"] / times["graph"]
+    # the march's work over all frames (each frame once through the counting per-context
+    # render), and the batched sequence without the occlusion pass (march + shading)
+    tot = {"march_steps": 0, "iso_hits": 0, "cells_visited": 0, "sphere_tests": 0}
+    for f in range(nframes):
+        v = vises[f // args.batch_views]
+        v.set_counting(True)
+        v.draw(out=False, projection=projs[f], modelview_=mvs[f])
+        c = v.counters()
+        v.set_counting(False)
+        for k in tot:
+            tot[k] += int(c[k])
+    for v in vises:
+        v.ambient_occlusion = {"isEnabled": False, "weight": 0.5}
+        v.update_buffers()
+    batch.set_views(projs, mvs, bg=bg)
+    batch.render(B.GRAPH)
+    batch.synchronize()
+    ts = []
+    for _ in range(args.batch_repeats):
+        batch.render(B.GRAPH)
+        ts.append(batch.last_time_us())
+    t_noao = float(np.median(ts)) * 1e-6
+    npx = nframes * 960 * 540
+    per = 4 * (8 * 2 + 4)  # 4 components (density + float3 field), 8 FP16 taps + the brick scale each
+    march_b = per * tot["march_steps"] + 7 * per * tot["iso_hits"] + 8 * tot["cells_visited"] + 16 * tot["sphere_tests"] \
+        + 36 * npx
+    seq_b = march_b + 44 * npx
+    ach = seq_b / t_noao / 1e9
+    res["march_steps"] = tot["march_steps"]
+    res["roofline"] = {"kernel": "batched march + shading (occlusion off), one graph replay", "bound": "hbm",
+                       "ms": t_noao * 1e3, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes": seq_b}
     batch.close()
     for v in vises:
         v.close()
@@ -385,11 +481,13 @@ def main() -> int:
     ap.add_argument("--march-waves", type=int, default=0, help="march occupancy target (0, 6, 8)")
     ap.add_argument("--no-tune", action="store_true", help="keep --prefetch / --march-waves instead of autotuning")
     ap.add_argument("--no-learned-ao", action="store_true", help="skip training / timing the learned occlusion pass")
-    ap.add_argument("--ao-train-steps", type=int, default=600)
+    ap.add_argument("--ao-train-steps", type=int, default=1250, help="16 crops each: 20,000 triples")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sub-volumes x views workload")
     ap.add_argument("--batch-count", type=int, default=48)
     ap.add_argument("--batch-views", type=int, default=8)
     ap.add_argument("--batch-repeats", type=int, default=5)
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the 512^3, 3840x2160, step 0.25 precision-form workload")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain device copy of this size first (scales profiler byte counters)")
     args = ap.parse_args()
@@ -519,6 +617,8 @@ def main() -> int:
     vis.close()
     if not args.no_batch:
         line["batched"] = batched_workload(args)
+    if not args.no_precision:
+        line["precision"] = precision_workload()
     line["compositing"] = compositing()
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or host_cpus()["nproc"]

@@ -191,10 +191,16 @@ extern "C"
         float nz);
     int isaac_hip_remove_clipping(isaac_hip_ctx* ctx, uint32_t nr);
     int isaac_hip_clipping_count(const isaac_hip_ctx* ctx);
+    /* plane nr as set: its position and its normal before scaling and normalisation
+     * (clipping.elem[nr].position and clippingSavedNormals[nr], the values the
+     * periodic message reports, lib/isaac.hpp:2799-2816) */
+    int isaac_hip_get_clipping(const isaac_hip_ctx* ctx, uint32_t nr, float position[3], float normal[3]);
     /* updatePosition / updateLocalSize / updateLocalParticleSize (lib/isaac.hpp:1056-1076) */
     int isaac_hip_update_position(isaac_hip_ctx* ctx, const int32_t position[3]);
     int isaac_hip_update_local_size(isaac_hip_ctx* ctx, const int64_t local_size[3]);
     int isaac_hip_update_local_particle_size(isaac_hip_ctx* ctx, const int64_t local_particle_size[3]);
+    /* globalSizeScaled (lib/isaac.hpp:2181-2202), the size the register message reports */
+    int isaac_hip_global_size_scaled(const isaac_hip_ctx* ctx, int32_t out[3]);
 
     /* ---------------------------------------------------------------- per frame
      * doVisualization's buffer section (lib/isaac.hpp:1805-1948): non-persistent
@@ -239,9 +245,14 @@ extern "C"
     {
         ISAAC_HIP_TUNE_MARCH_PREFETCH = 0, /* 1: fetch step i+1 before compositing step i (default 0) */
         ISAAC_HIP_TUNE_MARCH_WAVES = 1, /* march occupancy target: 0 (compiler default), 6 or 8 waves per SIMD */
-        ISAAC_HIP_TUNE_MARCH_TILES = 2 /* march tile order: 0 one band of tile rows per XCD; 2, 4 (default), 8 or
+        ISAAC_HIP_TUNE_MARCH_TILES = 2, /* march tile order: 0 one band of tile rows per XCD; 2, 4 (default), 8 or
                                           16: super-tiles of that many 16x16-pixel tiles a side, dealt
                                           round-robin to the XCDs */
+        ISAAC_HIP_TUNE_TEX_LAYOUT = 3 /* texture the combined-form march reads: 0 (default) the linear texture
+                                         (x fastest); 8: a copy in 8^3 bricks with a one-texel apron, built by
+                                         update_buffers after classification (the linear texture stays what
+                                         isaac_hip_get_texture returns); a change needs update_buffers before
+                                         the next render; next-step prefetch applies to the linear layout only */
     };
     int isaac_hip_set_tuning(isaac_hip_ctx* ctx, int32_t key, int32_t value);
 
@@ -477,9 +488,13 @@ extern "C"
         int32_t* time_step);
     /* device pointers of a particle source's exact cell list: per-cell count
      * (int32) and exclusive offset (int32, ncells+1), and the stable
-     * permutation of particle indices (int32) */
+     * permutation of particle indices (int32).  update_buffers builds only what
+     * the frame reads (each cell's records of the particles whose transfer-function
+     * alpha is >= 0.5, in ascending index); the first call after it builds the full
+     * list on the context stream from the particle buffers of that update, which
+     * must still hold the same data.  ISAAC_HIP_ERR_STATE before any update. */
     int isaac_hip_get_cell_list(
-        const isaac_hip_ctx* ctx,
+        isaac_hip_ctx* ctx,
         int particle_idx,
         const void** counts,
         const void** offsets,

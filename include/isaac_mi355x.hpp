@@ -227,6 +227,7 @@ namespace isaac
 
         ~IsaacVisualization()
         {
+            isaac_hip_session_destroy(session);
             isaac_hip_destroy(ctx);
         }
 
@@ -376,6 +377,43 @@ namespace isaac
             return std::string(message.data(), size_t(n));
         }
 
+        // The master rank's messages to the ISAAC server, as text (jansson's
+        // json_dumps(root, 0)): the register message init() sends
+        // (lib/isaac.hpp:834-892, 1276-1292), the periodic message of one step with
+        // the sections in `send` (ISAAC_HIP_SEND_* flags, the reference's send*
+        // members; lib/isaac.hpp:2546-2838), numbered by metaNr, which it advances,
+        // and the exit message of the destructor (lib/isaac.hpp:2110-2118).
+        // `metadata` is the JSON text of the metadata object (getJsonMetaRoot).
+        std::string registerMessage(int nodes = 1)
+        {
+            if(!session)
+                check(isaac_hip_session_create(&session) == 0 ? 0 : -1, "session_create");
+            MessageState st(*this);
+            const int32_t fb[2] = {int32_t(framebufferSize.x), int32_t(framebufferSize.y)};
+            int32_t gs[3] = {0, 0, 0};
+            check(isaac_hip_global_size_scaled(ctx, gs), "global_size_scaled");
+            return messageText([&](char* m, int64_t cap, int64_t* n)
+                               { return isaac_hip_register_message(session, &st.s, name.c_str(), nodes, fb, gs, m, cap, n); });
+        }
+
+        std::string periodMessage(uint32_t send)
+        {
+            if(!session)
+                throw IsaacError("periodMessage: registerMessage first (init)");
+            MessageState st(*this);
+            const int64_t nr = metaNr++;
+            return messageText([&](char* m, int64_t cap, int64_t* n)
+                               { return isaac_hip_period_message(session, &st.s, nr, send, m, cap, n); });
+        }
+
+        static std::string exitMessage()
+        {
+            return messageText([](char* m, int64_t cap, int64_t* n) { return isaac_hip_exit_message(m, cap, n); });
+        }
+
+        std::string metadata = "{}";
+        int64_t metaNr = 0;
+
         isaac_hip_ctx* handle() const
         {
             return ctx;
@@ -417,6 +455,124 @@ namespace isaac
             if(rc < 0)
                 throw IsaacError(std::string(what) + ": " + isaac_hip_last_error(ctx));
         }
+
+        template<typename F>
+        static std::string messageText(F&& write)
+        {
+            std::vector<char> buf(1 << 16);
+            int64_t n = 0;
+            int rc = write(buf.data(), int64_t(buf.size()), &n);
+            if(rc == ISAAC_HIP_ERR_ARG && n + 1 > int64_t(buf.size()))
+            {
+                buf.resize(size_t(n) + 1);
+                rc = write(buf.data(), int64_t(buf.size()), &n);
+            }
+            if(rc != 0)
+                throw IsaacError("server message: error " + std::to_string(rc));
+            return std::string(buf.data(), size_t(n));
+        }
+
+        // the visualization state the messages report, in the C-ABI's form
+        struct MessageState
+        {
+            isaac_hip_steering_state s{};
+            std::vector<std::string> names, funcs;
+            std::vector<const char*> namePtr, funcPtr;
+            std::vector<int32_t> dims, errs, counts;
+            std::vector<uint32_t> values;
+            std::vector<float> lut, rgba, minMax, clipPos, clipNormal;
+
+            template<typename T_Tuple, std::size_t... I>
+            void collect(const T_Tuple& t, std::index_sequence<I...>)
+            {
+                (names.push_back(std::tuple_element_t<I, T_Tuple>::getName()), ...);
+                (dims.push_back(int32_t(std::tuple_element_t<I, T_Tuple>::featureDim)), ...);
+                (void) t;
+            }
+
+            explicit MessageState(const IsaacVisualization& v)
+            {
+                collect(v.volumeSources, std::make_index_sequence<vSourceListSize>{});
+                collect(v.fieldSources, std::make_index_sequence<fSourceListSize>{});
+                collect(v.particleSources, std::make_index_sequence<pSourceListSize>{});
+                const int n = combinedSourceListSize;
+                lut.resize(size_t(std::max(n, 1)) * T_transferSize * 4);
+                for(int i = 0; i < n; i++)
+                {
+                    funcs.push_back(v.functions[i].source);
+                    errs.push_back(v.functions[i].errorCode);
+                    std::vector<uint32_t> pos;
+                    std::vector<float> col;
+                    for(const auto& kv : v.transferPoints[i])
+                    {
+                        pos.push_back(kv.first);
+                        col.insert(col.end(), {kv.second.x, kv.second.y, kv.second.z, kv.second.w});
+                    }
+                    isaac_hip_build_transfer_lut(T_transferSize, int32_t(pos.size()), pos.data(), col.data(),
+                                                 lut.data() + size_t(i) * T_transferSize * 4);
+                    counts.push_back(int32_t(pos.size()));
+                    values.insert(values.end(), pos.begin(), pos.end());
+                    rgba.insert(rgba.end(), col.begin(), col.end());
+                    minMax.push_back(v.minMaxArray.min[i]);
+                    minMax.push_back(v.minMaxArray.max[i]);
+                }
+                for(const std::string& x : names)
+                    namePtr.push_back(x.c_str());
+                for(const std::string& x : funcs)
+                    funcPtr.push_back(x.c_str());
+                const int nclip = isaac_hip_clipping_count(v.ctx);
+                for(int i = 0; i < nclip; i++)
+                {
+                    float p[3], q[3];
+                    isaac_hip_get_clipping(v.ctx, uint32_t(i), p, q);
+                    clipPos.insert(clipPos.end(), p, p + 3);
+                    clipNormal.insert(clipNormal.end(), q, q + 3);
+                }
+                s.n_sources = n;
+                s.n_volume_field = volumeFieldSourceListSize;
+                s.source_names = namePtr.data();
+                s.feature_dims = dims.data();
+                s.functions = funcPtr.data();
+                s.function_errors = errs.data();
+                s.transfer_size = T_transferSize;
+                s.transfer_lut = lut.data();
+                s.transfer_point_counts = counts.data();
+                s.transfer_point_values = values.data();
+                s.transfer_point_rgba = rgba.data();
+                s.weights = v.sourceWeight;
+                s.iso_thresholds = v.isoThreshold;
+                s.min_max = minMax.data();
+                s.step = v.step;
+                s.interpolation = v.interpolation ? 1 : 0;
+                s.render_optimization = 1;
+                s.seed_points = v.seedPoints;
+                s.advection_step = v.advectionStepFactor;
+                s.advection_weight = v.advectionHistoryWeight;
+                s.advection_seeding_period = v.advectionSeedingPeriod;
+                s.advection_seeding_duration = v.advectionSeedingTime;
+                s.advection_on_pause = v.advectionOnPause ? 1 : 0;
+                s.advection_border = 0;
+                for(int k = 0; k < 3; k++)
+                    s.background[k] = v.backgroundColor[k];
+                s.n_clipping = nclip;
+                s.clipping_position = clipPos.data();
+                s.clipping_normal = clipNormal.data();
+                s.ao_enabled = v.ambientOcclusion.isEnabled ? 1 : 0;
+                s.ao_weight = v.ambientOcclusion.weight;
+                s.render_mode = v.renderMode;
+                s.dither_mode = v.ditherMode;
+                for(int k = 0; k < 16; k++)
+                    s.projection[k] = v.projection[k];
+                for(int k = 0; k < 3; k++)
+                    s.look_at[k] = v.lookAt[size_t(k)];
+                for(int k = 0; k < 9; k++)
+                    s.rotation[k] = v.rotation[size_t(k)];
+                s.distance = v.distance;
+                s.metadata = v.metadata.c_str();
+            }
+        };
+
+        isaac_hip_session* session = nullptr;
 
         void updateModelview()
         {

@@ -163,6 +163,8 @@ _SIGS = {
     "isaac_hip_default_transfer": (ctypes.c_int, [I32, I32, I32, I32, P, P]),
     "isaac_hip_version": (ctypes.c_char_p, []),
     "isaac_hip_copy_to_host": (ctypes.c_int, [P, P, P, ctypes.c_size_t]),
+    "isaac_hip_get_clipping": (ctypes.c_int, [P, ctypes.c_uint32, P, P]),
+    "isaac_hip_global_size_scaled": (ctypes.c_int, [P, P]),
     "isaac_hip_session_create": (ctypes.c_int, [P]),
     "isaac_hip_session_destroy": (None, [P]),
     "isaac_hip_register_message": (ctypes.c_int, [P, P, ctypes.c_char_p, I32, P, P, P, ctypes.c_int64, P]),

@@ -19,6 +19,9 @@ namespace isaac_hip
     constexpr int kMaxFunctors = 3;
     constexpr int kMaxClipping = 16;
     constexpr float kZNear = 0.002f; // ISAAC_Z_NEAR lib/isaac/isaac_defines.hpp:53
+    constexpr int kTexBrick = 8; // side of a texture brick (texture layout 8)
+    constexpr int kTexBrickRow = kTexBrick + 1, kTexBrickSlice = kTexBrickRow * kTexBrickRow;
+    constexpr int kTexBrickTexels = kTexBrickSlice * kTexBrickRow; // 729
 
     enum Op : int32_t
     {
@@ -45,6 +48,7 @@ namespace isaac_hip
         int32_t active; // weight + iso > 0
         float weight;
         float iso;
+        float invIso; // RN(1 / iso) (0 when iso is 0)
         ChainDesc chain;
         int32_t lutSlot; // index of the LUT in the staged table
         const uint8_t* adv; // field source: advection texture (alpha *= adv / 255), else nullptr
@@ -55,6 +59,7 @@ namespace isaac_hip
         int32_t X, Y, Z;
         int32_t nsrc;
         float totalWeight;
+        float invTotalWeight; // RN(1 / totalWeight) (0 when totalWeight is 0)
         int32_t ditherMode;
         int32_t T;
         int32_t nLut; // LUTs staged in LDS (0 = read from global)
@@ -90,6 +95,11 @@ namespace isaac_hip
         // textures
         int32_t X, Y, Z;
         const uint2* tex; // interleaved combined textures, x fastest
+        // the same texels in 8^3 bricks with a one-texel apron on the + side (9^3 texels per
+        // brick, x fastest inside, bricks x fastest; apron and partial bricks hold the
+        // clamped texel), or nullptr: every trilinear sample reads one brick
+        const uint2* texB;
+        uint32_t nbx, nbxy; // bricks per row and per slice of the brick layout
         int32_t isoEnabled;
         int32_t volumeEnabled;
         float minSize;

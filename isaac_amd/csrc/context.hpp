@@ -37,7 +37,12 @@ namespace isaac_hip
         bool set = false;
         int64_t capN = 0, capCells = 0;
         BinBuffers B{};
-        void* blocks[12] = {};
+        void* blocks[14] = {};
+        // the frame builds only the visible particles' records; the full cell list of
+        // all particles is built on request (isaac_hip_get_cell_list) with the inputs
+        // of the last update_buffers
+        BinParams last{};
+        bool lastValid = false, fullValid = false;
     };
 } // namespace isaac_hip
 
@@ -87,6 +92,8 @@ struct isaac_hip_ctx
     // device buffers
     uint2* tex = nullptr; // per voxel: combined volume RGBA8, combined iso RGBA8
     size_t texVox = 0;
+    uint2* texBricks = nullptr; // the same texels in 8^3 bricks with a +1 apron (texture layout 8)
+    size_t texBrickCount = 0;
     uint32_t* color = nullptr;
     float* depth = nullptr;
     float* normal = nullptr;
@@ -98,6 +105,8 @@ struct isaac_hip_ctx
     int tunePrefetch = 0;
     int tuneWaves = 0;
     int tuneTiles = 4; // super-tiles of 4x4 tiles (measured fastest at 1080p)
+    int tuneTexLayout = 0; // march texture: 0 linear (x fastest), 8 bricks of 8^3 texels
+    int builtTexLayout = -1; // layout whose texels the last combined-form update_buffers wrote
     int form = ISAAC_HIP_FORM_COMBINED;
     int brickDtype = ISAAC_HIP_F32;
     int builtForm = -1; // form prepared by the last update_buffers

@@ -448,10 +448,10 @@ namespace
             return 0;
         free_part(p);
         c->generation++;
-        const size_t sizes[11] = {size_t(n) * 4, size_t(n) * 4, size_t(n) * 16, size_t(nc) * 8, size_t(nc) * 4,
+        const size_t sizes[12] = {size_t(n) * 4, size_t(n) * 4, size_t(n) * 16, size_t(nc) * 8, size_t(nc) * 4,
                                   size_t(nc + 1) * 4, size_t(nc) * 8, size_t(n) * 4, size_t(n) * 16,
-                                  needRad ? size_t(n) * 4 : 0, bin_scan_workspace_elems(nc) * 8};
-        for(int i = 0; i < 11; i++)
+                                  needRad ? size_t(n) * 4 : 0, bin_scan_workspace_elems(nc) * 8, size_t((n + 63) / 64) * 8};
+        for(int i = 0; i < 12; i++)
             if(sizes[i])
                 HIPCHK(c, hipMalloc(&p.blocks[i], sizes[i]));
         p.B.cellOf = (int32_t*) p.blocks[0];
@@ -466,6 +466,7 @@ namespace
         p.B.recs = (float4*) p.blocks[8];
         p.B.recRadius = (float*) p.blocks[9];
         p.B.scanTmp = (unsigned long long*) p.blocks[10];
+        p.B.visMask = (unsigned long long*) p.blocks[11];
         p.capN = n;
         p.capCells = nc;
         return 0;
@@ -576,6 +577,12 @@ namespace isaac_hip
         F.Y = int32_t(c->local[1]);
         F.Z = int32_t(c->local[2]);
         F.tex = c->tex;
+        if(c->builtTexLayout == kTexBrick && c->texBricks)
+        {
+            F.texB = c->texBricks;
+            F.nbx = uint32_t((c->local[0] + kTexBrick - 1) / kTexBrick);
+            F.nbxy = F.nbx * uint32_t((c->local[1] + kTexBrick - 1) / kTexBrick);
+        }
         const int nvf = c->nv + c->nf;
         F.isoEnabled = 0;
         for(int s = 0; s < nvf; s++)
@@ -889,8 +896,9 @@ extern "C"
         for(auto& p : c->parts)
             free_part(p);
         aonet_destroy(c);
-        void* bufs[] = {c->lutDev, c->tex,      c->color,      c->depth,   c->normal,  c->ao,
-                        c->volColor, c->counters, c->outFloat, c->mmPartials, c->mmResult, c->noise, c->noiseTmp};
+        void* bufs[] = {c->lutDev,   c->tex,        c->texBricks, c->color,    c->depth,
+                        c->normal,   c->ao,         c->volColor,  c->counters, c->outFloat,
+                        c->mmPartials, c->mmResult, c->noise,     c->noiseTmp};
         for(void* b : bufs)
             if(b)
                 (void) hipFree(b);
@@ -1170,6 +1178,18 @@ extern "C"
         return c ? c->nclip : ISAAC_HIP_ERR_ARG;
     }
 
+    int isaac_hip_get_clipping(const isaac_hip_ctx* c, uint32_t nr, float position[3], float normal[3])
+    {
+        if(!c || !position || !normal || nr >= uint32_t(c->nclip))
+            return ISAAC_HIP_ERR_ARG;
+        for(int k = 0; k < 3; k++)
+        {
+            position[k] = c->clipPos[nr][k];
+            normal[k] = c->clipSaved[nr][k];
+        }
+        return 0;
+    }
+
     int isaac_hip_update_position(isaac_hip_ctx* c, const int32_t position[3])
     {
         if(!c || !position)
@@ -1202,6 +1222,15 @@ extern "C"
                 return fail(c, ISAAC_HIP_ERR_ARG, "bad particle grid");
         for(int i = 0; i < 3; i++)
             set_tracked(c, c->pgrid[i], ps[i]);
+        return 0;
+    }
+
+    int isaac_hip_global_size_scaled(const isaac_hip_ctx* c, int32_t out[3])
+    {
+        if(!c || !out)
+            return ISAAC_HIP_ERR_ARG;
+        for(int i = 0; i < 3; i++)
+            out[i] = int32_t(c->gsScaled[i]);
         return 0;
     }
 
@@ -1311,7 +1340,10 @@ extern "C"
             P.lut = c->lutDev + size_t(gi) * T;
             P.T = T;
             P.filter = 1;
-            HIPCHK(c, launch_bin(P, p.B, c->stream));
+            HIPCHK(c, launch_bin_visible(P, p.B, c->stream));
+            p.last = P;
+            p.lastValid = true;
+            p.fullValid = false;
         }
         HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
         if(c->form == ISAAC_HIP_FORM_BRICKS)
@@ -1375,6 +1407,7 @@ extern "C"
         for(int s = 0; s < nvf; s++)
             tw += c->weight[s];
         P.totalWeight = tw;
+        P.invTotalWeight = tw != 0.0f ? 1.0f / tw : 0.0f;
         P.ditherMode = c->ditherMode;
         P.T = T;
         make_fastdiv(uint32_t(P.X), P.divX_mul, P.divX_shift);
@@ -1389,6 +1422,7 @@ extern "C"
             const VolSlot& v = c->vols[s];
             S.weight = c->weight[s];
             S.iso = c->iso[s];
+            S.invIso = S.iso > 0.0f ? 1.0f / S.iso : 0.0f;
             S.active = (S.weight + S.iso > 0.0f) ? 1 : 0;
             S.lutSlot = s;
             if(!v.set)
@@ -1422,10 +1456,28 @@ extern "C"
             S.adv = (s >= c->nv && S.active) ? v.adv[v.advCur] : nullptr;
         }
         HIPCHK(c, launch_classify(P, c->stream));
+        if(c->tuneTexLayout == kTexBrick)
+        {
+            const size_t nb = size_t((c->local[0] + kTexBrick - 1) / kTexBrick) * size_t((c->local[1] + kTexBrick - 1) / kTexBrick)
+                * size_t((c->local[2] + kTexBrick - 1) / kTexBrick);
+            if(nb != c->texBrickCount)
+            {
+                if(c->texBricks)
+                    HIPCHK(c, hipFree(c->texBricks));
+                c->texBricks = nullptr;
+                c->texBrickCount = 0;
+                // one padding texel: the last brick's last row is read 16 bytes at a time too
+                HIPCHK(c, hipMalloc(&c->texBricks, (nb * kTexBrickTexels + 1) * sizeof(uint2)));
+                HIPCHK(c, hipMemsetAsync(c->texBricks + nb * kTexBrickTexels, 0, sizeof(uint2), c->stream));
+                c->texBrickCount = nb;
+            }
+            HIPCHK(c, launch_tex_bricks(c->tex, c->texBricks, int(c->local[0]), int(c->local[1]), int(c->local[2]), c->stream));
+        }
         HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
         c->evRecorded[0] = true;
         c->classified = true;
         c->builtForm = ISAAC_HIP_FORM_COMBINED;
+        c->builtTexLayout = c->tuneTexLayout;
         return 0;
     }
 
@@ -1445,6 +1497,8 @@ extern "C"
             return fail(c, ISAAC_HIP_ERR_STATE, "isaac_hip_update_buffers must run before the first render");
         if(c->builtForm != c->form)
             return fail(c, ISAAC_HIP_ERR_STATE, "render form changed; call isaac_hip_update_buffers");
+        if(c->form == ISAAC_HIP_FORM_COMBINED && c->builtTexLayout != c->tuneTexLayout)
+            return fail(c, ISAAC_HIP_ERR_STATE, "texture layout changed; call isaac_hip_update_buffers");
         const int W = c->cfg.framebuffer_size[0], H = c->cfg.framebuffer_size[1];
         FrameTargets tg{W, H, c->color, c->depth, c->normal, c->ao, c->volColor, c->floatOut ? c->outFloat : nullptr, c->counters};
         FrameParams F;
@@ -1585,6 +1639,11 @@ extern "C"
             if(value != 0 && value != 2 && value != 4 && value != 8 && value != 16)
                 return fail(c, ISAAC_HIP_ERR_ARG, "march tile order: 0 (XCD bands) or a super-tile side of 2, 4, 8 or 16 tiles");
             c->tuneTiles = value;
+            return 0;
+        case ISAAC_HIP_TUNE_TEX_LAYOUT:
+            if(value != 0 && value != kTexBrick)
+                return fail(c, ISAAC_HIP_ERR_ARG, "texture layout: 0 (linear) or 8 (8^3 bricks)");
+            c->tuneTexLayout = value;
             return 0;
         default:
             return fail(c, ISAAC_HIP_ERR_ARG, "unknown tuning key");
@@ -1778,11 +1837,21 @@ extern "C"
         return 0;
     }
 
-    int isaac_hip_get_cell_list(const isaac_hip_ctx* c, int idx, const void** counts, const void** offsets, const void** perm)
+    int isaac_hip_get_cell_list(isaac_hip_ctx* c, int idx, const void** counts, const void** offsets, const void** perm)
     {
         if(!c || idx < 0 || idx >= c->np)
             return ISAAC_HIP_ERR_ARG;
-        const PartSlot& p = c->parts[idx];
+        PartSlot& p = c->parts[idx];
+        if(!p.lastValid)
+            return fail(c, ISAAC_HIP_ERR_STATE, "no cell list: the particle source was not binned by update_buffers");
+        if(!p.fullValid)
+        {
+            // the full list of all particles from the last update's inputs; its render
+            // records equal the visible build's
+            HIPCHK(c, hipSetDevice(c->device));
+            HIPCHK(c, launch_bin(p.last, p.B, c->stream));
+            p.fullValid = true;
+        }
         if(counts)
             *counts = p.B.count;
         if(offsets)

@@ -20,6 +20,7 @@ namespace isaac_hip
     // the fused march (clear, particles, iso + volume), the SSAO pass and shading
     // + volume blend; the host records its per-kernel events between them
     hipError_t launch_march(const FrameParams& F, const ParticleParams& PP, bool counting, hipStream_t s);
+    hipError_t launch_tex_bricks(const uint2* tex, uint2* out, int X, int Y, int Z, hipStream_t s);
     hipError_t launch_ssao(const FrameParams& F, hipStream_t s);
     hipError_t launch_shade(const FrameParams& F, hipStream_t s);
 
@@ -76,6 +77,7 @@ namespace isaac_hip
         float4* recs; // n: each cell's visible records from its offset on, ascending particle index
         float* recRadius; // per record radius or nullptr
         unsigned long long* scanTmp; // scan workspace (bin_scan_workspace_elems entries)
+        unsigned long long* visMask; // ceil(n / 64): visible-records build, bit i % 64 of word i / 64: particle i is visible
     };
 
     // ---- field sources: advection noise (updateNoiseTexture, lib/isaac.hpp:499-562) and
@@ -167,5 +169,8 @@ namespace isaac_hip
 
     size_t bin_scan_workspace_elems(int64_t ncells);
     hipError_t launch_bin(const BinParams& P, const BinBuffers& B, hipStream_t s);
+    // only the render records of the visible particles (count, offset, vrange, recs per
+    // cell over visible particles); count, offset and perm then do not hold the full list
+    hipError_t launch_bin_visible(const BinParams& P, const BinBuffers& B, hipStream_t s);
     hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s);
 } // namespace isaac_hip

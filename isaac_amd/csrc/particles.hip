@@ -3,7 +3,9 @@
 // The reference asks the application for a per-cell particle iterator
 // (getIterator(cell), example/example.cpp:148-228) and walks it in
 // ParticleRenderKernel (lib/isaac/isaac_particle_kernel.hpp:32-114).  This
-// library builds those lists itself from raw particles:
+// library builds those lists itself from raw particles.  Every frame builds only
+// the render records of the visible particles (vis_bin, scan, vis_scatter,
+// vis_sort below); the full cell list of all particles, on request:
 //
 //   1. bin_count   per particle: cell = clamp(floor(p / cw)), rel = p / cw - cell,
 //                  TF index of the functor-applied attribute; ONE returning 32-bit
@@ -292,6 +294,93 @@ namespace isaac_hip
         }
     }
 
+    // ---- the visible-records build: the render needs, per cell, only the records of
+    // the particles whose transfer-function alpha is >= 0.5, in ascending particle
+    // index (the reference's iteration order decides ties).  Invisible particles cost
+    // one attribute read and one bit.
+    //
+    // step 1, per particle: the TF index of its functor-applied attribute; each wave
+    // stores its 64 visibility bits (one word); a visible particle counts itself in its
+    // cell (a returning add: its arrival rank) and keeps cell, rank and record at its index
+    __global__ __launch_bounds__(256) void vis_bin_kernel(const BinParams P, const BinBuffers B)
+    {
+        uint32_t* cnt = reinterpret_cast<uint32_t*>(B.count64);
+        const uint32_t lane = threadIdx.x & 63u;
+        for(int64_t base = int64_t(blockIdx.x) * blockDim.x; base < P.n; base += int64_t(gridDim.x) * blockDim.x)
+        {
+            const int64_t i = base + threadIdx.x;
+            bool visible = false;
+            int li = 0;
+            if(i < P.n)
+            {
+                const float v = apply_chain_p(P.chain, P.attr + i * P.dim, P.dim);
+                const float lv = roundf(v * float(P.T));
+                li = (lv != lv) ? 0 : (int) gclamp(lv, 0.0f, float(P.T - 1));
+                visible = !P.filter || P.lut[li].w >= 0.5f;
+            }
+            const unsigned long long mask = __ballot(visible);
+            if(lane == 0u && i < P.n)
+                B.visMask[i >> 6] = mask;
+            if(visible)
+            {
+                float qx, qy, qz;
+                const int cx = cell_coord(P.pos[3 * i + 0], P.cw[0], P.grid[0], qx);
+                const int cy = cell_coord(P.pos[3 * i + 1], P.cw[1], P.grid[1], qy);
+                const int cz = cell_coord(P.pos[3 * i + 2], P.cw[2], P.grid[2], qz);
+                const int c = cx + P.grid[0] * (cy + P.grid[1] * cz);
+                B.cellOf[i] = c;
+                B.rank[i] = atomicAdd(cnt + c, 1u);
+                B.rel[i] = make_float4(qx - float(cx), qy - float(cy), qz - float(cz), __int_as_float(li));
+            }
+        }
+    }
+
+    // step 3 (after the scan of the visible counts): each visible particle's index at
+    // its cell's offset plus its rank
+    __global__ __launch_bounds__(256) void vis_scatter_kernel(const BinParams P, const BinBuffers B)
+    {
+        for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < P.n; i += int64_t(gridDim.x) * blockDim.x)
+            if((B.visMask[i >> 6] >> (i & 63)) & 1ull)
+                B.perm[B.offset[B.cellOf[i]] + int32_t(B.rank[i])] = int32_t(i);
+    }
+
+    // step 4, per cell: its visible particles in ascending index, then their records
+    // and the cell's record range
+    __global__ __launch_bounds__(256) void vis_sort_kernel(const BinParams P, const BinBuffers B, int64_t ncells)
+    {
+        for(int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; c < ncells; c += int64_t(gridDim.x) * blockDim.x)
+        {
+            const int k0 = B.offset[c], k1 = B.offset[c + 1];
+            int32_t* a = B.perm + k0;
+            const int len = k1 - k0;
+            // insertion sort, with Shell sort's Ciura gaps first for crowded cells
+            const int gaps[8] = {701, 301, 132, 57, 23, 10, 4, 1};
+            for(int g = len > 64 ? 0 : 7; g < 8; g++)
+            {
+                const int gap = gaps[g];
+                for(int i = gap; i < len; i++)
+                {
+                    const int32_t key = a[i];
+                    int j = i;
+                    while(j >= gap && a[j - gap] > key)
+                    {
+                        a[j] = a[j - gap];
+                        j -= gap;
+                    }
+                    a[j] = key;
+                }
+            }
+            for(int k = 0; k < len; k++)
+            {
+                const int32_t i = a[k];
+                B.recs[k0 + k] = B.rel[i];
+                if(B.recRadius)
+                    B.recRadius[k0 + k] = P.radius[i];
+            }
+            B.vrange[c] = make_int2(k0, k1);
+        }
+    }
+
     __global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* p, uint32_t value, size_t n)
     {
         for(size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
@@ -336,6 +425,31 @@ namespace isaac_hip
         if(cb > 8192)
             cb = 8192;
         hipLaunchKernelGGL(bin_sort_kernel, dim3(unsigned(cb)), dim3(256), 0, s, P, B, ncells);
+        return hipGetLastError();
+    }
+
+    hipError_t launch_bin_visible(const BinParams& P, const BinBuffers& B, hipStream_t s)
+    {
+        const int64_t ncells = int64_t(P.grid[0]) * P.grid[1] * P.grid[2];
+        hipError_t e;
+        if((e = launch_fill_u32(reinterpret_cast<uint32_t*>(B.count64), 0u, size_t(ncells), s)) != hipSuccess)
+            return e;
+        int64_t pb = (P.n + 255) / 256;
+        if(pb > 8192)
+            pb = 8192;
+        if(P.n > 0)
+            hipLaunchKernelGGL(vis_bin_kernel, dim3(unsigned(pb)), dim3(256), 0, s, P, B);
+        const int nb = int((ncells + kScanBlock - 1) / kScanBlock);
+        hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(kScanBlock), 0, s, B.count64, ncells, B.scanTmp);
+        hipLaunchKernelGGL(scan_blocksums_kernel, dim3(1), dim3(kScanBlock), 0, s, B.scanTmp, nb);
+        hipLaunchKernelGGL(scan_apply_kernel, dim3(nb), dim3(kScanBlock), 0, s, B.count64, ncells, B.scanTmp, B.count,
+                           B.offset);
+        if(P.n > 0)
+            hipLaunchKernelGGL(vis_scatter_kernel, dim3(unsigned(pb)), dim3(256), 0, s, P, B);
+        int64_t cb = (ncells + 255) / 256;
+        if(cb > 8192)
+            cb = 8192;
+        hipLaunchKernelGGL(vis_sort_kernel, dim3(unsigned(cb)), dim3(256), 0, s, P, B, ncells);
         return hipGetLastError();
     }
 } // namespace isaac_hip

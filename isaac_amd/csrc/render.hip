@@ -92,6 +92,20 @@ namespace isaac_hip
         return v[0];
     }
 
+    // x / d for a divisor fixed per launch, with r = RN(1/d) computed once on the
+    // host: x*r plus one fma residual correction (Markstein's correction), three
+    // instructions instead of the correctly rounded division sequence.  Equal to the
+    // IEEE quotient for finite operands away from overflow and underflow
+    // (tests/cpp/divq_check.c: 0 of 2e8 seeded quotients differ, a quarter of them
+    // next to rounding boundaries; for d = 255 every x in [0, 256],
+    // tests/cpp/div255_check.c).
+    __device__ inline float div_by(float x, float d, float r)
+    {
+        const float q0 = x * r;
+        const float e = __builtin_fmaf(-q0, d, x);
+        return __builtin_fmaf(e, r, q0);
+    }
+
     __device__ inline int tf_index(float v, int T)
     {
         float lv = roundf(v * float(T));
@@ -169,7 +183,7 @@ namespace isaac_hip
                 // (MergeAdvectionToCombinedTextureIterator, lib/isaac/isaac_common_kernel.hpp:655-656)
                 float ca = col.w;
                 if(S.adv)
-                    ca = ca * (float(S.adv[idx]) / 255.0f);
+                    ca = ca * div_by(float(S.adv[idx]), 255.0f, 1.0f / 255.0f);
                 if(S.weight > 0.0f)
                 {
                     float a = ca * S.weight;
@@ -180,7 +194,7 @@ namespace isaac_hip
                 }
                 if(S.iso > 0.0f)
                 {
-                    float a = ca / S.iso * 0.5f;
+                    float a = div_by(ca, S.iso, S.invIso) * 0.5f;
                     if(ia < a)
                     {
                         ir = col.x;
@@ -192,10 +206,10 @@ namespace isaac_hip
             }
             if(P.totalWeight != 0.0f)
             {
-                vr = vr / P.totalWeight;
-                vg = vg / P.totalWeight;
-                vb = vb / P.totalWeight;
-                va = va / P.totalWeight;
+                vr = div_by(vr, P.totalWeight, P.invTotalWeight);
+                vg = div_by(vg, P.totalWeight, P.invTotalWeight);
+                vb = div_by(vb, P.totalWeight, P.invTotalWeight);
+                va = div_by(va, P.totalWeight, P.invTotalWeight);
             }
             else
             {
@@ -373,15 +387,29 @@ namespace isaac_hip
         uint2 a, b;
     };
 
+    // index of texel (x, y, z) in the brick layout (FrameParams::texB)
+    __device__ inline uint32_t brick_texel(const FrameParams& F, uint32_t x, uint32_t y, uint32_t z)
+    {
+        const uint32_t b = (z >> 3) * F.nbxy + (y >> 3) * F.nbx + (x >> 3);
+        return b * uint32_t(kTexBrickTexels) + (z & 7u) * uint32_t(kTexBrickSlice) + (y & 7u) * uint32_t(kTexBrickRow)
+            + (x & 7u);
+    }
+
     // Sampler<LINEAR|NEAREST, CLAMP> addressing (lib/isaac/isaac_texture.hpp:182-259).  Every
     // coordinate is clamped into the texture, so a fetch at any position is in bounds.
-    template<bool LINEAR>
-    __device__ inline void fetch_taps(const uint2* __restrict__ tex, int X, int Y, int Z, float px, float py, float pz, Taps& T)
+    // BRICK reads the brick layout (texels x and x + 1 of a row stay adjacent there:
+    // x & 7 = 7 is followed by the apron texel).
+    template<bool LINEAR, bool BRICK>
+    __device__ inline void fetch_taps(const FrameParams& F, float px, float py, float pz, Taps& T)
     {
+        const uint2* __restrict__ tex = BRICK ? F.texB : F.tex;
+        const int X = F.X, Y = F.Y, Z = F.Z;
+        auto index = [&](uint32_t x, uint32_t y, uint32_t z)
+        { return BRICK ? brick_texel(F, x, y, z) : x + uint32_t(X) * (y + uint32_t(Y) * z); };
         if(!LINEAR)
         {
             const uint32_t x = iclamp(int(px), 0, X - 1), y = iclamp(int(py), 0, Y - 1), z = iclamp(int(pz), 0, Z - 1);
-            T.t[0] = tex[x + uint32_t(X) * (y + uint32_t(Y) * z)];
+            T.t[0] = tex[index(x, y, z)];
             T.fx = T.fy = T.fz = 0.0f;
             return;
         }
@@ -394,15 +422,13 @@ namespace isaac_hip
         const uint32_t x0 = iclamp(bx, 0, X - 1), x1 = iclamp(bx + 1, 0, X - 1);
         const uint32_t y0 = iclamp(by, 0, Y - 1), y1 = iclamp(by + 1, 0, Y - 1);
         const uint32_t z0 = iclamp(bz, 0, Z - 1), z1 = iclamp(bz + 1, 0, Z - 1);
-        const uint32_t r00 = uint32_t(X) * (y0 + uint32_t(Y) * z0), r01 = uint32_t(X) * (y0 + uint32_t(Y) * z1);
-        const uint32_t r10 = uint32_t(X) * (y1 + uint32_t(Y) * z0), r11 = uint32_t(X) * (y1 + uint32_t(Y) * z1);
         // texels x0 and x0+1 are adjacent: one 16-byte load per (y, z) row; at the
-        // x edge (x1 == x0) the second texel (the next row's first, or the padding
-        // texel after the last) is replaced by the first
-        const TexPair p00 = *reinterpret_cast<const TexPair*>(tex + (x0 + r00));
-        const TexPair p01 = *reinterpret_cast<const TexPair*>(tex + (x0 + r01));
-        const TexPair p10 = *reinterpret_cast<const TexPair*>(tex + (x0 + r10));
-        const TexPair p11 = *reinterpret_cast<const TexPair*>(tex + (x0 + r11));
+        // x edge (x1 == x0) the second texel (the next row's first, the padding
+        // texel after the last, or an apron texel) is replaced by the first
+        const TexPair p00 = *reinterpret_cast<const TexPair*>(tex + index(x0, y0, z0));
+        const TexPair p01 = *reinterpret_cast<const TexPair*>(tex + index(x0, y0, z1));
+        const TexPair p10 = *reinterpret_cast<const TexPair*>(tex + index(x0, y1, z0));
+        const TexPair p11 = *reinterpret_cast<const TexPair*>(tex + index(x0, y1, z1));
         const bool dup = x1 == x0;
         T.t[0] = p00.a;
         T.t[1] = p01.a;
@@ -444,11 +470,11 @@ namespace isaac_hip
         return o;
     }
 
-    template<bool LINEAR>
+    template<bool LINEAR, bool BRICK>
     __device__ inline float iso_alpha_at(const FrameParams& F, float px, float py, float pz)
     {
         Taps T;
-        fetch_taps<LINEAR>(F.tex, F.X, F.Y, F.Z, px, py, pz, T);
+        fetch_taps<LINEAR, BRICK>(F, px, py, pz, T);
         return channel<LINEAR, true>(T, 24);
     }
 
@@ -468,23 +494,16 @@ namespace isaac_hip
     };
 
     // the corners of the sample at c: for LINEAR the position minus 0.5 (the texel
-    // centres' offset), for NEAREST the position itself
-    template<bool LINEAR>
-    __device__ inline void fetch_corners(
-        const uint2* __restrict__ tex,
-        int X,
-        int Y,
-        int Z,
-        uint32_t XY,
-        float cx,
-        float cy,
-        float cz,
-        Corners& C)
+    // centres' offset), for NEAREST the position itself; BRICK reads the brick layout
+    template<bool LINEAR, bool BRICK>
+    __device__ inline void fetch_corners(const FrameParams& F, uint32_t XY, float cx, float cy, float cz, Corners& C)
     {
+        const uint2* __restrict__ tex = BRICK ? F.texB : F.tex;
+        const int X = F.X, Y = F.Y, Z = F.Z;
         if(!LINEAR)
         {
             const uint32_t x = iclamp(int(cx), 0, X - 1), y = iclamp(int(cy), 0, Y - 1), z = iclamp(int(cz), 0, Z - 1);
-            C.t[0] = tex[x + uint32_t(X) * y + XY * z];
+            C.t[0] = tex[BRICK ? brick_texel(F, x, y, z) : x + uint32_t(X) * y + XY * z];
             return;
         }
         const float flx = floorf(cx), fly = floorf(cy), flz = floorf(cz);
@@ -494,9 +513,11 @@ namespace isaac_hip
         const bool xin = uint32_t(bx) < uint32_t(X - 1);
         const float fx = xin ? cx - flx : 0.0f, fy = cy - fly, fz = cz - flz;
         const uint32_t x0 = iclamp(bx, 0, X - 1), y0 = iclamp(by, 0, Y - 1), z0 = iclamp(bz, 0, Z - 1);
-        const uint32_t dy = uint32_t(by) < uint32_t(Y - 1) ? uint32_t(X) : 0u;
-        const uint32_t dz = uint32_t(bz) < uint32_t(Z - 1) ? XY : 0u;
-        const uint32_t i00 = x0 + __umul24(y0, uint32_t(X)) + z0 * XY;
+        // rows y + 1 and z + 1: the next row / slice, or the same one at a clamped edge
+        // (in the brick layout the next row of y & 7 = 7 is the apron row)
+        const uint32_t dy = uint32_t(by) < uint32_t(Y - 1) ? (BRICK ? uint32_t(kTexBrickRow) : uint32_t(X)) : 0u;
+        const uint32_t dz = uint32_t(bz) < uint32_t(Z - 1) ? (BRICK ? uint32_t(kTexBrickSlice) : XY) : 0u;
+        const uint32_t i00 = BRICK ? brick_texel(F, x0, y0, z0) : x0 + __umul24(y0, uint32_t(X)) + z0 * XY;
         const uint32_t i01 = i00 + dz, i10 = i00 + dy, i11 = i10 + dz;
         const uint4 p00 = *reinterpret_cast<const uint4*>(tex + i00);
         const uint4 p01 = *reinterpret_cast<const uint4*>(tex + i01);
@@ -747,7 +768,7 @@ namespace isaac_hip
     // order (particles, iso, SSAO, shading, volume) for every pixel.
     // The taps of step i+1 are fetched before step i is composited (clamped
     // addresses, so a fetch past the last step stays in bounds).
-    template<bool LINEAR, bool COUNT, bool PREFETCH>
+    template<bool LINEAR, bool COUNT, bool PREFETCH, bool BRICK>
     __device__ __attribute__((always_inline)) inline void march_body(const FrameParams& F, const ParticleParams& PP)
     {
         int px, py;
@@ -808,8 +829,9 @@ namespace isaac_hip
                         auto at = [&](int k, Corners& C)
                         {
                             const float fk = float(k);
-                            fetch_corners<LINEAR>(F.tex, F.X, F.Y, F.Z, XY, __builtin_fmaf(stepVec.x, fk, startC.x),
-                                                  __builtin_fmaf(stepVec.y, fk, startC.y), __builtin_fmaf(stepVec.z, fk, startC.z), C);
+                            fetch_corners<LINEAR, BRICK>(F, XY, __builtin_fmaf(stepVec.x, fk, startC.x),
+                                                         __builtin_fmaf(stepVec.y, fk, startC.y),
+                                                         __builtin_fmaf(stepVec.z, fk, startC.z), C);
                         };
                         Corners cur;
                         at(i, cur);
@@ -894,13 +916,13 @@ namespace isaac_hip
                         // (lib/isaac/isaac_iso_kernel.hpp:97-119)
                         const F3 pu = divv(add(ray.start, muls(ray.dir, depth)), sc);
                         Taps h;
-                        fetch_taps<LINEAR>(F.tex, F.X, F.Y, F.Z, pu.x, pu.y, pu.z, h);
+                        fetch_taps<LINEAR, BRICK>(F, pu.x, pu.y, pu.z, h);
                         const float hr = channel<LINEAR, true>(h, 0), hg = channel<LINEAR, true>(h, 8);
                         const float hb = channel<LINEAR, true>(h, 16);
                         F3 g;
-                        g.x = iso_alpha_at<LINEAR>(F, pu.x + 1.0f, pu.y, pu.z) - iso_alpha_at<LINEAR>(F, pu.x - 1.0f, pu.y, pu.z);
-                        g.y = iso_alpha_at<LINEAR>(F, pu.x, pu.y + 1.0f, pu.z) - iso_alpha_at<LINEAR>(F, pu.x, pu.y - 1.0f, pu.z);
-                        g.z = iso_alpha_at<LINEAR>(F, pu.x, pu.y, pu.z + 1.0f) - iso_alpha_at<LINEAR>(F, pu.x, pu.y, pu.z - 1.0f);
+                        g.x = iso_alpha_at<LINEAR, BRICK>(F, pu.x + 1.0f, pu.y, pu.z) - iso_alpha_at<LINEAR, BRICK>(F, pu.x - 1.0f, pu.y, pu.z);
+                        g.y = iso_alpha_at<LINEAR, BRICK>(F, pu.x, pu.y + 1.0f, pu.z) - iso_alpha_at<LINEAR, BRICK>(F, pu.x, pu.y - 1.0f, pu.z);
+                        g.z = iso_alpha_at<LINEAR, BRICK>(F, pu.x, pu.y, pu.z + 1.0f) - iso_alpha_at<LINEAR, BRICK>(F, pu.x, pu.y, pu.z - 1.0f);
                         float gl = len3(g);
                         if(isoFirst || gl == 0.0f)
                         {
@@ -948,20 +970,20 @@ namespace isaac_hip
     // The march at the compiler's register allocation, and at occupancy targets of
     // 6 and 8 waves per SIMD (at most 80 / 64 VGPRs); the launcher picks one by the
     // tuning key ISAAC_HIP_TUNE_MARCH_WAVES.
-    template<bool LINEAR, bool COUNT, bool PREFETCH>
+    template<bool LINEAR, bool COUNT, bool PREFETCH, bool BRICK>
     __global__ __launch_bounds__(256) void march_kernel(const FrameParams F, const ParticleParams PP)
     {
-        march_body<LINEAR, COUNT, PREFETCH>(F, PP);
+        march_body<LINEAR, COUNT, PREFETCH, BRICK>(F, PP);
     }
-    template<bool LINEAR>
+    template<bool LINEAR, bool BRICK>
     __global__ __launch_bounds__(256, 6) void march_kernel_w6(const FrameParams F, const ParticleParams PP)
     {
-        march_body<LINEAR, false, false>(F, PP);
+        march_body<LINEAR, false, false, BRICK>(F, PP);
     }
-    template<bool LINEAR>
+    template<bool LINEAR, bool BRICK>
     __global__ __launch_bounds__(256, 8) void march_kernel_w8(const FrameParams F, const ParticleParams PP)
     {
-        march_body<LINEAR, false, false>(F, PP);
+        march_body<LINEAR, false, false, BRICK>(F, PP);
     }
 
     // ------------------------------------------------------------------ SSAO
@@ -2100,7 +2122,7 @@ namespace isaac_hip
                 }
                 if(S.iso > 0.0f)
                 {
-                    float a = ca / S.iso * 0.5f;
+                    float a = div_by(ca, S.iso, S.invIso) * 0.5f;
                     if(ia < a)
                     {
                         ir = col.x;
@@ -2112,10 +2134,10 @@ namespace isaac_hip
             }
             if(P.totalWeight != 0.0f)
             {
-                vr = vr / P.totalWeight;
-                vg = vg / P.totalWeight;
-                vb = vb / P.totalWeight;
-                va = va / P.totalWeight;
+                vr = div_by(vr, P.totalWeight, P.invTotalWeight);
+                vg = div_by(vg, P.totalWeight, P.invTotalWeight);
+                vb = div_by(vb, P.totalWeight, P.invTotalWeight);
+                va = div_by(va, P.totalWeight, P.invTotalWeight);
             }
             else
             {
@@ -2473,42 +2495,94 @@ namespace isaac_hip
         return hipGetLastError();
     }
 
-    hipError_t launch_march(const FrameParams& F, const ParticleParams& PP, bool counting, hipStream_t s)
+    template<bool BRICK>
+    hipError_t launch_march_layout(const FrameParams& F, const ParticleParams& PP, bool counting, hipStream_t s)
     {
         const unsigned nb = march_grid(F);
         if(nb == 0)
             return hipSuccess;
         const dim3 grid(nb), block(256);
-        if(!counting && !F.prefetch && (F.waves == 6 || F.waves == 8))
+        // next-step prefetch is compiled for the linear layout only
+        const bool prefetch = F.prefetch && !BRICK;
+        if(!counting && !prefetch && (F.waves == 6 || F.waves == 8))
         {
             if(F.waves == 6)
             {
                 if(F.linear)
-                    hipLaunchKernelGGL((march_kernel_w6<true>), grid, block, 0, s, F, PP);
+                    hipLaunchKernelGGL((march_kernel_w6<true, BRICK>), grid, block, 0, s, F, PP);
                 else
-                    hipLaunchKernelGGL((march_kernel_w6<false>), grid, block, 0, s, F, PP);
+                    hipLaunchKernelGGL((march_kernel_w6<false, BRICK>), grid, block, 0, s, F, PP);
             }
             else
             {
                 if(F.linear)
-                    hipLaunchKernelGGL((march_kernel_w8<true>), grid, block, 0, s, F, PP);
+                    hipLaunchKernelGGL((march_kernel_w8<true, BRICK>), grid, block, 0, s, F, PP);
                 else
-                    hipLaunchKernelGGL((march_kernel_w8<false>), grid, block, 0, s, F, PP);
+                    hipLaunchKernelGGL((march_kernel_w8<false, BRICK>), grid, block, 0, s, F, PP);
             }
             return hipGetLastError();
         }
-        const int v = (F.linear ? 4 : 0) | (counting ? 2 : 0) | (F.prefetch ? 1 : 0);
+        const int v = (F.linear ? 4 : 0) | (counting ? 2 : 0) | (prefetch ? 1 : 0);
         switch(v)
         {
-        case 0: hipLaunchKernelGGL((march_kernel<false, false, false>), grid, block, 0, s, F, PP); break;
-        case 1: hipLaunchKernelGGL((march_kernel<false, false, true>), grid, block, 0, s, F, PP); break;
-        case 2: hipLaunchKernelGGL((march_kernel<false, true, false>), grid, block, 0, s, F, PP); break;
-        case 3: hipLaunchKernelGGL((march_kernel<false, true, true>), grid, block, 0, s, F, PP); break;
-        case 4: hipLaunchKernelGGL((march_kernel<true, false, false>), grid, block, 0, s, F, PP); break;
-        case 5: hipLaunchKernelGGL((march_kernel<true, false, true>), grid, block, 0, s, F, PP); break;
-        case 6: hipLaunchKernelGGL((march_kernel<true, true, false>), grid, block, 0, s, F, PP); break;
-        default: hipLaunchKernelGGL((march_kernel<true, true, true>), grid, block, 0, s, F, PP); break;
+        case 0: hipLaunchKernelGGL((march_kernel<false, false, false, BRICK>), grid, block, 0, s, F, PP); break;
+        case 2: hipLaunchKernelGGL((march_kernel<false, true, false, BRICK>), grid, block, 0, s, F, PP); break;
+        case 4: hipLaunchKernelGGL((march_kernel<true, false, false, BRICK>), grid, block, 0, s, F, PP); break;
+        case 6: hipLaunchKernelGGL((march_kernel<true, true, false, BRICK>), grid, block, 0, s, F, PP); break;
+        default:
+            if constexpr(!BRICK)
+            {
+                if(v == 1)
+                    hipLaunchKernelGGL((march_kernel<false, false, true, false>), grid, block, 0, s, F, PP);
+                else if(v == 3)
+                    hipLaunchKernelGGL((march_kernel<false, true, true, false>), grid, block, 0, s, F, PP);
+                else if(v == 5)
+                    hipLaunchKernelGGL((march_kernel<true, false, true, false>), grid, block, 0, s, F, PP);
+                else
+                    hipLaunchKernelGGL((march_kernel<true, true, true, false>), grid, block, 0, s, F, PP);
+            }
+            break;
         }
+        return hipGetLastError();
+    }
+
+    hipError_t launch_march(const FrameParams& F, const ParticleParams& PP, bool counting, hipStream_t s)
+    {
+        return F.texB ? launch_march_layout<true>(F, PP, counting, s) : launch_march_layout<false>(F, PP, counting, s);
+    }
+
+    // Linear texture -> brick layout (FrameParams::texB): one workgroup per brick, the
+    // 729 texels of brick b (apron and parts beyond the volume take the clamped texel)
+    // written in order.
+    __global__ __launch_bounds__(256) void tex_brick_kernel(
+        const uint2* __restrict__ tex,
+        uint2* __restrict__ out,
+        int X,
+        int Y,
+        int Z,
+        uint32_t nbx,
+        uint32_t nby)
+    {
+        const uint32_t b = blockIdx.x;
+        const uint32_t bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+        for(uint32_t t = threadIdx.x; t < uint32_t(kTexBrickTexels); t += 256u)
+        {
+            const uint32_t lz = t / uint32_t(kTexBrickSlice), ly = (t / uint32_t(kTexBrickRow)) % uint32_t(kTexBrickRow);
+            const uint32_t lx = t % uint32_t(kTexBrickRow);
+            const uint32_t x = min(bx * uint32_t(kTexBrick) + lx, uint32_t(X - 1));
+            const uint32_t y = min(by * uint32_t(kTexBrick) + ly, uint32_t(Y - 1));
+            const uint32_t z = min(bz * uint32_t(kTexBrick) + lz, uint32_t(Z - 1));
+            out[size_t(b) * kTexBrickTexels + t] = tex[x + size_t(X) * (y + size_t(Y) * z)];
+        }
+    }
+
+    hipError_t launch_tex_bricks(const uint2* tex, uint2* out, int X, int Y, int Z, hipStream_t s)
+    {
+        const uint32_t nbx = uint32_t(X + kTexBrick - 1) / kTexBrick, nby = uint32_t(Y + kTexBrick - 1) / kTexBrick;
+        const uint32_t nbz = uint32_t(Z + kTexBrick - 1) / kTexBrick;
+        if(nbx * nby * nbz == 0)
+            return hipSuccess;
+        hipLaunchKernelGGL(tex_brick_kernel, dim3(nbx * nby * nbz), dim3(256), 0, s, tex, out, X, Y, Z, nbx, nby);
         return hipGetLastError();
     }
 

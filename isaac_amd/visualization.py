@@ -445,9 +445,16 @@ class IsaacVisualization:
     TUNE_MARCH_PREFETCH = 0
     TUNE_MARCH_WAVES = 1
     TUNE_MARCH_TILES = 2
+    TUNE_TEX_LAYOUT = 3
+    tex_layout = 0
+    render_form = 0
 
     def set_tuning(self, key: int, value: int) -> None:
+        """TUNE_TEX_LAYOUT takes effect at the next update_buffers (a render before it
+        raises)."""
         check(lib().isaac_hip_set_tuning(self._ctx, int(key), int(value)), self._ctx, "set_tuning")
+        if key == self.TUNE_TEX_LAYOUT:
+            self.tex_layout = int(value)
 
     # render forms (no counterpart in the reference; include/isaac_hip.h)
     FORM_COMBINED = 0
@@ -459,6 +466,7 @@ class IsaacVisualization:
         march over 8^3 field bricks stored in brick_dtype (F32/F16/BF16 + FP32 scale).
         Takes effect at the next update_buffers."""
         check(lib().isaac_hip_set_render_form(self._ctx, int(form), int(brick_dtype)), self._ctx, "set_render_form")
+        self.render_form = int(form)
 
     def bricks(self, source: int):
         """Host copies of a source's bricks: (pairs, scales) with pairs shaped

@@ -103,6 +103,17 @@ int main(int argc, char** argv)
         std::ofstream mout(std::string(argv[5]) + ".msg", std::ios::binary);
         mout.write(msg.data(), std::streamsize(msg.size()));
         std::printf("video frame message %zu bytes\n", msg.size());
+        // the server messages of this visualization: register (init), one periodic
+        // message with every section, exit
+        const std::string reg = vis.registerMessage(1);
+        const std::string per = vis.periodMessage(0x1ffffffu);
+        const std::string fin = decltype(vis)::exitMessage();
+        for(const auto& kv : {std::make_pair(".register", &reg), std::make_pair(".period", &per), std::make_pair(".exit", &fin)})
+        {
+            std::ofstream f(std::string(argv[5]) + kv.first, std::ios::binary);
+            f.write(kv.second->data(), std::streamsize(kv.second->size()));
+        }
+        std::printf("server messages %zu %zu %zu bytes\n", reg.size(), per.size(), fin.size());
     }
     (void) hipFree(src.ptr);
     return 0;

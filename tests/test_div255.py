@@ -15,3 +15,16 @@ def test_div255_identity_exhaustive(tmp_path):
     print(r.stdout)
     assert r.returncode == 0, r.stdout
     assert "mismatches 0" in r.stdout
+
+
+def test_general_division_identity_sampled(tmp_path):
+    """The classification kernel divides by per-source constants (iso thresholds,
+    the sum of source weights) with the same one-correction scheme and r = RN(1/d):
+    2e8 seeded (x, d) pairs, d over [1e-3, 1e3] and extreme mantissas, a quarter
+    of them next to rounding boundaries, all equal to IEEE x / d."""
+    exe = tmp_path / "divq_check"
+    subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-o", str(exe), os.path.join(HERE, "cpp", "divq_check.c"), "-lm"])
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=600)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout
+    assert "mismatches 0" in r.stdout

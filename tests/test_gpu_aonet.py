@@ -125,20 +125,28 @@ def test_render_path_occlusion_matches_network(gpu):
     vis.close()
 
 
-def _training_scene():
+def _evaluation_scene():
     # particles and an iso-surface give the depth structure the occlusion term depends on
     return scenes.prod_scene(n=128, fb=(1920, 1080), n_particles=250_000)
 
 
 @pytest.fixture(scope="module")
 def trained(gpu):
+    """The network trained on 20,000 triples (1250 steps x 16 crops of 128^2) of
+    16 production-style 128^3 scenes (fields, particles, clipping planes) whose seeds
+    differ from the evaluation scene's, at random poses at 1920x1080, then set on a
+    context of the evaluation scene for the gate."""
     from isaac_amd import aonet
     from isaac_amd import visualization as V
 
-    spec = _training_scene()
-    vis = V.from_scene(spec, gpu.index or 0)
-    vis.aonet_set_params(aonet.init_params(0))
-    losses = aonet.train(vis, steps=600, seed=1, log_every=100)
+    ts = aonet.TrainingSet(gpu.index or 0, fb=(1920, 1080), n=128, particles=250_000)
+    ts.vis.aonet_set_params(aonet.init_params(0))
+    losses = aonet.train(ts.vis, steps=1250, seed=1, log_every=250, prepare=ts.prepare)
+    params = ts.vis.aonet_get_params()
+    ts.close()
+    vis = V.from_scene(_evaluation_scene(), gpu.index or 0)
+    vis.update_buffers()
+    vis.aonet_set_params(params)
     yield vis, losses
     vis.close()
 
@@ -156,13 +164,34 @@ def test_learned_pass_gate(trained):
 
     vis, _ = trained
     poses = [scenes.DEFAULT_POSE, scenes.FILL_POSE, scenes.random_pose(np.random.default_rng(99))]
-    r = aonet.gate(vis, poses, floor_db=40.0)
+    r = aonet.gate(vis, poses, floor_db=40.0, min_gain_db=3.0)
     print(f"learned occlusion: image PSNR {r['image_psnr']:.2f} dB (floor 40), occlusion map PSNR "
-          f"{r['ao_psnr']:.2f} dB; without occlusion the image is at {r['image_psnr_no_ao']:.2f} dB", flush=True)
-    # what the network buys over the method without it
-    assert r["image_psnr"] > r["image_psnr_no_ao"] + 6.0
+          f"{r['ao_psnr']:.2f} dB; without occlusion the image is at {r['image_psnr_no_ao']:.2f} dB; "
+          f"gain over no occlusion (worst pose) {r['gain_db']:.2f} dB", flush=True)
+    # what the network buys over the method without it, on a scene it was not trained on
+    assert r["gain_db"] >= 3.0
     assert r["passed"], r
     assert vis.ao_mode == vis.AO_LEARNED
+
+
+def test_gate_rejects_a_network_that_adds_nothing(gpu):
+    """All-zero parameters give an occlusion term of 0 everywhere: the image equals the
+    one without occlusion, which can sit above the 40 dB floor, but it gains 0 dB, so
+    the gate falls back to the plain pass."""
+    from isaac_amd import aonet
+    from isaac_amd import visualization as V
+
+    vis = V.from_scene(_evaluation_scene(), gpu.index or 0)
+    vis.update_buffers()
+    vis.aonet_set_params(np.zeros(aonet.param_count(), np.float32))
+    poses = [scenes.DEFAULT_POSE, scenes.FILL_POSE]
+    r = aonet.gate(vis, poses, floor_db=40.0, min_gain_db=3.0)
+    print(f"zero network: image {r['image_psnr']:.2f} dB, without occlusion {r['image_psnr_no_ao']:.2f} dB, "
+          f"gain {r['gain_db']:.3f} dB", flush=True)
+    assert abs(r["gain_db"]) < 0.01
+    assert not r["passed"]
+    assert vis.ao_mode == vis.AO_PLAIN
+    vis.close()
 
 
 def test_learned_pass_frame_time(trained):

@@ -42,3 +42,20 @@ def test_cpp_dropin_example_matches_oracle(tmp_path, oracle_lib, hip_lib):
 
     msg = (tmp_path / "frame.rgba.msg").read_bytes()
     assert msg == jpeg_oracle.frame_message(img, 90)
+    # the server messages of the drop-in against the message oracle, from the state the
+    # example set: one density source, its 4-point transfer function, interpolation,
+    # step 0.5, black background, the default camera and the min/max it requested
+    from oracle import message_oracle as M
+
+    pts = [(0, (0.0, 0.0, 0.4, 0.0)), (256, (0.0, 0.5, 1.0, 0.05)), (614, (1.0, 0.85, 0.1, 0.35)),
+           (1024, (1.0, 0.15, 0.1, 0.9))]
+    st = M.State(
+        source_names=["density"], feature_dims=[1], n_volume_field=1, functions=["idem"], function_errors=[0],
+        transfer_size=1024, transfer_lut=oracle_lib.build_lut(pts, 1024)[None], transfer_points=[pts],
+        weights=[2.0], iso_thresholds=[0.0], min_max=[(float(line[1]), float(line[2]))], step=0.5, interpolation=True,
+        background=(0.0, 0.0, 0.0), projection=[float(x) for x in oracle_lib.default_projection(W, H)],
+    )
+    sess = M.Session()
+    assert (tmp_path / "frame.rgba.register").read_text() == sess.register(st, "shim_example", 1, (W, H), (64, 64, 64))
+    assert (tmp_path / "frame.rgba.period").read_text() == sess.period(st, 0, M.SEND)
+    assert (tmp_path / "frame.rgba.exit").read_text() == M.exit_message()

@@ -259,6 +259,59 @@ def test_schedules_do_not_change_the_frame(gpu, oracle_lib, form):
     vis.close()
 
 
+@pytest.mark.parametrize("n,linear,pose", [(64, True, "default"), (64, True, "fill"), (37, True, "default"),
+                                           (37, False, "fill")])
+def test_texture_brick_layout_renders_the_same_frame(gpu, oracle_lib, n, linear, pose):
+    """The march reading the 8^3-brick copy of the combined texture (apron texels
+    and partial bricks at a side of 37 hold clamped texels) renders the frame and
+    G-buffer of the linear texture bit for bit, at every occupancy target and with
+    the counting march, and executes the same steps; at a side of 37 the frame is
+    also checked against the oracle."""
+    from oracle import oracle
+
+    p = scenes.FILL_POSE if pose == "fill" else scenes.DEFAULT_POSE
+    spec = dataclasses.replace(_reduced_prod(n=n, fb=(330, 250), particles=20000, visible_tf=True, pose=p),
+                               interpolation=linear)
+    vis = _vis(spec, gpu)
+    vis.update_buffers()
+    vis.set_counting(True)
+    ref = vis.draw()
+    ref_g = [a.copy() for a in vis.gbuffer()]
+    ref_c = {k: v for k, v in vis.counters().items()}
+    vis.set_counting(False)
+    vis.set_tuning(vis.TUNE_TEX_LAYOUT, 8)
+    vis.update_buffers()
+    for waves in (0, 6, 8):
+        vis.set_tuning(vis.TUNE_MARCH_WAVES, waves)
+        img = vis.draw()
+        assert np.array_equal(img, ref), waves
+        for a, b in zip(vis.gbuffer(), ref_g):
+            assert np.array_equal(a, b), waves
+    vis.set_tuning(vis.TUNE_MARCH_WAVES, 0)
+    vis.set_counting(True)
+    assert np.array_equal(vis.draw(), ref)
+    assert vis.counters() == ref_c
+    vis.close()
+    if n == 37:
+        compare(f"prod{n}-{pose} brick layout", spec, ref, oracle.render(spec, 32))
+
+
+def test_texture_layout_change_needs_update(gpu):
+    from isaac_amd._lib import IsaacHipError
+
+    vis = _vis(_reduced_prod(n=16, fb=(32, 32), particles=100), gpu)
+    vis.update_buffers()
+    vis.draw()
+    with pytest.raises(IsaacHipError):
+        vis.set_tuning(vis.TUNE_TEX_LAYOUT, 4)
+    vis.set_tuning(vis.TUNE_TEX_LAYOUT, 8)
+    with pytest.raises(IsaacHipError):
+        vis.draw()
+    vis.update_buffers()
+    vis.draw()
+    vis.close()
+
+
 def test_tuning_rejects_unknown_values(gpu):
     from isaac_amd._lib import IsaacHipError
 
