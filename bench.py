@@ -159,16 +159,22 @@ def cpu_baseline(spec, poses, threads, frames=3):
     }
 
 
+def set_pose_of(vis, p):
+    import numpy as np
+
+    vis.look_at = np.asarray(p.look_at, np.float64)
+    vis.rotation = np.asarray(p.rotation, np.float64)
+    vis.distance = float(p.distance)
+    vis._update_modelview()
+
+
 def measure(vis, poses, out, steps, warmup):
     """Counting frames per pose (untimed), warmup, then `steps` timed frames."""
     import numpy as np
     import torch
 
     def set_pose(p):
-        vis.look_at = np.asarray(p.look_at, np.float64)
-        vis.rotation = np.asarray(p.rotation, np.float64)
-        vis.distance = float(p.distance)
-        vis._update_modelview()
+        set_pose_of(vis, p)
 
     counts = []
     vis.set_counting(True)
@@ -205,6 +211,35 @@ def measure(vis, poses, out, steps, warmup):
     avg_us = {k: float(np.mean(v)) for k, v in per.items()}
     avg_counts = {k: float(np.mean([c[k] for c in counts])) for k in counts[0]}
     return wall, total_samples, avg_us, avg_counts, counts
+
+
+def tex_layout_study(vis, poses, frames=6):
+    """March time and classification time (which holds the brick copy) of each texture
+    layout at the tuned schedule, per pose; medians over `frames` updated frames."""
+    import numpy as np
+
+    keep = vis.tex_layout
+    res = {}
+    for layout in (0, 8):
+        vis.set_tuning(vis.TUNE_TEX_LAYOUT, layout)
+        vis.update_buffers()
+        per = {}
+        for name, p in zip(("default", "fill"), poses):
+            set_pose_of(vis, p)
+            vis.update_buffers()
+            vis.draw(out=False)
+            m, c = [], []
+            for _ in range(frames):
+                vis.update_buffers()
+                vis.draw(out=False)
+                t = vis.timings()
+                m.append(t["march_us"])
+                c.append(t["optimization_buffer_us"])
+            per[name] = {"march_us": float(np.median(m)), "classify_us": float(np.median(c))}
+        res[f"layout={layout}"] = per
+    vis.set_tuning(vis.TUNE_TEX_LAYOUT, keep)
+    vis.update_buffers()
+    return res
 
 
 def learned_ao(vis, poses, out, args):
@@ -532,6 +567,7 @@ def main() -> int:
 
         vis.update_buffers()
         tuning = autotune.tune_march(vis, poses)
+        tuning["tex_layout_study"] = tex_layout_study(vis, poses)
     wall, total_samples, avg_us, avg_counts, counts = measure(vis, poses, out, args.steps, args.warmup)
     ms_per_step = wall / args.steps * 1e3
     value = total_samples / wall
