@@ -248,11 +248,13 @@ extern "C"
         ISAAC_HIP_TUNE_MARCH_TILES = 2, /* march tile order: 0 one band of tile rows per XCD; 2, 4 (default), 8 or
                                           16: super-tiles of that many 16x16-pixel tiles a side, dealt
                                           round-robin to the XCDs */
-        ISAAC_HIP_TUNE_TEX_LAYOUT = 3 /* texture the combined-form march reads: 0 (default) the linear texture
-                                         (x fastest); 8: a copy in 8^3 bricks with a one-texel apron, built by
-                                         update_buffers after classification (the linear texture stays what
-                                         isaac_hip_get_texture returns); a change needs update_buffers before
-                                         the next render; next-step prefetch applies to the linear layout only */
+        ISAAC_HIP_TUNE_TEX_LAYOUT = 3, /* texture the combined-form march reads: 0 (default) the linear texture
+                                          (x fastest); 8: a copy in 8^3 bricks with a one-texel apron, built by
+                                          update_buffers after classification (the linear texture stays what
+                                          isaac_hip_get_texture returns); a change needs update_buffers before
+                                          the next render; next-step prefetch applies to the linear layout only */
+        ISAAC_HIP_TUNE_MARCH_THREADS = 4 /* combined-form march workgroup: 256 (default; 4 waves render one 16x16
+                                            tile) or 64 (one wave per workgroup, one 8x8 quarter of a tile) */
     };
     int isaac_hip_set_tuning(isaac_hip_ctx* ctx, int32_t key, int32_t value);
 

@@ -3,7 +3,7 @@
 Times the march kernel of a visualization under each candidate schedule
 (include/isaac_hip.h, "tuning": march occupancy target, next-step prefetch,
 tile order over the XCDs and, in the combined form, the texture layout: linear
-or 8^3 bricks) on the caller's own frames and keeps the fastest.  Results do not
+or 8^3 bricks, and the workgroup size: 4 waves or one) on the caller's own frames and keeps the fastest.  Results do not
 depend on the schedule, only speed does; choices are cached per workload key.
 """
 from __future__ import annotations
@@ -20,6 +20,7 @@ CANDIDATES = {
     "march_prefetch": (0, 1),
     "march_tiles": (0, 4, 8),
     "tex_layout": (0, 8),
+    "march_threads": (256, 64),
 }
 
 
@@ -32,6 +33,7 @@ def _apply(vis, choice: Dict[str, int]) -> None:
     vis.set_tuning(vis.TUNE_MARCH_WAVES, choice["march_waves"])
     vis.set_tuning(vis.TUNE_MARCH_PREFETCH, choice["march_prefetch"])
     vis.set_tuning(vis.TUNE_MARCH_TILES, choice.get("march_tiles", 4))
+    vis.set_tuning(vis.TUNE_MARCH_THREADS, choice.get("march_threads", 256))
     layout = choice.get("tex_layout", 0)
     if layout != vis.tex_layout:
         vis.set_tuning(vis.TUNE_TEX_LAYOUT, layout)
@@ -49,12 +51,15 @@ def tune_march(vis, poses: Sequence, frames: int = 4, use_cache: bool = True, up
         _apply(vis, _CACHE[key])
         return {"choice": dict(_CACHE[key]), "march_us": {}, "cached": True}
     results = {}
-    layouts = CANDIDATES["tex_layout"] if vis.render_form == vis.FORM_COMBINED else (vis.tex_layout,)
-    for layout, waves, pf, tiles in itertools.product(layouts, CANDIDATES["march_waves"],
-                                                      CANDIDATES["march_prefetch"], CANDIDATES["march_tiles"]):
+    combined = vis.render_form == vis.FORM_COMBINED
+    layouts = CANDIDATES["tex_layout"] if combined else (vis.tex_layout,)
+    threads = CANDIDATES["march_threads"] if combined else (256,)
+    for layout, nt, waves, pf, tiles in itertools.product(layouts, threads, CANDIDATES["march_waves"],
+                                                          CANDIDATES["march_prefetch"], CANDIDATES["march_tiles"]):
         if pf and (waves or layout):
             continue  # the occupancy-targeted and brick-layout kernels have no prefetch variant
-        choice = {"march_waves": waves, "march_prefetch": pf, "march_tiles": tiles, "tex_layout": layout}
+        choice = {"march_waves": waves, "march_prefetch": pf, "march_tiles": tiles, "tex_layout": layout,
+                  "march_threads": nt}
         _apply(vis, choice)
         per_pose = []
         for p in poses:
@@ -72,14 +77,15 @@ def tune_march(vis, poses: Sequence, frames: int = 4, use_cache: bool = True, up
                 times.append(t["march_us"] + (t["optimization_buffer_us"] if update_each_frame else 0.0))
             per_pose.append(float(np.median(times)))
         # the mean over the poses of each pose's median: every pose weighs the same
-        results[(layout, waves, pf, tiles)] = (float(np.mean(per_pose)), per_pose)
+        results[(layout, nt, waves, pf, tiles)] = (float(np.mean(per_pose)), per_pose)
     best = min(results, key=lambda k: results[k][0])
-    choice = {"march_waves": best[1], "march_prefetch": best[2], "march_tiles": best[3], "tex_layout": best[0]}
+    choice = {"march_waves": best[2], "march_prefetch": best[3], "march_tiles": best[4], "tex_layout": best[0],
+              "march_threads": best[1]}
     _CACHE[key] = choice
     _apply(vis, choice)
 
     def name(k):
-        return f"layout={k[0]},waves={k[1]},prefetch={k[2]},tiles={k[3]}"
+        return f"layout={k[0]},threads={k[1]},waves={k[2]},prefetch={k[3]},tiles={k[4]}"
 
     return {"choice": choice, "cached": False,
             "march_us": {name(k): t[0] for k, t in results.items()},

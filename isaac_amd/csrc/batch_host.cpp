@@ -303,6 +303,7 @@ extern "C"
                 return bfail(b, rc, c->err);
             b->hF[f].waves = 0;
             b->hF[f].prefetch = 0;
+            b->hF[f].waveGroups = 4;
             b->hInvScale[f] = 1.0f / float(c->maxGsScaled);
             // the persistent march stages frame 0's transfer functions for every frame:
             // every frame must bind the same sources in the same slots

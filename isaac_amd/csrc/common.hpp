@@ -92,6 +92,7 @@ namespace isaac_hip
         int32_t prefetch; // fetch the next step's taps before compositing the current one
         int32_t waves; // march occupancy target: 0 compiler default, 6 or 8 waves per SIMD
         int32_t tileOrder; // march tile order: 0 one band of tile rows per XCD, S: SxS-tile super-tiles dealt to the XCDs
+        int32_t waveGroups; // combined-form march workgroups: 1 one wave each, else 4 waves (one 16x16 tile)
         // textures
         int32_t X, Y, Z;
         const uint2* tex; // interleaved combined textures, x fastest
@@ -259,9 +260,30 @@ namespace isaac_hip
     {
         return float((0.0f < x) ? 1 : 0) - float((x < 0.0f) ? 1 : 0);
     }
+    // v clamped to [lo, hi] (lo <= hi): one max and one min
     __host__ __device__ inline int iclamp(int v, int lo, int hi)
     {
-        return v < lo ? lo : (v > hi ? hi : v);
+        const int a = v < lo ? lo : v;
+        return a > hi ? hi : a;
+    }
+
+    // (uint32_t) roundf(gclamp(v, 0, 1) * 255): round half away from zero of a value
+    // x in [0, 255] as x < 0.5 ? 0 : trunc(x + 0.5) -- x + 0.5 is exact or rounds
+    // without crossing an integer for every x >= 0.5 (all binary32 v checked,
+    // tests/cpp/round_check.c); a NaN v gives the conversion of NaN in both forms
+    __host__ __device__ inline uint32_t unorm8(float v)
+    {
+        const float x = gclamp(v, 0.0f, 1.0f) * 255.0f;
+        return x < 0.5f ? 0u : (uint32_t) (x + 0.5f);
+    }
+
+    // transfer-function index of x = value * T: roundf(x), NaN -> 0, clamped to
+    // [0, T - 1] (lib/isaac/isaac_common_kernel.hpp:590-592), as
+    // x >= 0.5 ? trunc(min(x + 0.5, T - 1)) : 0 (every binary32 x in [-2, T + 2] and
+    // the infinities checked for T = 1024, tests/cpp/round_check.c)
+    __host__ __device__ inline int tf_index_of(float x, int T)
+    {
+        return x >= 0.5f ? (int) gmin(x + 0.5f, float(T - 1)) : 0;
     }
 
     // transformColor float4 -> byte4 (truncating, lib/isaac/isaac_helper.hpp:71-74)

@@ -106,6 +106,7 @@ struct isaac_hip_ctx
     int tuneWaves = 0;
     int tuneTiles = 4; // super-tiles of 4x4 tiles (measured fastest at 1080p)
     int tuneTexLayout = 0; // march texture: 0 linear (x fastest), 8 bricks of 8^3 texels
+    int tuneMarchThreads = 256; // combined-form march workgroup: 256 (4 waves, one tile) or 64 (one wave)
     int builtTexLayout = -1; // layout whose texels the last combined-form update_buffers wrote
     int form = ISAAC_HIP_FORM_COMBINED;
     int brickDtype = ISAAC_HIP_F32;

@@ -573,6 +573,7 @@ namespace isaac_hip
         F.prefetch = c->tunePrefetch;
         F.waves = c->tuneWaves;
         F.tileOrder = c->tuneTiles;
+        F.waveGroups = (c->form == ISAAC_HIP_FORM_COMBINED && c->tuneMarchThreads == 64) ? 1 : 4;
         F.X = int32_t(c->local[0]);
         F.Y = int32_t(c->local[1]);
         F.Z = int32_t(c->local[2]);
@@ -1639,6 +1640,11 @@ extern "C"
             if(value != 0 && value != 2 && value != 4 && value != 8 && value != 16)
                 return fail(c, ISAAC_HIP_ERR_ARG, "march tile order: 0 (XCD bands) or a super-tile side of 2, 4, 8 or 16 tiles");
             c->tuneTiles = value;
+            return 0;
+        case ISAAC_HIP_TUNE_MARCH_THREADS:
+            if(value != 64 && value != 256)
+                return fail(c, ISAAC_HIP_ERR_ARG, "march workgroup: 64 or 256 threads");
+            c->tuneMarchThreads = value;
             return 0;
         case ISAAC_HIP_TUNE_TEX_LAYOUT:
             if(value != 0 && value != kTexBrick)

@@ -93,8 +93,7 @@ namespace isaac_hip
             const int cz = cell_coord(P.pos[3 * i + 2], P.cw[2], P.grid[2], qz);
             const int c = cx + P.grid[0] * (cy + P.grid[1] * cz);
             const float v = apply_chain_p(P.chain, P.attr + i * P.dim, P.dim);
-            const float lv = roundf(v * float(P.T));
-            const int li = (lv != lv) ? 0 : (int) gclamp(lv, 0.0f, float(P.T - 1));
+            const int li = tf_index_of(v * float(P.T), P.T);
             const bool visible = !P.filter || P.lut[li].w >= 0.5f;
             B.cellOf[i] = c;
             B.rel[i] = make_float4(qx - float(cx), qy - float(cy), qz - float(cz), __int_as_float(visible ? li : -1));
@@ -314,8 +313,7 @@ namespace isaac_hip
             if(i < P.n)
             {
                 const float v = apply_chain_p(P.chain, P.attr + i * P.dim, P.dim);
-                const float lv = roundf(v * float(P.T));
-                li = (lv != lv) ? 0 : (int) gclamp(lv, 0.0f, float(P.T - 1));
+                li = tf_index_of(v * float(P.T), P.T);
                 visible = !P.filter || P.lut[li].w >= 0.5f;
             }
             const unsigned long long mask = __ballot(visible);

@@ -108,10 +108,7 @@ namespace isaac_hip
 
     __device__ inline int tf_index(float v, int T)
     {
-        float lv = roundf(v * float(T));
-        if(lv != lv)
-            return 0;
-        return (int) gclamp(lv, 0.0f, float(T - 1));
+        return tf_index_of(v * float(T), T);
     }
 
     // ------------------------------------------------------------------ classification
@@ -119,11 +116,7 @@ namespace isaac_hip
     {
         if(mode == 0)
         {
-            uint32_t q0 = (uint32_t) roundf(gclamp(r, 0.0f, 1.0f) * 255.0f);
-            uint32_t q1 = (uint32_t) roundf(gclamp(g, 0.0f, 1.0f) * 255.0f);
-            uint32_t q2 = (uint32_t) roundf(gclamp(b, 0.0f, 1.0f) * 255.0f);
-            uint32_t q3 = (uint32_t) roundf(gclamp(a, 0.0f, 1.0f) * 255.0f);
-            return q0 | (q1 << 8) | (q2 << 16) | (q3 << 24);
+            return unorm8(r) | (unorm8(g) << 8) | (unorm8(b) << 16) | (unorm8(a) << 24);
         }
         // threshold of the mode's mask (applyDither3D, lib/isaac/isaac_dither_kernel.hpp:20-323):
         // 1 ordered 4^3, 2 blue noise 4^3, 3 blue noise 8^3, any other mode blue noise 16^3
@@ -216,10 +209,7 @@ namespace isaac_hip
                 vr = vg = vb = va = 0.0f;
             }
             const uint32_t vol = dither_b4(P.ditherMode, x, y, z, vr, vg, vb, va);
-            uint32_t q0 = (uint32_t) roundf(gclamp(ir, 0.0f, 1.0f) * 255.0f);
-            uint32_t q1 = (uint32_t) roundf(gclamp(ig, 0.0f, 1.0f) * 255.0f);
-            uint32_t q2 = (uint32_t) roundf(gclamp(ib, 0.0f, 1.0f) * 255.0f);
-            uint32_t q3 = (uint32_t) roundf(gclamp(ia, 0.0f, 1.0f) * 255.0f);
+            const uint32_t q0 = unorm8(ir), q1 = unorm8(ig), q2 = unorm8(ib), q3 = unorm8(ia);
             // one 8-byte texel per voxel: combined volume RGBA8, combined iso RGBA8
             P.tex[idx] = make_uint2(vol, q0 | (q1 << 8) | (q2 << 16) | (q3 << 24));
         }
@@ -578,19 +568,38 @@ namespace isaac_hip
     // ------------------------------------------------------------------ pixel tiling
     // workgroup -> 16x16 tile, XCD-banded; returns false for threads outside the viewport
     // pixel of this thread in 16x16 tile `b` of the viewport (row-major tiles)
-    __device__ inline bool tile_pixel_at(const FrameParams& F, uint32_t b, int& px, int& py)
+    // pixel of this lane in 16x16 tile b, whose 8x8 quarter `wave` (0..3) its wave renders
+    __device__ inline bool tile_pixel_at(const FrameParams& F, uint32_t b, uint32_t wave, int& px, int& py)
     {
         const uint32_t tx = b % uint32_t(F.tilesX), ty = b / uint32_t(F.tilesX);
-        const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+        const uint32_t lane = threadIdx.x & 63u;
         px = F.vx0 + int(tx * 16u + (wave & 1u) * 8u + (lane & 7u));
         py = F.vy0 + int(ty * 16u + (wave >> 1) * 8u + (lane >> 3));
         return b < uint32_t(F.tilesX) * uint32_t(F.tilesY) && px < F.vx1 && py < F.vy1;
     }
 
-    __device__ inline uint32_t xcd_tile(const FrameParams& F)
+    __device__ inline bool tile_pixel_at(const FrameParams& F, uint32_t b, int& px, int& py)
+    {
+        return tile_pixel_at(F, b, threadIdx.x >> 6, px, py);
+    }
+
+    // The march's workgroups: one 16x16 tile of 4 waves each, or (F.waveGroups == 1)
+    // one wave each: workgroup b is then quarter (b / 8) % 4 of the tile of logical
+    // workgroup (b / 32) * 8 + b % 8, and runs on the XCD b % 8 of that logical
+    // workgroup, so the dealing of tiles to the XCDs is unchanged while the
+    // dispatcher fills any SIMD with a free wave slot.
+    __device__ inline uint32_t march_block(const FrameParams& F)
+    {
+        return F.waveGroups == 1 ? ((blockIdx.x >> 5) << 3) | (blockIdx.x & 7u) : blockIdx.x;
+    }
+    __device__ inline uint32_t march_wave(const FrameParams& F)
+    {
+        return F.waveGroups == 1 ? (blockIdx.x >> 3) & 3u : threadIdx.x >> 6;
+    }
+
+    __device__ inline uint32_t xcd_tile(const FrameParams& F, uint32_t b)
     {
         const uint32_t nb = uint32_t(F.tilesX) * uint32_t(F.tilesY);
-        uint32_t b = blockIdx.x;
         const uint32_t full = nb & ~7u;
         if(b < full)
             b = (b & 7u) * (full >> 3) + (b >> 3);
@@ -603,12 +612,12 @@ namespace isaac_hip
     // super-tiles from all over the frame (balanced when the work is concentrated,
     // e.g. a volume in the middle of the frame) while the rays of one super-tile
     // share its L2.  Grid: march_grid workgroups; padding workgroups map to no tile.
-    __device__ inline uint32_t supertile_tile(const FrameParams& F)
+    __device__ inline uint32_t supertile_tile(const FrameParams& F, uint32_t b)
     {
         const uint32_t S = uint32_t(F.tileOrder), sh = uint32_t(__ffs(int(S)) - 1);
         const uint32_t tX = uint32_t(F.tilesX), tY = uint32_t(F.tilesY);
         const uint32_t sx = (tX + S - 1u) >> sh, sy = (tY + S - 1u) >> sh;
-        const uint32_t b = blockIdx.x, i = b >> 3;
+        const uint32_t i = b >> 3;
         const uint32_t s = (b & 7u) + 8u * (i >> (2u * sh)), w = i & (S * S - 1u);
         if(s >= sx * sy)
             return 0xffffffffu;
@@ -620,12 +629,13 @@ namespace isaac_hip
 
     __device__ inline uint32_t march_tile(const FrameParams& F)
     {
-        return F.tileOrder != 0 ? supertile_tile(F) : xcd_tile(F);
+        const uint32_t b = march_block(F);
+        return F.tileOrder != 0 ? supertile_tile(F, b) : xcd_tile(F, b);
     }
 
     __device__ inline bool tile_pixel(const FrameParams& F, int& px, int& py)
     {
-        return tile_pixel_at(F, xcd_tile(F), px, py);
+        return tile_pixel_at(F, xcd_tile(F, blockIdx.x), px, py);
     }
 
     __device__ inline void block_count_add(unsigned long long* dst, unsigned long long v)
@@ -639,7 +649,9 @@ namespace isaac_hip
         __syncthreads();
         if(threadIdx.x == 0)
         {
-            unsigned long long s = red[0] + red[1] + red[2] + red[3];
+            unsigned long long s = 0;
+            for(uint32_t w = 0; w < (blockDim.x >> 6); w++)
+                s += red[w];
             if(s)
                 atomicAdd(dst, s);
         }
@@ -772,7 +784,7 @@ namespace isaac_hip
     __device__ __attribute__((always_inline)) inline void march_body(const FrameParams& F, const ParticleParams& PP)
     {
         int px, py;
-        const bool valid = tile_pixel_at(F, march_tile(F), px, py);
+        const bool valid = tile_pixel_at(F, march_tile(F), march_wave(F), px, py);
         unsigned long long nCells = 0, nTests = 0, nIso = 0, nIsoHits = 0, nIsoRays = 0, nPartRays = 0;
         unsigned long long nVol = 0, nVolRays = 0, nSteps = 0;
         if(valid)
@@ -814,6 +826,7 @@ namespace isaac_hip
                         nIsoRays++;
                     float cr = 0.f, cg = 0.f, cb = 0.f, ca = 0.f;
                     float oldValue = 0.0f;
+                    bool oldKnown = true; // oldValue holds the previous step's iso sum (0 before the first step)
                     bool isoHit = false, isoFirst = false;
                     int i = startSteps;
                     int last = isoOn ? endIso : endVol;
@@ -845,15 +858,28 @@ namespace isaac_hip
                                 at(i + 1, nxt);
                             if(isoOn)
                             {
-                                // the iso alpha in byte units (eight zero alpha bytes give exactly 0);
-                                // alpha >= 0.5 is a sum >= 127.5
-                                const float sum = (corners_or<LINEAR, true>(cur) >> 24) ? corner_sum<LINEAR, true>(cur, 24) : 0.0f;
+                                // the iso alpha in byte units; alpha >= 0.5 is a sum >= 127.5, which
+                                // needs an alpha byte >= 128 (the corner weights sum to 1 up to a few
+                                // ulps, so eight bytes <= 127 stay below 127.1).  Without such a byte
+                                // the sum is not formed: it is exactly 0 when all alpha bytes are 0,
+                                // else it is formed again from the step's corners if the next step hits
+                                const uint32_t alphas = corners_or<LINEAR, true>(cur);
+                                const bool high = (alphas >> 31) != 0u;
+                                const float sum = high ? corner_sum<LINEAR, true>(cur, 24) : 0.0f;
                                 if(COUNT)
                                     nIso++;
-                                const float tmpSum = oldValue;
+                                float tmpSum = oldValue;
+                                const bool tmpKnown = oldKnown;
                                 oldValue = sum;
+                                oldKnown = high || (alphas >> 24) == 0u;
                                 if(sum >= 127.5f)
                                 {
+                                    if(!tmpKnown)
+                                    {
+                                        Corners pc;
+                                        at(i - 1, pc);
+                                        tmpSum = corner_sum<LINEAR, true>(pc, 24);
+                                    }
                                     const bool first = ray.isClipped && i == startSteps;
                                     const float tt = float(i) * stepU;
                                     const float value = div255(sum), tmp = div255(tmpSum);
@@ -1011,7 +1037,7 @@ namespace isaac_hip
 
     __global__ __launch_bounds__(256) void ssao_kernel(const FrameParams F)
     {
-        ssao_body(F, xcd_tile(F));
+        ssao_body(F, xcd_tile(F, blockIdx.x));
     }
 
     // ------------------------------------------------------------------ shading + blend
@@ -1109,7 +1135,7 @@ namespace isaac_hip
 
     __global__ __launch_bounds__(256) void shade_kernel(const FrameParams F)
     {
-        shade_body(F, xcd_tile(F));
+        shade_body(F, xcd_tile(F, blockIdx.x));
     }
 
     // ------------------------------------------------------------------ precision form
@@ -2144,10 +2170,7 @@ namespace isaac_hip
                 vr = vg = vb = va = 0.0f;
             }
             texel[2 * k] = dither_b4(P.ditherMode, x + k, y, z, vr, vg, vb, va);
-            const uint32_t q0 = (uint32_t) roundf(gclamp(ir, 0.0f, 1.0f) * 255.0f);
-            const uint32_t q1 = (uint32_t) roundf(gclamp(ig, 0.0f, 1.0f) * 255.0f);
-            const uint32_t q2 = (uint32_t) roundf(gclamp(ib, 0.0f, 1.0f) * 255.0f);
-            const uint32_t q3 = (uint32_t) roundf(gclamp(ia, 0.0f, 1.0f) * 255.0f);
+            const uint32_t q0 = unorm8(ir), q1 = unorm8(ig), q2 = unorm8(ib), q3 = unorm8(ia);
             texel[2 * k + 1] = q0 | (q1 << 8) | (q2 << 16) | (q3 << 24);
         }
         uint4* o = reinterpret_cast<uint4*>(P.tex + u);
@@ -2501,7 +2524,8 @@ namespace isaac_hip
         const unsigned nb = march_grid(F);
         if(nb == 0)
             return hipSuccess;
-        const dim3 grid(nb), block(256);
+        // one-wave workgroups: 4 per logical workgroup, the logical grid rounded to 8
+        const dim3 grid(F.waveGroups == 1 ? (nb + 7u) / 8u * 32u : nb), block(F.waveGroups == 1 ? 64 : 256);
         // next-step prefetch is compiled for the linear layout only
         const bool prefetch = F.prefetch && !BRICK;
         if(!counting && !prefetch && (F.waves == 6 || F.waves == 8))

@@ -28,3 +28,16 @@ def test_general_division_identity_sampled(tmp_path):
     print(r.stdout)
     assert r.returncode == 0, r.stdout
     assert "mismatches 0" in r.stdout
+
+
+def test_rounding_shortcuts_exhaustive(tmp_path):
+    """Classification and the transfer-function lookups replace roundf by
+    x < 0.5 ? 0 : trunc(x + 0.5) (byte quantisation of [0, 1] x 255) and
+    x >= 0.5 ? trunc(min(x + 0.5, T - 1)) : 0 (transfer index): every binary32
+    input in the ranges where the two forms could differ."""
+    exe = tmp_path / "round_check"
+    subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-o", str(exe), os.path.join(HERE, "cpp", "round_check.c"), "-lm"])
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=600)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout
+    assert "mismatches 0;" in r.stdout and r.stdout.rstrip().endswith("mismatches 0")

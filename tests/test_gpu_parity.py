@@ -231,31 +231,33 @@ def test_production_full_size(gpu, oracle_lib, pose):
 
 @pytest.mark.parametrize("form", ["combined", "bricks-f16"])
 def test_schedules_do_not_change_the_frame(gpu, oracle_lib, form):
-    """Every march schedule (tile order over the XCDs, occupancy target, prefetch)
-    renders the same frame; the counting march executes the same steps in every
-    tile order."""
+    """Every march schedule (tile order over the XCDs, occupancy target, prefetch,
+    workgroups of 4 waves or of one wave) renders the same frame; the counting
+    march executes the same steps in every tile order and workgroup size."""
     spec = _reduced_prod(n=64, fb=(330, 250), particles=20000, visible_tf=True)
     vis = _vis(spec, gpu)
     if form != "combined":
         vis.set_render_form(vis.FORM_BRICKS, vis.F16)
     vis.update_buffers()
     ref = ref_counts = None
-    for tiles in (0, 2, 4, 8, 16):
-        vis.set_tuning(vis.TUNE_MARCH_TILES, tiles)
-        for waves, prefetch in ((0, 0), (6, 0), (8, 0), (0, 1)):
-            vis.set_tuning(vis.TUNE_MARCH_WAVES, waves)
-            vis.set_tuning(vis.TUNE_MARCH_PREFETCH, prefetch)
-            img = vis.draw()
-            if ref is None:
-                ref = img
-            assert np.array_equal(img, ref), (tiles, waves, prefetch)
-        vis.set_counting(True)
-        assert np.array_equal(vis.draw(), ref), tiles
-        c = {k: v for k, v in vis.counters().items() if k != "march_wave_steps"}
-        vis.set_counting(False)
-        if ref_counts is None:
-            ref_counts = c
-        assert c == ref_counts, tiles
+    for threads in ((256, 64) if form == "combined" else (256,)):
+        vis.set_tuning(vis.TUNE_MARCH_THREADS, threads)
+        for tiles in (0, 2, 4, 8, 16):
+            vis.set_tuning(vis.TUNE_MARCH_TILES, tiles)
+            for waves, prefetch in ((0, 0), (6, 0), (8, 0), (0, 1)):
+                vis.set_tuning(vis.TUNE_MARCH_WAVES, waves)
+                vis.set_tuning(vis.TUNE_MARCH_PREFETCH, prefetch)
+                img = vis.draw()
+                if ref is None:
+                    ref = img
+                assert np.array_equal(img, ref), (threads, tiles, waves, prefetch)
+            vis.set_counting(True)
+            assert np.array_equal(vis.draw(), ref), (threads, tiles)
+            c = {k: v for k, v in vis.counters().items() if k != "march_wave_steps"}
+            vis.set_counting(False)
+            if ref_counts is None:
+                ref_counts = c
+            assert c == ref_counts, (threads, tiles)
     vis.close()
 
 
@@ -312,11 +314,39 @@ def test_texture_layout_change_needs_update(gpu):
     vis.close()
 
 
+@pytest.mark.parametrize("pose", ["default", "fill"])
+def test_iso_hit_depth_matches_oracle(gpu, oracle_lib, pose):
+    """Iso-surface hit depths, interpolated between the hit step and the step
+    before it (lib/isaac/isaac_iso_kernel.hpp:84-92), against the oracle: the march
+    forms the previous step's iso sum only when the hit needs it, which must give
+    the same depths as forming it at every step (a wrong previous sum moves a hit
+    by up to one step, about 1e-3 of the depth)."""
+    from oracle import oracle
+
+    p = scenes.FILL_POSE if pose == "fill" else scenes.DEFAULT_POSE
+    spec = _reduced_prod(n=64, fb=(640, 480), particles=0, pose=p)
+    # iso threshold 0.1: a surface over a few thousand pixels at both poses
+    density = dataclasses.replace(spec.volumes[0], iso=0.1)
+    spec = dataclasses.replace(spec, particles=[], volumes=[density], ao_enabled=False)
+    vis, img = gpu_frame(spec, gpu)
+    ora = oracle.render(spec, 32)
+    _, depth, _, _ = vis.gbuffer()
+    vis.close()
+    both = (ora.depth < 3e38) & (depth < 3e38)
+    assert both.sum() > 3000, both.sum()
+    rel = np.abs(depth[both] - ora.depth[both]) / np.abs(ora.depth[both])
+    print(f"iso hits {both.sum()}: median rel depth diff {np.median(rel):.3g}, 99th pct {np.quantile(rel, 0.99):.3g}, "
+          f"max {rel.max():.3g}", flush=True)
+    assert (rel <= 1e-5).mean() >= 0.99
+    assert np.array_equal(depth >= 3e38, ora.depth >= 3e38) or ((depth >= 3e38) != (ora.depth >= 3e38)).mean() < 1e-3
+
+
 def test_tuning_rejects_unknown_values(gpu):
     from isaac_amd._lib import IsaacHipError
 
     vis = _vis(_reduced_prod(n=16, fb=(32, 32), particles=100), gpu)
-    for key, bad in ((vis.TUNE_MARCH_TILES, 3), (vis.TUNE_MARCH_TILES, 32), (vis.TUNE_MARCH_WAVES, 5), (99, 0)):
+    for key, bad in ((vis.TUNE_MARCH_TILES, 3), (vis.TUNE_MARCH_TILES, 32), (vis.TUNE_MARCH_WAVES, 5),
+                     (vis.TUNE_MARCH_THREADS, 128), (99, 0)):
         with pytest.raises(IsaacHipError):
             vis.set_tuning(key, bad)
     for tiles in (0, 2, 4, 8, 16):
