@@ -267,13 +267,15 @@ namespace isaac_hip
         return a > hi ? hi : a;
     }
 
-    // (uint32_t) roundf(gclamp(v, 0, 1) * 255): round half away from zero of a value
-    // x in [0, 255] as x < 0.5 ? 0 : trunc(x + 0.5) -- x + 0.5 is exact or rounds
-    // without crossing an integer for every x >= 0.5 (all binary32 v checked,
-    // tests/cpp/round_check.c); a NaN v gives the conversion of NaN in both forms
-    __host__ __device__ inline uint32_t unorm8(float v)
+    // (uint32_t) roundf(gclamp(v, 0, 1) * 255): the clamp is one median-of-three, and
+    // round half away from zero of x in [0, 255] is x < 0.5 ? 0 : trunc(x + 0.5) --
+    // x + 0.5 is exact or rounds without crossing an integer for every x >= 0.5 (all
+    // binary32 v in [0, 1] checked, tests/cpp/round_check.c).  A NaN v gives 0 here
+    // (the median of NaN, 0, 1 is the minimum of the other two) as the conversion of
+    // NaN gives 0 in the direct form.
+    __device__ inline uint32_t unorm8(float v)
     {
-        const float x = gclamp(v, 0.0f, 1.0f) * 255.0f;
+        const float x = __builtin_amdgcn_fmed3f(v, 0.0f, 1.0f) * 255.0f;
         return x < 0.5f ? 0u : (uint32_t) (x + 0.5f);
     }
 
@@ -283,7 +285,8 @@ namespace isaac_hip
     // the infinities checked for T = 1024, tests/cpp/round_check.c)
     __host__ __device__ inline int tf_index_of(float x, int T)
     {
-        return x >= 0.5f ? (int) gmin(x + 0.5f, float(T - 1)) : 0;
+        // x + 0.5 is not NaN on this branch, so fminf is gmin
+        return x >= 0.5f ? (int) fminf(x + 0.5f, float(T - 1)) : 0;
     }
 
     // transformColor float4 -> byte4 (truncating, lib/isaac/isaac_helper.hpp:71-74)

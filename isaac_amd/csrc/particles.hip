@@ -299,36 +299,54 @@ namespace isaac_hip
     // one attribute read and one bit.
     //
     // step 1, per particle: the TF index of its functor-applied attribute; each wave
-    // stores its 64 visibility bits (one word); a visible particle counts itself in its
-    // cell (a returning add: its arrival rank) and keeps cell, rank and record at its index
+    // takes 4 x 64 consecutive particles (all attribute reads, then all transfer
+    // lookups in flight together) and stores 4 words of 64 visibility bits; a visible
+    // particle counts itself in its cell (a returning add: its arrival rank) and keeps
+    // cell, rank and record at its index
+    constexpr int kVisPerLane = 4;
     __global__ __launch_bounds__(256) void vis_bin_kernel(const BinParams P, const BinBuffers B)
     {
         uint32_t* cnt = reinterpret_cast<uint32_t*>(B.count64);
         const uint32_t lane = threadIdx.x & 63u;
-        for(int64_t base = int64_t(blockIdx.x) * blockDim.x; base < P.n; base += int64_t(gridDim.x) * blockDim.x)
+        const int64_t waves = int64_t(gridDim.x) * (blockDim.x >> 6);
+        for(int64_t w = int64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); w * 64 * kVisPerLane < P.n; w += waves)
         {
-            const int64_t i = base + threadIdx.x;
-            bool visible = false;
-            int li = 0;
-            if(i < P.n)
+            const int64_t base = w * 64 * kVisPerLane;
+            float v[kVisPerLane];
+#pragma unroll
+            for(int k = 0; k < kVisPerLane; k++)
             {
-                const float v = apply_chain_p(P.chain, P.attr + i * P.dim, P.dim);
-                li = tf_index_of(v * float(P.T), P.T);
-                visible = !P.filter || P.lut[li].w >= 0.5f;
+                const int64_t i = base + k * 64 + lane;
+                v[k] = i < P.n ? apply_chain_p(P.chain, P.attr + i * P.dim, P.dim) : 0.0f;
             }
-            const unsigned long long mask = __ballot(visible);
-            if(lane == 0u && i < P.n)
-                B.visMask[i >> 6] = mask;
-            if(visible)
+            int li[kVisPerLane];
+            bool visible[kVisPerLane];
+#pragma unroll
+            for(int k = 0; k < kVisPerLane; k++)
             {
-                float qx, qy, qz;
-                const int cx = cell_coord(P.pos[3 * i + 0], P.cw[0], P.grid[0], qx);
-                const int cy = cell_coord(P.pos[3 * i + 1], P.cw[1], P.grid[1], qy);
-                const int cz = cell_coord(P.pos[3 * i + 2], P.cw[2], P.grid[2], qz);
-                const int c = cx + P.grid[0] * (cy + P.grid[1] * cz);
-                B.cellOf[i] = c;
-                B.rank[i] = atomicAdd(cnt + c, 1u);
-                B.rel[i] = make_float4(qx - float(cx), qy - float(cy), qz - float(cz), __int_as_float(li));
+                const int64_t i = base + k * 64 + lane;
+                li[k] = tf_index_of(v[k] * float(P.T), P.T);
+                visible[k] = i < P.n && (!P.filter || P.lut[li[k]].w >= 0.5f);
+            }
+#pragma unroll
+            for(int k = 0; k < kVisPerLane; k++)
+            {
+                const int64_t i0 = base + k * 64;
+                const unsigned long long mask = __ballot(visible[k]);
+                if(lane == 0u && i0 < P.n)
+                    B.visMask[i0 >> 6] = mask;
+                if(visible[k])
+                {
+                    const int64_t i = i0 + lane;
+                    float qx, qy, qz;
+                    const int cx = cell_coord(P.pos[3 * i + 0], P.cw[0], P.grid[0], qx);
+                    const int cy = cell_coord(P.pos[3 * i + 1], P.cw[1], P.grid[1], qy);
+                    const int cz = cell_coord(P.pos[3 * i + 2], P.cw[2], P.grid[2], qz);
+                    const int c = cx + P.grid[0] * (cy + P.grid[1] * cz);
+                    B.cellOf[i] = c;
+                    B.rank[i] = atomicAdd(cnt + c, 1u);
+                    B.rel[i] = make_float4(qx - float(cx), qy - float(cy), qz - float(cz), __int_as_float(li[k]));
+                }
             }
         }
     }
@@ -435,8 +453,12 @@ namespace isaac_hip
         int64_t pb = (P.n + 255) / 256;
         if(pb > 8192)
             pb = 8192;
+        // vis_bin: 4 waves per workgroup, 4 x 64 particles per wave
+        int64_t vb = (P.n + 256 * kVisPerLane - 1) / (256 * kVisPerLane);
+        if(vb > 8192)
+            vb = 8192;
         if(P.n > 0)
-            hipLaunchKernelGGL(vis_bin_kernel, dim3(unsigned(pb)), dim3(256), 0, s, P, B);
+            hipLaunchKernelGGL(vis_bin_kernel, dim3(unsigned(vb)), dim3(256), 0, s, P, B);
         const int nb = int((ncells + kScanBlock - 1) / kScanBlock);
         hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(kScanBlock), 0, s, B.count64, ncells, B.scanTmp);
         hipLaunchKernelGGL(scan_blocksums_kernel, dim3(1), dim3(kScanBlock), 0, s, B.scanTmp, nb);

@@ -35,7 +35,7 @@ static int tf_ref(float x, int T)
 
 static int tf_new(float x, int T)
 {
-    return x >= 0.5f ? (int) gmin(x + 0.5f, (float) (T - 1)) : 0;
+    return x >= 0.5f ? (int) fminf(x + 0.5f, (float) (T - 1)) : 0;
 }
 
 static uint64_t check_tf(float x, int T, uint64_t* n)
