@@ -332,9 +332,15 @@ extern "C"
         b->nviews = nviews;
         // the source signature selects a march compiled for it when every frame shares it
         int sig = post_signature(b->hQ[0]);
-        for(int f = 1; f < nframes; f++)
+        bool natural = true;
+        for(int f = 0; f < nframes; f++)
+        {
             if(post_signature(b->hQ[f]) != sig)
                 sig = 0;
+            natural = natural && natural_chains(b->hQ[f]);
+        }
+        if(sig > 0 && natural)
+            sig += kSigNatural;
         const size_t lutLds = size_t(b->hQ[0].nsrc) * size_t(b->hQ[0].T) * sizeof(float4);
         // launch arguments the captured graph holds
         if(c0->brickDtype != b->dtype || c0->interpolation != b->linear || sig != b->sig || lutLds != b->lutLds)

@@ -19,6 +19,10 @@ namespace isaac_hip
     constexpr int kMaxFunctors = 3;
     constexpr int kMaxClipping = 16;
     constexpr float kZNear = 0.002f; // ISAAC_Z_NEAR lib/isaac/isaac_defines.hpp:53
+    // added to a source signature of a compiled march: every source's functor chain is
+    // its natural one (none for a scalar source, one `length` for a vector source),
+    // evaluated inline in the interpreter's operation order instead of interpreted
+    constexpr int kSigNatural = 1000;
     constexpr int kTexBrick = 8; // side of a texture brick (texture layout 8)
     constexpr int kTexBrickRow = kTexBrick + 1, kTexBrickSlice = kTexBrickRow * kTexBrickRow;
     constexpr int kTexBrickTexels = kTexBrickSlice * kTexBrickRow; // 729

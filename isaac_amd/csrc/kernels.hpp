@@ -43,6 +43,7 @@ namespace isaac_hip
     hipError_t launch_batch_march(const BatchParams& B, int dtype, bool linear, int sig, size_t lds, int grid, hipStream_t s);
     // component counts of the enabled sources as base-5 digits, slot 0 lowest (-1: no source)
     int post_signature(const PostParams& Q);
+    bool natural_chains(const PostParams& Q);
     hipError_t launch_batch_ssao(const FrameParams* frames, int nframes, int maxTiles, hipStream_t s);
     hipError_t launch_batch_shade(const FrameParams* frames, int nframes, int maxTiles, hipStream_t s);
     hipError_t launch_brick_march(const FrameParams& F, const PostParams& Q, const ParticleParams& PP, bool counting, hipStream_t s);

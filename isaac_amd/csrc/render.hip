@@ -1448,19 +1448,40 @@ namespace isaac_hip
     // order, as base-5 digits (slot 0 lowest), 0 = resolved at run time.  A
     // march compiled for one signature fetches the rows of all its sources
     // before interpolating any of them, so their latencies overlap.
+    constexpr bool sig_natural(int sig)
+    {
+        return sig >= kSigNatural;
+    }
+
     constexpr int sig_count(int sig)
     {
         int n = 0;
-        for(; sig > 0; sig /= 5)
+        for(sig %= kSigNatural; sig > 0; sig /= 5)
             n++;
         return n;
     }
 
     constexpr int sig_dim(int sig, int slot)
     {
+        sig %= kSigNatural;
         for(int i = 0; i < slot; i++)
             sig /= 5;
         return sig % 5;
+    }
+
+    // the natural chain's value of a D-component sample: the value itself, or the length
+    // with apply_chain's association ((v0 v0 + v1 v1) + v2 v2)
+    template<int D>
+    __device__ inline float natural_value(const float (&v)[4])
+    {
+        if constexpr(D == 1)
+            return v[0];
+        else if constexpr(D == 2)
+            return sqrtf(v[0] * v[0] + v[1] * v[1]);
+        else if constexpr(D == 3)
+            return sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        else
+            return sqrtf((v[0] * v[0] + v[1] * v[1]) + (v[2] * v[2] + v[3] * v[3]));
     }
 
     template<int DT, bool LINEAR, int SIG, int S>
@@ -1512,7 +1533,7 @@ namespace isaac_hip
             constexpr int D = sig_dim(SIG, S);
             float v[4] = {0.f, 0.f, 0.f, 0.f};
             slot_interp<DT, LINEAR, D>(R[S], t, v);
-            const float tv = apply_chain(Q.src[S].chain, v[0], v[1], v[2], v[3], D);
+            const float tv = sig_natural(SIG) ? natural_value<D>(v) : apply_chain(Q.src[S].chain, v[0], v[1], v[2], v[3], D);
             post_accumulate(o, Q.src[S], lut[S * Q.T + tf_index(tv, Q.T)]);
             sig_classify<DT, LINEAR, SIG, S + 1>(Q, lut, t, R, o);
         }
@@ -1859,7 +1880,7 @@ namespace isaac_hip
         void* out)
     {
         __shared__ float reg[729 * DIM];
-        __shared__ float sc[DIM];
+        __shared__ float sc[DIM], rc[DIM];
         const int64_t brick = blockIdx.x;
         const int Bx = int(brick % nbx), By = int((brick / nbx) % nby), Bz = int(brick / (int64_t(nbx) * nby));
         for(int i = threadIdx.x; i < 729 * DIM; i += 256)
@@ -1883,6 +1904,9 @@ namespace isaac_hip
             if(lane == 0)
             {
                 sc[wave] = s;
+                // RN(1/s) for the quotients, 0 where s is outside [2^-60, 2^60] (there
+                // the quotients take the IEEE division)
+                rc[wave] = (s >= 0x1p-60f && s <= 0x1p+60f) ? 1.0f / s : 0.0f;
                 scale[brick * DIM + wave] = s;
             }
         }
@@ -1897,8 +1921,16 @@ namespace isaac_hip
                 reinterpret_cast<float2*>(out)[i] = make_float2(v0, v1);
             else
             {
-                const float s = sc[c];
-                const float q0 = v0 / s, q1 = v1 / s;
+                // v / s as x*r plus one fma residual step (div_by), with v's sign (the
+                // residual step turns -0 into +0): the IEEE quotient for |v| <= s with s in
+                // [2^-60, 2^60]; for BF16, whose range keeps tiny quotients, only where
+                // |v| >= 2^-60 (the residual stays normal)
+                const float s = sc[c], r = rc[c];
+                auto quot = [&](float v) {
+                    const bool fast = r != 0.0f && (DT == kDtF16 || fabsf(v) >= 0x1p-60f);
+                    return fast ? copysignf(div_by(v, s, r), v) : v / s;
+                };
+                const float q0 = quot(v0), q1 = quot(v1);
                 uint32_t w;
                 if(DT == kDtF16)
                     w = uint32_t(__half_as_ushort(__float2half_rn(q0))) | (uint32_t(__half_as_ushort(__float2half_rn(q1))) << 16);
@@ -1964,6 +1996,27 @@ namespace isaac_hip
         return (ns + 7u) / 8u * 8u * S * S;
     }
 
+    // a source's chain is its natural one: no operations for a scalar source, one
+    // `length` (then nothing) for a vector source
+    bool natural_chain(const ChainDesc& c, int dim)
+    {
+        for(int k = 0; k < kMaxFunctors; k++)
+        {
+            const int want = (k == 0 && dim > 1) ? OP_LENGTH : OP_IDEM;
+            if(c.op[k] != want)
+                return false;
+        }
+        return true;
+    }
+
+    bool natural_chains(const PostParams& Q)
+    {
+        for(int i = 0; i < Q.nsrc; i++)
+            if(!natural_chain(Q.src[i].chain, Q.src[i].dim))
+                return false;
+        return true;
+    }
+
     int post_signature(const PostParams& Q)
     {
         int sig = 0;
@@ -1984,10 +2037,16 @@ namespace isaac_hip
             // signatures with a compiled march: density (1), vector (3), and pairs of them
             const int sig = post_signature(Q);
             const int v = Q.dtype * 8 + (sig == 1 ? 0 : sig == 3 ? 1 : sig == 6 ? 2 : sig == 8 ? 3 : sig == 16 ? 4 : 7);
+            const bool natural = natural_chains(Q);
             switch(v)
             {
 #define ISAAC_SIG_CASE(dt, k, sg)                                                                                       \
-    case dt * 8 + k: hipLaunchKernelGGL((brick_march_sig_kernel<dt, sg>), grid, block, lds, s, F, Q, PP); return hipGetLastError();
+    case dt * 8 + k:                                                                                                    \
+        if(natural)                                                                                                     \
+            hipLaunchKernelGGL((brick_march_sig_kernel<dt, sg + kSigNatural>), grid, block, lds, s, F, Q, PP);          \
+        else                                                                                                            \
+            hipLaunchKernelGGL((brick_march_sig_kernel<dt, sg>), grid, block, lds, s, F, Q, PP);                        \
+        return hipGetLastError();
                 ISAAC_SIG_CASE(0, 0, 1)
                 ISAAC_SIG_CASE(0, 1, 3)
                 ISAAC_SIG_CASE(0, 2, 6)
@@ -2030,13 +2089,20 @@ namespace isaac_hip
     {
         if(B.nframes <= 0 || grid <= 0)
             return hipSuccess;
+        const bool natural = sig_natural(sig);
+        sig %= kSigNatural;
         if(linear && (sig == 1 || sig == 3 || sig == 16))
         {
             const int k = dtype * 3 + (sig == 1 ? 0 : sig == 3 ? 1 : 2);
             switch(k)
             {
 #define ISAAC_SIG_CASE(dt, j, sg)                                                                                       \
-    case dt * 3 + j: hipLaunchKernelGGL((batch_march_kernel<dt, true, sg>), dim3(grid), dim3(256), lds, s, B); return hipGetLastError();
+    case dt * 3 + j:                                                                                                    \
+        if(natural)                                                                                                     \
+            hipLaunchKernelGGL((batch_march_kernel<dt, true, sg + kSigNatural>), dim3(grid), dim3(256), lds, s, B);     \
+        else                                                                                                            \
+            hipLaunchKernelGGL((batch_march_kernel<dt, true, sg>), dim3(grid), dim3(256), lds, s, B);                   \
+        return hipGetLastError();
                 ISAAC_SIG_CASE(0, 0, 1)
                 ISAAC_SIG_CASE(0, 1, 3)
                 ISAAC_SIG_CASE(0, 2, 16)
@@ -2225,6 +2291,7 @@ namespace isaac_hip
                     f[s][j] = g[s][j];
         }
     }
+
 
     // signature of the active sources for classify_sig_kernel, 0 when the generic kernel applies;
     // Pc receives the active sources in slot order

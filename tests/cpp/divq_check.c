@@ -1,11 +1,12 @@
-/* Check of the general division used by the gfx950 classification kernel:
- * for binary32 x >= 0 and d > 0 with r = RN(1/d),
- *     q0 = RN(x * r); e = fma(-q0, d, x); q = fma(e, r, q0)
- * equals RN(x / d).  Divisors: 64 with extreme mantissas (all ones, lowest bit,
- * half) over 16 binades, the rest log-uniform in [1e-3, 1e3]; per divisor,
- * seeded x uniform in [0, d], uniform in [0, 1], and x = RN(q * d) for q within
- * 2 ulps of a random float in [0, 1] (inputs whose quotient sits next to a
- * rounding boundary).  Prints the mismatch count; exit status 0 when it is 0. */
+/* Check of the general division used by the gfx950 classification and brick-build
+ * kernels: for binary32 x and d > 0 with r = RN(1/d),
+ *     q0 = RN(x * r); e = fma(-q0, d, x); q = copysign(fma(e, r, q0), x)
+ * equals RN(x / d) bit for bit (for x >= 0 the copysign changes nothing).  Divisors:
+ * 64 with extreme mantissas (all ones, lowest bit, half) over 16 binades, the rest
+ * log-uniform in [1e-3, 1e3]; per divisor, seeded x uniform in [0, d], uniform in
+ * [0, 1], and x = RN(q * d) for q within 2 ulps of a random float in [0, 1] (inputs
+ * whose quotient sits next to a rounding boundary), every other one negated, and
+ * x = +0 and -0.  Prints the mismatch count; exit status 0 when it is 0. */
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -70,9 +71,13 @@ int main(int argc, char** argv)
                 q = f_of(u_of(q) + (uint32_t) (next(&s) % 5) - 2u);
                 x = q * d;
             }
+            if(i & 1)
+                x = -x;
+            if(i < 2)
+                x = i ? -0.0f : 0.0f;
             const float q0 = x * r;
             const float e = fmaf(-q0, d, x);
-            const float q = fmaf(e, r, q0);
+            const float q = copysignf(fmaf(e, r, q0), x);
             const float ref = x / d;
             if(u_of(q) != u_of(ref))
             {

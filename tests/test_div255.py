@@ -19,9 +19,11 @@ def test_div255_identity_exhaustive(tmp_path):
 
 def test_general_division_identity_sampled(tmp_path):
     """The classification kernel divides by per-source constants (iso thresholds,
-    the sum of source weights) with the same one-correction scheme and r = RN(1/d):
-    2e8 seeded (x, d) pairs, d over [1e-3, 1e3] and extreme mantissas, a quarter
-    of them next to rounding boundaries, all equal to IEEE x / d."""
+    the sum of source weights), and the brick build by each brick's scale, with the
+    same one-correction scheme and r = RN(1/d) (the sign of x restored for -0):
+    2e8 seeded (x, d) pairs, d over [1e-3, 1e3] and extreme mantissas, x of both
+    signs and +-0, a quarter of them next to rounding boundaries, all equal to IEEE
+    x / d bit for bit."""
     exe = tmp_path / "divq_check"
     subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-o", str(exe), os.path.join(HERE, "cpp", "divq_check.c"), "-lm"])
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=600)

@@ -140,6 +140,10 @@ def test_reduced_production_bricks(gpu, oracle_lib, dtype, pose):
     assert (np.abs(depth[both] - ora.depth[both]) <= 1e-3 * np.maximum(1.0, np.abs(ora.depth[both]))).mean() >= FRACTION
     # iso edges flip between precisions; the image stays above the floor
     assert p >= 30.0
+    # the march compiled for the source signature (natural chains inlined) renders the
+    # counting march's frame bit for bit
+    vis.set_counting(False)
+    assert np.array_equal(vis.draw(), img)
     vis.close()
 
 
@@ -158,6 +162,9 @@ def test_bricks_nonpersistent_and_chains(gpu, oracle_lib):
     ora = oracle.render_post(spec, 32, F32)
     ora64 = oracle.render_post(spec, 64, oracle.BRICK_RAW)
     compare_post("chains", spec, img, fimg, vis, F32, ora, ora64)
+    # the signature-compiled march with the chains interpreted per sample: the same frame
+    vis.set_counting(False)
+    assert np.array_equal(vis.draw(), img)
     vis.close()
 
 
