@@ -5,7 +5,7 @@
 //                          the march of CombinedVolumeRenderKernel, fused per pixel
 //                          lib/isaac/isaac_common_kernel.hpp:195-217, lib/isaac/isaac_particle_kernel.hpp:117-260,
 //                          lib/isaac/isaac_iso_kernel.hpp:25-128, lib/isaac/isaac_volume_kernel.hpp:23-82
-//   ssao_kernel            SSAOKernel (live 7x7 part)     lib/isaac/isaac_ssao_kernel.hpp:161-182
+//   ssao_tiled_kernel      SSAOKernel (live 7x7 part)     lib/isaac/isaac_ssao_kernel.hpp:161-182
 //   shade_kernel           ShadingKernel + the volume pass's final blend
 //                          lib/isaac/isaac_common_kernel.hpp:220-320, lib/isaac/isaac_volume_kernel.hpp:83-86
 //
@@ -15,8 +15,8 @@
 //
 // Launch shape: 256-thread workgroups = four wave64s, each wave an 8x8 pixel
 // tile (16x16 pixels per workgroup), so the 64 rays of a wave sample
-// neighbouring voxels.  Workgroups are remapped so that each of the 8 XCDs
-// (each with its own L2) renders a contiguous band of tiles.
+// neighbouring voxels; the march deals its tiles to the 8 XCDs (each with its
+// own L2) in super-tiles or bands (march_tile), and can run one-wave workgroups.
 #include "kernels.hpp"
 
 namespace isaac_hip
@@ -1035,9 +1035,53 @@ namespace isaac_hip
         F.ao[size_t(py) * F.W + px] = gclamp(occ / 42.0f, 0.0f, 1.0f);
     }
 
-    __global__ __launch_bounds__(256) void ssao_kernel(const FrameParams F)
+    // ssao_body's count with the depth taps from LDS: a workgroup takes a 64x32 block
+    // of the viewport and stages the 124x92 depth region around it (the 30-pixel reach
+    // of the 7x7 taps at a stride of 10), each region texel read at its framebuffer-
+    // clamped position -- so region[px + 10 i - x0 + 30] is the clamped tap itself and
+    // the 49 taps are LDS reads at fixed offsets.  The count is exact (at most 49
+    // increments of 1), so the occlusion equals ssao_body's.
+    constexpr int kSsaoBW = 64, kSsaoBH = 32, kSsaoReach = 30;
+    constexpr int kSsaoRW = kSsaoBW + 2 * kSsaoReach, kSsaoRH = kSsaoBH + 2 * kSsaoReach; // 124 x 92
+    __global__ __launch_bounds__(256) void ssao_tiled_kernel(const FrameParams F)
     {
-        ssao_body(F, xcd_tile(F, blockIdx.x));
+        __shared__ float region[kSsaoRW * kSsaoRH];
+        const int nbX = (F.vx1 - F.vx0 + kSsaoBW - 1) / kSsaoBW;
+        const int x0 = F.vx0 + int(blockIdx.x % uint32_t(nbX)) * kSsaoBW;
+        const int y0 = F.vy0 + int(blockIdx.x / uint32_t(nbX)) * kSsaoBH;
+        // staging: thread t loads column t % 124 of rows t / 124 + 2 k (248 threads)
+        const int col = int(threadIdx.x) % kSsaoRW, r0 = int(threadIdx.x) / kSsaoRW;
+        if(r0 < 2)
+        {
+            const int gx = iclamp(x0 - kSsaoReach + col, 0, F.W - 1);
+            const float* src = F.depth + gx;
+#pragma unroll 4
+            for(int r = r0; r < kSsaoRH; r += 2)
+            {
+                const int gy = iclamp(y0 - kSsaoReach + r, 0, F.H - 1);
+                region[r * kSsaoRW + col] = src[size_t(gy) * F.W];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for(int k = 0; k < kSsaoBW * kSsaoBH / 256; k++)
+        {
+            const int q = int(threadIdx.x) + 256 * k;
+            const int px = x0 + (q & (kSsaoBW - 1)), py = y0 + q / kSsaoBW;
+            if(px >= F.vx1 || py >= F.vy1)
+                continue;
+            // the tap (-3, -3) of this pixel; every tap is a fixed offset from it
+            const float* t0 = region + (py - y0) * kSsaoRW + (px - x0);
+            const float ref = t0[kSsaoReach * kSsaoRW + kSsaoReach];
+            float occ = 0.0f;
+#pragma unroll
+            for(int i = 0; i < 7; ++i)
+#pragma unroll
+                for(int j = 0; j < 7; ++j)
+                    if(t0[(j * 10) * kSsaoRW + i * 10] < ref)
+                        occ = occ + 1.0f;
+            F.ao[size_t(py) * F.W + px] = gclamp(occ / 42.0f, 0.0f, 1.0f);
+        }
     }
 
     // ------------------------------------------------------------------ shading + blend
@@ -1881,14 +1925,24 @@ namespace isaac_hip
     {
         __shared__ float reg[729 * DIM];
         __shared__ float sc[DIM], rc[DIM];
+        __shared__ int64_t rowOff[81]; // element offset of region row (ly, lz), x = 0
         const int64_t brick = blockIdx.x;
         const int Bx = int(brick % nbx), By = int((brick / nbx) % nby), Bz = int(brick / (int64_t(nbx) * nby));
+        if(threadIdx.x < 81)
+        {
+            const int lz = int(threadIdx.x) / 9, ly = int(threadIdx.x) - lz * 9;
+            const int y = min(By * 8 + ly, Y - 1), z = min(Bz * 8 + lz, Z - 1);
+            rowOff[threadIdx.x] = (int64_t(y) * sy + int64_t(z) * sz) * DIM;
+        }
+        __syncthreads();
+        const int x0 = Bx * 8, xLast = X - 1 - x0; // last local x inside the box
+        const int64_t sxD = sx * DIM;
         for(int i = threadIdx.x; i < 729 * DIM; i += 256)
         {
             const int vox = i / DIM, c = i - vox * DIM;
-            const int lz = vox / 81, ly = (vox / 9) % 9, lx = vox % 9;
-            const int x = min(Bx * 8 + lx, X - 1), y = min(By * 8 + ly, Y - 1), z = min(Bz * 8 + lz, Z - 1);
-            reg[i] = src[(int64_t(x) * sx + int64_t(y) * sy + int64_t(z) * sz) * DIM + c];
+            const int row = vox / 9, lx = vox - row * 9;
+            const int x = x0 + min(lx, xLast);
+            reg[i] = src[rowOff[row] + int64_t(x) * sxD + c];
         }
         __syncthreads();
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -2679,10 +2733,11 @@ namespace isaac_hip
 
     hipError_t launch_ssao(const FrameParams& F, hipStream_t s)
     {
-        const unsigned nb = unsigned(F.tilesX) * unsigned(F.tilesY);
-        if(nb == 0)
+        const unsigned bw = unsigned(F.vx1 - F.vx0 + kSsaoBW - 1) / kSsaoBW;
+        const unsigned bh = unsigned(F.vy1 - F.vy0 + kSsaoBH - 1) / kSsaoBH;
+        if(bw * bh == 0)
             return hipSuccess;
-        hipLaunchKernelGGL(ssao_kernel, dim3(nb), dim3(256), 0, s, F);
+        hipLaunchKernelGGL(ssao_tiled_kernel, dim3(bw * bh), dim3(256), 0, s, F);
         return hipGetLastError();
     }
 
