@@ -514,7 +514,9 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--prefetch", type=int, default=0, help="march kernel: fetch step i+1 before compositing step i")
     ap.add_argument("--march-waves", type=int, default=0, help="march occupancy target (0, 6, 8)")
-    ap.add_argument("--no-tune", action="store_true", help="keep --prefetch / --march-waves instead of autotuning")
+    ap.add_argument("--march-threads", type=int, default=256, help="march workgroup: 256 (4 waves) or 64 (one wave)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep --prefetch / --march-waves / --march-threads instead of autotuning")
     ap.add_argument("--no-learned-ao", action="store_true", help="skip training / timing the learned occlusion pass")
     ap.add_argument("--ao-train-steps", type=int, default=1250, help="16 crops each: 20,000 triples")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sub-volumes x views workload")
@@ -559,6 +561,7 @@ def main() -> int:
     n_visible = int(np.count_nonzero(spec.particles[0].attr[:, 0] >= 0.995))
 
     vis.set_tuning(vis.TUNE_MARCH_WAVES, args.march_waves)
+    vis.set_tuning(vis.TUNE_MARCH_THREADS, args.march_threads)
     form, dt = FORMS[args.form]
     vis.set_render_form(form, dt)
     tuning = None

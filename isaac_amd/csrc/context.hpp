@@ -127,8 +127,8 @@ struct isaac_hip_ctx
     // min/max request: per-workgroup partials and one (min, max) per source
     isaac_hip::MinMaxPartial* mmPartials = nullptr;
     float2* mmResult = nullptr;
-    // events: 0 update start, 1 after source copies, 2 after binning, 3 after
-    // classification (or brick build); 4 draw start, 5 after the march, 6 after the
+    // events: 0 update start, 1 after source copies, 2 after classification (or brick
+    // build), 3 after the particle records; 4 draw start, 5 after the march, 6 after the
     // occlusion pass, 7 after shading, 8 after the framebuffer copy
     hipEvent_t ev[12] = {};
     bool evRecorded[2] = {false, false};

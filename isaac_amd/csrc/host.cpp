@@ -695,6 +695,44 @@ namespace isaac_hip
         }
         return 0;
     }
+    // the visible particles' records of every enabled particle source (launch_bin_visible)
+    int bin_particles(isaac_hip_ctx* c)
+    {
+        const int T = c->cfg.transfer_size;
+        const int nvf = c->nv + c->nf;
+        // particle cell lists
+        for(int s = 0; s < c->np; s++)
+        {
+            PartSlot& p = c->parts[s];
+            const int gi = nvf + s;
+            if(!p.set || c->weight[gi] == 0.0f)
+                continue;
+            int rc = ensure_part_buffers(c, p);
+            if(rc)
+                return rc;
+            BinParams P{};
+            P.pos = p.desc.position;
+            P.attr = p.desc.attribute;
+            P.radius = p.desc.radius;
+            P.dim = p.desc.feature_dim;
+            P.n = p.desc.count;
+            for(int i = 0; i < 3; i++)
+            {
+                P.grid[i] = int32_t(c->pgrid[i]);
+                P.cw[i] = float(c->local[i]) / float(c->pgrid[i]);
+            }
+            P.chain = c->chain[gi];
+            P.lut = c->lutDev + size_t(gi) * T;
+            P.T = T;
+            P.filter = 1;
+            HIPCHK(c, launch_bin_visible(P, p.B, c->stream));
+            p.last = P;
+            p.lastValid = true;
+            p.fullValid = false;
+        }
+        return 0;
+    }
+
 } // namespace isaac_hip
 
 extern "C"
@@ -1316,37 +1354,6 @@ extern "C"
             }
         }
         HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-        // particle cell lists
-        for(int s = 0; s < c->np; s++)
-        {
-            PartSlot& p = c->parts[s];
-            const int gi = nvf + s;
-            if(!p.set || c->weight[gi] == 0.0f)
-                continue;
-            int rc = ensure_part_buffers(c, p);
-            if(rc)
-                return rc;
-            BinParams P{};
-            P.pos = p.desc.position;
-            P.attr = p.desc.attribute;
-            P.radius = p.desc.radius;
-            P.dim = p.desc.feature_dim;
-            P.n = p.desc.count;
-            for(int i = 0; i < 3; i++)
-            {
-                P.grid[i] = int32_t(c->pgrid[i]);
-                P.cw[i] = float(c->local[i]) / float(c->pgrid[i]);
-            }
-            P.chain = c->chain[gi];
-            P.lut = c->lutDev + size_t(gi) * T;
-            P.T = T;
-            P.filter = 1;
-            HIPCHK(c, launch_bin_visible(P, p.B, c->stream));
-            p.last = P;
-            p.lastValid = true;
-            p.fullValid = false;
-        }
-        HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
         if(c->form == ISAAC_HIP_FORM_BRICKS)
         {
             // brick form: re-lay every enabled volume/field source (the raw
@@ -1389,6 +1396,11 @@ extern "C"
                                         v.brickScale, v.brick, c->stream));
                 v.bricked = true;
             }
+            // the particle records after the GPU-heavy brick build, so the host enqueues
+            // their short launches while the build runs
+            HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+            if(int rc = bin_particles(c))
+                return rc;
             HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
             c->evRecorded[0] = true;
             c->classified = true;
@@ -1474,6 +1486,11 @@ extern "C"
             }
             HIPCHK(c, launch_tex_bricks(c->tex, c->texBricks, int(c->local[0]), int(c->local[1]), int(c->local[2]), c->stream));
         }
+        // the particle records after the classification, so the host enqueues their short
+        // launches while the classification runs
+        HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+        if(int rc = bin_particles(c))
+            return rc;
         HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
         c->evRecorded[0] = true;
         c->classified = true;
@@ -1597,10 +1614,12 @@ extern "C"
         *out = isaac_hip_timings{};
         if(c->evRecorded[0])
         {
+            // update order: source copies (ev 0-1), classification or brick build (1-2),
+            // particle records (2-3)
             out->source_copy_us = elapsed_us(c->ev[0], c->ev[1]);
-            out->binning_us = elapsed_us(c->ev[1], c->ev[2]);
-            out->buffer_us = elapsed_us(c->ev[0], c->ev[2]);
-            out->optimization_buffer_us = elapsed_us(c->ev[2], c->ev[3]);
+            out->optimization_buffer_us = elapsed_us(c->ev[1], c->ev[2]);
+            out->binning_us = elapsed_us(c->ev[2], c->ev[3]);
+            out->buffer_us = out->source_copy_us + out->binning_us;
         }
         if(c->evRecorded[1])
         {

@@ -317,6 +317,17 @@ class IsaacVisualization:
         self.modelview = modelview(self.look_at, self.rotation, self.distance)
 
     def _push_state(self) -> None:
+        """Hand the steering members to the library; skipped when none changed since
+        the last push (these setters are the only writers of that state)."""
+        ao = self.ambient_occlusion
+        key = (tuple(float(w) for w in self.source_weight), tuple(float(t) for t in self.iso_threshold), float(self.step),
+               bool(self.interpolation), int(self.dither_mode), bool(ao["isEnabled"]), float(ao["weight"]),
+               int(self.render_mode))
+        if self.field_sources:
+            key += (float(self.advection_step), float(self.advection_weight), int(self.advection_seeding_period),
+                    int(self.advection_seeding_duration), int(self.seed_points))
+        if key == getattr(self, "_pushed", None):
+            return
         c = self._ctx
         L = lib()
         for i in range(self.combined_source_list_size):
@@ -333,6 +344,7 @@ class IsaacVisualization:
                                             int(self.advection_seeding_period), int(self.advection_seeding_duration)),
                   c, "set_advection")
             check(L.isaac_hip_set_seed_points(c, int(self.seed_points)), c, "set_seed_points")
+        self._pushed = key
 
     def _bind_sources(self, pointer=None) -> None:
         c = self._ctx
