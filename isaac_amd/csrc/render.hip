@@ -698,18 +698,62 @@ namespace isaac_hip
         float pdepth = FLT_MAX;
         float4 pcol = make_float4(0.f, 0.f, 0.f, 0.f);
         F3 pn = f3(0.f, 0.f, 0.f);
-        while(cell[0] < uint32_t(PP.grid[0]) && cell[1] < uint32_t(PP.grid[1]) && cell[2] < uint32_t(PP.grid[2])
-              && !hit && tested <= rayLength)
+        const uint32_t gx = uint32_t(PP.grid[0]), gy = uint32_t(PP.grid[1]), gz = uint32_t(PP.grid[2]);
+        auto inside = [&](const uint32_t (&c)[3]) { return c[0] < gx && c[1] < gy && c[2] < gz; };
+        auto cell_id = [&](const uint32_t (&c)[3]) { return int(c[0] + gx * (c[1] + gy * c[2])); };
+        // record ranges of the current cell, requested one cell ahead: the walk's next
+        // cell depends on the ray only, so the range of cell i + 1 is in flight while
+        // the spheres of cell i are tested
+        int2 kr[4] = {};
+        if(inside(cell))
+        {
+            const int cid = cell_id(cell);
+#pragma unroll
+            for(int s = 0; s < 4; s++)
+                if(s < PP.nsrc)
+                    kr[s] = PP.src[s].vrange[cid];
+        }
+        while(inside(cell) && !hit && tested <= rayLength)
         {
             if(COUNT)
                 nCells++;
-            const int cid = int(cell[0] + uint32_t(PP.grid[0]) * (cell[1] + uint32_t(PP.grid[1]) * cell[2]));
-            const F3 cf = f3(float(cell[0]), float(cell[1]), float(cell[2]));
-            for(int s = 0; s < PP.nsrc; s++)
+            // the walk's next cell (ties between axes go towards z)
+            uint32_t next[3] = {cell[0], cell[1], cell[2]};
+            float nextTested;
+            const bool ax0 = t[0] < t[1] && t[0] < t[2];
+            const bool ax1 = !ax0 && t[1] < t[0] && t[1] < t[2];
+            if(ax0)
             {
+                next[0] += uint32_t(ds[0]);
+                nextTested = t[0];
+            }
+            else if(ax1)
+            {
+                next[1] += uint32_t(ds[1]);
+                nextTested = t[1];
+            }
+            else
+            {
+                next[2] += uint32_t(ds[2]);
+                nextTested = t[2];
+            }
+            int2 krn[4] = {};
+            if(inside(next))
+            {
+                const int nid = cell_id(next);
+#pragma unroll
+                for(int s = 0; s < 4; s++)
+                    if(s < PP.nsrc)
+                        krn[s] = PP.src[s].vrange[nid];
+            }
+            const F3 cf = f3(float(cell[0]), float(cell[1]), float(cell[2]));
+#pragma unroll
+            for(int s = 0; s < 4; s++)
+            {
+                if(s >= PP.nsrc)
+                    break;
                 const ParticleSourceDev& S = PP.src[s];
-                const int2 kr = S.vrange[cid];
-                for(int k = kr.x; k < kr.y; k++)
+                for(int k = kr[s].x; k < kr[s].y; k++)
                 {
                     if(COUNT)
                         nTests++;
@@ -741,24 +785,19 @@ namespace isaac_hip
                     }
                 }
             }
-            if(t[0] < t[1] && t[0] < t[2])
-            {
-                cell[0] += uint32_t(ds[0]);
-                tested = t[0];
+            cell[0] = next[0];
+            cell[1] = next[1];
+            cell[2] = next[2];
+            tested = nextTested;
+            if(ax0)
                 t[0] = t[0] + dt[0];
-            }
-            else if(t[1] < t[0] && t[1] < t[2])
-            {
-                cell[1] += uint32_t(ds[1]);
-                tested = t[1];
+            else if(ax1)
                 t[1] = t[1] + dt[1];
-            }
             else
-            {
-                cell[2] += uint32_t(ds[2]);
-                tested = t[2];
                 t[2] = t[2] + dt[2];
-            }
+#pragma unroll
+            for(int s = 0; s < 4; s++)
+                kr[s] = krn[s];
         }
         if(hit)
         {
