@@ -860,13 +860,21 @@ namespace isaac_hip
                     int endVol = endIso;
                     const F3 stepVec = divv(muls(ray.dir, stepU), sc);
                     const F3 startU = divv(ray.start, sc);
-                    bool volOn = volBlend;
                     if(isoOn)
                         nIsoRays++;
                     float cr = 0.f, cg = 0.f, cb = 0.f, ca = 0.f;
+                    // The traversal state is kept in plain numbers rather than flags, so
+                    // the loop carries no lane masks: the iso test runs while i <= isoEnd and
+                    // the volume while i <= volEnd (either set below the first step to switch
+                    // it off); hitStep is the step of the iso hit (-1: none); oldValue is the
+                    // previous step's iso sum, -1 where it was not formed; blendW is the
+                    // volume colour's w marker (0: the volume pass blends, -1: it does not).
+                    const int off = -0x7fffffff;
+                    int isoEnd = isoOn ? endIso : off;
+                    int volEnd = volBlend ? endVol : off;
+                    int hitStep = -1;
                     float oldValue = 0.0f;
-                    bool oldKnown = true; // oldValue holds the previous step's iso sum (0 before the first step)
-                    bool isoHit = false, isoFirst = false;
+                    float blendW = volBlend ? 0.0f : -1.0f;
                     int i = startSteps;
                     int last = isoOn ? endIso : endVol;
                     if(i <= last)
@@ -895,7 +903,7 @@ namespace isaac_hip
                             const bool ahead = PREFETCH && i + 1 <= last;
                             if(ahead)
                                 at(i + 1, nxt);
-                            if(isoOn)
+                            if(i <= isoEnd)
                             {
                                 // the iso alpha in byte units; alpha >= 0.5 is a sum >= 127.5, which
                                 // needs an alpha byte >= 128 (the corner weights sum to 1 up to a few
@@ -908,12 +916,10 @@ namespace isaac_hip
                                 if(COUNT)
                                     nIso++;
                                 float tmpSum = oldValue;
-                                const bool tmpKnown = oldKnown;
-                                oldValue = sum;
-                                oldKnown = high || (alphas >> 24) == 0u;
+                                oldValue = (high || (alphas >> 24) == 0u) ? sum : -1.0f;
                                 if(sum >= 127.5f)
                                 {
-                                    if(!tmpKnown)
+                                    if(tmpSum < 0.0f)
                                     {
                                         Corners pc;
                                         at(i - 1, pc);
@@ -928,44 +934,42 @@ namespace isaac_hip
                                     else
                                         d = tt + stepU * (0.5f - tmp) / (value - tmp);
                                     // colour and gradient at the hit are sampled after the loop
-                                    isoHit = true;
-                                    isoFirst = first;
+                                    hitStep = i;
                                     depth = d;
                                     if(COUNT)
                                         nIsoHits++;
-                                    isoOn = false;
+                                    isoEnd = off;
                                     // the volume pass ends at min(clip end, iso depth)
                                     const float ve = gmin(ray.endDepth, d);
                                     if(!(ve > ray.startDepth))
-                                        volBlend = volOn = false;
-                                    else
-                                        endVol = int(floorf(ve / stepU));
+                                    {
+                                        volEnd = off;
+                                        blendW = -1.0f;
+                                    }
+                                    else if(volEnd != off)
+                                        volEnd = int(floorf(ve / stepU));
                                 }
                             }
-                            // eight zero volume texels give the sample (0,0,0,0), which leaves
-                            // the accumulated colour unchanged: skip its arithmetic
-                            if(volOn && i <= endVol)
+                            // (eight zero volume texels give the sample (0,0,0,0), which leaves
+                            // the accumulated colour unchanged, so no test for them: the branch
+                            // cost more than the arithmetic it skipped)
+                            if(i <= volEnd)
                             {
                                 if(COUNT)
                                     nVol++;
-                                if(corners_or<LINEAR, false>(cur) != 0u)
-                                {
-                                    const float vr = corner_sum<LINEAR, false>(cur, 0);
-                                    const float vg = corner_sum<LINEAR, false>(cur, 8);
-                                    const float vb = corner_sum<LINEAR, false>(cur, 16);
-                                    const float va = corner_sum<LINEAR, false>(cur, 24);
-                                    const float w = (1.0f - ca) * vscale;
-                                    cr = cr + w * vr;
-                                    cg = cg + w * vg;
-                                    cb = cb + w * vb;
-                                    ca = ca + w * va;
-                                    if(ca > 0.99f)
-                                        volOn = false;
-                                }
+                                const float vr = corner_sum<LINEAR, false>(cur, 0);
+                                const float vg = corner_sum<LINEAR, false>(cur, 8);
+                                const float vb = corner_sum<LINEAR, false>(cur, 16);
+                                const float va = corner_sum<LINEAR, false>(cur, 24);
+                                const float w = (1.0f - ca) * vscale;
+                                cr = cr + w * vr;
+                                cg = cg + w * vg;
+                                cb = cb + w * vb;
+                                ca = ca + w * va;
+                                if(ca > 0.99f)
+                                    volEnd = off;
                             }
-                            last = isoOn ? endIso : -0x7fffffff;
-                            if(volOn && endVol > last)
-                                last = endVol;
+                            last = isoEnd > volEnd ? isoEnd : volEnd;
                             if(i + 1 > last)
                                 break;
                             i++;
@@ -975,6 +979,8 @@ namespace isaac_hip
                                 at(i, cur);
                         }
                     }
+                    const bool isoHit = hitStep >= 0;
+                    const bool isoFirst = isoHit && ray.isClipped && hitStep == startSteps;
                     if(isoHit)
                     {
                         // hit colour and central-difference gradient of the iso alpha
@@ -997,7 +1003,7 @@ namespace isaac_hip
                         normal = f3(-g.x / gl, -g.y / gl, -g.z / gl);
                         color = to_byte4(hr, hg, hb, 1.0f);
                     }
-                    if(volBlend)
+                    if(blendW >= 0.0f)
                     {
                         nVolRays++;
                         vcol = make_float4(cr, cg, cb, ca);
