@@ -1731,25 +1731,27 @@ namespace isaac_hip
                     int endVol = endIso;
                     const F3 stepVec = divv(muls(ray.dir, stepU), sc);
                     const F3 startU = divv(ray.start, sc);
-                    bool volOn = volBlend;
                     if(isoOn)
                         nIsoRays++;
                     float cr = 0.f, cg = 0.f, cb = 0.f, ca = 0.f;
                     float oldValue = 0.0f;
-                    bool isoHit = false, isoFirst = false;
+                    // traversal state as step bounds, not flags (no lane masks carried by the
+                    // loop): the iso test runs while i <= isoEnd, the volume while i <= volEnd,
+                    // either set below every step to switch it off; hitStep is the hit's step
+                    const int off = -0x7fffffff;
+                    int isoEnd = isoOn ? endIso : off;
+                    int volEnd = volBlend ? endVol : off;
+                    int hitStep = -1;
                     for(int i = startSteps;; i++)
                     {
-                        int last = isoOn ? endIso : -0x7fffffff;
-                        if(volOn && endVol > last)
-                            last = endVol;
-                        if(i > last)
+                        if(i > (isoEnd > volEnd ? isoEnd : volEnd))
                             break;
                         if(COUNT)
                             nSteps++;
                         const float fi = float(i);
                         const PostSample sm = post_classify<DT, LINEAR, SIG>(
                             Q, lut, F.X, F.Y, F.Z, startU.x + stepVec.x * fi, startU.y + stepVec.y * fi, startU.z + stepVec.z * fi);
-                        if(isoOn)
+                        if(i <= isoEnd)
                         {
                             const float value = gmin(sm.ia, 1.0f);
                             if(COUNT)
@@ -1758,20 +1760,21 @@ namespace isaac_hip
                             oldValue = value;
                             if(value >= 0.5f)
                             {
-                                isoFirst = ray.isClipped && i == startSteps;
-                                depth = isoFirst ? ray.startDepth : fi * stepU + stepU * (0.5f - tmp) / (value - tmp);
-                                isoHit = true;
-                                isoOn = false;
+                                const bool first = ray.isClipped && i == startSteps;
+                                depth = first ? ray.startDepth : fi * stepU + stepU * (0.5f - tmp) / (value - tmp);
+                                hitStep = i;
+                                isoEnd = off;
                                 if(COUNT)
                                     nIsoHits++;
+                                // the volume pass ends at min(clip end, iso depth)
                                 const double ve = fmin(rd.endDepth, double(depth));
                                 if(!(ve > rd.startDepth))
-                                    volBlend = volOn = false;
-                                else
-                                    endVol = int(floor(ve / stepUd));
+                                    volEnd = off;
+                                else if(volEnd != off)
+                                    volEnd = int(floor(ve / stepUd));
                             }
                         }
-                        if(volOn && i <= endVol)
+                        if(i <= volEnd)
                         {
                             if(COUNT)
                                 nVol++;
@@ -1781,9 +1784,14 @@ namespace isaac_hip
                             cb = cb + w * (sm.b * factor);
                             ca = ca + w * (sm.a * factor);
                             if(ca > 0.99f)
-                                volOn = false;
+                                volEnd = off;
                         }
                     }
+                    const bool isoHit = hitStep >= 0;
+                    const bool isoFirst = isoHit && ray.isClipped && hitStep == startSteps;
+                    // a hit at or before the ray's start leaves no volume segment to blend
+                    if(isoHit && !(fmin(rd.endDepth, double(depth)) > rd.startDepth))
+                        volBlend = false;
                     if(isoHit)
                     {
                         const F3 pu = divv(add(ray.start, muls(ray.dir, depth)), sc);
