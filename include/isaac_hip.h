@@ -253,8 +253,11 @@ extern "C"
                                           update_buffers after classification (the linear texture stays what
                                           isaac_hip_get_texture returns); a change needs update_buffers before
                                           the next render; next-step prefetch applies to the linear layout only */
-        ISAAC_HIP_TUNE_MARCH_THREADS = 4 /* combined-form march workgroup: 256 (default; 4 waves render one 16x16
+        ISAAC_HIP_TUNE_MARCH_THREADS = 4, /* combined-form march workgroup: 256 (default; 4 waves render one 16x16
                                             tile) or 64 (one wave per workgroup, one 8x8 quarter of a tile) */
+        ISAAC_HIP_TUNE_MARCH_OFFSETS = 5 /* combined-form trilinear march loads: 32 (default) a 32-bit byte offset
+                                            from the texture's base where the texture spans < 4 GiB (larger ones
+                                            always take 64-bit addresses); 64: a 64-bit address per lane */
     };
     int isaac_hip_set_tuning(isaac_hip_ctx* ctx, int32_t key, int32_t value);
 

@@ -105,6 +105,7 @@ namespace isaac_hip
         // clamped texel), or nullptr: every trilinear sample reads one brick
         const uint2* texB;
         uint32_t nbx, nbxy; // bricks per row and per slice of the brick layout
+        int32_t texOff32; // the layout the march reads spans < 4 GiB: its loads take 32-bit byte offsets
         int32_t isoEnabled;
         int32_t volumeEnabled;
         float minSize;

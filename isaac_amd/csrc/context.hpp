@@ -107,6 +107,7 @@ struct isaac_hip_ctx
     int tuneTiles = 4; // super-tiles of 4x4 tiles (measured fastest at 1080p)
     int tuneTexLayout = 0; // march texture: 0 linear (x fastest), 8 bricks of 8^3 texels
     int tuneMarchThreads = 256; // combined-form march workgroup: 256 (4 waves, one tile) or 64 (one wave)
+    int tuneMarchOffsets = 32; // combined-form march loads: 32-bit byte offsets where the texture allows, or 64-bit addresses
     int builtTexLayout = -1; // layout whose texels the last combined-form update_buffers wrote
     int form = ISAAC_HIP_FORM_COMBINED;
     int brickDtype = ISAAC_HIP_F32;

@@ -584,6 +584,12 @@ namespace isaac_hip
             F.nbx = uint32_t((c->local[0] + kTexBrick - 1) / kTexBrick);
             F.nbxy = F.nbx * uint32_t((c->local[1] + kTexBrick - 1) / kTexBrick);
         }
+        {
+            // bytes of the layout the march reads, its padding texel included
+            const uint64_t texels = F.texB ? uint64_t(c->texBrickCount) * kTexBrickTexels
+                                           : uint64_t(c->local[0]) * c->local[1] * c->local[2];
+            F.texOff32 = (c->tuneMarchOffsets == 32 && (texels + 1) * sizeof(uint2) <= 0xffffffffull) ? 1 : 0;
+        }
         const int nvf = c->nv + c->nf;
         F.isoEnabled = 0;
         for(int s = 0; s < nvf; s++)
@@ -1664,6 +1670,11 @@ extern "C"
             if(value != 64 && value != 256)
                 return fail(c, ISAAC_HIP_ERR_ARG, "march workgroup: 64 or 256 threads");
             c->tuneMarchThreads = value;
+            return 0;
+        case ISAAC_HIP_TUNE_MARCH_OFFSETS:
+            if(value != 32 && value != 64)
+                return fail(c, ISAAC_HIP_ERR_ARG, "march load addressing: 32 (byte offsets) or 64 (addresses)");
+            c->tuneMarchOffsets = value;
             return 0;
         case ISAAC_HIP_TUNE_TEX_LAYOUT:
             if(value != 0 && value != kTexBrick)

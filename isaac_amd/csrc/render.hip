@@ -485,7 +485,9 @@ namespace isaac_hip
 
     // the corners of the sample at c: for LINEAR the position minus 0.5 (the texel
     // centres' offset), for NEAREST the position itself; BRICK reads the brick layout
-    template<bool LINEAR, bool BRICK>
+    // OFF32: the texture spans < 4 GiB, so the loads take a 32-bit byte offset from the
+    // texture's (uniform) base address instead of a 64-bit address per lane
+    template<bool LINEAR, bool BRICK, bool OFF32 = false>
     __device__ inline void fetch_corners(const FrameParams& F, uint32_t XY, float cx, float cy, float cz, Corners& C)
     {
         const uint2* __restrict__ tex = BRICK ? F.texB : F.tex;
@@ -508,11 +510,24 @@ namespace isaac_hip
         const uint32_t dy = uint32_t(by) < uint32_t(Y - 1) ? (BRICK ? uint32_t(kTexBrickRow) : uint32_t(X)) : 0u;
         const uint32_t dz = uint32_t(bz) < uint32_t(Z - 1) ? (BRICK ? uint32_t(kTexBrickSlice) : XY) : 0u;
         const uint32_t i00 = BRICK ? brick_texel(F, x0, y0, z0) : x0 + __umul24(y0, uint32_t(X)) + z0 * XY;
-        const uint32_t i01 = i00 + dz, i10 = i00 + dy, i11 = i10 + dz;
-        const uint4 p00 = *reinterpret_cast<const uint4*>(tex + i00);
-        const uint4 p01 = *reinterpret_cast<const uint4*>(tex + i01);
-        const uint4 p10 = *reinterpret_cast<const uint4*>(tex + i10);
-        const uint4 p11 = *reinterpret_cast<const uint4*>(tex + i11);
+        uint4 p00, p01, p10, p11;
+        if constexpr(OFF32)
+        {
+            const char* base = reinterpret_cast<const char*>(tex);
+            const uint32_t o00 = i00 << 3, o01 = o00 + (dz << 3), o10 = o00 + (dy << 3), o11 = o10 + (dz << 3);
+            p00 = *reinterpret_cast<const uint4*>(base + o00);
+            p01 = *reinterpret_cast<const uint4*>(base + o01);
+            p10 = *reinterpret_cast<const uint4*>(base + o10);
+            p11 = *reinterpret_cast<const uint4*>(base + o11);
+        }
+        else
+        {
+            const uint32_t i01 = i00 + dz, i10 = i00 + dy, i11 = i10 + dz;
+            p00 = *reinterpret_cast<const uint4*>(tex + i00);
+            p01 = *reinterpret_cast<const uint4*>(tex + i01);
+            p10 = *reinterpret_cast<const uint4*>(tex + i10);
+            p11 = *reinterpret_cast<const uint4*>(tex + i11);
+        }
         C.t[0] = make_uint2(p00.x, p00.y);
         C.t[1] = make_uint2(p01.x, p01.y);
         C.t[2] = make_uint2(p10.x, p10.y);
@@ -819,7 +834,7 @@ namespace isaac_hip
     // order (particles, iso, SSAO, shading, volume) for every pixel.
     // The taps of step i+1 are fetched before step i is composited (clamped
     // addresses, so a fetch past the last step stays in bounds).
-    template<bool LINEAR, bool COUNT, bool PREFETCH, bool BRICK>
+    template<bool LINEAR, bool COUNT, bool PREFETCH, bool BRICK, bool OFF32 = false>
     __device__ __attribute__((always_inline)) inline void march_body(const FrameParams& F, const ParticleParams& PP)
     {
         int px, py;
@@ -889,7 +904,7 @@ namespace isaac_hip
                         auto at = [&](int k, Corners& C)
                         {
                             const float fk = float(k);
-                            fetch_corners<LINEAR, BRICK>(F, XY, __builtin_fmaf(stepVec.x, fk, startC.x),
+                            fetch_corners<LINEAR, BRICK, OFF32>(F, XY, __builtin_fmaf(stepVec.x, fk, startC.x),
                                                          __builtin_fmaf(stepVec.y, fk, startC.y),
                                                          __builtin_fmaf(stepVec.z, fk, startC.z), C);
                         };
@@ -1041,20 +1056,20 @@ namespace isaac_hip
     // The march at the compiler's register allocation, and at occupancy targets of
     // 6 and 8 waves per SIMD (at most 80 / 64 VGPRs); the launcher picks one by the
     // tuning key ISAAC_HIP_TUNE_MARCH_WAVES.
-    template<bool LINEAR, bool COUNT, bool PREFETCH, bool BRICK>
+    template<bool LINEAR, bool COUNT, bool PREFETCH, bool BRICK, bool OFF32 = false>
     __global__ __launch_bounds__(256) void march_kernel(const FrameParams F, const ParticleParams PP)
     {
-        march_body<LINEAR, COUNT, PREFETCH, BRICK>(F, PP);
+        march_body<LINEAR, COUNT, PREFETCH, BRICK, OFF32>(F, PP);
     }
-    template<bool LINEAR, bool BRICK>
+    template<bool LINEAR, bool BRICK, bool OFF32>
     __global__ __launch_bounds__(256, 6) void march_kernel_w6(const FrameParams F, const ParticleParams PP)
     {
-        march_body<LINEAR, false, false, BRICK>(F, PP);
+        march_body<LINEAR, false, false, BRICK, OFF32>(F, PP);
     }
-    template<bool LINEAR, bool BRICK>
+    template<bool LINEAR, bool BRICK, bool OFF32>
     __global__ __launch_bounds__(256, 8) void march_kernel_w8(const FrameParams F, const ParticleParams PP)
     {
-        march_body<LINEAR, false, false, BRICK>(F, PP);
+        march_body<LINEAR, false, false, BRICK, OFF32>(F, PP);
     }
 
     // ------------------------------------------------------------------ SSAO
@@ -2706,17 +2721,21 @@ namespace isaac_hip
         {
             if(F.waves == 6)
             {
-                if(F.linear)
-                    hipLaunchKernelGGL((march_kernel_w6<true, BRICK>), grid, block, 0, s, F, PP);
+                if(F.linear && F.texOff32)
+                    hipLaunchKernelGGL((march_kernel_w6<true, BRICK, true>), grid, block, 0, s, F, PP);
+                else if(F.linear)
+                    hipLaunchKernelGGL((march_kernel_w6<true, BRICK, false>), grid, block, 0, s, F, PP);
                 else
-                    hipLaunchKernelGGL((march_kernel_w6<false, BRICK>), grid, block, 0, s, F, PP);
+                    hipLaunchKernelGGL((march_kernel_w6<false, BRICK, false>), grid, block, 0, s, F, PP);
             }
             else
             {
-                if(F.linear)
-                    hipLaunchKernelGGL((march_kernel_w8<true, BRICK>), grid, block, 0, s, F, PP);
+                if(F.linear && F.texOff32)
+                    hipLaunchKernelGGL((march_kernel_w8<true, BRICK, true>), grid, block, 0, s, F, PP);
+                else if(F.linear)
+                    hipLaunchKernelGGL((march_kernel_w8<true, BRICK, false>), grid, block, 0, s, F, PP);
                 else
-                    hipLaunchKernelGGL((march_kernel_w8<false, BRICK>), grid, block, 0, s, F, PP);
+                    hipLaunchKernelGGL((march_kernel_w8<false, BRICK, false>), grid, block, 0, s, F, PP);
             }
             return hipGetLastError();
         }
@@ -2725,7 +2744,12 @@ namespace isaac_hip
         {
         case 0: hipLaunchKernelGGL((march_kernel<false, false, false, BRICK>), grid, block, 0, s, F, PP); break;
         case 2: hipLaunchKernelGGL((march_kernel<false, true, false, BRICK>), grid, block, 0, s, F, PP); break;
-        case 4: hipLaunchKernelGGL((march_kernel<true, false, false, BRICK>), grid, block, 0, s, F, PP); break;
+        case 4:
+            if(F.texOff32)
+                hipLaunchKernelGGL((march_kernel<true, false, false, BRICK, true>), grid, block, 0, s, F, PP);
+            else
+                hipLaunchKernelGGL((march_kernel<true, false, false, BRICK, false>), grid, block, 0, s, F, PP);
+            break;
         case 6: hipLaunchKernelGGL((march_kernel<true, true, false, BRICK>), grid, block, 0, s, F, PP); break;
         default:
             if constexpr(!BRICK)

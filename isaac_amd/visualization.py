@@ -459,6 +459,7 @@ class IsaacVisualization:
     TUNE_MARCH_TILES = 2
     TUNE_TEX_LAYOUT = 3
     TUNE_MARCH_THREADS = 4
+    TUNE_MARCH_OFFSETS = 5
     tex_layout = 0
     render_form = 0
 

@@ -232,8 +232,9 @@ def test_production_full_size(gpu, oracle_lib, pose):
 @pytest.mark.parametrize("form", ["combined", "bricks-f16"])
 def test_schedules_do_not_change_the_frame(gpu, oracle_lib, form):
     """Every march schedule (tile order over the XCDs, occupancy target, prefetch,
-    workgroups of 4 waves or of one wave) renders the same frame; the counting
-    march executes the same steps in every tile order and workgroup size."""
+    workgroups of 4 waves or of one wave, loads by 32-bit offset or 64-bit address)
+    renders the same frame; the counting march executes the same steps in every
+    tile order and workgroup size."""
     spec = _reduced_prod(n=64, fb=(330, 250), particles=20000, visible_tf=True)
     vis = _vis(spec, gpu)
     if form != "combined":
@@ -247,10 +248,13 @@ def test_schedules_do_not_change_the_frame(gpu, oracle_lib, form):
             for waves, prefetch in ((0, 0), (6, 0), (8, 0), (0, 1)):
                 vis.set_tuning(vis.TUNE_MARCH_WAVES, waves)
                 vis.set_tuning(vis.TUNE_MARCH_PREFETCH, prefetch)
-                img = vis.draw()
-                if ref is None:
-                    ref = img
-                assert np.array_equal(img, ref), (threads, tiles, waves, prefetch)
+                for offsets in (32, 64):
+                    vis.set_tuning(vis.TUNE_MARCH_OFFSETS, offsets)
+                    img = vis.draw()
+                    if ref is None:
+                        ref = img
+                    assert np.array_equal(img, ref), (threads, tiles, waves, prefetch, offsets)
+                vis.set_tuning(vis.TUNE_MARCH_OFFSETS, 32)
             vis.set_counting(True)
             assert np.array_equal(vis.draw(), ref), (threads, tiles)
             c = {k: v for k, v in vis.counters().items() if k != "march_wave_steps"}
@@ -346,7 +350,7 @@ def test_tuning_rejects_unknown_values(gpu):
 
     vis = _vis(_reduced_prod(n=16, fb=(32, 32), particles=100), gpu)
     for key, bad in ((vis.TUNE_MARCH_TILES, 3), (vis.TUNE_MARCH_TILES, 32), (vis.TUNE_MARCH_WAVES, 5),
-                     (vis.TUNE_MARCH_THREADS, 128), (99, 0)):
+                     (vis.TUNE_MARCH_THREADS, 128), (vis.TUNE_MARCH_OFFSETS, 16), (99, 0)):
         with pytest.raises(IsaacHipError):
             vis.set_tuning(key, bad)
     for tiles in (0, 2, 4, 8, 16):

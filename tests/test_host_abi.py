@@ -85,3 +85,19 @@ def test_cpp_dropin_header_is_hip_free():
         assert f"#include <{bad}" not in text and f'#include "{bad}' not in text
     text = open(os.path.join(ROOT, "include", "isaac_hip.h")).read()
     assert "#include <hip" not in text
+
+
+def test_python_constants_match_the_header():
+    """The Python mirror's tuning keys, render forms and occlusion modes carry the
+    values include/isaac_hip.h gives them."""
+    import re
+
+    from isaac_amd.visualization import IsaacVisualization as V
+
+    text = open(os.path.join(ROOT, "include", "isaac_hip.h")).read()
+    found = 0
+    for prefix in ("TUNE", "FORM", "AO"):
+        for name, value in re.findall(rf"ISAAC_HIP_({prefix}_\w+)\s*=\s*(\d+)", text):
+            assert getattr(V, name) == int(value), name
+            found += 1
+    assert found >= 10
