@@ -214,13 +214,14 @@ def measure(vis, poses, out, steps, warmup):
 
 
 def tex_layout_study(vis, poses, frames=6):
-    """March time and classification time (which holds the brick copy) of each texture
-    layout at the tuned schedule, per pose; medians over `frames` updated frames."""
+    """March time and classification time (which holds the brick copy, or the binary16
+    copy's stores) of each texture layout at the tuned schedule, per pose; medians over
+    `frames` updated frames."""
     import numpy as np
 
     keep = vis.tex_layout
     res = {}
-    for layout in (0, 8):
+    for layout in (0, 8, 16):
         vis.set_tuning(vis.TUNE_TEX_LAYOUT, layout)
         vis.update_buffers()
         per = {}
@@ -515,6 +516,8 @@ def main() -> int:
     ap.add_argument("--prefetch", type=int, default=0, help="march kernel: fetch step i+1 before compositing step i")
     ap.add_argument("--march-waves", type=int, default=0, help="march occupancy target (0, 6, 8)")
     ap.add_argument("--march-threads", type=int, default=256, help="march workgroup: 256 (4 waves) or 64 (one wave)")
+    ap.add_argument("--tex-layout", type=int, default=0,
+                    help="texture the march reads: 0 linear, 8 bricks of 8^3, 16 binary16 copy of the volume RGBA")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep --prefetch / --march-waves / --march-threads instead of autotuning")
     ap.add_argument("--no-learned-ao", action="store_true", help="skip training / timing the learned occlusion pass")
@@ -562,6 +565,7 @@ def main() -> int:
 
     vis.set_tuning(vis.TUNE_MARCH_WAVES, args.march_waves)
     vis.set_tuning(vis.TUNE_MARCH_THREADS, args.march_threads)
+    vis.set_tuning(vis.TUNE_TEX_LAYOUT, args.tex_layout)
     form, dt = FORMS[args.form]
     vis.set_render_form(form, dt)
     tuning = None

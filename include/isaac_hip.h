@@ -250,9 +250,13 @@ extern "C"
                                           round-robin to the XCDs */
         ISAAC_HIP_TUNE_TEX_LAYOUT = 3, /* texture the combined-form march reads: 0 (default) the linear texture
                                           (x fastest); 8: a copy in 8^3 bricks with a one-texel apron, built by
-                                          update_buffers after classification (the linear texture stays what
-                                          isaac_hip_get_texture returns); a change needs update_buffers before
-                                          the next render; next-step prefetch applies to the linear layout only */
+                                          update_buffers after classification; 16: the classification also
+                                          writes the volume RGBA of every texel as four binary16 values, which
+                                          the trilinear march steps over (the iso sums of steps that can reach
+                                          the threshold, hit colours and the nearest sampler read the linear
+                                          texture).  The linear texture stays what isaac_hip_get_texture
+                                          returns; a change needs update_buffers before the next render;
+                                          next-step prefetch applies to layouts 0 and 16 */
         ISAAC_HIP_TUNE_MARCH_THREADS = 4, /* combined-form march workgroup: 256 (default; 4 waves render one 16x16
                                             tile) or 64 (one wave per workgroup, one 8x8 quarter of a tile) */
         ISAAC_HIP_TUNE_MARCH_OFFSETS = 5 /* combined-form trilinear march loads: 32 (default) a 32-bit byte offset

@@ -19,7 +19,7 @@ CANDIDATES = {
     "march_waves": (0, 6, 8),
     "march_prefetch": (0, 1),
     "march_tiles": (0, 4, 8),
-    "tex_layout": (0, 8),
+    "tex_layout": (0, 8, 16),
     "march_threads": (256, 64),
 }
 
@@ -44,7 +44,7 @@ def tune_march(vis, poses: Sequence, frames: int = 4, use_cache: bool = True, up
     """Pick the fastest march schedule over `poses` (frames per pose per candidate).
     With `update_each_frame` (the render loop updates the sources every frame) each
     timed frame runs update_buffers first and the score adds its classification time,
-    which holds the brick layout's copy.  Returns {"choice": {...}, "march_us":
+    which holds the brick layout's copy and the binary16 copy's stores.  Returns {"choice": {...}, "march_us":
     {candidate: score in us}}."""
     key = _key(vis)
     if use_cache and key in _CACHE:
@@ -56,7 +56,7 @@ def tune_march(vis, poses: Sequence, frames: int = 4, use_cache: bool = True, up
     threads = CANDIDATES["march_threads"] if combined else (256,)
     for layout, nt, waves, pf, tiles in itertools.product(layouts, threads, CANDIDATES["march_waves"],
                                                           CANDIDATES["march_prefetch"], CANDIDATES["march_tiles"]):
-        if pf and (waves or layout):
+        if pf and (waves or layout == 8):
             continue  # the occupancy-targeted and brick-layout kernels have no prefetch variant
         choice = {"march_waves": waves, "march_prefetch": pf, "march_tiles": tiles, "tex_layout": layout,
                   "march_threads": nt}

@@ -24,6 +24,7 @@ namespace isaac_hip
     // evaluated inline in the interpreter's operation order instead of interpreted
     constexpr int kSigNatural = 1000;
     constexpr int kTexBrick = 8; // side of a texture brick (texture layout 8)
+    constexpr int kTexHalf = 16; // texture layout 16: the march steps over a binary16 copy of the volume RGBA
     constexpr int kTexBrickRow = kTexBrick + 1, kTexBrickSlice = kTexBrickRow * kTexBrickRow;
     constexpr int kTexBrickTexels = kTexBrickSlice * kTexBrickRow; // 729
 
@@ -70,6 +71,7 @@ namespace isaac_hip
         uint32_t divX_mul, divX_shift, divXY_mul, divXY_shift;
         const float4* luts; // nsrc * T
         uint2* tex; // per voxel: combined volume RGBA8 (.x), combined iso RGBA8 (.y)
+        uint2* texH; // or nullptr: per voxel the volume RGBA8 as four binary16 values (see half_texel)
         int64_t nvox;
         ClassifySource src[kMaxSources];
     };
@@ -100,6 +102,9 @@ namespace isaac_hip
         // textures
         int32_t X, Y, Z;
         const uint2* tex; // interleaved combined textures, x fastest
+        // or nullptr: the volume RGBA of the same texels as binary16 values, the alpha's sign
+        // set where the iso alpha byte is >= 128 (half_texel); the trilinear march steps over it
+        const uint2* texH;
         // the same texels in 8^3 bricks with a one-texel apron on the + side (9^3 texels per
         // brick, x fastest inside, bricks x fastest; apron and partial bricks hold the
         // clamped texel), or nullptr: every trilinear sample reads one brick
@@ -282,6 +287,19 @@ namespace isaac_hip
     {
         const float x = __builtin_amdgcn_fmed3f(v, 0.0f, 1.0f) * 255.0f;
         return x < 0.5f ? 0u : (uint32_t) (x + 0.5f);
+    }
+
+    // The march texel of texture layout 16: the volume RGBA8 bytes as binary16 values
+    // (every integer 0..255 is exact there), so a trilinear sum takes each one straight into
+    // an fma with mixed operand formats, and the sign of the alpha set where the iso alpha
+    // byte is >= 128 (the only steps where the iso alpha can reach 0.5; the march reads
+    // the alpha through its magnitude).  .x = (r, g), .y = (b, a) with the first in the low half.
+    __device__ inline uint2 half_texel(uint32_t vol, uint32_t iso)
+    {
+        auto h = [](uint32_t b) { return uint32_t(__half_as_ushort(__float2half_rn(float(b)))); };
+        const uint32_t flag = (iso >> 31) << 15;
+        return make_uint2(h(vol & 0xffu) | (h((vol >> 8) & 0xffu) << 16),
+                          h((vol >> 16) & 0xffu) | ((h(vol >> 24) | flag) << 16));
     }
 
     // transfer-function index of x = value * T: roundf(x), NaN -> 0, clamped to

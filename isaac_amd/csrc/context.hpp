@@ -93,6 +93,7 @@ struct isaac_hip_ctx
     uint2* tex = nullptr; // per voxel: combined volume RGBA8, combined iso RGBA8
     size_t texVox = 0;
     uint2* texBricks = nullptr; // the same texels in 8^3 bricks with a +1 apron (texture layout 8)
+    uint2* texH = nullptr; // the volume RGBA as binary16 per texel (texture layout 16), texVox + 1 texels
     size_t texBrickCount = 0;
     uint32_t* color = nullptr;
     float* depth = nullptr;
@@ -105,7 +106,7 @@ struct isaac_hip_ctx
     int tunePrefetch = 0;
     int tuneWaves = 0;
     int tuneTiles = 4; // super-tiles of 4x4 tiles (measured fastest at 1080p)
-    int tuneTexLayout = 0; // march texture: 0 linear (x fastest), 8 bricks of 8^3 texels
+    int tuneTexLayout = 0; // march texture: 0 linear (x fastest), 8 bricks of 8^3 texels, 16 linear binary16 copy
     int tuneMarchThreads = 256; // combined-form march workgroup: 256 (4 waves, one tile) or 64 (one wave)
     int tuneMarchOffsets = 32; // combined-form march loads: 32-bit byte offsets where the texture allows, or 64-bit addresses
     int builtTexLayout = -1; // layout whose texels the last combined-form update_buffers wrote

@@ -322,6 +322,9 @@ namespace
         if(c->tex)
             (void) hipFree(c->tex);
         c->tex = nullptr;
+        if(c->texH)
+            (void) hipFree(c->texH);
+        c->texH = nullptr;
         // one texel of padding: the march reads texels x and x+1 with one 16-byte load,
         // also at the last texel (whose x+1 it then ignores)
         c->generation++;
@@ -578,6 +581,8 @@ namespace isaac_hip
         F.Y = int32_t(c->local[1]);
         F.Z = int32_t(c->local[2]);
         F.tex = c->tex;
+        if(c->builtTexLayout == kTexHalf && c->texH)
+            F.texH = c->texH;
         if(c->builtTexLayout == kTexBrick && c->texBricks)
         {
             F.texB = c->texBricks;
@@ -941,7 +946,7 @@ extern "C"
         for(auto& p : c->parts)
             free_part(p);
         aonet_destroy(c);
-        void* bufs[] = {c->lutDev,   c->tex,        c->texBricks, c->color,    c->depth,
+        void* bufs[] = {c->lutDev,   c->tex,        c->texBricks, c->texH,     c->color,    c->depth,
                         c->normal,   c->ao,         c->volColor,  c->counters, c->outFloat,
                         c->mmPartials, c->mmResult, c->noise,     c->noiseTmp};
         for(void* b : bufs)
@@ -1433,6 +1438,16 @@ extern "C"
         make_fastdiv(uint32_t(P.X * P.Y), P.divXY_mul, P.divXY_shift);
         P.luts = c->lutDev;
         P.tex = c->tex;
+        if(c->tuneTexLayout == kTexHalf)
+        {
+            if(!c->texH)
+            {
+                // one padding texel, as for the combined texture
+                HIPCHK(c, hipMalloc(&c->texH, (c->texVox + 1) * sizeof(uint2)));
+                HIPCHK(c, hipMemsetAsync(c->texH + c->texVox, 0, sizeof(uint2), c->stream));
+            }
+            P.texH = c->texH;
+        }
         const int maxLds = int((64 * 1024) / (size_t(T) * sizeof(float4)));
         P.nLut = std::min(nvf, std::max(0, maxLds));
         for(int s = 0; s < nvf; s++)
@@ -1677,8 +1692,8 @@ extern "C"
             c->tuneMarchOffsets = value;
             return 0;
         case ISAAC_HIP_TUNE_TEX_LAYOUT:
-            if(value != 0 && value != kTexBrick)
-                return fail(c, ISAAC_HIP_ERR_ARG, "texture layout: 0 (linear) or 8 (8^3 bricks)");
+            if(value != 0 && value != kTexBrick && value != kTexHalf)
+                return fail(c, ISAAC_HIP_ERR_ARG, "texture layout: 0 (linear), 8 (8^3 bricks) or 16 (binary16 copy)");
             c->tuneTexLayout = value;
             return 0;
         default:

@@ -211,7 +211,10 @@ namespace isaac_hip
             const uint32_t vol = dither_b4(P.ditherMode, x, y, z, vr, vg, vb, va);
             const uint32_t q0 = unorm8(ir), q1 = unorm8(ig), q2 = unorm8(ib), q3 = unorm8(ia);
             // one 8-byte texel per voxel: combined volume RGBA8, combined iso RGBA8
-            P.tex[idx] = make_uint2(vol, q0 | (q1 << 8) | (q2 << 16) | (q3 << 24));
+            const uint32_t iso = q0 | (q1 << 8) | (q2 << 16) | (q3 << 24);
+            P.tex[idx] = make_uint2(vol, iso);
+            if(P.texH)
+                P.texH[idx] = half_texel(vol, iso);
         }
     }
 
@@ -486,11 +489,13 @@ namespace isaac_hip
     // the corners of the sample at c: for LINEAR the position minus 0.5 (the texel
     // centres' offset), for NEAREST the position itself; BRICK reads the brick layout
     // OFF32: the texture spans < 4 GiB, so the loads take a 32-bit byte offset from the
-    // texture's (uniform) base address instead of a 64-bit address per lane
-    template<bool LINEAR, bool BRICK, bool OFF32 = false>
+    // texture's (uniform) base address instead of a 64-bit address per lane.  HALF reads
+    // the binary16 copy of the linear layout (same texel indices) instead.
+    template<bool LINEAR, bool BRICK, bool OFF32 = false, bool HALF = false>
     __device__ inline void fetch_corners(const FrameParams& F, uint32_t XY, float cx, float cy, float cz, Corners& C)
     {
-        const uint2* __restrict__ tex = BRICK ? F.texB : F.tex;
+        static_assert(!(HALF && BRICK), "the binary16 copy has the linear layout");
+        const uint2* __restrict__ tex = HALF ? F.texH : (BRICK ? F.texB : F.tex);
         const int X = F.X, Y = F.Y, Z = F.Z;
         if(!LINEAR)
         {
@@ -556,6 +561,26 @@ namespace isaac_hip
         if(!LINEAR)
             return d(0);
         float v = d(0) * C.w[0];
+#pragma unroll
+        for(int k = 1; k < 8; k++)
+            v = __builtin_fmaf(d(k), C.w[k], v);
+        return v;
+    }
+
+    // channel ch (0..3: r, g, b, a) of the volume RGBA from corners of the binary16 copy,
+    // in byte units: the same sum as corner_sum<true, false>(C, 8 * ch) (each binary16 value
+    // is its byte exactly; the first product goes through an fma with +0, which returns the
+    // product itself as both factors are >= +0); the alpha is read through its magnitude
+    template<int CH>
+    __device__ inline float half_sum(const Corners& C)
+    {
+        auto d = [&](int k)
+        {
+            const uint32_t u = CH < 2 ? C.t[k].x : C.t[k].y;
+            const float v = float(__ushort_as_half((unsigned short) ((CH & 1) ? (u >> 16) : (u & 0xffffu))));
+            return CH == 3 ? fabsf(v) : v;
+        };
+        float v = __builtin_fmaf(d(0), C.w[0], 0.0f);
 #pragma unroll
         for(int k = 1; k < 8; k++)
             v = __builtin_fmaf(d(k), C.w[k], v);
@@ -834,7 +859,7 @@ namespace isaac_hip
     // order (particles, iso, SSAO, shading, volume) for every pixel.
     // The taps of step i+1 are fetched before step i is composited (clamped
     // addresses, so a fetch past the last step stays in bounds).
-    template<bool LINEAR, bool COUNT, bool PREFETCH, bool BRICK, bool OFF32 = false>
+    template<bool LINEAR, bool COUNT, bool PREFETCH, bool BRICK, bool OFF32 = false, bool HALF = false>
     __device__ __attribute__((always_inline)) inline void march_body(const FrameParams& F, const ParticleParams& PP)
     {
         int px, py;
@@ -901,12 +926,21 @@ namespace isaac_hip
                         // texel centres' -0.5 for the trilinear sampler
                         const float sh = LINEAR ? 0.5f : 0.0f;
                         const F3 startC = f3(startU.x - sh, startU.y - sh, startU.z - sh);
+                        // the step's corners (from the binary16 copy for HALF) and, for the iso
+                        // sums of HALF, the combined texture's corners at the same position
                         auto at = [&](int k, Corners& C)
                         {
                             const float fk = float(k);
-                            fetch_corners<LINEAR, BRICK, OFF32>(F, XY, __builtin_fmaf(stepVec.x, fk, startC.x),
-                                                         __builtin_fmaf(stepVec.y, fk, startC.y),
-                                                         __builtin_fmaf(stepVec.z, fk, startC.z), C);
+                            fetch_corners<LINEAR, BRICK, OFF32, HALF>(F, XY, __builtin_fmaf(stepVec.x, fk, startC.x),
+                                                                      __builtin_fmaf(stepVec.y, fk, startC.y),
+                                                                      __builtin_fmaf(stepVec.z, fk, startC.z), C);
+                        };
+                        auto atIso = [&](int k, Corners& C)
+                        {
+                            const float fk = float(k);
+                            fetch_corners<LINEAR, BRICK, OFF32, false>(F, XY, __builtin_fmaf(stepVec.x, fk, startC.x),
+                                                                       __builtin_fmaf(stepVec.y, fk, startC.y),
+                                                                       __builtin_fmaf(stepVec.z, fk, startC.z), C);
                         };
                         Corners cur;
                         at(i, cur);
@@ -925,19 +959,32 @@ namespace isaac_hip
                                 // ulps, so eight bytes <= 127 stay below 127.1).  Without such a byte
                                 // the sum is not formed: it is exactly 0 when all alpha bytes are 0,
                                 // else it is formed again from the step's corners if the next step hits
+                                // (HALF: the flag is the sign of the binary16 alpha, and the sum
+                                // comes from the combined texture's corners of this step)
                                 const uint32_t alphas = corners_or<LINEAR, true>(cur);
                                 const bool high = (alphas >> 31) != 0u;
-                                const float sum = high ? corner_sum<LINEAR, true>(cur, 24) : 0.0f;
+                                float sum = 0.0f;
+                                if(HALF)
+                                {
+                                    if(high)
+                                    {
+                                        Corners ic;
+                                        atIso(i, ic);
+                                        sum = corner_sum<LINEAR, true>(ic, 24);
+                                    }
+                                }
+                                else if(high)
+                                    sum = corner_sum<LINEAR, true>(cur, 24);
                                 if(COUNT)
                                     nIso++;
                                 float tmpSum = oldValue;
-                                oldValue = (high || (alphas >> 24) == 0u) ? sum : -1.0f;
+                                oldValue = (high || (!HALF && (alphas >> 24) == 0u)) ? sum : -1.0f;
                                 if(sum >= 127.5f)
                                 {
                                     if(tmpSum < 0.0f)
                                     {
                                         Corners pc;
-                                        at(i - 1, pc);
+                                        atIso(i - 1, pc);
                                         tmpSum = corner_sum<LINEAR, true>(pc, 24);
                                     }
                                     const bool first = ray.isClipped && i == startSteps;
@@ -972,10 +1019,10 @@ namespace isaac_hip
                             {
                                 if(COUNT)
                                     nVol++;
-                                const float vr = corner_sum<LINEAR, false>(cur, 0);
-                                const float vg = corner_sum<LINEAR, false>(cur, 8);
-                                const float vb = corner_sum<LINEAR, false>(cur, 16);
-                                const float va = corner_sum<LINEAR, false>(cur, 24);
+                                const float vr = HALF ? half_sum<0>(cur) : corner_sum<LINEAR, false>(cur, 0);
+                                const float vg = HALF ? half_sum<1>(cur) : corner_sum<LINEAR, false>(cur, 8);
+                                const float vb = HALF ? half_sum<2>(cur) : corner_sum<LINEAR, false>(cur, 16);
+                                const float va = HALF ? half_sum<3>(cur) : corner_sum<LINEAR, false>(cur, 24);
                                 const float w = (1.0f - ca) * vscale;
                                 cr = cr + w * vr;
                                 cg = cg + w * vg;
@@ -1056,20 +1103,20 @@ namespace isaac_hip
     // The march at the compiler's register allocation, and at occupancy targets of
     // 6 and 8 waves per SIMD (at most 80 / 64 VGPRs); the launcher picks one by the
     // tuning key ISAAC_HIP_TUNE_MARCH_WAVES.
-    template<bool LINEAR, bool COUNT, bool PREFETCH, bool BRICK, bool OFF32 = false>
+    template<bool LINEAR, bool COUNT, bool PREFETCH, bool BRICK, bool OFF32 = false, bool HALF = false>
     __global__ __launch_bounds__(256) void march_kernel(const FrameParams F, const ParticleParams PP)
     {
-        march_body<LINEAR, COUNT, PREFETCH, BRICK, OFF32>(F, PP);
+        march_body<LINEAR, COUNT, PREFETCH, BRICK, OFF32, HALF>(F, PP);
     }
-    template<bool LINEAR, bool BRICK, bool OFF32>
+    template<bool LINEAR, bool BRICK, bool OFF32, bool HALF = false>
     __global__ __launch_bounds__(256, 6) void march_kernel_w6(const FrameParams F, const ParticleParams PP)
     {
-        march_body<LINEAR, false, false, BRICK, OFF32>(F, PP);
+        march_body<LINEAR, false, false, BRICK, OFF32, HALF>(F, PP);
     }
-    template<bool LINEAR, bool BRICK, bool OFF32>
+    template<bool LINEAR, bool BRICK, bool OFF32, bool HALF = false>
     __global__ __launch_bounds__(256, 8) void march_kernel_w8(const FrameParams F, const ParticleParams PP)
     {
-        march_body<LINEAR, false, false, BRICK, OFF32>(F, PP);
+        march_body<LINEAR, false, false, BRICK, OFF32, HALF>(F, PP);
     }
 
     // ------------------------------------------------------------------ SSAO
@@ -2364,6 +2411,16 @@ namespace isaac_hip
         uint4* o = reinterpret_cast<uint4*>(P.tex + u);
         o[0] = make_uint4(texel[0], texel[1], texel[2], texel[3]);
         o[1] = make_uint4(texel[4], texel[5], texel[6], texel[7]);
+        if(P.texH)
+        {
+            uint2 h[4];
+#pragma unroll
+            for(int k = 0; k < 4; k++)
+                h[k] = half_texel(texel[2 * k], texel[2 * k + 1]);
+            uint4* oh = reinterpret_cast<uint4*>(P.texH + u);
+            oh[0] = make_uint4(h[0].x, h[0].y, h[1].x, h[1].y);
+            oh[1] = make_uint4(h[2].x, h[2].y, h[3].x, h[3].y);
+        }
     }
 
     // Classification compiled for the signature of its active sources (component
@@ -2707,6 +2764,23 @@ namespace isaac_hip
         return hipGetLastError();
     }
 
+    // the trilinear march over the binary16 copy (texture layout 16), at every schedule
+    template<bool OFF32>
+    void launch_march_half(const FrameParams& F, const ParticleParams& PP, bool counting, bool prefetch, dim3 grid, dim3 block,
+                           hipStream_t s)
+    {
+        if(counting)
+            hipLaunchKernelGGL((march_kernel<true, true, false, false, OFF32, true>), grid, block, 0, s, F, PP);
+        else if(prefetch)
+            hipLaunchKernelGGL((march_kernel<true, false, true, false, OFF32, true>), grid, block, 0, s, F, PP);
+        else if(F.waves == 6)
+            hipLaunchKernelGGL((march_kernel_w6<true, false, OFF32, true>), grid, block, 0, s, F, PP);
+        else if(F.waves == 8)
+            hipLaunchKernelGGL((march_kernel_w8<true, false, OFF32, true>), grid, block, 0, s, F, PP);
+        else
+            hipLaunchKernelGGL((march_kernel<true, false, false, false, OFF32, true>), grid, block, 0, s, F, PP);
+    }
+
     template<bool BRICK>
     hipError_t launch_march_layout(const FrameParams& F, const ParticleParams& PP, bool counting, hipStream_t s)
     {
@@ -2717,6 +2791,17 @@ namespace isaac_hip
         const dim3 grid(F.waveGroups == 1 ? (nb + 7u) / 8u * 32u : nb), block(F.waveGroups == 1 ? 64 : 256);
         // next-step prefetch is compiled for the linear layout only
         const bool prefetch = F.prefetch && !BRICK;
+        if constexpr(!BRICK)
+        {
+            if(F.texH && F.linear)
+            {
+                if(F.texOff32)
+                    launch_march_half<true>(F, PP, counting, prefetch, grid, block, s);
+                else
+                    launch_march_half<false>(F, PP, counting, prefetch, grid, block, s);
+                return hipGetLastError();
+            }
+        }
         if(!counting && !prefetch && (F.waves == 6 || F.waves == 8))
         {
             if(F.waves == 6)

@@ -222,9 +222,14 @@ def test_production_full_size(gpu, oracle_lib, pose):
     vis, img = gpu_frame(spec, gpu)
     img2 = vis.do_visualization()
     assert np.array_equal(img, img2), "two renders of the same frame differ"
+    c = vis.counters()
+    # the march over the binary16 copy of the volume texture renders the same frame and
+    # executes the same steps
+    vis.set_tuning(vis.TUNE_TEX_LAYOUT, 16)
+    assert np.array_equal(vis.do_visualization(), img), "texture layout 16"
+    assert vis.counters() == c
     ora = oracle.render(spec, 32)
     compare(f"prod256-{pose}", spec, img, ora)
-    c = vis.counters()
     print("gpu counters", c, "oracle", ora.counters, flush=True)
     assert abs(c["vol_samples"] - ora.counters["vol_samples"]) <= 1e-3 * ora.counters["vol_samples"]
 
@@ -267,12 +272,15 @@ def test_schedules_do_not_change_the_frame(gpu, oracle_lib, form):
 
 @pytest.mark.parametrize("n,linear,pose", [(64, True, "default"), (64, True, "fill"), (37, True, "default"),
                                            (37, False, "fill")])
-def test_texture_brick_layout_renders_the_same_frame(gpu, oracle_lib, n, linear, pose):
-    """The march reading the 8^3-brick copy of the combined texture (apron texels
-    and partial bricks at a side of 37 hold clamped texels) renders the frame and
-    G-buffer of the linear texture bit for bit, at every occupancy target and with
-    the counting march, and executes the same steps; at a side of 37 the frame is
-    also checked against the oracle."""
+@pytest.mark.parametrize("layout", [8, 16])
+def test_texture_layouts_render_the_same_frame(gpu, oracle_lib, n, linear, pose, layout):
+    """The march reading the 8^3-brick copy of the combined texture (layout 8: apron
+    texels and partial bricks at a side of 37 hold clamped texels) or the binary16
+    copy of its volume RGBA (layout 16; the nearest sampler keeps reading the combined
+    texture) renders the frame and G-buffer of the linear texture bit for bit, at
+    every occupancy target, with prefetch (layout 16) and with the counting march,
+    and executes the same steps; at a side of 37 the frame is also checked against
+    the oracle."""
     from oracle import oracle
 
     p = scenes.FILL_POSE if pose == "fill" else scenes.DEFAULT_POSE
@@ -285,21 +293,23 @@ def test_texture_brick_layout_renders_the_same_frame(gpu, oracle_lib, n, linear,
     ref_g = [a.copy() for a in vis.gbuffer()]
     ref_c = {k: v for k, v in vis.counters().items()}
     vis.set_counting(False)
-    vis.set_tuning(vis.TUNE_TEX_LAYOUT, 8)
+    vis.set_tuning(vis.TUNE_TEX_LAYOUT, layout)
     vis.update_buffers()
-    for waves in (0, 6, 8):
+    for waves, prefetch in ((0, 0), (6, 0), (8, 0)) + (((0, 1),) if layout == 16 else ()):
         vis.set_tuning(vis.TUNE_MARCH_WAVES, waves)
+        vis.set_tuning(vis.TUNE_MARCH_PREFETCH, prefetch)
         img = vis.draw()
-        assert np.array_equal(img, ref), waves
+        assert np.array_equal(img, ref), (waves, prefetch)
         for a, b in zip(vis.gbuffer(), ref_g):
-            assert np.array_equal(a, b), waves
+            assert np.array_equal(a, b), (waves, prefetch)
     vis.set_tuning(vis.TUNE_MARCH_WAVES, 0)
+    vis.set_tuning(vis.TUNE_MARCH_PREFETCH, 0)
     vis.set_counting(True)
     assert np.array_equal(vis.draw(), ref)
     assert vis.counters() == ref_c
     vis.close()
     if n == 37:
-        compare(f"prod{n}-{pose} brick layout", spec, ref, oracle.render(spec, 32))
+        compare(f"prod{n}-{pose} layout {layout}", spec, ref, oracle.render(spec, 32))
 
 
 def test_texture_layout_change_needs_update(gpu):
@@ -308,8 +318,9 @@ def test_texture_layout_change_needs_update(gpu):
     vis = _vis(_reduced_prod(n=16, fb=(32, 32), particles=100), gpu)
     vis.update_buffers()
     vis.draw()
-    with pytest.raises(IsaacHipError):
-        vis.set_tuning(vis.TUNE_TEX_LAYOUT, 4)
+    for bad in (4, 32):
+        with pytest.raises(IsaacHipError):
+            vis.set_tuning(vis.TUNE_TEX_LAYOUT, bad)
     vis.set_tuning(vis.TUNE_TEX_LAYOUT, 8)
     with pytest.raises(IsaacHipError):
         vis.draw()
@@ -334,7 +345,14 @@ def test_iso_hit_depth_matches_oracle(gpu, oracle_lib, pose):
     spec = dataclasses.replace(spec, particles=[], volumes=[density], ao_enabled=False)
     vis, img = gpu_frame(spec, gpu)
     ora = oracle.render(spec, 32)
-    _, depth, _, _ = vis.gbuffer()
+    gb = [a.copy() for a in vis.gbuffer()]
+    depth = gb[1]
+    # the march over the binary16 copy forms the iso sums from the combined texture at the
+    # flagged steps only: the same hits, depths and normals
+    vis.set_tuning(vis.TUNE_TEX_LAYOUT, 16)
+    assert np.array_equal(vis.do_visualization(), img)
+    for a, b in zip(vis.gbuffer(), gb):
+        assert np.array_equal(a, b)
     vis.close()
     both = (ora.depth < 3e38) & (depth < 3e38)
     assert both.sum() > 3000, both.sum()
