@@ -2409,8 +2409,20 @@ namespace isaac_hip
             texel[2 * k + 1] = q0 | (q1 << 8) | (q2 << 16) | (q3 << 24);
         }
         uint4* o = reinterpret_cast<uint4*>(P.tex + u);
-        o[0] = make_uint4(texel[0], texel[1], texel[2], texel[3]);
-        o[1] = make_uint4(texel[4], texel[5], texel[6], texel[7]);
+        if(P.texH)
+        {
+            // the march steps over the binary16 copy and reads the combined texture only at
+            // steps that can hit the iso-surface: its stores are non-temporal, so they do not
+            // displace the copy from the caches
+            typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+            __builtin_nontemporal_store(v4u{texel[0], texel[1], texel[2], texel[3]}, reinterpret_cast<v4u*>(o));
+            __builtin_nontemporal_store(v4u{texel[4], texel[5], texel[6], texel[7]}, reinterpret_cast<v4u*>(o) + 1);
+        }
+        else
+        {
+            o[0] = make_uint4(texel[0], texel[1], texel[2], texel[3]);
+            o[1] = make_uint4(texel[4], texel[5], texel[6], texel[7]);
+        }
         if(P.texH)
         {
             uint2 h[4];
