@@ -11,6 +11,9 @@ callback, with the camera alternating between the default pose and a
 volume-filling pose.  Inputs are resident in HBM before timing starts and the
 framebuffer stays on the device (no PCIe in the timed region); fps_host_output
 repeats the frames with the framebuffer copied to pinned host memory each step.
+The timed frames record HIP events around the march only (its time for the
+roofline); the per-kernel breakdown comes from the same frames run just before
+with events between all kernels (ms_per_step_all_timing_events).
 
 Metric: trilinear samples per second (volume + iso march samples actually
 executed, after early ray termination), with frames/s alongside.  The same run
@@ -168,8 +171,12 @@ def set_pose_of(vis, p):
     vis._update_modelview()
 
 
-def measure(vis, poses, out, steps, warmup):
-    """Counting frames per pose (untimed), warmup, then `steps` timed frames."""
+def measure(vis, poses, out, steps, warmup, instrumented=None):
+    """Counting frames per pose (untimed), warmup, `steps` frames with timing events
+    between all kernels (the per-kernel breakdown), then `steps` timed frames with timing
+    events around the march only (each event costs a few us on the stream; the march's
+    HIP-event time in avg_us comes from the timed frames).  `instrumented`, a dict,
+    receives the frame time of the fully instrumented frames."""
     import numpy as np
     import torch
 
@@ -191,11 +198,9 @@ def measure(vis, poses, out, steps, warmup):
     vis.synchronize()
     torch.cuda.synchronize()
     per = {"classify": [], "binning": [], "march": [], "ssao": [], "shade": []}
-    total_samples = 0
     t0 = time.perf_counter()
     for k in range(steps):
-        pi = k % 2
-        set_pose(poses[pi])
+        set_pose(poses[k % 2])
         vis.update_buffers()
         vis.draw(out=out)
         t = vis.timings()  # waits for the step: each frame completes before the next (as the reference)
@@ -204,11 +209,29 @@ def measure(vis, poses, out, steps, warmup):
         per["march"].append(t["march_us"])
         per["ssao"].append(t["ssao_us"])
         per["shade"].append(t["shade_us"])
+    vis.synchronize()
+    if instrumented is not None:
+        instrumented["ms_per_step"] = (time.perf_counter() - t0) / steps * 1e3
+        instrumented["march_us"] = float(np.mean(per["march"]))
+    vis.set_tuning(vis.TUNE_TIMING_EVENTS, 2)
+    march = []
+    total_samples = 0
+    vis.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        pi = k % 2
+        set_pose(poses[pi])
+        vis.update_buffers()
+        vis.draw(out=out)
+        march.append(vis.timings()["march_us"])  # waits for the step
         total_samples += counts[pi]["vol_samples"] + counts[pi]["iso_samples"]
     vis.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    vis.set_tuning(vis.TUNE_TIMING_EVENTS, 1)
     avg_us = {k: float(np.mean(v)) for k, v in per.items()}
+    avg_us["march"] = float(np.mean(march))
     avg_counts = {k: float(np.mean([c[k] for c in counts])) for k in counts[0]}
     return wall, total_samples, avg_us, avg_counts, counts
 
@@ -575,7 +598,8 @@ def main() -> int:
         vis.update_buffers()
         tuning = autotune.tune_march(vis, poses)
         tuning["tex_layout_study"] = tex_layout_study(vis, poses)
-    wall, total_samples, avg_us, avg_counts, counts = measure(vis, poses, out, args.steps, args.warmup)
+    instrumented = {}
+    wall, total_samples, avg_us, avg_counts, counts = measure(vis, poses, out, args.steps, args.warmup, instrumented)
     ms_per_step = wall / args.steps * 1e3
     value = total_samples / wall
     # the same frames with the framebuffer copied to (pinned) host memory each step, as
@@ -617,6 +641,8 @@ def main() -> int:
         "fps": args.steps / wall,
         "fps_host_output": args.steps / wall_h,
         "ms_per_step_host_output": wall_h / args.steps * 1e3,
+        # the same frames with timing events between all kernels (per-kernel breakdown below)
+        "ms_per_step_all_timing_events": instrumented["ms_per_step"],
         "samples_per_frame": {"default": counts[0]["vol_samples"] + counts[0]["iso_samples"],
                               "fill": counts[1]["vol_samples"] + counts[1]["iso_samples"]},
         "kernel_ms": {k: v / 1e3 for k, v in avg_us.items()},

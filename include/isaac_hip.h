@@ -259,9 +259,13 @@ extern "C"
                                           next-step prefetch applies to layouts 0 and 16 */
         ISAAC_HIP_TUNE_MARCH_THREADS = 4, /* combined-form march workgroup: 256 (default; 4 waves render one 16x16
                                             tile) or 64 (one wave per workgroup, one 8x8 quarter of a tile) */
-        ISAAC_HIP_TUNE_MARCH_OFFSETS = 5 /* combined-form trilinear march loads: 32 (default) a 32-bit byte offset
+        ISAAC_HIP_TUNE_MARCH_OFFSETS = 5, /* combined-form trilinear march loads: 32 (default) a 32-bit byte offset
                                             from the texture's base where the texture spans < 4 GiB (larger ones
                                             always take 64-bit addresses); 64: a 64-bit address per lane */
+        ISAAC_HIP_TUNE_TIMING_EVENTS = 6 /* 1 (default): update_buffers and render record timing events between
+                                            their kernels (isaac_hip_get_timings); 2: only the two around the
+                                            march (get_timings reports march_us, zeros elsewhere); 0: none (each
+                                            event costs a few microseconds on the stream; all timings zero) */
     };
     int isaac_hip_set_tuning(isaac_hip_ctx* ctx, int32_t key, int32_t value);
 

@@ -53,6 +53,10 @@ def tune_march(vis, poses: Sequence, frames: int = 4, use_cache: bool = True, up
     results = {}
     combined = vis.render_form == vis.FORM_COMBINED
     layouts = CANDIDATES["tex_layout"] if combined else (vis.tex_layout,)
+    if update_each_frame:
+        # layout 16's binary16 copy costs the classification about what it saves the march
+        # (measured at the 256^3 workload), so it is a candidate where the sources stay put
+        layouts = tuple(x for x in layouts if x != 16) or (0,)
     threads = CANDIDATES["march_threads"] if combined else (256,)
     for layout, nt, waves, pf, tiles in itertools.product(layouts, threads, CANDIDATES["march_waves"],
                                                           CANDIDATES["march_prefetch"], CANDIDATES["march_tiles"]):

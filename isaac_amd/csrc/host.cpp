@@ -307,6 +307,14 @@ namespace
         c->maxGsScaled = std::max(c->gsScaled[0], std::max(c->gsScaled[1], c->gsScaled[2]));
     }
 
+    // timing marker k on the context's stream: every marker (timing events 1), the two
+    // around the march (2, markers 4 and 5) or none (0)
+    hipError_t mark(isaac_hip_ctx* c, int k)
+    {
+        const bool on = c->timingEvents == 1 || (c->timingEvents == 2 && (k == 4 || k == 5));
+        return on ? hipEventRecord(c->ev[k], c->stream) : hipSuccess;
+    }
+
     // assigns v to dst and bumps the context's generation when the value changes
     template<class T>
     void set_tracked(isaac_hip_ctx* c, T& dst, const T& v)
@@ -1303,7 +1311,7 @@ extern "C"
             HIPCHK(c, hipStreamSynchronize(c->stream));
             c->lutDirty = false;
         }
-        HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+        HIPCHK(c, mark(c, 0));
         // non-persistent sources (UpdatePersistentTextureIterator, lib/isaac.hpp:249-290)
         for(int s = 0; s < nvf; s++)
         {
@@ -1364,7 +1372,7 @@ extern "C"
                 HIPCHK(c, launch_advect(A, c->stream));
             }
         }
-        HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+        HIPCHK(c, mark(c, 1));
         if(c->form == ISAAC_HIP_FORM_BRICKS)
         {
             // brick form: re-lay every enabled volume/field source (the raw
@@ -1409,11 +1417,11 @@ extern "C"
             }
             // the particle records after the GPU-heavy brick build, so the host enqueues
             // their short launches while the build runs
-            HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+            HIPCHK(c, mark(c, 2));
             if(int rc = bin_particles(c))
                 return rc;
-            HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-            c->evRecorded[0] = true;
+            HIPCHK(c, mark(c, 3));
+            c->evRecorded[0] = c->timingEvents == 1;
             c->classified = true;
             c->builtForm = ISAAC_HIP_FORM_BRICKS;
             return 0;
@@ -1509,11 +1517,11 @@ extern "C"
         }
         // the particle records after the classification, so the host enqueues their short
         // launches while the classification runs
-        HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+        HIPCHK(c, mark(c, 2));
         if(int rc = bin_particles(c))
             return rc;
-        HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-        c->evRecorded[0] = true;
+        HIPCHK(c, mark(c, 3));
+        c->evRecorded[0] = c->timingEvents == 1;
         c->classified = true;
         c->builtForm = ISAAC_HIP_FORM_COMBINED;
         c->builtTexLayout = c->tuneTexLayout;
@@ -1550,12 +1558,12 @@ extern "C"
         }
         if(c->counting)
             HIPCHK(c, launch_fill_u32(reinterpret_cast<uint32_t*>(c->counters), 0u, 2 * ISAAC_HIP_CNT_COUNT, c->stream));
-        HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+        HIPCHK(c, mark(c, 4));
         if(c->form == ISAAC_HIP_FORM_BRICKS)
             HIPCHK(c, launch_brick_march(F, Q, PP, c->counting, c->stream));
         else
             HIPCHK(c, launch_march(F, PP, c->counting, c->stream));
-        HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
+        HIPCHK(c, mark(c, 5));
         if(F.aoEnabled)
         {
             if(c->aoMode == ISAAC_HIP_AO_LEARNED)
@@ -1567,16 +1575,17 @@ extern "C"
             else
                 HIPCHK(c, launch_ssao(F, c->stream));
         }
-        HIPCHK(c, hipEventRecord(c->ev[6], c->stream));
+        HIPCHK(c, mark(c, 6));
         HIPCHK(c, launch_shade(F, c->stream));
-        HIPCHK(c, hipEventRecord(c->ev[7], c->stream));
+        HIPCHK(c, mark(c, 7));
         if(out_rgba)
         {
             const size_t bytes = size_t(W) * size_t(H) * 4;
             HIPCHK(c, hipMemcpyAsync(out_rgba, c->color, bytes, out_is_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
         }
-        HIPCHK(c, hipEventRecord(c->ev[8], c->stream));
-        c->evRecorded[1] = true;
+        HIPCHK(c, mark(c, 8));
+        c->evRecorded[1] = c->timingEvents != 0;
+        c->evMarchOnly = c->timingEvents == 2;
         c->rendered = true;
         if(out_rgba && !out_is_device)
             HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1643,8 +1652,9 @@ extern "C"
             out->buffer_us = out->source_copy_us + out->binning_us;
         }
         if(c->evRecorded[1])
-        {
             out->march_us = elapsed_us(c->ev[4], c->ev[5]);
+        if(c->evRecorded[1] && !c->evMarchOnly)
+        {
             out->ssao_us = elapsed_us(c->ev[5], c->ev[6]);
             out->shade_us = elapsed_us(c->ev[6], c->ev[7]);
             out->kernel_us = elapsed_us(c->ev[4], c->ev[7]);
@@ -1685,6 +1695,11 @@ extern "C"
             if(value != 64 && value != 256)
                 return fail(c, ISAAC_HIP_ERR_ARG, "march workgroup: 64 or 256 threads");
             c->tuneMarchThreads = value;
+            return 0;
+        case ISAAC_HIP_TUNE_TIMING_EVENTS:
+            if(value < 0 || value > 2)
+                return fail(c, ISAAC_HIP_ERR_ARG, "timing events: 0 (none), 1 (all) or 2 (around the march)");
+            c->timingEvents = value;
             return 0;
         case ISAAC_HIP_TUNE_MARCH_OFFSETS:
             if(value != 32 && value != 64)

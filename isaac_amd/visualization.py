@@ -460,6 +460,7 @@ class IsaacVisualization:
     TUNE_TEX_LAYOUT = 3
     TUNE_MARCH_THREADS = 4
     TUNE_MARCH_OFFSETS = 5
+    TUNE_TIMING_EVENTS = 6
     tex_layout = 0
     render_form = 0
 

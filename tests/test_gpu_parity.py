@@ -376,6 +376,30 @@ def test_tuning_rejects_unknown_values(gpu):
     vis.close()
 
 
+def test_timing_event_modes(gpu):
+    """Timing events between all kernels (1), around the march only (2) or none (0):
+    the same frame each time; get_timings reports what was recorded and zeros for the rest."""
+    from isaac_amd._lib import IsaacHipError
+
+    vis = _vis(_reduced_prod(n=32, fb=(96, 64), particles=2000), gpu)
+    ref = vis.do_visualization()
+    t = vis.timings()
+    assert t["march_us"] > 0 and t["shade_us"] > 0 and t["optimization_buffer_us"] > 0
+    for mode in (2, 0, 1):
+        vis.set_tuning(vis.TUNE_TIMING_EVENTS, mode)
+        assert np.array_equal(vis.do_visualization(), ref), mode
+        t = vis.timings()
+        if mode == 0:
+            assert all(v == 0 for v in t.values()), t
+        elif mode == 2:
+            assert t["march_us"] > 0 and t["shade_us"] == 0 and t["optimization_buffer_us"] == 0, t
+        else:
+            assert t["march_us"] > 0 and t["shade_us"] > 0 and t["optimization_buffer_us"] > 0, t
+    with pytest.raises(IsaacHipError):
+        vis.set_tuning(vis.TUNE_TIMING_EVENTS, 3)
+    vis.close()
+
+
 @pytest.mark.parametrize("fb", [(1, 1), (17, 9), (300, 1), (1, 200), (129, 65)])
 def test_odd_framebuffers(gpu, oracle_lib, fb):
     """Framebuffers of one pixel, one row or column, and sizes that leave partial
