@@ -262,10 +262,14 @@ extern "C"
         ISAAC_HIP_TUNE_MARCH_OFFSETS = 5, /* combined-form trilinear march loads: 32 (default) a 32-bit byte offset
                                             from the texture's base where the texture spans < 4 GiB (larger ones
                                             always take 64-bit addresses); 64: a 64-bit address per lane */
-        ISAAC_HIP_TUNE_TIMING_EVENTS = 6 /* 1 (default): update_buffers and render record timing events between
+        ISAAC_HIP_TUNE_TIMING_EVENTS = 6, /* 1 (default): update_buffers and render record timing events between
                                             their kernels (isaac_hip_get_timings); 2: only the two around the
                                             march (get_timings reports march_us, zeros elsewhere); 0: none (each
                                             event costs a few microseconds on the stream; all timings zero) */
+        ISAAC_HIP_TUNE_FUSED_SHADING = 7 /* 1 (default): with the plain occlusion pass in render modes 0, 1 and 3
+                                            (those that keep the depth buffer), the occlusion kernel shades its
+                                            own pixels (get_timings: ssao_us holds both, shade_us is 0); 0: a
+                                            separate shading kernel */
     };
     int isaac_hip_set_tuning(isaac_hip_ctx* ctx, int32_t key, int32_t value);
 

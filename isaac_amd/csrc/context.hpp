@@ -134,6 +134,7 @@ struct isaac_hip_ctx
     // occlusion pass, 7 after shading, 8 after the framebuffer copy
     hipEvent_t ev[12] = {};
     bool evRecorded[2] = {false, false};
+    bool tuneFusedShading = true; // the occlusion kernel shades its pixels where the render mode allows
     int timingEvents = 1; // timing markers between the frame's kernels: 1 all, 2 around the march only, 0 none
     bool evMarchOnly = false; // the last render recorded the march's markers only
 };

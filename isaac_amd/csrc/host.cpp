@@ -1564,6 +1564,7 @@ extern "C"
         else
             HIPCHK(c, launch_march(F, PP, c->counting, c->stream));
         HIPCHK(c, mark(c, 5));
+        bool fused = false;
         if(F.aoEnabled)
         {
             if(c->aoMode == ISAAC_HIP_AO_LEARNED)
@@ -1573,10 +1574,15 @@ extern "C"
                     return rc;
             }
             else
-                HIPCHK(c, launch_ssao(F, c->stream));
+            {
+                // the occlusion kernel shades its own pixels where the mode allows
+                fused = c->tuneFusedShading && ssao_shades(F);
+                HIPCHK(c, launch_ssao(F, c->stream, fused));
+            }
         }
         HIPCHK(c, mark(c, 6));
-        HIPCHK(c, launch_shade(F, c->stream));
+        if(!fused)
+            HIPCHK(c, launch_shade(F, c->stream));
         HIPCHK(c, mark(c, 7));
         if(out_rgba)
         {
@@ -1695,6 +1701,9 @@ extern "C"
             if(value != 64 && value != 256)
                 return fail(c, ISAAC_HIP_ERR_ARG, "march workgroup: 64 or 256 threads");
             c->tuneMarchThreads = value;
+            return 0;
+        case ISAAC_HIP_TUNE_FUSED_SHADING:
+            c->tuneFusedShading = value != 0;
             return 0;
         case ISAAC_HIP_TUNE_TIMING_EVENTS:
             if(value < 0 || value > 2)

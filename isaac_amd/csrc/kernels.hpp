@@ -21,7 +21,9 @@ namespace isaac_hip
     // + volume blend; the host records its per-kernel events between them
     hipError_t launch_march(const FrameParams& F, const ParticleParams& PP, bool counting, hipStream_t s);
     hipError_t launch_tex_bricks(const uint2* tex, uint2* out, int X, int Y, int Z, hipStream_t s);
-    hipError_t launch_ssao(const FrameParams& F, hipStream_t s);
+    // shade: the occlusion kernel also shades its pixels (only where ssao_shades(F))
+    hipError_t launch_ssao(const FrameParams& F, hipStream_t s, bool shade = false);
+    bool ssao_shades(const FrameParams& F);
     hipError_t launch_shade(const FrameParams& F, hipStream_t s);
 
     // ---- precision form: field bricks and the post-classification march

@@ -6,6 +6,7 @@
 //                          lib/isaac/isaac_common_kernel.hpp:195-217, lib/isaac/isaac_particle_kernel.hpp:117-260,
 //                          lib/isaac/isaac_iso_kernel.hpp:25-128, lib/isaac/isaac_volume_kernel.hpp:23-82
 //   ssao_tiled_kernel      SSAOKernel (live 7x7 part)     lib/isaac/isaac_ssao_kernel.hpp:161-182
+//                          (with the shading of its pixels fused in render modes 0, 1, 3)
 //   shade_kernel           ShadingKernel + the volume pass's final blend
 //                          lib/isaac/isaac_common_kernel.hpp:220-320, lib/isaac/isaac_volume_kernel.hpp:83-86
 //
@@ -1150,6 +1151,11 @@ namespace isaac_hip
     // increments of 1), so the occlusion equals ssao_body's.
     constexpr int kSsaoBW = 64, kSsaoBH = 32, kSsaoReach = 30;
     constexpr int kSsaoRW = kSsaoBW + 2 * kSsaoReach, kSsaoRH = kSsaoBH + 2 * kSsaoReach; // 124 x 92
+    // SHADE: the block also shades its pixels with the occlusion term it just formed (render
+    // modes that leave the depth buffer alone, so no block changes a depth another one reads)
+    __device__ __attribute__((always_inline)) inline void shade_pixel(const FrameParams& F, int px, int py, float ao, float depth);
+
+    template<bool SHADE>
     __global__ __launch_bounds__(256) void ssao_tiled_kernel(const FrameParams F)
     {
         __shared__ float region[kSsaoRW * kSsaoRH];
@@ -1187,7 +1193,10 @@ namespace isaac_hip
                 for(int j = 0; j < 7; ++j)
                     if(t0[(j * 10) * kSsaoRW + i * 10] < ref)
                         occ = occ + 1.0f;
-            F.ao[size_t(py) * F.W + px] = gclamp(occ / 42.0f, 0.0f, 1.0f);
+            const float ao = gclamp(occ / 42.0f, 0.0f, 1.0f);
+            F.ao[size_t(py) * F.W + px] = ao;
+            if(SHADE)
+                shade_pixel(F, px, py, ao, ref);
         }
     }
 
@@ -1195,18 +1204,14 @@ namespace isaac_hip
     // ShadingKernel (lib/isaac/isaac_common_kernel.hpp:220-320), then the volume
     // pass's final blend color + (1 - color.a) * solid
     // (lib/isaac/isaac_volume_kernel.hpp:83-86) for pixels whose volume pass ran.
-    __device__ __attribute__((always_inline)) inline void shade_body(const FrameParams& F, uint32_t tile)
+    // one pixel, given its occlusion term and depth
+    __device__ __attribute__((always_inline)) inline void shade_pixel(const FrameParams& F, int px, int py, float ao, float depth)
     {
-        int px, py;
-        if(!tile_pixel_at(F, tile, px, py))
-            return;
         const size_t p = size_t(py) * size_t(F.W) + size_t(px);
         const uint32_t cb = F.color[p];
         F4 color = F4{div255(float(cb & 0xffu)), div255(float((cb >> 8) & 0xffu)), div255(float((cb >> 16) & 0xffu)),
                       div255(float(cb >> 24))};
         const F3 normal = f3(F.normal[3 * p + 0], F.normal[3 * p + 1], F.normal[3 * p + 2]);
-        float depth = F.depth[p];
-        const float ao = F.aoEnabled ? F.ao[p] : 0.0f;
         const float aoStrength = 1.0f - ao;
         const int mode = F.renderMode;
         uint32_t solid = cb;
@@ -1282,6 +1287,15 @@ namespace isaac_hip
         F.color[p] = out;
         if(depthChanged)
             F.depth[p] = depth;
+    }
+
+    __device__ __attribute__((always_inline)) inline void shade_body(const FrameParams& F, uint32_t tile)
+    {
+        int px, py;
+        if(!tile_pixel_at(F, tile, px, py))
+            return;
+        const size_t p = size_t(py) * size_t(F.W) + size_t(px);
+        shade_pixel(F, px, py, F.aoEnabled ? F.ao[p] : 0.0f, F.depth[p]);
     }
 
     __global__ __launch_bounds__(256) void shade_kernel(const FrameParams F)
@@ -2905,13 +2919,22 @@ namespace isaac_hip
         return hipGetLastError();
     }
 
-    hipError_t launch_ssao(const FrameParams& F, hipStream_t s)
+    bool ssao_shades(const FrameParams& F)
+    {
+        // render modes 0, 1 and 3 leave the depth buffer as the march wrote it
+        return F.aoEnabled && (F.renderMode == 0 || F.renderMode == 1 || F.renderMode == 3);
+    }
+
+    hipError_t launch_ssao(const FrameParams& F, hipStream_t s, bool shade)
     {
         const unsigned bw = unsigned(F.vx1 - F.vx0 + kSsaoBW - 1) / kSsaoBW;
         const unsigned bh = unsigned(F.vy1 - F.vy0 + kSsaoBH - 1) / kSsaoBH;
         if(bw * bh == 0)
             return hipSuccess;
-        hipLaunchKernelGGL(ssao_tiled_kernel, dim3(bw * bh), dim3(256), 0, s, F);
+        if(shade)
+            hipLaunchKernelGGL(ssao_tiled_kernel<true>, dim3(bw * bh), dim3(256), 0, s, F);
+        else
+            hipLaunchKernelGGL(ssao_tiled_kernel<false>, dim3(bw * bh), dim3(256), 0, s, F);
         return hipGetLastError();
     }
 

@@ -461,6 +461,7 @@ class IsaacVisualization:
     TUNE_MARCH_THREADS = 4
     TUNE_MARCH_OFFSETS = 5
     TUNE_TIMING_EVENTS = 6
+    TUNE_FUSED_SHADING = 7
     tex_layout = 0
     render_form = 0
 

@@ -376,6 +376,26 @@ def test_tuning_rejects_unknown_values(gpu):
     vis.close()
 
 
+@pytest.mark.parametrize("mode", [0, 1, 3])
+def test_fused_shading_matches_separate_kernel(gpu, mode):
+    """In render modes 0, 1 and 3 the occlusion kernel shades its own pixels: the frame,
+    G-buffer and float output equal those of the separate shading kernel bit for bit."""
+    spec = dataclasses.replace(_reduced_prod(n=48, fb=(333, 201), particles=5000, visible_tf=True), render_mode=mode)
+    vis = _vis(spec, gpu)
+    vis.update_buffers()
+    out = {}
+    for fused in (1, 0):
+        vis.set_tuning(vis.TUNE_FUSED_SHADING, fused)
+        img = vis.draw()
+        out[fused] = (img, [a.copy() for a in vis.gbuffer()])
+        if not fused:
+            assert vis.timings()["shade_us"] > 0
+    assert np.array_equal(out[0][0], out[1][0])
+    for a, b in zip(out[0][1], out[1][1]):
+        assert np.array_equal(a, b)
+    vis.close()
+
+
 def test_timing_event_modes(gpu):
     """Timing events between all kernels (1), around the march only (2) or none (0):
     the same frame each time; get_timings reports what was recorded and zeros for the rest."""
@@ -384,7 +404,8 @@ def test_timing_event_modes(gpu):
     vis = _vis(_reduced_prod(n=32, fb=(96, 64), particles=2000), gpu)
     ref = vis.do_visualization()
     t = vis.timings()
-    assert t["march_us"] > 0 and t["shade_us"] > 0 and t["optimization_buffer_us"] > 0
+    # (the occlusion kernel shades its pixels in this render mode: ssao_us holds the shading)
+    assert t["march_us"] > 0 and t["ssao_us"] > 0 and t["optimization_buffer_us"] > 0
     for mode in (2, 0, 1):
         vis.set_tuning(vis.TUNE_TIMING_EVENTS, mode)
         assert np.array_equal(vis.do_visualization(), ref), mode
@@ -392,9 +413,9 @@ def test_timing_event_modes(gpu):
         if mode == 0:
             assert all(v == 0 for v in t.values()), t
         elif mode == 2:
-            assert t["march_us"] > 0 and t["shade_us"] == 0 and t["optimization_buffer_us"] == 0, t
+            assert t["march_us"] > 0 and t["ssao_us"] == 0 and t["optimization_buffer_us"] == 0, t
         else:
-            assert t["march_us"] > 0 and t["shade_us"] > 0 and t["optimization_buffer_us"] > 0, t
+            assert t["march_us"] > 0 and t["ssao_us"] > 0 and t["optimization_buffer_us"] > 0, t
     with pytest.raises(IsaacHipError):
         vis.set_tuning(vis.TUNE_TIMING_EVENTS, 3)
     vis.close()
