@@ -943,16 +943,11 @@ namespace isaac_hip
                                                                        __builtin_fmaf(stepVec.y, fk, startC.y),
                                                                        __builtin_fmaf(stepVec.z, fk, startC.z), C);
                         };
-                        Corners cur;
-                        at(i, cur);
-                        for(;;)
+                        // one step: the iso test and the volume sample of the corners c of step i
+                        auto process = [&](const Corners& cur) __attribute__((always_inline))
                         {
                             if(COUNT)
                                 nSteps++;
-                            Corners nxt;
-                            const bool ahead = PREFETCH && i + 1 <= last;
-                            if(ahead)
-                                at(i + 1, nxt);
                             if(i <= isoEnd)
                             {
                                 // the iso alpha in byte units; alpha >= 0.5 is a sum >= 127.5, which
@@ -1032,14 +1027,50 @@ namespace isaac_hip
                                 if(ca > 0.99f)
                                     volEnd = off;
                             }
-                            last = isoEnd > volEnd ? isoEnd : volEnd;
-                            if(i + 1 > last)
-                                break;
-                            i++;
-                            if(PREFETCH)
-                                cur = nxt;
-                            else
+                        };
+                        Corners cur;
+                        at(i, cur);
+                        if constexpr(PREFETCH)
+                        {
+                            // two buffers in turn: the next step's corners are requested before
+                            // this step's are composited, with no copy between the buffers (a hit
+                            // can extend the volume pass by one step past the range requested)
+                            Corners nxt;
+                            for(;;)
+                            {
+                                bool ahead = i + 1 <= last;
+                                if(ahead)
+                                    at(i + 1, nxt);
+                                process(cur);
+                                last = isoEnd > volEnd ? isoEnd : volEnd;
+                                if(i + 1 > last)
+                                    break;
+                                i++;
+                                if(!ahead)
+                                    at(i, nxt);
+                                ahead = i + 1 <= last;
+                                if(ahead)
+                                    at(i + 1, cur);
+                                process(nxt);
+                                last = isoEnd > volEnd ? isoEnd : volEnd;
+                                if(i + 1 > last)
+                                    break;
+                                i++;
+                                if(!ahead)
+                                    at(i, cur);
+                            }
+                        }
+                        else
+                        {
+                            for(;;)
+                            {
+                                process(cur);
+                                last = isoEnd > volEnd ? isoEnd : volEnd;
+                                if(i + 1 > last)
+                                    break;
+                                i++;
                                 at(i, cur);
+                            }
                         }
                     }
                     const bool isoHit = hitStep >= 0;
@@ -2869,6 +2900,8 @@ namespace isaac_hip
                     hipLaunchKernelGGL((march_kernel<false, false, true, false>), grid, block, 0, s, F, PP);
                 else if(v == 3)
                     hipLaunchKernelGGL((march_kernel<false, true, true, false>), grid, block, 0, s, F, PP);
+                else if(v == 5 && F.texOff32)
+                    hipLaunchKernelGGL((march_kernel<true, false, true, false, true>), grid, block, 0, s, F, PP);
                 else if(v == 5)
                     hipLaunchKernelGGL((march_kernel<true, false, true, false>), grid, block, 0, s, F, PP);
                 else
