@@ -2415,7 +2415,10 @@ namespace isaac_hip
                 const float c0 = f[s][k * D];
                 const float c1 = D > 1 ? f[s][k * D + 1] : 0.f;
                 const float c2 = D > 2 ? f[s][k * D + 2] : 0.f;
-                const float tv = apply_chain(S.chain, c0, c1, c2, 0.f, D);
+                // natural chains (identity, or one `length`) evaluated inline, in the
+                // interpreter's order, instead of interpreting the chain per voxel
+                const float tv = sig_natural(SIG) ? (D == 1 ? c0 : sqrtf(c0 * c0 + c1 * c1 + c2 * c2))
+                                                  : apply_chain(S.chain, c0, c1, c2, 0.f, D);
                 const float4 col = lds_lut[S.lutSlot * T + tf_index(tv, T)];
                 const float ca = col.w;
                 if(S.weight > 0.0f)
@@ -2551,16 +2554,27 @@ namespace isaac_hip
             if(blocks > 2048)
                 blocks = 2048;
             const dim3 grid = dim3(unsigned(blocks)), block(512);
+            bool natural = true;
+            for(int k = 0; k < Pc.nsrc; k++)
+                natural = natural && natural_chain(Pc.src[k].chain, Pc.src[k].dim);
+#define ISAAC_CLS_SIG(sg)                                                                                            \
+    case sg:                                                                                                         \
+        if(natural)                                                                                                  \
+            hipLaunchKernelGGL((classify_sig_kernel<sg + kSigNatural>), grid, block, lds, s, Pc);                     \
+        else                                                                                                         \
+            hipLaunchKernelGGL((classify_sig_kernel<sg>), grid, block, lds, s, Pc);                                   \
+        return hipGetLastError();
             switch(sig)
             {
-            case 1: hipLaunchKernelGGL((classify_sig_kernel<1>), grid, block, lds, s, Pc); return hipGetLastError();
-            case 3: hipLaunchKernelGGL((classify_sig_kernel<3>), grid, block, lds, s, Pc); return hipGetLastError();
-            case 6: hipLaunchKernelGGL((classify_sig_kernel<6>), grid, block, lds, s, Pc); return hipGetLastError();
-            case 8: hipLaunchKernelGGL((classify_sig_kernel<8>), grid, block, lds, s, Pc); return hipGetLastError();
-            case 16: hipLaunchKernelGGL((classify_sig_kernel<16>), grid, block, lds, s, Pc); return hipGetLastError();
-            case 18: hipLaunchKernelGGL((classify_sig_kernel<18>), grid, block, lds, s, Pc); return hipGetLastError();
+                ISAAC_CLS_SIG(1)
+                ISAAC_CLS_SIG(3)
+                ISAAC_CLS_SIG(6)
+                ISAAC_CLS_SIG(8)
+                ISAAC_CLS_SIG(16)
+                ISAAC_CLS_SIG(18)
             default: break;
             }
+#undef ISAAC_CLS_SIG
         }
         int64_t blocks = (P.nvox + 255) / 256;
         if(blocks > 4096)
