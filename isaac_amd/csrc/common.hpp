@@ -278,15 +278,16 @@ namespace isaac_hip
     }
 
     // (uint32_t) roundf(gclamp(v, 0, 1) * 255): the clamp is one median-of-three, and
-    // round half away from zero of x in [0, 255] is x < 0.5 ? 0 : trunc(x + 0.5) --
-    // x + 0.5 is exact or rounds without crossing an integer for every x >= 0.5 (all
-    // binary32 v in [0, 1] checked, tests/cpp/round_check.c).  A NaN v gives 0 here
-    // (the median of NaN, 0, 1 is the minimum of the other two) as the conversion of
-    // NaN gives 0 in the direct form.
+    // round half away from zero of x = v * 255 is trunc(x + h) with h = 0x1.fffffep-2, the
+    // largest binary32 below 0.5: for x < 0.5 the sum stays below 1 (with h = 0.5 it rounds
+    // to 1 just below x = 0.5), and for x >= 0.5 it crosses the next integer exactly when
+    // x's fraction is >= 0.5 (every binary32 v in [0, 1] checked, tests/cpp/round_check.c).
+    // A NaN v gives 0 here (the median of NaN, 0, 1 is the minimum of the other two) as the
+    // conversion of NaN gives 0 in the direct form.
     __device__ inline uint32_t unorm8(float v)
     {
         const float x = __builtin_amdgcn_fmed3f(v, 0.0f, 1.0f) * 255.0f;
-        return x < 0.5f ? 0u : (uint32_t) (x + 0.5f);
+        return (uint32_t) (x + 0x1.fffffep-2f);
     }
 
     // The march texel of texture layout 16: the volume RGBA8 bytes as binary16 values

@@ -1,7 +1,9 @@
 /* Exhaustive checks of the rounding shortcuts used by the gfx950 classification
  * and transfer-function lookups (isaac_amd/csrc/common.hpp unorm8, tf_index_of):
  *  (1) byte quantisation: for every binary32 v in [0, 1],
- *        x = v * 255;  x < 0.5 ? 0 : (uint32_t)(x + 0.5)  ==  (uint32_t) roundf(x)
+ *        x = v * 255;  (uint32_t)(x + h)  ==  (uint32_t) roundf(x),
+ *      h = 0x1.fffffep-2 (the largest binary32 below 0.5); and, as a check that the
+ *      loop sees the inputs where it matters, x + 0.5 in place of x + h does differ
  *  (2) transfer-function index: for every binary32 x in [-2, T + 2] (T = 1024)
  *      and +-0, +-inf, +-FLT_MAX,
  *        x >= 0.5 ? (int) min(x + 0.5, T - 1) : 0
@@ -52,13 +54,15 @@ static uint64_t check_tf(float x, int T, uint64_t* n)
 
 int main(void)
 {
-    uint64_t bad8 = 0, n8 = 0;
+    uint64_t bad8 = 0, n8 = 0, half8 = 0;
     const uint32_t one = 0x3f800000u;
     for(uint32_t u = 0; u <= one; u++)
     {
         const float x = f_of(u) * 255.0f;
         const uint32_t a = (uint32_t) roundf(x);
-        const uint32_t b = x < 0.5f ? 0u : (uint32_t) (x + 0.5f);
+        const uint32_t b = (uint32_t) (x + 0x1.fffffep-2f);
+        if((uint32_t) (x + 0.5f) != a)
+            half8++;
         n8++;
         if(a != b)
         {
@@ -84,7 +88,8 @@ int main(void)
         badt += check_tf(special[i], T, &nt);
     if(tf_new(f_of(0x7fc00000u), T) != 0 || tf_new(f_of(0x7fa00000u), T) != 0)
         badt++;
+    printf("unorm8 with x + 0.5 in place of x + h: %llu inputs differ from roundf\n", (unsigned long long) half8);
     printf("unorm8: %llu inputs, mismatches %llu; tf_index: %llu inputs, mismatches %llu\n", (unsigned long long) n8,
            (unsigned long long) bad8, (unsigned long long) nt, (unsigned long long) badt);
-    return (bad8 == 0 && badt == 0) ? 0 : 1;
+    return (bad8 == 0 && badt == 0 && half8 > 0) ? 0 : 1;
 }

@@ -34,7 +34,8 @@ def test_general_division_identity_sampled(tmp_path):
 
 def test_rounding_shortcuts_exhaustive(tmp_path):
     """Classification and the transfer-function lookups replace roundf by
-    x < 0.5 ? 0 : trunc(x + 0.5) (byte quantisation of [0, 1] x 255) and
+    trunc(x + 0x1.fffffep-2) (byte quantisation of [0, 1] x 255; with 0.5 in place of
+    the largest binary32 below it the check must find differences) and
     x >= 0.5 ? trunc(min(x + 0.5, T - 1)) : 0 (transfer index): every binary32
     input in the ranges where the two forms could differ."""
     exe = tmp_path / "round_check"
