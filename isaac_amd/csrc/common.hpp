@@ -304,13 +304,13 @@ namespace isaac_hip
     }
 
     // transfer-function index of x = value * T: roundf(x), NaN -> 0, clamped to
-    // [0, T - 1] (lib/isaac/isaac_common_kernel.hpp:590-592), as
-    // x >= 0.5 ? trunc(min(x + 0.5, T - 1)) : 0 (every binary32 x in [-2, T + 2] and
-    // the infinities checked for T = 1024, tests/cpp/round_check.c)
-    __host__ __device__ inline int tf_index_of(float x, int T)
+    // [0, T - 1] (lib/isaac/isaac_common_kernel.hpp:590-592), as trunc(med3(x + h, 0, T - 1))
+    // with h = 0x1.fffffep-2 as in unorm8 (every binary32 x in [-2, T + 2] and the
+    // infinities checked for T = 1024, tests/cpp/round_check.c); a NaN x gives NaN + h, whose
+    // median with 0 and T - 1 is 0
+    __device__ inline int tf_index_of(float x, int T)
     {
-        // x + 0.5 is not NaN on this branch, so fminf is gmin
-        return x >= 0.5f ? (int) fminf(x + 0.5f, float(T - 1)) : 0;
+        return (int) __builtin_amdgcn_fmed3f(x + 0x1.fffffep-2f, 0.0f, float(T - 1));
     }
 
     // transformColor float4 -> byte4 (truncating, lib/isaac/isaac_helper.hpp:71-74)

@@ -6,7 +6,7 @@
  *      loop sees the inputs where it matters, x + 0.5 in place of x + h does differ
  *  (2) transfer-function index: for every binary32 x in [-2, T + 2] (T = 1024)
  *      and +-0, +-inf, +-FLT_MAX,
- *        x >= 0.5 ? (int) min(x + 0.5, T - 1) : 0
+ *        (int) med3(x + h, 0, T - 1)   (med3 as v_med3_f32: NaN -> the minimum of the others)
  *      equals the reference form lv = roundf(x); NaN -> 0; (int) clamp(lv, 0, T - 1),
  *      and, for NaN x, both are 0 (checked on one quiet and one signalling pattern).
  * Prints the mismatch counts; exit status 0 when both are 0. */
@@ -35,9 +35,16 @@ static int tf_ref(float x, int T)
     return (int) gclamp(lv, 0.0f, (float) (T - 1));
 }
 
+/* v_med3_f32: the median, or, when an input is NaN, the minimum of the others */
+static float hw_med3(float a, float b, float c)
+{
+    if(a != a)
+        return fminf(b, c);
+    return fmaxf(fminf(a, b), fminf(fmaxf(a, b), c));
+}
 static int tf_new(float x, int T)
 {
-    return x >= 0.5f ? (int) fminf(x + 0.5f, (float) (T - 1)) : 0;
+    return (int) hw_med3(x + 0x1.fffffep-2f, 0.0f, (float) (T - 1));
 }
 
 static uint64_t check_tf(float x, int T, uint64_t* n)

@@ -36,7 +36,7 @@ def test_rounding_shortcuts_exhaustive(tmp_path):
     """Classification and the transfer-function lookups replace roundf by
     trunc(x + 0x1.fffffep-2) (byte quantisation of [0, 1] x 255; with 0.5 in place of
     the largest binary32 below it the check must find differences) and
-    x >= 0.5 ? trunc(min(x + 0.5, T - 1)) : 0 (transfer index): every binary32
+    trunc(med3(x + 0x1.fffffep-2, 0, T - 1)) (transfer index): every binary32
     input in the ranges where the two forms could differ."""
     exe = tmp_path / "round_check"
     subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-o", str(exe), os.path.join(HERE, "cpp", "round_check.c"), "-lm"])
