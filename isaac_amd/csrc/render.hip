@@ -1184,7 +1184,17 @@ namespace isaac_hip
     constexpr int kSsaoRW = kSsaoBW + 2 * kSsaoReach, kSsaoRH = kSsaoBH + 2 * kSsaoReach; // 124 x 92
     // SHADE: the block also shades its pixels with the occlusion term it just formed (render
     // modes that leave the depth buffer alone, so no block changes a depth another one reads)
-    __device__ __attribute__((always_inline)) inline void shade_pixel(const FrameParams& F, int px, int py, float ao, float depth);
+    // the G-buffer values the shading of one pixel reads (besides occlusion and depth)
+    struct ShadeIn
+    {
+        uint32_t cb;
+        F3 normal;
+        float4 v;
+    };
+
+    __device__ __attribute__((always_inline)) inline ShadeIn shade_load(const FrameParams& F, size_t p);
+    __device__ __attribute__((always_inline)) inline void shade_finish(const FrameParams& F, int px, int py, const ShadeIn& in,
+                                                                      float ao, float depth);
 
     template<bool SHADE>
     __global__ __launch_bounds__(256) void ssao_tiled_kernel(const FrameParams F)
@@ -1214,6 +1224,10 @@ namespace isaac_hip
             const int px = x0 + (q & (kSsaoBW - 1)), py = y0 + q / kSsaoBW;
             if(px >= F.vx1 || py >= F.vy1)
                 continue;
+            // (SHADE: this pixel's G-buffer values are requested before its taps are counted)
+            ShadeIn in;
+            if(SHADE)
+                in = shade_load(F, size_t(py) * size_t(F.W) + size_t(px));
             // the tap (-3, -3) of this pixel; every tap is a fixed offset from it
             const float* t0 = region + (py - y0) * kSsaoRW + (px - x0);
             const float ref = t0[kSsaoReach * kSsaoRW + kSsaoReach];
@@ -1227,7 +1241,7 @@ namespace isaac_hip
             const float ao = gclamp(occ / 42.0f, 0.0f, 1.0f);
             F.ao[size_t(py) * F.W + px] = ao;
             if(SHADE)
-                shade_pixel(F, px, py, ao, ref);
+                shade_finish(F, px, py, in, ao, ref);
         }
     }
 
@@ -1235,14 +1249,24 @@ namespace isaac_hip
     // ShadingKernel (lib/isaac/isaac_common_kernel.hpp:220-320), then the volume
     // pass's final blend color + (1 - color.a) * solid
     // (lib/isaac/isaac_volume_kernel.hpp:83-86) for pixels whose volume pass ran.
-    // one pixel, given its occlusion term and depth
-    __device__ __attribute__((always_inline)) inline void shade_pixel(const FrameParams& F, int px, int py, float ao, float depth)
+    __device__ __attribute__((always_inline)) inline ShadeIn shade_load(const FrameParams& F, size_t p)
+    {
+        ShadeIn in;
+        in.cb = F.color[p];
+        in.normal = f3(F.normal[3 * p + 0], F.normal[3 * p + 1], F.normal[3 * p + 2]);
+        in.v = F.volColor[p];
+        return in;
+    }
+
+    // one pixel, given its G-buffer values, occlusion term and depth
+    __device__ __attribute__((always_inline)) inline void shade_finish(const FrameParams& F, int px, int py, const ShadeIn& in,
+                                                                      float ao, float depth)
     {
         const size_t p = size_t(py) * size_t(F.W) + size_t(px);
-        const uint32_t cb = F.color[p];
+        const uint32_t cb = in.cb;
         F4 color = F4{div255(float(cb & 0xffu)), div255(float((cb >> 8) & 0xffu)), div255(float((cb >> 16) & 0xffu)),
                       div255(float(cb >> 24))};
-        const F3 normal = f3(F.normal[3 * p + 0], F.normal[3 * p + 1], F.normal[3 * p + 2]);
+        const F3 normal = in.normal;
         const float aoStrength = 1.0f - ao;
         const int mode = F.renderMode;
         uint32_t solid = cb;
@@ -1303,7 +1327,7 @@ namespace isaac_hip
             depthChanged = true;
         }
         uint32_t out = solid;
-        const float4 v = F.volColor[p];
+        const float4 v = in.v;
         const float sr = div255(float(solid & 0xffu)), sg = div255(float((solid >> 8) & 0xffu));
         const float sb = div255(float((solid >> 16) & 0xffu)), sa = div255(float(solid >> 24));
         float4 fo = make_float4(sr, sg, sb, sa);
@@ -1326,7 +1350,7 @@ namespace isaac_hip
         if(!tile_pixel_at(F, tile, px, py))
             return;
         const size_t p = size_t(py) * size_t(F.W) + size_t(px);
-        shade_pixel(F, px, py, F.aoEnabled ? F.ao[p] : 0.0f, F.depth[p]);
+        shade_finish(F, px, py, shade_load(F, p), F.aoEnabled ? F.ao[p] : 0.0f, F.depth[p]);
     }
 
     __global__ __launch_bounds__(256) void shade_kernel(const FrameParams F)
