@@ -2,8 +2,9 @@
 
 Times the march kernel of a visualization under each candidate schedule
 (include/isaac_hip.h, "tuning": march occupancy target, next-step prefetch,
-tile order over the XCDs and, in the combined form, the texture layout: linear
-or 8^3 bricks, and the workgroup size: 4 waves or one) on the caller's own frames and keeps the fastest.  Results do not
+tile order over the XCDs and, in the combined form, the texture layout: linear,
+8^3 bricks or, where the sources stay put between frames, the binary16 copy, and the
+workgroup size: 4 waves or one) on the caller's own frames and keeps the fastest.  Results do not
 depend on the schedule, only speed does; choices are cached per workload key.
 """
 from __future__ import annotations
